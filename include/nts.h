@@ -1,0 +1,203 @@
+/* nts.h — C ABI of the MI355X (gfx950) differentiable Gaussian-splat
+ * rasteriser.
+ *
+ * This is the drop-in boundary for the public Rust surface of
+ * crates/nano-optimize (src/lib.rs:21-45).  Each declaration names the
+ * reference item it replaces.  A thin Rust FFI crate binds these symbols
+ * (extern "C" + safe wrappers); Python binds them with ctypes.
+ *
+ * Conventions
+ *  - Every call returns int: 0 = ok, negative = NTS_E_*.  Nothing aborts
+ *    or throws across the ABI; nts_last_error() gives the message of the
+ *    last failure on the calling thread.
+ *  - Work is asynchronous on the context's HIP stream (default priority).
+ *    nts_sync() and the *_download calls are the blocking calls.
+ *  - The library owns all device memory; the caller owns host arrays.
+ *  - One nts_ctx per host thread; handles are usable only with the context
+ *    that created them.
+ */
+#ifndef NTS_H
+#define NTS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NTS_OK 0
+#define NTS_E_INVALID (-1)      /* bad argument / size mismatch */
+#define NTS_E_HIP (-2)          /* HIP runtime error */
+#define NTS_E_NOMEM (-3)        /* device allocation or HBM budget exceeded */
+#define NTS_E_CAPACITY (-4)     /* a device capacity was exceeded (pairs, splats) */
+#define NTS_E_STATE (-5)        /* handle used in the wrong state */
+
+typedef struct nts_ctx nts_ctx;
+typedef struct nts_buffer nts_buffer;
+typedef struct nts_scene nts_scene;
+typedef struct nts_bins nts_bins;
+
+/* Mirror of CameraUniform (crates/nano-optimize/src/raster.rs:61-71), 96 B:
+ * column-major view matrix, camera position, near plane, focal length in
+ * pixels, image size, splat count. */
+typedef struct nts_camera {
+  float view[16];
+  float cam_pos[3];
+  float near_;
+  float focal;
+  float width;
+  float height;
+  uint32_t n_splats;
+} nts_camera;
+
+/* Mirror of TilingParams (crates/nano-optimize/src/tile_binner.rs:21-42). */
+typedef struct nts_tiling {
+  uint32_t width, height, tile_size;
+  float depth_max;
+} nts_tiling;
+
+/* Host splat arrays in the SplatBuffer layout
+ * (crates/nano-optimize/src/splat_store.rs:21-36): pos xyz, rotation wxyz,
+ * log-scale xyz, opacity logit, SH DC rgb, 45 SH-rest planar per splat. */
+typedef struct nts_splats_host {
+  uint32_t n;
+  float* pos;        /* 3n */
+  float* rot;        /* 4n */
+  float* log_scale;  /* 3n */
+  float* logit;      /* n  */
+  float* sh_dc;      /* 3n */
+  float* sh_rest;    /* 45n */
+} nts_splats_host;
+
+/* Mirror of AdamConfig (crates/nano-optimize/src/adam.rs:22-39). */
+typedef struct nts_adam_cfg {
+  float lr, beta1, beta2, eps;
+} nts_adam_cfg;
+
+/* Constraint bounds (crates/nano-optimize/src/train.rs:37-39). */
+typedef struct nts_constraints {
+  float log_scale_min, log_scale_max, logit_max;
+} nts_constraints;
+
+#define NTS_LOSS_L1 0
+#define NTS_LOSS_MSE 1
+
+#define NTS_BWD_REF_PARITY 0  /* rasterize_backward.wgsl semantics */
+#define NTS_BWD_RECOMPUTE 1   /* final-T + last-index recompute, full chain rule */
+
+/* ---- context (replaces WgpuCtx::new, gpu.rs:31-77) -------------------- */
+int nts_ctx_create(int device, uint64_t hbm_budget_bytes, nts_ctx** out);
+int nts_ctx_destroy(nts_ctx* ctx);
+int nts_sync(nts_ctx* ctx);
+const char* nts_last_error(void);
+int nts_version(void);
+/* Per-kernel HIP-event timing on the context stream (off by default). */
+int nts_prof_enable(nts_ctx* ctx, int on);
+int nts_prof_reset(nts_ctx* ctx);
+/* Total ms and launch count of kernels whose name starts with `prefix`. */
+int nts_prof_query(nts_ctx* ctx, const char* prefix, double* total_ms, uint64_t* launches);
+/* Newline-separated "name launches total_ms" report into buf. */
+int nts_prof_report(nts_ctx* ctx, char* buf, size_t cap);
+
+/* ---- device buffers (replaces WgpuCtx::storage_buffer[_zeroed] and
+ *      readback, gpu.rs:79-152) ------------------------------------------ */
+int nts_buffer_create(nts_ctx* ctx, uint64_t bytes, nts_buffer** out);
+int nts_buffer_destroy(nts_ctx* ctx, nts_buffer* buf);
+uint64_t nts_buffer_size(const nts_buffer* buf);
+int nts_buffer_upload(nts_ctx* ctx, nts_buffer* buf, uint64_t offset, const void* host, uint64_t bytes);
+int nts_buffer_download(nts_ctx* ctx, const nts_buffer* buf, uint64_t offset, void* host, uint64_t bytes);
+int nts_buffer_zero(nts_ctx* ctx, nts_buffer* buf);
+
+/* ---- scene: parameters + gradients + Adam moments + grad accumulator
+ *      (replaces GpuSplatBuffer, GradSplatBuffers, AdamMomentBuffers,
+ *      splat_gpu.rs:36-384) ---------------------------------------------- */
+int nts_scene_create(nts_ctx* ctx, uint32_t capacity, nts_scene** out);
+int nts_scene_destroy(nts_ctx* ctx, nts_scene* scene);
+uint32_t nts_scene_count(const nts_scene* scene);
+/* GpuSplatBuffer::upload / sync_from (splat_gpu.rs:94-150). Resets moments,
+ * gradients and grad_acc to zero. */
+int nts_scene_upload(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* host);
+/* GpuSplatBuffer::readback (splat_gpu.rs:152-187); host->n is set. */
+int nts_scene_download(nts_ctx* ctx, const nts_scene* scene, nts_splats_host* host);
+/* GradSplatBuffers readback (train.rs:370-389 reads them for norms). */
+int nts_scene_download_grads(nts_ctx* ctx, const nts_scene* scene, nts_splats_host* grads);
+/* Overwrite gradients (tests feed synthetic gradients into Adam). */
+int nts_scene_upload_grads(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* grads);
+/* GradSplatBuffers::zero (splat_gpu.rs:63-75). */
+int nts_scene_zero_grads(nts_ctx* ctx, nts_scene* scene);
+/* AdamMomentBuffers::{download_to_cpu, upload_from_cpu} (splat_gpu.rs:229-384). */
+int nts_scene_download_moments(nts_ctx* ctx, const nts_scene* scene, nts_splats_host* m, nts_splats_host* v);
+int nts_scene_upload_moments(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* m, const nts_splats_host* v);
+int nts_scene_download_grad_acc(nts_ctx* ctx, const nts_scene* scene, float* host);
+int nts_scene_upload_grad_acc(nts_ctx* ctx, nts_scene* scene, const float* host);
+
+/* ---- forward (Rasterizer::project, raster.rs:731-794) ----------------- */
+/* CameraUniform::from_pose (raster.rs:79-99): glam look_at_rh, near 0.01. */
+int nts_camera_from_pose(const float eye[3], const float target[3], const float up[3], float fov_y,
+                         uint32_t width, uint32_t height, uint32_t n_splats, nts_camera* out);
+/* projected: >= 48*n bytes of ProjectedSplat records (raster.rs:27-39). */
+int nts_project(nts_ctx* ctx, const nts_scene* scene, const nts_camera* cam, nts_buffer* projected);
+
+/* ---- binning (TileBinner::bin, tile_binner.rs:181-382) ---------------- */
+/* Count, scan, emit 64-bit (tile << 16 | depth16) keys, onesweep radix sort,
+ * per-tile ranges.  Like the reference it reads the pair total back. */
+int nts_bin(nts_ctx* ctx, const nts_buffer* projected, uint32_t n_splats, const nts_tiling* tiling,
+            nts_bins** out);
+int nts_bins_destroy(nts_ctx* ctx, nts_bins* bins);
+int nts_bins_info(const nts_bins* bins, uint64_t* total_pairs, uint32_t* num_tiles);
+/* Sorted keys in the reference's u32 format (tile_id << 16 | depth16);
+ * NTS_E_CAPACITY when the tile id does not fit 16 bits. */
+int nts_bins_download_keys_u32(nts_ctx* ctx, const nts_bins* bins, uint32_t* host);
+int nts_bins_download_keys_u64(nts_ctx* ctx, const nts_bins* bins, uint64_t* host);
+int nts_bins_download_payloads(nts_ctx* ctx, const nts_bins* bins, uint32_t* host);
+/* ranges: 2 u32 per tile [begin, end), empty tiles (0, 0). */
+int nts_bins_download_ranges(nts_ctx* ctx, const nts_bins* bins, uint32_t* host);
+
+/* PrefixScan::scan (prefix_scan.rs:127-186): exclusive, wrapping u32, in place. */
+int nts_scan_u32(nts_ctx* ctx, nts_buffer* data, uint64_t n);
+/* RadixSort::sort (radix_sort.rs:144-303): stable sort of u32 keys with u32
+ * payloads, in place.  key_bits limits the sorted low bits (32 = all). */
+int nts_sort_pairs_u32(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits);
+int nts_sort_pairs_u64(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits);
+
+/* ---- composite (Rasterizer::composite, raster.rs:468-529) ------------- */
+/* image: 16 B per pixel (r, g, b, 1 - T).  Any tile size in {8, 16, 32};
+ * the reference's kernel is fixed at 16. */
+int nts_composite(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,
+                  const nts_tiling* tiling);
+
+/* ---- loss (dL/dC contract, train.rs:180-196, loss.rs:163-179) ---------- */
+/* dl_dc: 16 B per pixel (w = 0).  *loss receives the scalar loss. */
+int nts_loss_grad(nts_ctx* ctx, const nts_buffer* image, const nts_buffer* target, nts_buffer* dl_dc,
+                  uint32_t width, uint32_t height, int kind, double* loss);
+
+/* ---- backward (Rasterizer::composite_backward, raster.rs:653-724;
+ *      project_backward, raster.rs:559-645) ---------------------------- */
+/* projected_grad: 48 B per splat, accumulated into (zero it first, like
+ * Rasterizer::zero_projected_grad).  Mode NTS_BWD_REF_PARITY only here. */
+int nts_composite_backward(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins,
+                           const nts_buffer* forward_out, const nts_buffer* dl_dc,
+                           nts_buffer* projected_grad, const nts_tiling* tiling);
+/* Writes the scene's parameter gradients. */
+int nts_project_backward(nts_ctx* ctx, nts_scene* scene, const nts_buffer* projected_grad,
+                         const nts_camera* cam);
+
+/* ---- optimiser (AdamGpu, adam_gpu.rs:334-541) ------------------------- */
+/* AdamGpu::step on a flat slab of n floats (adam_step.wgsl). */
+int nts_adam_step_flat(nts_ctx* ctx, nts_buffer* params, const nts_buffer* grads, nts_buffer* m,
+                       nts_buffer* v, uint64_t n, const nts_adam_cfg* cfg, uint64_t t);
+/* accumulate_grad + step_all + apply_constraints fused in one kernel:
+ * grad_acc += |dpos|_1, Adam on every attribute (positions use cfg_pos,
+ * the rest cfg_attr, bias corrections with step t), then quaternion
+ * renormalisation and the log-scale / logit clamps. */
+int nts_adam_step(nts_ctx* ctx, nts_scene* scene, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr,
+                  uint64_t t, const nts_constraints* cons);
+
+/* ---- viewer (Rasterizer::tonemap, raster.rs:405-455) ------------------- */
+int nts_tonemap(nts_ctx* ctx, const nts_buffer* image, nts_buffer* rgba8, uint32_t width, uint32_t height);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NTS_H */

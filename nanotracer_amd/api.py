@@ -1,0 +1,377 @@
+"""Host-side mirror of the public surface of crates/nano-optimize
+(src/lib.rs:21-45) over the C ABI.  Names follow the reference: GpuCtx for
+WgpuCtx, GpuSplatBuffer, Rasterizer, TileBinner, PrefixScan, RadixSort,
+AdamGpu, CameraUniform.from_pose, TilingParams.  Every call runs on the
+gfx950 kernels of libnts.so; there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .synth import HostSplats
+
+PROJECTED_BYTES = 48        # ProjectedSplat, raster.rs:27-39
+PROJECTED_GRAD_BYTES = 48   # ProjectedGrad, raster.rs:46-57
+
+
+class Buffer:
+    """A device allocation owned by the library (wgpu::Buffer stand-in)."""
+
+    def __init__(self, ctx: "GpuCtx", nbytes: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        L.check(L.lib().nts_buffer_create(ctx.h, nbytes, C.byref(h)))
+        self.h = h
+        self.nbytes = nbytes
+
+    def upload(self, a: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        L.check(L.lib().nts_buffer_upload(self.ctx.h, self.h, offset, L.vptr(a), a.nbytes))
+
+    def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        L.check(L.lib().nts_buffer_download(self.ctx.h, self.h, offset, L.vptr(out), out.nbytes))
+        return out
+
+    def zero(self) -> None:
+        L.check(L.lib().nts_buffer_zero(self.ctx.h, self.h))
+
+    def free(self) -> None:
+        if self.h:
+            L.lib().nts_buffer_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class GpuCtx:
+    """WgpuCtx (gpu.rs:21-152) over a HIP device and stream."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        L.check(L.lib().nts_ctx_create(device, hbm_budget, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            L.lib().nts_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def storage_buffer(self, a: np.ndarray) -> Buffer:
+        a = np.ascontiguousarray(a)
+        b = Buffer(self, max(a.nbytes, 4))
+        b.upload(a)
+        return b
+
+    def storage_buffer_zeroed(self, nbytes: int) -> Buffer:
+        return Buffer(self, max(int(nbytes), 4))
+
+    def readback(self, buf: Buffer, dtype, count: int) -> np.ndarray:
+        return buf.download(dtype, count)
+
+    def sync(self) -> None:
+        L.check(L.lib().nts_sync(self.h))
+
+    # per-kernel HIP-event timing
+    def prof_enable(self, on: bool = True) -> None:
+        L.check(L.lib().nts_prof_enable(self.h, int(on)))
+
+    def prof_reset(self) -> None:
+        L.check(L.lib().nts_prof_reset(self.h))
+
+    def prof_query(self, prefix: str):
+        ms, n = C.c_double(), C.c_uint64()
+        L.check(L.lib().nts_prof_query(self.h, prefix.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def prof_report(self) -> dict:
+        buf = C.create_string_buffer(1 << 16)
+        L.check(L.lib().nts_prof_report(self.h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.rsplit(" ", 2)
+            out[name] = (int(n), float(ms))
+        return out
+
+
+def _host_struct(s: HostSplats) -> L.SplatsHost:
+    hs = L.SplatsHost()
+    hs.n = s.n
+    hs.pos, hs.rot, hs.log_scale = L.fptr(s.pos), L.fptr(s.rot), L.fptr(s.log_scale)
+    hs.logit, hs.sh_dc, hs.sh_rest = L.fptr(s.logit), L.fptr(s.sh_dc), L.fptr(s.sh_rest)
+    return hs
+
+
+class GpuSplatBuffer:
+    """GpuSplatBuffer + GradSplatBuffers + AdamMomentBuffers + grad_acc
+    (splat_gpu.rs:36-384) as one device scene of attribute planes."""
+
+    def __init__(self, ctx: GpuCtx, capacity: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        L.check(L.lib().nts_scene_create(ctx.h, max(1, int(capacity)), C.byref(h)))
+        self.h = h
+        self.capacity = capacity
+
+    @classmethod
+    def upload(cls, ctx: GpuCtx, splats: HostSplats, capacity: int | None = None) -> "GpuSplatBuffer":
+        g = cls(ctx, capacity or max(1, splats.n))
+        g.sync_from(splats)
+        return g
+
+    @property
+    def n(self) -> int:
+        return int(L.lib().nts_scene_count(self.h))
+
+    def sync_from(self, splats: HostSplats) -> None:
+        L.check(L.lib().nts_scene_upload(self.ctx.h, self.h, C.byref(_host_struct(splats))))
+
+    def readback(self) -> HostSplats:
+        out = HostSplats.zeros(self.n)
+        L.check(L.lib().nts_scene_download(self.ctx.h, self.h, C.byref(_host_struct(out))))
+        return out
+
+    def grads(self) -> HostSplats:
+        out = HostSplats.zeros(self.n)
+        L.check(L.lib().nts_scene_download_grads(self.ctx.h, self.h, C.byref(_host_struct(out))))
+        return out
+
+    def set_grads(self, g: HostSplats) -> None:
+        L.check(L.lib().nts_scene_upload_grads(self.ctx.h, self.h, C.byref(_host_struct(g))))
+
+    def zero_grads(self) -> None:
+        L.check(L.lib().nts_scene_zero_grads(self.ctx.h, self.h))
+
+    def moments(self):
+        m, v = HostSplats.zeros(self.n), HostSplats.zeros(self.n)
+        L.check(L.lib().nts_scene_download_moments(self.ctx.h, self.h, C.byref(_host_struct(m)),
+                                                   C.byref(_host_struct(v))))
+        return m, v
+
+    def set_moments(self, m: HostSplats, v: HostSplats) -> None:
+        L.check(L.lib().nts_scene_upload_moments(self.ctx.h, self.h, C.byref(_host_struct(m)),
+                                                 C.byref(_host_struct(v))))
+
+    def grad_acc(self) -> np.ndarray:
+        out = np.zeros(self.n, np.float32)
+        L.check(L.lib().nts_scene_download_grad_acc(self.ctx.h, self.h, L.vptr(out)))
+        return out
+
+    def set_grad_acc(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        L.check(L.lib().nts_scene_upload_grad_acc(self.ctx.h, self.h, L.vptr(a)))
+
+    def free(self) -> None:
+        if self.h:
+            L.lib().nts_scene_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class CameraUniform:
+    """CameraUniform::from_pose (raster.rs:79-99)."""
+
+    @staticmethod
+    def from_pose(eye, target, up, fov_y: float, width: int, height: int, n_splats: int) -> L.Camera:
+        f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+        cam = L.Camera()
+        L.check(L.lib().nts_camera_from_pose(f3(eye), f3(target), f3(up), float(fov_y), int(width), int(height),
+                                             int(n_splats), C.byref(cam)))
+        return cam
+
+
+@dataclass
+class TilingParams:
+    """TilingParams (tile_binner.rs:21-42)."""
+
+    width: int
+    height: int
+    tile_size: int = 16
+    depth_max: float = 100.0
+
+    def tiles_x(self) -> int:
+        return -(-self.width // self.tile_size)
+
+    def tiles_y(self) -> int:
+        return -(-self.height // self.tile_size)
+
+    def num_tiles(self) -> int:
+        return self.tiles_x() * self.tiles_y()
+
+    def c(self) -> L.Tiling:
+        return L.Tiling(self.width, self.height, self.tile_size, self.depth_max)
+
+
+class TileBinningResult:
+    """TileBinningResult (tile_binner.rs:77-83); device-resident."""
+
+    def __init__(self, ctx: GpuCtx, h):
+        self.ctx = ctx
+        self.h = h
+        tp, nt = C.c_uint64(), C.c_uint32()
+        L.check(L.lib().nts_bins_info(h, C.byref(tp), C.byref(nt)))
+        self.total_pairs = tp.value
+        self.num_tiles = nt.value
+
+    def sorted_keys(self) -> np.ndarray:
+        out = np.zeros(self.total_pairs, np.uint32)
+        L.check(L.lib().nts_bins_download_keys_u32(self.ctx.h, self.h, L.vptr(out)))
+        return out
+
+    def sorted_keys_u64(self) -> np.ndarray:
+        out = np.zeros(self.total_pairs, np.uint64)
+        L.check(L.lib().nts_bins_download_keys_u64(self.ctx.h, self.h, L.vptr(out)))
+        return out
+
+    def sorted_payloads(self) -> np.ndarray:
+        out = np.zeros(self.total_pairs, np.uint32)
+        L.check(L.lib().nts_bins_download_payloads(self.ctx.h, self.h, L.vptr(out)))
+        return out
+
+    def tile_ranges(self) -> np.ndarray:
+        out = np.zeros(2 * self.num_tiles, np.uint32)
+        L.check(L.lib().nts_bins_download_ranges(self.ctx.h, self.h, L.vptr(out)))
+        return out
+
+    def free(self) -> None:
+        if self.h:
+            L.lib().nts_bins_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class TileBinner:
+    """TileBinner::bin (tile_binner.rs:181-382)."""
+
+    def __init__(self, ctx: GpuCtx | None = None):
+        pass
+
+    def bin(self, ctx: GpuCtx, projected: Buffer, n_splats: int, params: TilingParams) -> TileBinningResult:
+        h = C.c_void_p()
+        L.check(L.lib().nts_bin(ctx.h, projected.h, int(n_splats), C.byref(params.c()), C.byref(h)))
+        return TileBinningResult(ctx, h)
+
+
+class PrefixScan:
+    """PrefixScan::scan (prefix_scan.rs:127-186): exclusive u32, in place."""
+
+    def __init__(self, ctx: GpuCtx | None = None):
+        pass
+
+    def scan(self, ctx: GpuCtx, buf: Buffer, n: int) -> None:
+        L.check(L.lib().nts_scan_u32(ctx.h, buf.h, int(n)))
+
+
+class RadixSort:
+    """RadixSort::sort (radix_sort.rs:144-303): stable, in place."""
+
+    def __init__(self, ctx: GpuCtx | None = None):
+        pass
+
+    def sort(self, ctx: GpuCtx, keys: Buffer, vals: Buffer, n: int, key_bits: int = 32) -> None:
+        L.check(L.lib().nts_sort_pairs_u32(ctx.h, keys.h, vals.h, int(n), int(key_bits)))
+
+    def sort_u64(self, ctx: GpuCtx, keys: Buffer, vals: Buffer, n: int, key_bits: int = 64) -> None:
+        L.check(L.lib().nts_sort_pairs_u64(ctx.h, keys.h, vals.h, int(n), int(key_bits)))
+
+
+class Rasterizer:
+    """Rasterizer (raster.rs:114-804)."""
+
+    def __init__(self, ctx: GpuCtx | None = None):
+        pass
+
+    def alloc_image(self, ctx: GpuCtx, width: int, height: int) -> Buffer:
+        return ctx.storage_buffer_zeroed(width * height * 16)
+
+    def alloc_projected(self, ctx: GpuCtx, n: int) -> Buffer:
+        return ctx.storage_buffer_zeroed(max(n, 1) * PROJECTED_BYTES)
+
+    def alloc_projected_grad(self, ctx: GpuCtx, n: int) -> Buffer:
+        return ctx.storage_buffer_zeroed(max(n, 1) * PROJECTED_GRAD_BYTES)
+
+    def zero_projected_grad(self, ctx: GpuCtx, buf: Buffer, n: int | None = None) -> None:
+        buf.zero()
+
+    def project(self, ctx: GpuCtx, splats: GpuSplatBuffer, camera: L.Camera, projected: Buffer) -> None:
+        L.check(L.lib().nts_project(ctx.h, splats.h, C.byref(camera), projected.h))
+
+    def composite(self, ctx: GpuCtx, projected: Buffer, bins: TileBinningResult, image: Buffer,
+                  params: TilingParams) -> None:
+        L.check(L.lib().nts_composite(ctx.h, projected.h, bins.h, image.h, C.byref(params.c())))
+
+    def composite_backward(self, ctx: GpuCtx, projected: Buffer, bins: TileBinningResult, forward_out: Buffer,
+                           dl_dc: Buffer, projected_grad: Buffer, params: TilingParams) -> None:
+        L.check(L.lib().nts_composite_backward(ctx.h, projected.h, bins.h, forward_out.h, dl_dc.h,
+                                               projected_grad.h, C.byref(params.c())))
+
+    def project_backward(self, ctx: GpuCtx, splats: GpuSplatBuffer, projected_grad: Buffer,
+                         camera: L.Camera) -> None:
+        L.check(L.lib().nts_project_backward(ctx.h, splats.h, projected_grad.h, C.byref(camera)))
+
+    def tonemap(self, ctx: GpuCtx, composite: Buffer, target: Buffer, width: int, height: int) -> None:
+        L.check(L.lib().nts_tonemap(ctx.h, composite.h, target.h, width, height))
+
+
+def loss_grad(ctx: GpuCtx, image: Buffer, target: Buffer, dl_dc: Buffer, width: int, height: int,
+              kind: int = L.LOSS_L1) -> float:
+    """dL/dC on the device (the reference computes it on the CPU, train.rs:180-196)."""
+    out = C.c_double()
+    L.check(L.lib().nts_loss_grad(ctx.h, image.h, target.h, dl_dc.h, width, height, kind, C.byref(out)))
+    return out.value
+
+
+@dataclass
+class AdamConfig:
+    """AdamConfig (adam.rs:22-39); defaults as the reference."""
+
+    lr: float = 1.6e-4
+    beta1: float = 0.9
+    beta2: float = 0.999
+    eps: float = 1e-15
+
+    def c(self) -> L.AdamCfg:
+        return L.AdamCfg(self.lr, self.beta1, self.beta2, self.eps)
+
+
+LOG_SCALE_MIN, LOG_SCALE_MAX, OPACITY_LOGIT_MAX = -10.0, 5.0, 10.0  # train.rs:37-39
+
+
+class AdamGpu:
+    """AdamGpu (adam_gpu.rs)."""
+
+    def __init__(self, ctx: GpuCtx | None = None):
+        pass
+
+    def step(self, ctx: GpuCtx, params: Buffer, grads: Buffer, m: Buffer, v: Buffer, n: int, lr: float, t: int,
+             cfg: AdamConfig) -> None:
+        c = AdamConfig(lr, cfg.beta1, cfg.beta2, cfg.eps).c()
+        L.check(L.lib().nts_adam_step_flat(ctx.h, params.h, grads.h, m.h, v.h, int(n), C.byref(c), int(t)))
+
+    def step_all_fused(self, ctx: GpuCtx, splats: GpuSplatBuffer, t: int, cfg_pos: AdamConfig,
+                       cfg_attr: AdamConfig, log_scale_min: float = LOG_SCALE_MIN,
+                       log_scale_max: float = LOG_SCALE_MAX, opacity_logit_max: float = OPACITY_LOGIT_MAX) -> None:
+        """accumulate_grad + step_all + apply_constraints in one kernel."""
+        cons = L.Constraints(log_scale_min, log_scale_max, opacity_logit_max)
+        L.check(L.lib().nts_adam_step(ctx.h, splats.h, C.byref(cfg_pos.c()), C.byref(cfg_attr.c()), int(t),
+                                      C.byref(cons)))

@@ -1,0 +1,112 @@
+// Adam (adam_step.wgsl:24-36) and the fused per-splat optimiser step:
+// grad_acc += |dpos|_1 (accumulate_grad_pos.wgsl:17-23), Adam on all 59
+// attribute planes, then the constraints of apply_constraints.wgsl:60-77.
+#include "nts_internal.h"
+
+struct AdamK {
+  float lr, b1, b2, eps, bc1, bc2;
+};
+
+__device__ __forceinline__ float adam_lane(float p, float g, float& m, float& v, const AdamK& c) {
+  const float mn = c.b1 * m + (1.0f - c.b1) * g;
+  const float vn = c.b2 * v + (1.0f - c.b2) * g * g;
+  m = mn;
+  v = vn;
+  const float mh = mn / c.bc1;
+  const float vh = vn / c.bc2;
+  return p - c.lr * mh / (sqrtf(vh) + c.eps);
+}
+
+__global__ void __launch_bounds__(256) k_adam_flat(float* __restrict__ p, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, uint64_t n, AdamK c) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    float mm = m[i], vv = v[i];
+    p[i] = adam_lane(p[i], g[i], mm, vv, c);
+    m[i] = mm;
+    v[i] = vv;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_adam_scene(float* __restrict__ P, const float* __restrict__ G,
+                                                    float* __restrict__ M, float* __restrict__ V,
+                                                    float* __restrict__ grad_acc, uint32_t cap, uint32_t n,
+                                                    AdamK cpos, AdamK cattr, float smin, float smax, float omax) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t c = cap;
+  // densify gate accumulator (before the step, as train.rs:228)
+  grad_acc[i] = grad_acc[i] + fabsf(G[(P_POS + 0) * c + i]) + fabsf(G[(P_POS + 1) * c + i]) +
+                fabsf(G[(P_POS + 2) * c + i]);
+  float q[4], s[3], o = 0.0f;
+#pragma unroll 4
+  for (int k = 0; k < P_COUNT; ++k) {
+    const size_t j = (size_t)k * c + i;
+    float mm = M[j], vv = V[j];
+    const float np = adam_lane(P[j], G[j], mm, vv, k < P_ROT ? cpos : cattr);
+    M[j] = mm;
+    V[j] = vv;
+    if (k >= P_ROT && k < P_ROT + 4) q[k - P_ROT] = np;
+    else if (k >= P_SCALE && k < P_SCALE + 3) s[k - P_SCALE] = np;
+    else if (k == P_LOGIT) o = np;
+    else P[j] = np;
+  }
+  const float n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  const float nrm = nts_maxf(sqrtf(n2), 1e-8f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) P[(P_ROT + k) * c + i] = q[k] / nrm;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) P[(P_SCALE + k) * c + i] = nts_minf(nts_maxf(s[k], smin), smax);
+  P[P_LOGIT * c + i] = nts_minf(nts_maxf(o, -omax), omax);
+}
+
+// compiler-rt __powisf2, which Rust's f32::powi lowers to (adam_gpu.rs:275-276).
+static float powi_f32(float a, int32_t b) {
+  const int recip = b < 0;
+  float r = 1.0f;
+  while (1) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return recip ? 1.0f / r : r;
+}
+
+static AdamK make_adam(const nts_adam_cfg* cfg, uint64_t t) {
+  AdamK k;
+  k.lr = cfg->lr;
+  k.b1 = cfg->beta1;
+  k.b2 = cfg->beta2;
+  k.eps = cfg->eps;
+  k.bc1 = 1.0f - powi_f32(cfg->beta1, (int32_t)t);
+  k.bc2 = 1.0f - powi_f32(cfg->beta2, (int32_t)t);
+  return k;
+}
+
+extern "C" int nts_adam_step_flat(nts_ctx* ctx, nts_buffer* params, const nts_buffer* grads, nts_buffer* m,
+                                  nts_buffer* v, uint64_t n, const nts_adam_cfg* cfg, uint64_t t) {
+  NTS_CHECK_ARG(ctx && params && grads && m && v && cfg && t > 0);
+  NTS_CHECK_ARG(n * 4 <= params->bytes && n * 4 <= grads->bytes && n * 4 <= m->bytes && n * 4 <= v->bytes);
+  if (n == 0) return NTS_OK;
+  uint32_t grid = nts_div_up(n, 256);
+  if (grid > 8192) grid = 8192;
+  NTS_LAUNCH(ctx, "adam_flat", k_adam_flat, dim3(grid), dim3(256), 0, (float*)params->ptr,
+             (const float*)grads->ptr, (float*)m->ptr, (float*)v->ptr, n, make_adam(cfg, t));
+  return NTS_OK;
+}
+
+int nts_adam_scene_dev(nts_ctx* ctx, nts_scene* s, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr,
+                       uint64_t t, const nts_constraints* cons) {
+  if (s->n == 0) return NTS_OK;
+  NTS_LAUNCH(ctx, "adam_step", k_adam_scene, dim3(nts_div_up(s->n, 256)), dim3(256), 0, s->params,
+             (const float*)s->grads, s->m, s->v, s->grad_acc, s->cap, s->n, make_adam(cfg_pos, t),
+             make_adam(cfg_attr, t), cons->log_scale_min, cons->log_scale_max, cons->logit_max);
+  return NTS_OK;
+}
+
+extern "C" int nts_adam_step(nts_ctx* ctx, nts_scene* scene, const nts_adam_cfg* cfg_pos,
+                             const nts_adam_cfg* cfg_attr, uint64_t t, const nts_constraints* cons) {
+  NTS_CHECK_ARG(ctx && scene && cfg_pos && cfg_attr && cons && t > 0);
+  return nts_adam_scene_dev(ctx, scene, cfg_pos, cfg_attr, t, cons);
+}

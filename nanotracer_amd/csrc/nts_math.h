@@ -1,0 +1,104 @@
+// Scalar maths shared by the gfx950 kernels and the host-side restatement.
+//
+// Every function here is written with IEEE-754 operations only (+ - * /,
+// sqrt, fma, floor/ceil/rint, bit casts), so a translation unit compiled
+// with -ffp-contract=off produces the same bits on the host (x86-64, SSE/FMA)
+// and on gfx950 (hipcc rounds fp32 division and sqrt correctly and keeps
+// fp32 denormals).  That is what makes tile keys, tile ranges and rendered
+// images bit-identical between the device path and the host checker.
+//
+// nts_expf: Cody & Waite (1980) argument reduction x = n*ln2 + r with a
+// two-word ln2 (the high word has 9 significant bits, so n*ln2_hi is exact
+// for |n| <= 150), then the degree-7 Taylor polynomial of e^r on
+// |r| <= ln2/2 in Horner form with fused multiply-adds (truncation error
+// 5e-9 relative, below half an ulp), then scaling by 2^n built from the
+// exponent field.  Accuracy: <= 1 ulp over the normal range (checked against
+// a float64 exp in the test suite).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define NTS_HD __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#define NTS_HD static inline
+#endif
+
+NTS_HD float nts_bits_to_f32(uint32_t u) { return __builtin_bit_cast(float, u); }
+NTS_HD uint32_t nts_f32_to_bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+NTS_HD float nts_fmaf(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// 2^k for k in [-126, 127] as an exact float.
+NTS_HD float nts_pow2i(int k) { return nts_bits_to_f32((uint32_t)(k + 127) << 23); }
+
+NTS_HD float nts_expf(float x) {
+  if (!(x > -104.0f)) {            // deep underflow (and NaN)
+    return (x != x) ? x : 0.0f;
+  }
+  if (x > 88.72284f) return nts_bits_to_f32(0x7f800000u);  // +inf
+  const float n = __builtin_rintf(x * 1.44269504088896341f);
+  float r = nts_fmaf(n, -0.693359375f, x);          // ln2_hi = 355/512
+  r = nts_fmaf(n, 2.12194440e-4f, r);               // ln2 - ln2_hi
+  float p = 1.98412698e-4f;                         // 1/7!
+  p = nts_fmaf(p, r, 1.38888889e-3f);               // 1/6!
+  p = nts_fmaf(p, r, 8.33333333e-3f);               // 1/5!
+  p = nts_fmaf(p, r, 4.16666667e-2f);               // 1/4!
+  p = nts_fmaf(p, r, 1.66666667e-1f);               // 1/3!
+  p = nts_fmaf(p, r, 0.5f);
+  p = nts_fmaf(p, r, 1.0f);
+  p = nts_fmaf(p, r, 1.0f);
+  int k = (int)n;
+  if (k < -125) {                                   // result near/below FLT_MIN
+    p = p * nts_pow2i(-64);
+    k += 64;
+  } else if (k > 127) {
+    p = p * 2.0f;
+    k -= 1;
+  }
+  return p * nts_pow2i(k);
+}
+
+NTS_HD float nts_sigmoidf(float x) { return 1.0f / (1.0f + nts_expf(-x)); }
+
+// Saturating float -> int32 conversion of an integral-valued float (the
+// result of floor).  Out-of-range values clamp to the int32 range.
+NTS_HD int32_t nts_f2i_sat(float x) {
+  if (x != x) return 0;
+  if (x <= -2147483648.0f) return (int32_t)0x80000000u;
+  if (x >= 2147483520.0f) return 2147483520;
+  return (int32_t)x;
+}
+
+NTS_HD float nts_minf(float a, float b) { return (b < a) ? b : a; }
+NTS_HD float nts_maxf(float a, float b) { return (a < b) ? b : a; }
+NTS_HD int32_t nts_mini(int32_t a, int32_t b) { return (b < a) ? b : a; }
+NTS_HD int32_t nts_maxi(int32_t a, int32_t b) { return (a < b) ? b : a; }
+
+// Real spherical-harmonic constants of the rasteriser (the band-1 constant is
+// 0.488602, as in crates/nano-optimize/src/shaders/project_gaussians.wgsl:51-64).
+#define NTS_SH_C0 0.2820948f
+#define NTS_SH_C1 0.488602f
+#define NTS_SH_C2_0 1.0925485f
+#define NTS_SH_C2_1 (-1.0925485f)
+#define NTS_SH_C2_2 0.31539157f
+#define NTS_SH_C2_3 (-1.0925485f)
+#define NTS_SH_C2_4 0.54627424f
+#define NTS_SH_C3_0 (-0.5900436f)
+#define NTS_SH_C3_1 2.8906114f
+#define NTS_SH_C3_2 (-0.4570458f)
+#define NTS_SH_C3_3 0.37317634f
+#define NTS_SH_C3_4 (-0.4570458f)
+#define NTS_SH_C3_5 1.4453057f
+#define NTS_SH_C3_6 (-0.5900436f)
+
+// Blend thresholds (crates/nano-optimize/src/shaders/rasterize.wgsl:87-107,
+// rasterize_backward.wgsl:48-51).
+#define NTS_ALPHA_MIN (1.0f / 255.0f)
+#define NTS_ALPHA_MAX 0.99f
+#define NTS_T_MIN 1e-4f
+#define NTS_ONE_MINUS_ALPHA_EPS 1e-4f
+#define NTS_T_BAIL 1.0001f
+#define NTS_COV_DILATION 0.3f

@@ -1,0 +1,57 @@
+// Shared definitions of the tile alpha-blend kernels.
+#pragma once
+#include "nts_internal.h"
+
+#define NTS_MAX_VIEWS 32
+
+// One view of a launch: image size, tile grid, and where its tiles, its
+// projected records and its pixels start in the launch-wide arrays.
+struct ViewDesc {
+  uint32_t width, height;
+  uint32_t tiles_x, tiles_y;
+  uint32_t tile_base;   // first global tile id of the view
+  uint32_t proj_base;   // offset of the view's records in projected[] / projected_grad[]
+  uint64_t pix_base;    // offset of the view's first pixel in the image arrays
+};
+
+// Passed by value as a kernel argument (kernarg segment), so launches need
+// no upload and stay capturable.
+struct ViewTable {
+  uint32_t n;
+  uint32_t total_tiles;
+  uint32_t pad[2];
+  ViewDesc v[NTS_MAX_VIEWS];
+};
+
+// View of global tile gt (tile_base ascending).
+__device__ __forceinline__ uint32_t view_of_tile(const ViewTable& vt, uint32_t gt) {
+  uint32_t lo = 0, hi = vt.n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (vt.v[mid].tile_base <= gt) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// e^x for x <= 0: the arithmetic of nts_expf with the branches that cannot
+// fire for non-positive arguments removed (bit-identical on that domain).
+__device__ __forceinline__ float nts_expf_nonpos(float x) {
+  if (!(x > -104.0f)) return (x != x) ? x : 0.0f;
+  const float n = __builtin_rintf(x * 1.44269504088896341f);
+  float r = nts_fmaf(n, -0.693359375f, x);
+  r = nts_fmaf(n, 2.12194440e-4f, r);
+  float p = 1.98412698e-4f;
+  p = nts_fmaf(p, r, 1.38888889e-3f);
+  p = nts_fmaf(p, r, 8.33333333e-3f);
+  p = nts_fmaf(p, r, 4.16666667e-2f);
+  p = nts_fmaf(p, r, 1.66666667e-1f);
+  p = nts_fmaf(p, r, 0.5f);
+  p = nts_fmaf(p, r, 1.0f);
+  p = nts_fmaf(p, r, 1.0f);
+  int k = (int)n;
+  if (k < -125) {
+    p = p * nts_pow2i(-64);
+    k += 64;
+  }
+  return p * nts_pow2i(k);
+}

@@ -1,0 +1,60 @@
+// Device-wide primitives: block scans, the exclusive u32 scan and the
+// segmented onesweep LSD radix sort (Adinets & Merrill, "Onesweep: A Faster
+// Least Significant Digit Radix Sort for GPUs", 2022): one upfront pass
+// builds the 8-bit digit histograms of every pass in LDS, then each pass is a
+// single kernel in which a block ranks its 4096 keys per digit with wave64
+// ballots, publishes its per-digit counts and looks back over the preceding
+// blocks of its segment (decoupled look-back) for its global digit offsets,
+// reorders its keys in LDS and writes them out digit-contiguously.
+#pragma once
+#include "nts_internal.h"
+
+#define RS_BLOCK 256
+#define RS_ITEMS 16
+#define RS_TILE (RS_BLOCK * RS_ITEMS)
+#define RS_RADIX 256
+#define RS_MAX_SEGS 64
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Inclusive scan of v across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= (uint32_t)off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan across a block of NT threads (NT a multiple of 64).  s_wave
+// must hold NT/64 words.  Contains two barriers.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    const uint32_t x = s_wave[w];
+    if ((uint32_t)w < wave) pre += x;
+    tot += x;
+  }
+  __syncthreads();
+  total = tot;
+  return pre + incl - v;
+}
+
+// Binary search: largest s in [0, n) with a[s] <= x (a ascending, a[0] <= x).
+template <typename T>
+__device__ __forceinline__ uint32_t seg_of(const T* a, uint32_t n, T x) {
+  uint32_t lo = 0, hi = n;  // invariant a[lo] <= x, answer < hi
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}

@@ -1,0 +1,921 @@
+// Host restatement of nano-optimize's splat rasteriser.  TEST INFRASTRUCTURE
+// ONLY: the checker for the gfx950 path (see nts_oracle.h).
+//
+// Operation order follows the WGSL text: `a * b * c` is (a*b)*c, sums are
+// left to right, a WGSL matrix product (A*B)[c][r] = A[0][r]*B[c][0] +
+// A[1][r]*B[c][1] + A[2][r]*B[c][2].  Built with -ffp-contract=off.
+#include "nts_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "nts_math.h"
+
+namespace {
+
+template <typename T> struct M;
+template <> struct M<float> {
+  static float exp(float x) { return nts_expf(x); }
+  static float sqrt(float x) { return __builtin_sqrtf(x); }
+  static float ceil(float x) { return __builtin_ceilf(x); }
+  static float floor(float x) { return __builtin_floorf(x); }
+};
+template <> struct M<double> {
+  static double exp(double x) { return ::exp(x); }
+  static double sqrt(double x) { return ::sqrt(x); }
+  static double ceil(double x) { return ::ceil(x); }
+  static double floor(double x) { return ::floor(x); }
+};
+
+template <typename T> struct V3 { T x, y, z; };
+
+// Column-major 3x3: c[col][row].
+template <typename T> struct M3 { T c[3][3]; };
+
+template <typename T> M3<T> mul(const M3<T>& a, const M3<T>& b) {
+  M3<T> o;
+  for (int col = 0; col < 3; ++col)
+    for (int row = 0; row < 3; ++row)
+      o.c[col][row] = a.c[0][row] * b.c[col][0] + a.c[1][row] * b.c[col][1] +
+                      a.c[2][row] * b.c[col][2];
+  return o;
+}
+template <typename T> M3<T> transpose(const M3<T>& a) {
+  M3<T> o;
+  for (int col = 0; col < 3; ++col)
+    for (int row = 0; row < 3; ++row) o.c[col][row] = a.c[row][col];
+  return o;
+}
+template <typename T> V3<T> mulv(const M3<T>& a, const V3<T>& v) {
+  // WGSL mat * vec = col0*v.x + col1*v.y + col2*v.z
+  V3<T> o;
+  o.x = a.c[0][0] * v.x + a.c[1][0] * v.y + a.c[2][0] * v.z;
+  o.y = a.c[0][1] * v.x + a.c[1][1] * v.y + a.c[2][1] * v.z;
+  o.z = a.c[0][2] * v.x + a.c[1][2] * v.y + a.c[2][2] * v.z;
+  return o;
+}
+template <typename T> T dot3(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// project_gaussians.wgsl:68-78 (rotation stored w, x, y, z).
+template <typename T> M3<T> quat_to_mat3(const T* q) {
+  const T w = q[0], x = q[1], y = q[2], z = q[3];
+  M3<T> r;
+  r.c[0][0] = T(1) - T(2) * (y * y + z * z);
+  r.c[0][1] = T(2) * (x * y + w * z);
+  r.c[0][2] = T(2) * (x * z - w * y);
+  r.c[1][0] = T(2) * (x * y - w * z);
+  r.c[1][1] = T(1) - T(2) * (x * x + z * z);
+  r.c[1][2] = T(2) * (y * z + w * x);
+  r.c[2][0] = T(2) * (x * z + w * y);
+  r.c[2][1] = T(2) * (y * z - w * x);
+  r.c[2][2] = T(1) - T(2) * (x * x + y * y);
+  return r;
+}
+
+// SH basis of bands 1..3 (project_gaussians.wgsl:102-128,
+// project_backward.wgsl:166-182).
+template <typename T> void sh_basis(T x, T y, T z, T b[15]) {
+  const T xx = x * x, yy = y * y, zz = z * z;
+  b[0] = -T(NTS_SH_C1) * y;
+  b[1] = T(NTS_SH_C1) * z;
+  b[2] = -T(NTS_SH_C1) * x;
+  b[3] = T(NTS_SH_C2_0) * x * y;
+  b[4] = T(NTS_SH_C2_1) * y * z;
+  b[5] = T(NTS_SH_C2_2) * (T(2) * zz - xx - yy);
+  b[6] = T(NTS_SH_C2_3) * x * z;
+  b[7] = T(NTS_SH_C2_4) * (xx - yy);
+  b[8] = T(NTS_SH_C3_0) * y * (T(3) * xx - yy);
+  b[9] = T(NTS_SH_C3_1) * x * y * z;
+  b[10] = T(NTS_SH_C3_2) * y * (T(4) * zz - xx - yy);
+  b[11] = T(NTS_SH_C3_3) * z * (T(2) * zz - T(3) * xx - T(3) * yy);
+  b[12] = T(NTS_SH_C3_4) * x * (T(4) * zz - xx - yy);
+  b[13] = T(NTS_SH_C3_5) * z * (xx - yy);
+  b[14] = T(NTS_SH_C3_6) * x * (xx - T(3) * yy);
+}
+
+template <typename T> struct Splats {
+  const T *pos, *rot, *log_scale, *logit, *sh_dc, *sh_rest;
+};
+
+template <typename T> M3<T> view_rot(const orc_camera* cam) {
+  M3<T> w;
+  for (int col = 0; col < 3; ++col)
+    for (int row = 0; row < 3; ++row) w.c[col][row] = T(cam->view[col * 4 + row]);
+  return w;
+}
+
+// view * vec4(p, 1) then divide by w (project_gaussians.wgsl:170-172).
+template <typename T> V3<T> to_camera(const orc_camera* cam, const T* p) {
+  const float* v = cam->view;
+  T o[4];
+  for (int row = 0; row < 4; ++row)
+    o[row] = T(v[0 * 4 + row]) * p[0] + T(v[1 * 4 + row]) * p[1] + T(v[2 * 4 + row]) * p[2] +
+             T(v[3 * 4 + row]) * T(1);
+  return V3<T>{o[0] / o[3], o[1] / o[3], o[2] / o[3]};
+}
+
+// project_gaussians.wgsl:156-259.
+template <typename T> void project_one(const Splats<T>& s, uint32_t idx, const orc_camera* cam, T* out) {
+  for (int k = 0; k < 12; ++k) out[k] = T(0);
+  const T* pw = s.pos + 3 * (size_t)idx;
+  const V3<T> cx = to_camera<T>(cam, pw);
+  const T depth = -cx.z;
+  if (depth < T(cam->near_)) return;
+  const V3<T> t{cx.x, cx.y, depth};
+  const T W = T(cam->width), H = T(cam->height), f = T(cam->focal);
+  const T cxp = W * T(0.5), cyp = H * T(0.5);
+  const T mx = cxp + (f * t.x) / t.z;
+  const T my = cyp - (f * t.y) / t.z;
+
+  const M3<T> R = quat_to_mat3<T>(s.rot + 4 * (size_t)idx);
+  const T* ls = s.log_scale + 3 * (size_t)idx;
+  const T sg[3] = {M<T>::exp(ls[0]), M<T>::exp(ls[1]), M<T>::exp(ls[2])};
+  const T s2[3] = {sg[0] * sg[0], sg[1] * sg[1], sg[2] * sg[2]};
+  M3<T> Ms;
+  for (int col = 0; col < 3; ++col)
+    for (int row = 0; row < 3; ++row) Ms.c[col][row] = R.c[col][row] * s2[col];
+  const M3<T> cov_w = mul(Ms, transpose(R));
+  const M3<T> Wm = view_rot<T>(cam);
+  const M3<T> cov_c = mul(mul(Wm, cov_w), transpose(Wm));
+
+  const T inv_z = T(1) / t.z;
+  const T inv_z2 = inv_z * inv_z;
+  const V3<T> j0{f * inv_z, T(0), -f * t.x * inv_z2};
+  const V3<T> j1{T(0), -f * inv_z, f * t.y * inv_z2};
+  V3<T> r0, r1;
+  r0.x = cov_c.c[0][0] * j0.x + cov_c.c[1][0] * j0.y + cov_c.c[2][0] * j0.z;
+  r0.y = cov_c.c[0][1] * j0.x + cov_c.c[1][1] * j0.y + cov_c.c[2][1] * j0.z;
+  r0.z = cov_c.c[0][2] * j0.x + cov_c.c[1][2] * j0.y + cov_c.c[2][2] * j0.z;
+  r1.x = cov_c.c[0][0] * j1.x + cov_c.c[1][0] * j1.y + cov_c.c[2][0] * j1.z;
+  r1.y = cov_c.c[0][1] * j1.x + cov_c.c[1][1] * j1.y + cov_c.c[2][1] * j1.z;
+  r1.z = cov_c.c[0][2] * j1.x + cov_c.c[1][2] * j1.y + cov_c.c[2][2] * j1.z;
+  T s00 = dot3(r0, j0);
+  const T s01 = dot3(r0, j1);
+  T s11 = dot3(r1, j1);
+  s00 = s00 + T(NTS_COV_DILATION);
+  s11 = s11 + T(NTS_COV_DILATION);
+  const T det = s00 * s11 - s01 * s01;
+  if (det <= T(0)) return;
+  const T inv_det = T(1) / det;
+  const T ca = s11 * inv_det, cb = -s01 * inv_det, cc = s00 * inv_det;
+  const T mid = T(0.5) * (s00 + s11);
+  const T disc = mid * mid - det;
+  const T lambda = mid + M<T>::sqrt(disc > T(0) ? disc : T(0));
+  const T radius = M<T>::ceil(T(3) * M<T>::sqrt(lambda));
+  const T maxpix = (W > H ? W : H) + radius;
+  if (mx < -radius || mx >= W + radius || my < -radius || my >= H + radius || radius >= maxpix) return;
+
+  // colour (project_gaussians.wgsl:88-154, 250-252)
+  V3<T> d{T(cam->cam_pos[0]) - pw[0], T(cam->cam_pos[1]) - pw[1], T(cam->cam_pos[2]) - pw[2]};
+  const T inv_len = T(1) / M<T>::sqrt(dot3(d, d));
+  d.x = d.x * inv_len;
+  d.y = d.y * inv_len;
+  d.z = d.z * inv_len;
+  T b[15];
+  sh_basis<T>(d.x, d.y, d.z, b);
+  const T* dc = s.sh_dc + 3 * (size_t)idx;
+  const T* rest = s.sh_rest + 45 * (size_t)idx;
+  T rgb[3] = {T(NTS_SH_C0) * dc[0], T(NTS_SH_C0) * dc[1], T(NTS_SH_C0) * dc[2]};
+  T acc[3] = {T(0), T(0), T(0)};
+  for (int i = 0; i < 3; ++i)
+    for (int ch = 0; ch < 3; ++ch) acc[ch] = acc[ch] + rest[15 * ch + i] * b[i];
+  for (int i = 0; i < 5; ++i)
+    for (int ch = 0; ch < 3; ++ch) acc[ch] = acc[ch] + rest[15 * ch + 3 + i] * b[3 + i];
+  for (int i = 0; i < 7; ++i)
+    for (int ch = 0; ch < 3; ++ch) acc[ch] = acc[ch] + rest[15 * ch + 8 + i] * b[8 + i];
+  for (int ch = 0; ch < 3; ++ch) {
+    rgb[ch] = rgb[ch] + acc[ch];
+    const T c = rgb[ch] + T(0.5);
+    rgb[ch] = c > T(0) ? c : T(0);
+  }
+  const T lg = s.logit[idx];
+  const T alpha = T(1) / (T(1) + M<T>::exp(-lg));
+
+  out[0] = mx; out[1] = my; out[2] = depth; out[3] = radius;
+  out[4] = ca; out[5] = cb; out[6] = cc; out[7] = T(1);
+  out[8] = rgb[0]; out[9] = rgb[1]; out[10] = rgb[2]; out[11] = alpha;
+}
+
+template <typename T> void project_all(const Splats<T>& s, uint32_t n, const orc_camera* cam, T* projected) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) project_one<T>(s, (uint32_t)i, cam, projected + 12 * i);
+}
+
+// Tile bbox of a visible record (tile_count.wgsl:39-50); returns false when empty.
+template <typename T>
+bool tile_bbox(const T* p, const orc_tiling* t, int32_t* x0, int32_t* y0, int32_t* x1, int32_t* y1) {
+  if (p[7] == T(0)) return false;
+  const uint32_t tiles_x = (t->width + t->tile_size - 1) / t->tile_size;
+  const uint32_t tiles_y = (t->height + t->tile_size - 1) / t->tile_size;
+  const float mx = (float)p[0], my = (float)p[1], r = (float)p[3];
+  const float ts = (float)t->tile_size;
+  *x0 = nts_maxi(nts_f2i_sat(__builtin_floorf((mx - r) / ts)), 0);
+  *y0 = nts_maxi(nts_f2i_sat(__builtin_floorf((my - r) / ts)), 0);
+  *x1 = nts_mini(nts_f2i_sat(__builtin_floorf((mx + r) / ts)), (int32_t)tiles_x - 1);
+  *y1 = nts_mini(nts_f2i_sat(__builtin_floorf((my + r) / ts)), (int32_t)tiles_y - 1);
+  return !(*x1 < *x0 || *y1 < *y0);
+}
+
+template <typename T>
+void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* ranges,
+                 const orc_tiling* t, T* out, float* final_t, uint32_t* last_idx) {
+  const uint32_t ts = t->tile_size;
+  const uint32_t tiles_x = (t->width + ts - 1) / ts;
+  const uint32_t tiles_y = (t->height + ts - 1) / ts;
+  const int64_t ntiles = (int64_t)tiles_x * tiles_y;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t tile = 0; tile < ntiles; ++tile) {
+    const uint32_t tx = (uint32_t)(tile % tiles_x), ty = (uint32_t)(tile / tiles_x);
+    const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
+    for (uint32_t ly = 0; ly < ts; ++ly)
+      for (uint32_t lx = 0; lx < ts; ++lx) {
+        const uint32_t px = tx * ts + lx, py = ty * ts + ly;
+        if (px >= t->width || py >= t->height) continue;
+        const T pxf = T((float)px + 0.5f), pyf = T((float)py + 0.5f);
+        T C[3] = {T(0), T(0), T(0)};
+        T Tr = T(1);
+        uint32_t last = 0;
+        for (uint32_t i = begin; i < end; ++i) {
+          const T* p = projected + 12 * (size_t)payloads[i];
+          const T dx = pxf - p[0], dy = pyf - p[1];
+          const T power = T(-0.5) * (p[4] * dx * dx + T(2) * p[5] * dx * dy + p[6] * dy * dy);
+          if (power > T(0)) continue;
+          const T g = M<T>::exp(power);
+          const T ar = p[11] * g;
+          const T alpha = ar < T(NTS_ALPHA_MAX) ? ar : T(NTS_ALPHA_MAX);
+          if (alpha < T(NTS_ALPHA_MIN)) continue;
+          const T ta = Tr * alpha;
+          C[0] = C[0] + ta * p[8];
+          C[1] = C[1] + ta * p[9];
+          C[2] = C[2] + ta * p[10];
+          Tr = Tr * (T(1) - alpha);
+          last = i - begin + 1;
+          if (Tr < T(NTS_T_MIN)) break;
+        }
+        const size_t pix = (size_t)py * t->width + px;
+        out[4 * pix + 0] = C[0];
+        out[4 * pix + 1] = C[1];
+        out[4 * pix + 2] = C[2];
+        out[4 * pix + 3] = T(1) - Tr;
+        if (final_t) final_t[pix] = (float)Tr;
+        if (last_idx) last_idx[pix] = last;
+      }
+  }
+}
+
+// compiler-rt __powisf2 (what Rust's f32::powi lowers to).
+float powi_f32(float a, int32_t b) {
+  const int recip = b < 0;
+  float r = 1.0f;
+  while (1) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return recip ? 1.0f / r : r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_version(void) { return 1; }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_expf_array(const float* x, float* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = nts_expf(x[i]);
+}
+
+// raster.rs:79-99 CameraUniform::from_pose with glam's right-handed look-at
+// (view = [s | u | -f] rows, translation -eye.s, -eye.u, eye.f).
+void orc_camera_from_pose(const float eye[3], const float target[3], const float up[3], float fov_y,
+                          uint32_t width, uint32_t height, uint32_t n_splats, orc_camera* out) {
+  float f[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
+  float il = 1.0f / sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  for (int i = 0; i < 3; ++i) f[i] = f[i] * il;
+  float s[3] = {f[1] * up[2] - up[1] * f[2], f[2] * up[0] - up[2] * f[0], f[0] * up[1] - up[0] * f[1]};
+  il = 1.0f / sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  for (int i = 0; i < 3; ++i) s[i] = s[i] * il;
+  const float u[3] = {s[1] * f[2] - f[1] * s[2], s[2] * f[0] - f[2] * s[0], s[0] * f[1] - f[0] * s[1]};
+  float* v = out->view;
+  for (int c = 0; c < 3; ++c) {
+    v[c * 4 + 0] = s[c];
+    v[c * 4 + 1] = u[c];
+    v[c * 4 + 2] = -f[c];
+    v[c * 4 + 3] = 0.0f;
+  }
+  v[12] = -(eye[0] * s[0] + eye[1] * s[1] + eye[2] * s[2]);
+  v[13] = -(eye[0] * u[0] + eye[1] * u[1] + eye[2] * u[2]);
+  v[14] = eye[0] * f[0] + eye[1] * f[1] + eye[2] * f[2];
+  v[15] = 1.0f;
+  out->cam_pos[0] = eye[0];
+  out->cam_pos[1] = eye[1];
+  out->cam_pos[2] = eye[2];
+  out->near_ = 0.01f;
+  out->focal = (float)height / (2.0f * tanf(fov_y * 0.5f));
+  out->width = (float)width;
+  out->height = (float)height;
+  out->n_splats = n_splats;
+}
+
+void orc_project_f32(const orc_splats_f32* s, uint32_t n, const orc_camera* cam, float* projected) {
+  Splats<float> sp{s->pos, s->rot, s->log_scale, s->logit, s->sh_dc, s->sh_rest};
+  project_all<float>(sp, n, cam, projected);
+}
+
+void orc_project_f64(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, double* projected) {
+  Splats<double> sp{s->pos, s->rot, s->log_scale, s->logit, s->sh_dc, s->sh_rest};
+  project_all<double>(sp, n, cam, projected);
+}
+
+uint64_t orc_tile_count(const float* projected, uint32_t n, const orc_tiling* t, uint32_t* counts) {
+  uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    int32_t x0, y0, x1, y1;
+    uint32_t c = 0;
+    if (tile_bbox<float>(projected + 12 * i, t, &x0, &y0, &x1, &y1))
+      c = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1));
+    counts[i] = c;
+    total += c;
+  }
+  return total;
+}
+
+// scan_block.wgsl + scan_add_offsets.wgsl: exclusive, wrapping u32.
+void orc_exclusive_scan_u32(const uint32_t* in, uint64_t n, uint32_t* out) {
+  uint32_t acc = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t v = in[i];
+    out[i] = acc;
+    acc += v;
+  }
+}
+
+// tile_emit.wgsl:91-121.
+void orc_tile_emit(const float* projected, uint32_t n, const uint32_t* offsets, const orc_tiling* t,
+                   uint32_t* keys, uint32_t* payloads) {
+  const uint32_t tiles_x = (t->width + t->tile_size - 1) / t->tile_size;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    const float* p = projected + 12 * i;
+    int32_t x0, y0, x1, y1;
+    if (!tile_bbox<float>(p, t, &x0, &y0, &x1, &y1)) continue;
+    float dn = p[2] / t->depth_max;
+    dn = dn < 0.0f ? 0.0f : (dn > 1.0f ? 1.0f : dn);
+    const uint32_t d16 = (uint32_t)(dn * 65535.0f);
+    uint32_t w = offsets[i];
+    for (int32_t ty = y0; ty <= y1; ++ty)
+      for (int32_t tx = x0; tx <= x1; ++tx) {
+        const uint32_t tile = (uint32_t)ty * tiles_x + (uint32_t)tx;
+        keys[w] = (tile << 16) | d16;
+        payloads[w] = (uint32_t)i;
+        ++w;
+      }
+  }
+}
+
+// radix_sort.rs:1-26: 32 stable 1-bit LSD passes == a stable sort by key.
+void orc_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint64_t n) {
+  std::vector<uint64_t> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return keys[a] < keys[b]; });
+  std::vector<uint32_t> k2(n), v2(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    k2[i] = keys[idx[i]];
+    v2[i] = vals[idx[i]];
+  }
+  memcpy(keys, k2.data(), n * 4);
+  memcpy(vals, v2.data(), n * 4);
+}
+
+void orc_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint64_t n) {
+  std::vector<uint64_t> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return keys[a] < keys[b]; });
+  std::vector<uint64_t> k2(n);
+  std::vector<uint32_t> v2(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    k2[i] = keys[idx[i]];
+    v2[i] = vals[idx[i]];
+  }
+  memcpy(keys, k2.data(), n * 8);
+  memcpy(vals, v2.data(), n * 4);
+}
+
+// tile_ranges.wgsl:22-41 (ranges zero-initialised: empty tiles stay (0,0)).
+void orc_tile_ranges(const uint32_t* sorted_keys, uint64_t k, uint32_t num_tiles, uint32_t* ranges) {
+  memset(ranges, 0, sizeof(uint32_t) * 2 * (size_t)num_tiles);
+  for (uint64_t i = 0; i < k; ++i) {
+    const uint32_t tile = sorted_keys[i] >> 16;
+    const bool starts = (i == 0) || ((sorted_keys[i - 1] >> 16) != tile);
+    const bool ends = (i + 1 == k) || ((sorted_keys[i + 1] >> 16) != tile);
+    if (tile >= num_tiles) continue;
+    if (starts) ranges[2 * tile] = (uint32_t)i;
+    if (ends) ranges[2 * tile + 1] = (uint32_t)(i + 1);
+  }
+}
+
+uint64_t orc_bin(const float* projected, uint32_t n, const orc_tiling* t, uint32_t* keys,
+                 uint32_t* payloads, uint32_t* ranges) {
+  std::vector<uint32_t> counts(n > 0 ? n : 1), offsets(n > 0 ? n : 1);
+  const uint64_t total = orc_tile_count(projected, n, t, counts.data());
+  if (!keys) return total;
+  const uint32_t tiles_x = (t->width + t->tile_size - 1) / t->tile_size;
+  const uint32_t tiles_y = (t->height + t->tile_size - 1) / t->tile_size;
+  const uint32_t num_tiles = tiles_x * tiles_y;
+  if (total == 0) {
+    memset(ranges, 0, sizeof(uint32_t) * 2 * (size_t)num_tiles);
+    return 0;
+  }
+  orc_exclusive_scan_u32(counts.data(), n, offsets.data());
+  orc_tile_emit(projected, n, offsets.data(), t, keys, payloads);
+  orc_sort_pairs_u32(keys, payloads, total);
+  orc_tile_ranges(keys, total, num_tiles, ranges);
+  return total;
+}
+
+void orc_composite(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
+                   const orc_tiling* t, float* out, float* final_t, uint32_t* last_idx) {
+  composite_t<float>(projected, payloads, ranges, t, out, final_t, last_idx);
+}
+
+void orc_composite_f64(const double* projected, const uint32_t* payloads, const uint32_t* ranges,
+                       const orc_tiling* t, double* out) {
+  composite_t<double>(projected, payloads, ranges, t, out, nullptr, nullptr);
+}
+
+// rasterize_backward.wgsl:73-210.  Per-pixel terms are computed exactly as
+// the kernel does (fp32); the per-splat sums are accumulated in float64.
+void orc_composite_backward_ref(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
+                                const orc_tiling* t, uint32_t n, const float* forward_out,
+                                const float* dl_dc, float* projected_grad) {
+  const uint32_t ts = t->tile_size;
+  const uint32_t tiles_x = (t->width + ts - 1) / ts;
+  const uint32_t tiles_y = (t->height + ts - 1) / ts;
+  const int64_t ntiles = (int64_t)tiles_x * tiles_y;
+  std::vector<double> acc((size_t)n * 9, 0.0);
+  int nthreads = orc_num_threads();
+  std::vector<std::vector<double>> partial(nthreads);
+#pragma omp parallel
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+    std::vector<double>& mine = partial[tid];
+    mine.assign((size_t)n * 9, 0.0);
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t tile = 0; tile < ntiles; ++tile) {
+      const uint32_t tx = (uint32_t)(tile % tiles_x), ty = (uint32_t)(tile / tiles_x);
+      const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
+      if (end <= begin) continue;
+      for (uint32_t ly = 0; ly < ts; ++ly)
+        for (uint32_t lx = 0; lx < ts; ++lx) {
+          const uint32_t px = tx * ts + lx, py = ty * ts + ly;
+          if (px >= t->width || py >= t->height) continue;
+          const size_t pix = (size_t)py * t->width + px;
+          const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+          float T = 1.0f - forward_out[4 * pix + 3];
+          const float gC[3] = {dl_dc[4 * pix + 0], dl_dc[4 * pix + 1], dl_dc[4 * pix + 2]};
+          float S[3] = {0.0f, 0.0f, 0.0f};
+          for (int64_t i = (int64_t)end - 1; i >= (int64_t)begin; --i) {
+            const uint32_t sidx = payloads[i];
+            const float* p = projected + 12 * (size_t)sidx;
+            const float dx = pxf - p[0], dy = pyf - p[1];
+            const float ca = p[4], cb = p[5], cc = p[6];
+            const float power = -0.5f * (ca * dx * dx + 2.0f * cb * dx * dy + cc * dy * dy);
+            if (power > 0.0f) continue;
+            const float g = nts_expf(power);
+            const float sigma = p[11];
+            const float ar = sigma * g;
+            const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
+            if (alpha < NTS_ALPHA_MIN) continue;
+            const float oma = nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
+            T = T / oma;
+            if (T > NTS_T_BAIL) break;
+            const float ta = T * alpha;
+            const float col[3] = {p[8], p[9], p[10]};
+            const float contrib[3] = {ta * col[0], ta * col[1], ta * col[2]};
+            const float gcol[3] = {gC[0] * ta, gC[1] * ta, gC[2] * ta};
+            const float dCa[3] = {T * col[0] - S[0] / oma, T * col[1] - S[1] / oma, T * col[2] - S[2] / oma};
+            const float ga = gC[0] * dCa[0] + gC[1] * dCa[1] + gC[2] * dCa[2];
+            const float sat = (ar < NTS_ALPHA_MAX) ? 1.0f : 0.0f;
+            const float gar = ga * sat;
+            const float gsig = gar * g;
+            const float gpow = gar * sigma * g;
+            const float g_a = gpow * -0.5f * (dx * dx);
+            const float g_b = gpow * -0.5f * (2.0f * dx * dy);
+            const float g_c = gpow * -0.5f * (dy * dy);
+            const float gmx = gpow * (ca * dx + cb * dy);
+            const float gmy = gpow * (cb * dx + cc * dy);
+            double* a = &mine[(size_t)sidx * 9];
+            a[0] += gmx; a[1] += gmy; a[2] += gsig;
+            a[3] += g_a; a[4] += g_b; a[5] += g_c;
+            a[6] += gcol[0]; a[7] += gcol[1]; a[8] += gcol[2];
+            S[0] = S[0] + contrib[0];
+            S[1] = S[1] + contrib[1];
+            S[2] = S[2] + contrib[2];
+          }
+        }
+    }
+  }
+  for (int th = 0; th < nthreads; ++th) {
+    const std::vector<double>& pv = partial[th];
+    if (pv.empty()) continue;
+    for (size_t j = 0; j < acc.size(); ++j) acc[j] += pv[j];
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    float* o = projected_grad + 12 * (size_t)i;
+    const double* a = &acc[(size_t)i * 9];
+    o[0] = (float)a[0]; o[1] = (float)a[1]; o[2] = (float)a[2]; o[3] = 0.0f;
+    o[4] = (float)a[3]; o[5] = (float)a[4]; o[6] = (float)a[5]; o[7] = 0.0f;
+    o[8] = (float)a[6]; o[9] = (float)a[7]; o[10] = (float)a[8]; o[11] = 0.0f;
+  }
+}
+
+// project_backward.wgsl:79-304.
+void orc_project_backward(const orc_splats_f32* sp, uint32_t n, const orc_camera* cam,
+                          const float* projected_grad, orc_grads_f32* g) {
+#pragma omp parallel for schedule(static)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    const uint32_t idx = (uint32_t)ii;
+    float* gp = g->pos + 3 * (size_t)idx;
+    float* gr = g->rot + 4 * (size_t)idx;
+    float* gs = g->log_scale + 3 * (size_t)idx;
+    float* go = g->logit + idx;
+    float* gd = g->sh_dc + 3 * (size_t)idx;
+    float* grest = g->sh_rest + 45 * (size_t)idx;
+    const float* pw = sp->pos + 3 * (size_t)idx;
+    const V3<float> cx = to_camera<float>(cam, pw);
+    const float depth = -cx.z;
+    auto zero_all = [&]() {
+      for (int k = 0; k < 3; ++k) gp[k] = gs[k] = gd[k] = 0.0f;
+      for (int k = 0; k < 4; ++k) gr[k] = 0.0f;
+      *go = 0.0f;
+      for (int k = 0; k < 45; ++k) grest[k] = 0.0f;
+    };
+    if (depth < cam->near_) { zero_all(); continue; }
+    const V3<float> t{cx.x, cx.y, depth};
+    const float inv_z = 1.0f / t.z;
+    const float inv_z2 = inv_z * inv_z;
+    const float f = cam->focal;
+    const M3<float> R = quat_to_mat3<float>(sp->rot + 4 * (size_t)idx);
+    const float* ls = sp->log_scale + 3 * (size_t)idx;
+    const float sg[3] = {nts_expf(ls[0]), nts_expf(ls[1]), nts_expf(ls[2])};
+    const float s2[3] = {sg[0] * sg[0], sg[1] * sg[1], sg[2] * sg[2]};
+    M3<float> Ms;
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row) Ms.c[col][row] = R.c[col][row] * s2[col];
+    const M3<float> cov_world = mul(Ms, transpose(R));
+    const M3<float> Wm = view_rot<float>(cam);
+    const M3<float> cov_cam = mul(mul(Wm, cov_world), transpose(Wm));
+    const V3<float> j0{f * inv_z, 0.0f, -f * t.x * inv_z2};
+    const V3<float> j1{0.0f, -f * inv_z, f * t.y * inv_z2};
+    V3<float> r0, r1;
+    r0.x = cov_cam.c[0][0] * j0.x + cov_cam.c[1][0] * j0.y + cov_cam.c[2][0] * j0.z;
+    r0.y = cov_cam.c[0][1] * j0.x + cov_cam.c[1][1] * j0.y + cov_cam.c[2][1] * j0.z;
+    r0.z = cov_cam.c[0][2] * j0.x + cov_cam.c[1][2] * j0.y + cov_cam.c[2][2] * j0.z;
+    r1.x = cov_cam.c[0][0] * j1.x + cov_cam.c[1][0] * j1.y + cov_cam.c[2][0] * j1.z;
+    r1.y = cov_cam.c[0][1] * j1.x + cov_cam.c[1][1] * j1.y + cov_cam.c[2][1] * j1.z;
+    r1.z = cov_cam.c[0][2] * j1.x + cov_cam.c[1][2] * j1.y + cov_cam.c[2][2] * j1.z;
+    const float s00 = dot3(r0, j0) + NTS_COV_DILATION;
+    const float s01 = dot3(r0, j1);
+    const float s11 = dot3(r1, j1) + NTS_COV_DILATION;
+    const float det = s00 * s11 - s01 * s01;
+    if (det <= 0.0f) { zero_all(); continue; }
+    const float inv_det = 1.0f / det;
+    const float ca = s11 * inv_det, cb = -s01 * inv_det, cc = s00 * inv_det;
+
+    const float* pg = projected_grad + 12 * (size_t)idx;
+    const float dmx = pg[0], dmy = pg[1], dop = pg[2];
+    const float dca = pg[4], dcb = pg[5], dcc = pg[6];
+    const float dcol[3] = {pg[8], pg[9], pg[10]};
+
+    const float os = 1.0f / (1.0f + nts_expf(-sp->logit[idx]));
+    *go = dop * os * (1.0f - os);
+
+    V3<float> d{cam->cam_pos[0] - pw[0], cam->cam_pos[1] - pw[1], cam->cam_pos[2] - pw[2]};
+    const float inv_len = 1.0f / __builtin_sqrtf(dot3(d, d));
+    d.x = d.x * inv_len; d.y = d.y * inv_len; d.z = d.z * inv_len;
+    for (int ch = 0; ch < 3; ++ch) gd[ch] = dcol[ch] * NTS_SH_C0;
+    float b[15];
+    sh_basis<float>(d.x, d.y, d.z, b);
+    for (int k = 0; k < 15; ++k) {
+      grest[k] = dcol[0] * b[k];
+      grest[15 + k] = dcol[1] * b[k];
+      grest[30 + k] = dcol[2] * b[k];
+    }
+
+    // dSigma2d = -(cm * dconic_mat * cm), symmetric halving (203-217).
+    const float cm[2][2] = {{ca, cb}, {cb, cc}};                 // [col][row]
+    const float dm[2][2] = {{dca, dcb * 0.5f}, {dcb * 0.5f, dcc}};
+    float t1[2][2], full[2][2];
+    for (int col = 0; col < 2; ++col)
+      for (int row = 0; row < 2; ++row) t1[col][row] = cm[0][row] * dm[col][0] + cm[1][row] * dm[col][1];
+    for (int col = 0; col < 2; ++col)
+      for (int row = 0; row < 2; ++row)
+        full[col][row] = -1.0f * (t1[0][row] * cm[col][0] + t1[1][row] * cm[col][1]);
+    const float ds00 = full[0][0], ds11 = full[1][1];
+    const float ds01 = full[0][1] + full[1][0];
+    const float dsig[2][2] = {{ds00, ds01 * 0.5f}, {ds01 * 0.5f, ds11}};
+    float d00 = 0.0f, d01 = 0.0f, d02 = 0.0f, d11 = 0.0f, d12 = 0.0f, d22 = 0.0f;
+    for (int a = 0; a < 2; ++a)
+      for (int bb = 0; bb < 2; ++bb) {
+        const float coef = dsig[bb][a];
+        const V3<float> ja = (a == 0) ? j0 : j1;
+        const V3<float> jb = (bb == 0) ? j0 : j1;
+        d00 = d00 + ja.x * coef * jb.x;
+        d01 = d01 + ja.x * coef * jb.y;
+        d02 = d02 + ja.x * coef * jb.z;
+        d11 = d11 + ja.y * coef * jb.y;
+        d12 = d12 + ja.y * coef * jb.z;
+        d22 = d22 + ja.z * coef * jb.z;
+      }
+    M3<float> dcov_cam;
+    dcov_cam.c[0][0] = d00; dcov_cam.c[0][1] = d01; dcov_cam.c[0][2] = d02;
+    dcov_cam.c[1][0] = d01; dcov_cam.c[1][1] = d11; dcov_cam.c[1][2] = d12;
+    dcov_cam.c[2][0] = d02; dcov_cam.c[2][1] = d12; dcov_cam.c[2][2] = d22;
+    const M3<float> dcw = mul(mul(transpose(Wm), dcov_cam), Wm);
+    M3<float> RS2;
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row) RS2.c[col][row] = R.c[col][row] * s2[col];
+    M3<float> sym;
+    const M3<float> dcwT = transpose(dcw);
+    for (int col = 0; col < 3; ++col)
+      for (int row = 0; row < 3; ++row) sym.c[col][row] = dcw.c[col][row] + dcwT.c[col][row];
+    const M3<float> dR = mul(sym, RS2);
+    const float G00 = dR.c[0][0], G10 = dR.c[0][1], G20 = dR.c[0][2];
+    const float G01 = dR.c[1][0], G11 = dR.c[1][1], G21 = dR.c[1][2];
+    const float G02 = dR.c[2][0], G12 = dR.c[2][1], G22 = dR.c[2][2];
+    for (int k = 0; k < 3; ++k) {
+      const V3<float> rk{R.c[k][0], R.c[k][1], R.c[k][2]};
+      const float ds2 = dot3(rk, mulv(dcw, rk));
+      gs[k] = ds2 * 2.0f * s2[k];
+    }
+    const float* q = sp->rot + 4 * (size_t)idx;
+    const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+    gr[0] = 2.0f * (qz * (G10 - G01) + qy * (G02 - G20) + qx * (G21 - G12));
+    gr[1] = 2.0f * (qy * (G01 + G10) + qz * (G02 + G20) + qw * (G21 - G12)) - 4.0f * qx * (G11 + G22);
+    gr[2] = -4.0f * qy * (G00 + G22) + 2.0f * (qx * (G01 + G10) + qw * (G02 - G20) + qz * (G12 + G21));
+    gr[3] = -4.0f * qz * (G00 + G11) + 2.0f * (qw * (G10 - G01) + qx * (G02 + G20) + qy * (G12 + G21));
+
+    const V3<float> dtx{f * inv_z, 0.0f, -f * t.x * inv_z2};
+    const V3<float> dty{0.0f, -f * inv_z, f * t.y * inv_z2};
+    const V3<float> dLdt{dtx.x * dmx + dty.x * dmy, dtx.y * dmx + dty.y * dmy, dtx.z * dmx + dty.z * dmy};
+    const V3<float> dLdcam{dLdt.x, dLdt.y, -dLdt.z};
+    const V3<float> dp = mulv(transpose(Wm), dLdcam);
+    gp[0] = dp.x; gp[1] = dp.y; gp[2] = dp.z;
+  }
+}
+
+float orc_powi_f32(float a, int32_t b) { return powi_f32(a, b); }
+
+// adam.rs:70-90; bias corrections computed once per step (adam_gpu.rs:275-276).
+void orc_adam_step(float* params, const float* grads, float* m, float* v, uint64_t n, float lr,
+                   float beta1, float beta2, float eps, uint64_t t) {
+  const float bc1 = 1.0f - powi_f32(beta1, (int32_t)t);
+  const float bc2 = 1.0f - powi_f32(beta2, (int32_t)t);
+  for (uint64_t i = 0; i < n; ++i) {
+    const float g = grads[i];
+    const float mn = beta1 * m[i] + (1.0f - beta1) * g;
+    const float vn = beta2 * v[i] + (1.0f - beta2) * g * g;
+    m[i] = mn;
+    v[i] = vn;
+    const float mh = mn / bc1;
+    const float vh = vn / bc2;
+    params[i] = params[i] - lr * mh / (__builtin_sqrtf(vh) + eps);
+  }
+}
+
+// apply_constraints.wgsl:60-77.
+void orc_apply_constraints(float* rot, float* log_scale, float* logit, uint32_t n, float smin, float smax,
+                           float omax) {
+  for (uint32_t i = 0; i < n; ++i) {
+    float* q = rot + 4 * (size_t)i;
+    const float n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    const float nrm = nts_maxf(__builtin_sqrtf(n2), 1e-8f);
+    for (int k = 0; k < 4; ++k) q[k] = q[k] / nrm;
+    for (int k = 0; k < 3; ++k) {
+      float s = log_scale[3 * (size_t)i + k];
+      s = nts_minf(nts_maxf(s, smin), smax);
+      log_scale[3 * (size_t)i + k] = s;
+    }
+    logit[i] = nts_minf(nts_maxf(logit[i], -omax), omax);
+  }
+}
+
+// accumulate_grad_pos.wgsl:17-23.
+void orc_accumulate_grad(const float* dpos, float* acc, uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i)
+    acc[i] = acc[i] + fabsf(dpos[3 * (size_t)i]) + fabsf(dpos[3 * (size_t)i + 1]) + fabsf(dpos[3 * (size_t)i + 2]);
+}
+
+double orc_loss_grad(const float* pred, const float* target, uint32_t w, uint32_t h, int kind, float* dl_dc) {
+  const float n_scalars = (float)w * (float)h * 3.0f;
+  const uint64_t np = (uint64_t)w * h;
+  double sum = 0.0;
+  if (kind == 1) {  // MSE, loss.rs:163-179
+    const float inv_n = 2.0f / n_scalars;
+    for (uint64_t i = 0; i < np; ++i) {
+      float d[3];
+      for (int c = 0; c < 3; ++c) d[c] = pred[4 * i + c] - target[4 * i + c];
+      sum += (double)(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      for (int c = 0; c < 3; ++c) dl_dc[4 * i + c] = d[c] * inv_n;
+      dl_dc[4 * i + 3] = 0.0f;
+    }
+  } else {  // L1: mean |d| over 3WH, d/dC = sign(d)/(3WH)
+    const float inv_n = 1.0f / n_scalars;
+    for (uint64_t i = 0; i < np; ++i) {
+      for (int c = 0; c < 3; ++c) {
+        const float d = pred[4 * i + c] - target[4 * i + c];
+        sum += (double)fabsf(d);
+        dl_dc[4 * i + c] = d > 0.0f ? inv_n : (d < 0.0f ? -inv_n : 0.0f);
+      }
+      dl_dc[4 * i + 3] = 0.0f;
+    }
+  }
+  return sum / ((double)np * 3.0);
+}
+
+// -- prune / densify (train.rs:414-602) ------------------------------------
+
+namespace {
+const int kStride[6] = {3, 4, 3, 1, 3, 45};
+
+float* attr_ptr(orc_splats_mut* s, int a) {
+  switch (a) {
+    case 0: return s->pos;
+    case 1: return s->rot;
+    case 2: return s->log_scale;
+    case 3: return s->logit;
+    case 4: return s->sh_dc;
+    default: return s->sh_rest;
+  }
+}
+
+// SplatBuffer::swap_remove + AdamState::swap_remove_range + grad_acc swap/pop.
+void swap_remove(orc_splats_mut* s, uint32_t n, uint32_t i, float** moments, float* grad_acc) {
+  const uint32_t last = n - 1;
+  for (int a = 0; a < 6; ++a) {
+    const int st = kStride[a];
+    float* p = attr_ptr(s, a);
+    for (int k = 0; k < st; ++k) p[(size_t)i * st + k] = p[(size_t)last * st + k];
+    if (moments)
+      for (int mm = 0; mm < 2; ++mm) {
+        float* q = moments[2 * a + mm];
+        for (int k = 0; k < st; ++k) q[(size_t)i * st + k] = q[(size_t)last * st + k];
+      }
+  }
+  if (grad_acc) grad_acc[i] = grad_acc[last];
+}
+
+void push_splat(orc_splats_mut* s, uint32_t n, const float* pos, const float* rot, const float* scale,
+                float logit, const float* dc, const float* rest, float** moments, float* grad_acc) {
+  const float* src[6] = {pos, rot, scale, &logit, dc, rest};
+  for (int a = 0; a < 6; ++a) {
+    const int st = kStride[a];
+    float* p = attr_ptr(s, a);
+    for (int k = 0; k < st; ++k) p[(size_t)n * st + k] = src[a][k];
+    if (moments)
+      for (int mm = 0; mm < 2; ++mm) {
+        float* q = moments[2 * a + mm];
+        for (int k = 0; k < st; ++k) q[(size_t)n * st + k] = 0.0f;
+      }
+  }
+  if (grad_acc) grad_acc[n] = 0.0f;
+}
+
+// train.rs:552-564
+float lcg01(uint32_t* st) {
+  *st = *st * 1664525u + 1013904223u;
+  return (float)(*st) / (4294967295.0f + 1.0f);
+}
+float box_muller(uint32_t* st) {
+  const float u1 = nts_maxf(lcg01(st), 1e-8f);
+  const float u2 = lcg01(st);
+  return sqrtf(-2.0f * logf(u1)) * cosf(2.0f * 3.14159265358979323846f * u2);
+}
+}  // namespace
+
+uint32_t orc_prune(orc_splats_mut* s, uint32_t n, float threshold, float** moments, float* grad_acc) {
+  // Indices collected ascending, applied high-to-low (train.rs:586-601).
+  std::vector<uint32_t> rm;
+  for (uint32_t i = 0; i < n; ++i)
+    if (s->logit[i] < threshold) rm.push_back(i);
+  for (size_t j = rm.size(); j-- > 0;) {
+    swap_remove(s, n, rm[j], moments, grad_acc);
+    --n;
+  }
+  return n;
+}
+
+uint32_t orc_densify(orc_splats_mut* s, uint32_t n, uint32_t grad_count, float grad_threshold,
+                     float split_thr, uint32_t max_splats, uint32_t iter, float** moments, float* grad_acc,
+                     uint32_t counts_out[3]) {
+  const float count_f = (float)(grad_count > 1 ? grad_count : 1);
+  std::vector<uint32_t> to_clone, to_split;
+  uint32_t projected = n;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float avg = grad_acc[i] / count_f;
+    if (avg <= grad_threshold) continue;
+    const float* ls = s->log_scale + 3 * (size_t)i;
+    float mx = -INFINITY;
+    for (int k = 0; k < 3; ++k) mx = nts_maxf(mx, ls[k]);
+    if (mx > split_thr) {
+      if (projected < max_splats) { to_split.push_back(i); ++projected; }
+    } else if (projected < max_splats) {
+      to_clone.push_back(i);
+      ++projected;
+    }
+  }
+  uint32_t rng = iter * 1664525u + 1013904223u;
+  const float half = -0.6931471805599453f;
+  const float log_split = -logf(1.6f);
+  for (uint32_t i : to_clone) {
+    const float op_new = s->logit[i] + half;
+    float pos[3], rot[4], sc[3], dc[3], rest[45];
+    memcpy(pos, s->pos + 3 * (size_t)i, 12);
+    memcpy(rot, s->rot + 4 * (size_t)i, 16);
+    memcpy(sc, s->log_scale + 3 * (size_t)i, 12);
+    memcpy(dc, s->sh_dc + 3 * (size_t)i, 12);
+    memcpy(rest, s->sh_rest + 45 * (size_t)i, 180);
+    push_splat(s, n, pos, rot, sc, op_new, dc, rest, moments, grad_acc);
+    ++n;
+    s->logit[i] = op_new;
+  }
+  std::vector<uint32_t> parents;
+  for (uint32_t i : to_split) {
+    float pos[3], rot[4], sc[3], dc[3], rest[45];
+    memcpy(pos, s->pos + 3 * (size_t)i, 12);
+    memcpy(rot, s->rot + 4 * (size_t)i, 16);
+    memcpy(sc, s->log_scale + 3 * (size_t)i, 12);
+    memcpy(dc, s->sh_dc + 3 * (size_t)i, 12);
+    memcpy(rest, s->sh_rest + 45 * (size_t)i, 180);
+    const float op = s->logit[i];
+    const float nsc[3] = {sc[0] + log_split, sc[1] + log_split, sc[2] + log_split};
+    const float sg[3] = {expf(sc[0]), expf(sc[1]), expf(sc[2])};
+    // glam Quat (x, y, z, w) * Vec3: v + 2w(q x v) + 2 q x (q x v)
+    const float qw = rot[0], qx = rot[1], qy = rot[2], qz = rot[3];
+    for (int child = 0; child < 2; ++child) {
+      float off[3];
+      for (int k = 0; k < 3; ++k) off[k] = box_muller(&rng) * sg[k];
+      const float tx = 2.0f * (qy * off[2] - qz * off[1]);
+      const float ty = 2.0f * (qz * off[0] - qx * off[2]);
+      const float tz = 2.0f * (qx * off[1] - qy * off[0]);
+      const float rx = off[0] + qw * tx + (qy * tz - qz * ty);
+      const float ry = off[1] + qw * ty + (qz * tx - qx * tz);
+      const float rz = off[2] + qw * tz + (qx * ty - qy * tx);
+      const float np[3] = {pos[0] + rx, pos[1] + ry, pos[2] + rz};
+      push_splat(s, n, np, rot, nsc, op, dc, rest, moments, grad_acc);
+      ++n;
+    }
+    parents.push_back(i);
+  }
+  std::sort(parents.begin(), parents.end(), [](uint32_t a, uint32_t b) { return a > b; });
+  for (uint32_t i : parents) {
+    swap_remove(s, n, i, moments, grad_acc);
+    --n;
+  }
+  if (counts_out) {
+    counts_out[0] = (uint32_t)to_clone.size();
+    counts_out[1] = (uint32_t)to_split.size() * 2;
+    counts_out[2] = (uint32_t)parents.size();
+  }
+  return n;
+}
+
+// tonemap.wgsl:27-41.
+void orc_tonemap(const float* image, uint32_t w, uint32_t h, uint8_t* out) {
+  for (uint64_t i = 0; i < (uint64_t)w * h; ++i) {
+    for (int c = 0; c < 3; ++c) {
+      const float x = image[4 * i + c];
+      float tm = x / (1.0f + nts_maxf(x, 0.0f));
+      tm = nts_minf(nts_maxf(tm, 0.0f), 1.0f);
+      const float s = tm > 0.0031308f ? 1.055f * powf(tm, 1.0f / 2.4f) - 0.055f : tm * 12.92f;
+      out[4 * i + c] = (uint8_t)lrintf(nts_minf(nts_maxf(s, 0.0f), 1.0f) * 255.0f);
+    }
+    const float a = nts_minf(nts_maxf(image[4 * i + 3], 0.0f), 1.0f);
+    out[4 * i + 3] = (uint8_t)lrintf(a * 255.0f);
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// Host restatement of the nano-optimize differentiable splat rasteriser.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
+// check and the CPU-baseline leg of bench.py load it to compare the gfx950
+// path against.  No product code links or calls it.
+//
+// It restates, operation by operation, the WGSL kernels and Rust dispatch of
+// crates/nano-optimize (each function cites the file:line it follows), in
+// fp32 (bit parity with the device path: both sides use nts_math.h and
+// -ffp-contract=off) and in fp64 (finite-difference checks).
+//
+// Layouts:
+//   splats (host SoA, crates/nano-optimize/src/splat_store.rs:21-36):
+//     pos[3n] xyz, rot[4n] wxyz, log_scale[3n], logit[n], sh_dc[3n],
+//     sh_rest[45n] planar per splat (R[15] G[15] B[15]).
+//   projected record (raster.rs:27-39): 12 values per splat
+//     {mean.x, mean.y, depth, radius, conic.a, conic.b, conic.c, visible,
+//      r, g, b, opacity}.
+//   projected grad (raster.rs:46-57): 12 values per splat
+//     {dmean.x, dmean.y, dopacity, pad, dconic.a, .b, .c, pad, dr, dg, db, pad}.
+//   images: 4 floats per pixel, row-major.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_camera {   // raster.rs:61-71 CameraUniform (96 bytes)
+  float view[16];             // column-major 4x4 (glam to_cols_array)
+  float cam_pos[3];
+  float near_;
+  float focal;
+  float width;
+  float height;
+  uint32_t n_splats;
+} orc_camera;
+
+typedef struct orc_tiling {   // tile_binner.rs:21-42 TilingParams
+  uint32_t width, height, tile_size;
+  float depth_max;
+} orc_tiling;
+
+typedef struct orc_splats_f32 {
+  const float *pos, *rot, *log_scale, *logit, *sh_dc, *sh_rest;
+} orc_splats_f32;
+
+typedef struct orc_splats_f64 {
+  const double *pos, *rot, *log_scale, *logit, *sh_dc, *sh_rest;
+} orc_splats_f64;
+
+typedef struct orc_grads_f32 {
+  float *pos, *rot, *log_scale, *logit, *sh_dc, *sh_rest;
+} orc_grads_f32;
+
+int orc_version(void);
+int orc_num_threads(void);
+
+// exp / sigmoid from the shared scalar header (pinned against float64 exp).
+void orc_expf_array(const float* x, float* y, uint64_t n);
+
+void orc_camera_from_pose(const float eye[3], const float target[3], const float up[3],
+                          float fov_y, uint32_t width, uint32_t height, uint32_t n_splats,
+                          orc_camera* out);
+
+void orc_project_f32(const orc_splats_f32* s, uint32_t n, const orc_camera* cam, float* projected);
+void orc_project_f64(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, double* projected);
+
+// Binning over a single view; keys in the reference u32 format
+// (tile_id << 16 | depth16).  orc_tile_count returns the total pair count.
+uint64_t orc_tile_count(const float* projected, uint32_t n, const orc_tiling* t, uint32_t* counts);
+void orc_exclusive_scan_u32(const uint32_t* in, uint64_t n, uint32_t* out);
+void orc_tile_emit(const float* projected, uint32_t n, const uint32_t* offsets,
+                   const orc_tiling* t, uint32_t* keys, uint32_t* payloads);
+void orc_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint64_t n);
+void orc_sort_pairs_u64(uint64_t* keys, uint32_t* vals, uint64_t n);
+void orc_tile_ranges(const uint32_t* sorted_keys, uint64_t k, uint32_t num_tiles, uint32_t* ranges);
+// Full bin (count, scan, emit, stable sort, ranges).  Call once with
+// keys == NULL to get the pair count, then again with buffers of that size.
+uint64_t orc_bin(const float* projected, uint32_t n, const orc_tiling* t,
+                 uint32_t* keys, uint32_t* payloads, uint32_t* ranges);
+
+// Forward alpha blend (rasterize.wgsl:48-118).  out: 4 floats/pixel
+// (r, g, b, 1 - T).  final_t / last_idx may be NULL; last_idx is the number
+// of entries of the tile's slice up to and including the last splat that
+// contributed to the pixel.
+void orc_composite(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
+                   const orc_tiling* t, float* out, float* final_t, uint32_t* last_idx);
+void orc_composite_f64(const double* projected, const uint32_t* payloads, const uint32_t* ranges,
+                       const orc_tiling* t, double* out);
+
+// Reference-parity backward (rasterize_backward.wgsl:73-210).  Accumulates
+// (in float64, then rounds) into projected_grad[12n], which is overwritten.
+void orc_composite_backward_ref(const float* projected, const uint32_t* payloads,
+                                const uint32_t* ranges, const orc_tiling* t, uint32_t n,
+                                const float* forward_out, const float* dl_dc, float* projected_grad);
+
+// project_backward.wgsl:79-304.  Writes every gradient of every splat.
+void orc_project_backward(const orc_splats_f32* s, uint32_t n, const orc_camera* cam,
+                          const float* projected_grad, orc_grads_f32* g);
+
+// Adam (adam.rs:70-90) on a flat slab; bias corrections as adam_gpu.rs:275-276.
+float orc_powi_f32(float a, int32_t b);
+void orc_adam_step(float* params, const float* grads, float* m, float* v, uint64_t n,
+                   float lr, float beta1, float beta2, float eps, uint64_t t);
+void orc_apply_constraints(float* rot, float* log_scale, float* logit, uint32_t n,
+                           float log_scale_min, float log_scale_max, float logit_max);
+void orc_accumulate_grad(const float* dpos, float* acc, uint32_t n);
+
+// dL/dC for a rendered image against a target (both 4 floats/pixel, w
+// ignored).  kind 0 = L1 (mean |d| over 3WH, sign(0) = 0), 1 = MSE
+// (loss.rs:163-179).  Returns the loss (float64 accumulation).
+double orc_loss_grad(const float* pred, const float* target, uint32_t w, uint32_t h,
+                     int kind, float* dl_dc);
+
+// Prune / densify on the host SoA (train.rs:414-602).  Arrays must have
+// capacity for the result.  moments: 12 slabs (m,v per attribute, in the
+// attribute order pos, rot, scale, logit, dc, rest), may be NULL.
+typedef struct orc_splats_mut {
+  float *pos, *rot, *log_scale, *logit, *sh_dc, *sh_rest;
+} orc_splats_mut;
+uint32_t orc_prune(orc_splats_mut* s, uint32_t n, float threshold, float** moments, float* grad_acc);
+uint32_t orc_densify(orc_splats_mut* s, uint32_t n, uint32_t grad_count, float grad_threshold,
+                     float split_log_scale_threshold, uint32_t max_splats, uint32_t iter,
+                     float** moments, float* grad_acc, uint32_t counts_out[3]);
+
+// Reinhard + sRGB + rgba8 (tonemap.wgsl:27-41).
+void orc_tonemap(const float* image, uint32_t w, uint32_t h, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,311 @@
+"""ctypes wrapper of the host restatement (oracle/lib/libnts_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker.  Imported by tests/, by the smoke
+check and by bench.py's CPU-baseline leg; never by the product path.
+
+Each function restates a piece of crates/nano-optimize (file:line in
+nts_oracle.h / nts_oracle.cpp).  All parts are pinned by tests:
+  - exp: against float64 exp (<= 1 ulp);
+  - scan / sort / count / emit / ranges: the reference's known-answer tests
+    (prefix_scan.rs:291-333, radix_sort.rs:357-408, tile_binner.rs:443-541),
+    closed forms and brute force;
+  - projection: the closed form of raster.rs:999-1026 and a float64 EWA
+    restatement written independently in tests/;
+  - forward blend: brute force per pixel and the analytic single splat;
+  - backward: fp64 central finite differences of the forward;
+  - Adam / constraints / grad_acc: closed forms and the adam_gpu_parity.rs
+    sequences;
+  - prune / densify: the train.rs:626-743 known-answer tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "lib" / "libnts_oracle.so"
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("view", C.c_float * 16),
+        ("cam_pos", C.c_float * 3),
+        ("near_", C.c_float),
+        ("focal", C.c_float),
+        ("width", C.c_float),
+        ("height", C.c_float),
+        ("n_splats", C.c_uint32),
+    ]
+
+
+class Tiling(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tile_size", C.c_uint32), ("depth_max", C.c_float)]
+
+
+class _Splats32(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("pos", "rot", "log_scale", "logit", "sh_dc", "sh_rest")]
+
+
+_h = None
+
+
+def lib():
+    global _h
+    if _h is None:
+        if not LIB.exists():
+            build()
+        _h = C.CDLL(str(LIB))
+        for name in ("orc_tile_count", "orc_bin"):
+            getattr(_h, name).restype = C.c_uint64
+        _h.orc_loss_grad.restype = C.c_double
+        _h.orc_powi_f32.restype = C.c_float
+        _h.orc_powi_f32.argtypes = [C.c_float, C.c_int32]
+        _h.orc_prune.restype = C.c_uint32
+        _h.orc_densify.restype = C.c_uint32
+    return _h
+
+
+def _p(a: np.ndarray):
+    assert a.flags.c_contiguous, "array must be contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def expf(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_expf_array(_p(x), _p(y), C.c_uint64(x.size))
+    return y
+
+
+def camera_from_pose(eye, target, up, fov_y, width, height, n_splats) -> Camera:
+    f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+    cam = Camera()
+    lib().orc_camera_from_pose(f3(eye), f3(target), f3(up), C.c_float(fov_y), C.c_uint32(width),
+                               C.c_uint32(height), C.c_uint32(n_splats), C.byref(cam))
+    return cam
+
+
+def camera_like(cam) -> Camera:
+    """Copy any 96-byte camera structure into the oracle's type."""
+    out = Camera()
+    C.memmove(C.byref(out), C.byref(cam), C.sizeof(Camera))
+    return out
+
+
+def tiling(width, height, tile_size=16, depth_max=100.0) -> Tiling:
+    return Tiling(width, height, tile_size, depth_max)
+
+
+def _splats(s, dtype):
+    arrs = [np.ascontiguousarray(a, dtype) for a in (s.pos, s.rot, s.log_scale, s.logit, s.sh_dc, s.sh_rest)]
+    st = _Splats32(*[a.ctypes.data for a in arrs])
+    return st, arrs
+
+
+def project(s, cam: Camera, dtype=np.float32) -> np.ndarray:
+    st, keep = _splats(s, dtype)
+    n = s.pos.shape[0]
+    out = np.zeros((n, 12), dtype)
+    fn = lib().orc_project_f32 if dtype == np.float32 else lib().orc_project_f64
+    fn(C.byref(st), C.c_uint32(n), C.byref(cam), _p(out))
+    return out
+
+
+def tile_count(projected: np.ndarray, t: Tiling):
+    n = projected.shape[0]
+    counts = np.zeros(max(n, 1), np.uint32)
+    total = lib().orc_tile_count(_p(np.ascontiguousarray(projected, np.float32)), C.c_uint32(n), C.byref(t),
+                                 _p(counts))
+    return counts[:n], int(total)
+
+
+def exclusive_scan(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty_like(a)
+    lib().orc_exclusive_scan_u32(_p(a), _p(out), C.c_uint64(a.size))
+    return out
+
+
+def sort_pairs(keys: np.ndarray, vals: np.ndarray):
+    k = np.array(keys, copy=True)
+    v = np.array(vals, dtype=np.uint32, copy=True)
+    if k.dtype == np.uint32:
+        lib().orc_sort_pairs_u32(_p(k), _p(v), C.c_uint64(k.size))
+    else:
+        k = k.astype(np.uint64)
+        lib().orc_sort_pairs_u64(_p(k), _p(v), C.c_uint64(k.size))
+    return k, v
+
+
+def bin(projected: np.ndarray, t: Tiling):
+    """Full TileBinner::bin: (sorted u32 keys, payloads, ranges[2*tiles])."""
+    projected = np.ascontiguousarray(projected, np.float32)
+    n = projected.shape[0]
+    total = lib().orc_bin(_p(projected), C.c_uint32(n), C.byref(t), None, None, None)
+    ntiles = (-(-t.width // t.tile_size)) * (-(-t.height // t.tile_size))
+    keys = np.zeros(max(total, 1), np.uint32)
+    pay = np.zeros(max(total, 1), np.uint32)
+    ranges = np.zeros(2 * ntiles, np.uint32)
+    lib().orc_bin(_p(projected), C.c_uint32(n), C.byref(t), _p(keys), _p(pay), _p(ranges))
+    return keys[:total], pay[:total], ranges
+
+
+def composite(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t: Tiling, with_state=False):
+    projected = np.ascontiguousarray(projected, np.float32)
+    payloads = np.ascontiguousarray(payloads, np.uint32)
+    if payloads.size == 0:
+        payloads = np.zeros(1, np.uint32)
+    ranges = np.ascontiguousarray(ranges, np.uint32)
+    img = np.zeros((t.height, t.width, 4), np.float32)
+    ft = np.zeros((t.height, t.width), np.float32)
+    li = np.zeros((t.height, t.width), np.uint32)
+    lib().orc_composite(_p(projected), _p(payloads), _p(ranges), C.byref(t), _p(img), _p(ft), _p(li))
+    return (img, ft, li) if with_state else img
+
+
+def composite_f64(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t: Tiling) -> np.ndarray:
+    projected = np.ascontiguousarray(projected, np.float64)
+    payloads = np.ascontiguousarray(payloads, np.uint32)
+    if payloads.size == 0:
+        payloads = np.zeros(1, np.uint32)
+    img = np.zeros((t.height, t.width, 4), np.float64)
+    lib().orc_composite_f64(_p(projected), _p(payloads), _p(np.ascontiguousarray(ranges, np.uint32)), C.byref(t),
+                            _p(img))
+    return img
+
+
+def composite_backward_ref(projected, payloads, ranges, t: Tiling, forward_out, dl_dc) -> np.ndarray:
+    projected = np.ascontiguousarray(projected, np.float32)
+    n = projected.shape[0]
+    payloads = np.ascontiguousarray(payloads, np.uint32)
+    if payloads.size == 0:
+        payloads = np.zeros(1, np.uint32)
+    out = np.zeros((n, 12), np.float32)
+    lib().orc_composite_backward_ref(_p(projected), _p(payloads), _p(np.ascontiguousarray(ranges, np.uint32)),
+                                     C.byref(t), C.c_uint32(n), _p(np.ascontiguousarray(forward_out, np.float32)),
+                                     _p(np.ascontiguousarray(dl_dc, np.float32)), _p(out))
+    return out
+
+
+def project_backward(s, cam: Camera, projected_grad: np.ndarray):
+    """Returns gradients in the host SoA layout (HostSplats-like tuple of arrays)."""
+    from nanotracer_amd.synth import HostSplats  # data container only
+
+    st, keep = _splats(s, np.float32)
+    n = s.pos.shape[0]
+    g = HostSplats.zeros(n)
+    gs = _Splats32(*[a.ctypes.data for a in g.arrays()])
+    lib().orc_project_backward(C.byref(st), C.c_uint32(n), C.byref(cam),
+                               _p(np.ascontiguousarray(projected_grad, np.float32)), C.byref(gs))
+    return g
+
+
+def powi(a: float, b: int) -> float:
+    return lib().orc_powi_f32(a, b)
+
+
+def adam_step(params, grads, m, v, lr, beta1, beta2, eps, t):
+    """In place on float32 arrays (adam.rs:70-90)."""
+    lib().orc_adam_step(_p(params), _p(np.ascontiguousarray(grads, np.float32)), _p(m), _p(v),
+                        C.c_uint64(params.size), C.c_float(lr), C.c_float(beta1), C.c_float(beta2),
+                        C.c_float(eps), C.c_uint64(t))
+
+
+def apply_constraints(rot, log_scale, logit, smin=-10.0, smax=5.0, omax=10.0):
+    lib().orc_apply_constraints(_p(rot), _p(log_scale), _p(logit), C.c_uint32(logit.size), C.c_float(smin),
+                                C.c_float(smax), C.c_float(omax))
+
+
+def accumulate_grad(dpos, acc):
+    lib().orc_accumulate_grad(_p(np.ascontiguousarray(dpos, np.float32)), _p(acc), C.c_uint32(acc.size))
+
+
+def loss_grad(pred, target, kind=0):
+    pred = np.ascontiguousarray(pred, np.float32)
+    h, w = pred.shape[:2]
+    dl = np.zeros_like(pred)
+    loss = lib().orc_loss_grad(_p(pred), _p(np.ascontiguousarray(target, np.float32)), C.c_uint32(w),
+                               C.c_uint32(h), C.c_int(kind), _p(dl))
+    return loss, dl
+
+
+class _Mut(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("pos", "rot", "log_scale", "logit", "sh_dc", "sh_rest")]
+
+
+def _mut(s, cap):
+    arrs = []
+    for a, w in zip(s.arrays(), (3, 4, 3, 1, 3, 45)):
+        b = np.zeros(cap * w, np.float32)
+        b[: a.size] = a.reshape(-1)
+        arrs.append(b)
+    return _Mut(*[a.ctypes.data for a in arrs]), arrs
+
+
+def _unmut(arrs, n):
+    from nanotracer_amd.synth import HostSplats
+
+    ws = (3, 4, 3, 1, 3, 45)
+    out = [arrs[k][: n * ws[k]].reshape(n, ws[k]) if ws[k] > 1 else arrs[k][:n].copy() for k in range(6)]
+    return HostSplats(*[np.ascontiguousarray(a) for a in out])
+
+
+def _moments(moments, cap):
+    """moments: list of 12 flat arrays (m, v per attribute) or None."""
+    if moments is None:
+        return None, None
+    ws = (3, 4, 3, 1, 3, 45)
+    bufs = []
+    for k in range(12):
+        b = np.zeros(cap * ws[k // 2], np.float32)
+        b[: moments[k].size] = moments[k]
+        bufs.append(b)
+    ptrs = (C.c_void_p * 12)(*[b.ctypes.data for b in bufs])
+    return ptrs, bufs
+
+
+def prune(s, threshold=-5.3, moments=None, grad_acc=None):
+    n = s.n
+    st, arrs = _mut(s, n)
+    mp, mb = _moments(moments, n)
+    ga = None if grad_acc is None else np.ascontiguousarray(grad_acc, np.float32).copy()
+    n2 = lib().orc_prune(C.byref(st), C.c_uint32(n), C.c_float(threshold), mp, None if ga is None else _p(ga))
+    ws = (3, 4, 3, 1, 3, 45)
+    mo = None if mb is None else [mb[k][: n2 * ws[k // 2]] for k in range(12)]
+    return _unmut(arrs, n2), mo, (None if ga is None else ga[:n2])
+
+
+def densify(s, grad_count, grad_threshold, split_thr, max_splats, it, moments=None, grad_acc=None):
+    n = s.n
+    cap = max(n * 3, max_splats) + 4
+    st, arrs = _mut(s, cap)
+    mp, mb = _moments(moments, cap)
+    ga = np.zeros(cap, np.float32)
+    ga[:n] = grad_acc
+    counts = (C.c_uint32 * 3)()
+    n2 = lib().orc_densify(C.byref(st), C.c_uint32(n), C.c_uint32(grad_count), C.c_float(grad_threshold),
+                           C.c_float(split_thr), C.c_uint32(max_splats), C.c_uint32(it), mp, _p(ga), counts)
+    ws = (3, 4, 3, 1, 3, 45)
+    mo = None if mb is None else [mb[k][: n2 * ws[k // 2]] for k in range(12)]
+    return _unmut(arrs, n2), mo, ga[:n2], tuple(counts)
+
+
+def tonemap(image: np.ndarray) -> np.ndarray:
+    image = np.ascontiguousarray(image, np.float32)
+    h, w = image.shape[:2]
+    out = np.zeros((h, w, 4), np.uint8)
+    lib().orc_tonemap(_p(image), C.c_uint32(w), C.c_uint32(h), _p(out))
+    return out
