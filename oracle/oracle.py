@@ -127,15 +127,15 @@ def project(s, cam: Camera, dtype=np.float32) -> np.ndarray:
 def tile_count(projected: np.ndarray, t: Tiling):
     n = projected.shape[0]
     counts = np.zeros(max(n, 1), np.uint32)
-    total = lib().orc_tile_count(_p(np.ascontiguousarray(projected, np.float32)), C.c_uint32(n), C.byref(t),
-                                 _p(counts))
+    pj = np.ascontiguousarray(projected, np.float32)
+    total = lib().orc_tile_count(_p(pj), C.c_uint32(n), C.byref(t), _p(counts))
     return counts[:n], int(total)
 
 
 def exclusive_scan(a: np.ndarray) -> np.ndarray:
     a = np.ascontiguousarray(a, np.uint32)
     out = np.empty_like(a)
-    lib().orc_exclusive_scan_u32(_p(a), _p(out), C.c_uint64(a.size))
+    lib().orc_exclusive_scan_u32(_p(a), C.c_uint64(a.size), _p(out))
     return out
 
 
@@ -182,8 +182,8 @@ def composite_f64(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarra
     if payloads.size == 0:
         payloads = np.zeros(1, np.uint32)
     img = np.zeros((t.height, t.width, 4), np.float64)
-    lib().orc_composite_f64(_p(projected), _p(payloads), _p(np.ascontiguousarray(ranges, np.uint32)), C.byref(t),
-                            _p(img))
+    rg = np.ascontiguousarray(ranges, np.uint32)
+    lib().orc_composite_f64(_p(projected), _p(payloads), _p(rg), C.byref(t), _p(img))
     return img
 
 
@@ -194,9 +194,11 @@ def composite_backward_ref(projected, payloads, ranges, t: Tiling, forward_out, 
     if payloads.size == 0:
         payloads = np.zeros(1, np.uint32)
     out = np.zeros((n, 12), np.float32)
-    lib().orc_composite_backward_ref(_p(projected), _p(payloads), _p(np.ascontiguousarray(ranges, np.uint32)),
-                                     C.byref(t), C.c_uint32(n), _p(np.ascontiguousarray(forward_out, np.float32)),
-                                     _p(np.ascontiguousarray(dl_dc, np.float32)), _p(out))
+    rg = np.ascontiguousarray(ranges, np.uint32)
+    fo = np.ascontiguousarray(forward_out, np.float32)
+    dl = np.ascontiguousarray(dl_dc, np.float32)
+    lib().orc_composite_backward_ref(_p(projected), _p(payloads), _p(rg), C.byref(t), C.c_uint32(n), _p(fo),
+                                     _p(dl), _p(out))
     return out
 
 
@@ -208,8 +210,8 @@ def project_backward(s, cam: Camera, projected_grad: np.ndarray):
     n = s.pos.shape[0]
     g = HostSplats.zeros(n)
     gs = _Splats32(*[a.ctypes.data for a in g.arrays()])
-    lib().orc_project_backward(C.byref(st), C.c_uint32(n), C.byref(cam),
-                               _p(np.ascontiguousarray(projected_grad, np.float32)), C.byref(gs))
+    pg = np.ascontiguousarray(projected_grad, np.float32)
+    lib().orc_project_backward(C.byref(st), C.c_uint32(n), C.byref(cam), _p(pg), C.byref(gs))
     return g
 
 
@@ -219,7 +221,8 @@ def powi(a: float, b: int) -> float:
 
 def adam_step(params, grads, m, v, lr, beta1, beta2, eps, t):
     """In place on float32 arrays (adam.rs:70-90)."""
-    lib().orc_adam_step(_p(params), _p(np.ascontiguousarray(grads, np.float32)), _p(m), _p(v),
+    gr = np.ascontiguousarray(grads, np.float32)
+    lib().orc_adam_step(_p(params), _p(gr), _p(m), _p(v),
                         C.c_uint64(params.size), C.c_float(lr), C.c_float(beta1), C.c_float(beta2),
                         C.c_float(eps), C.c_uint64(t))
 
@@ -230,14 +233,16 @@ def apply_constraints(rot, log_scale, logit, smin=-10.0, smax=5.0, omax=10.0):
 
 
 def accumulate_grad(dpos, acc):
-    lib().orc_accumulate_grad(_p(np.ascontiguousarray(dpos, np.float32)), _p(acc), C.c_uint32(acc.size))
+    dp = np.ascontiguousarray(dpos, np.float32)
+    lib().orc_accumulate_grad(_p(dp), _p(acc), C.c_uint32(acc.size))
 
 
 def loss_grad(pred, target, kind=0):
     pred = np.ascontiguousarray(pred, np.float32)
     h, w = pred.shape[:2]
     dl = np.zeros_like(pred)
-    loss = lib().orc_loss_grad(_p(pred), _p(np.ascontiguousarray(target, np.float32)), C.c_uint32(w),
+    tg = np.ascontiguousarray(target, np.float32)
+    loss = lib().orc_loss_grad(_p(pred), _p(tg), C.c_uint32(w),
                                C.c_uint32(h), C.c_int(kind), _p(dl))
     return loss, dl
 

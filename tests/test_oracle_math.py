@@ -1,0 +1,96 @@
+"""Pins the checker's scalar maths: exp against float64, powi against the
+closed form, and the SH basis against the real spherical harmonics."""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+from oracle import oracle as O
+
+
+def test_expf_within_one_ulp():
+    x = np.concatenate([np.linspace(-87.3, 88.7, 3_000_001), np.linspace(-1, 1, 200_001)]).astype(np.float32)
+    y = O.expf(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert (np.abs(y - ref) / ulp).max() <= 1.0
+
+
+def test_expf_special_values():
+    x = np.array([0.0, -0.0, -200.0, -104.5, 89.0, np.inf, -np.inf, np.nan, 1.0, -1.0], np.float32)
+    y = O.expf(x)
+    assert y[0] == 1.0 and y[1] == 1.0
+    assert y[2] == 0.0 and y[3] == 0.0
+    assert np.isinf(y[4]) and np.isinf(y[5]) and y[6] == 0.0 and np.isnan(y[7])
+    assert abs(y[8] - math.e) <= 2.5e-7 and abs(y[9] - 1 / math.e) <= 1e-7
+
+
+def test_expf_subnormal_range():
+    x = np.linspace(-103.9, -87.4, 20001).astype(np.float32)
+    y = O.expf(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    # one rounding into the subnormal grid (spacing 2^-149)
+    assert np.abs(y - ref).max() <= 2.0 ** -149
+
+
+@pytest.mark.parametrize("b,t", [(0.9, 1), (0.9, 7), (0.999, 1), (0.999, 100), (0.999, 12345), (0.5, 30)])
+def test_powi_matches_closed_form(b, t):
+    assert abs(O.powi(b, t) - float(np.float32(b)) ** t) <= 1e-6 * max(1.0, float(np.float32(b)) ** t) + 1e-12
+
+
+def _real_sh(l, m, theta, phi):
+    """Real SH with the Condon-Shortley phase, from scipy's complex Y."""
+    if hasattr(special, "sph_harm_y"):
+        Y = lambda mm: special.sph_harm_y(l, mm, theta, phi)
+    else:
+        Y = lambda mm: special.sph_harm(mm, l, phi, theta)
+    if m > 0:
+        return math.sqrt(2) * (-1) ** m * np.real(Y(m))
+    if m < 0:
+        return math.sqrt(2) * (-1) ** m * np.imag(Y(-m))
+    return np.real(Y(0))
+
+
+def test_sh_basis_is_the_real_spherical_harmonics():
+    """Colours from a splat with one unit SH coefficient equal the basis
+    function (+0.5 offset) of the real SH for that (l, m), up to the sign
+    convention of the 3DGS basis and the 6-digit constants."""
+    from nanotracer_amd import synth
+
+    rng = np.random.default_rng(0)
+    dirs = rng.normal(size=(64, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    # (index in sh_rest band order) -> (l, m); 3DGS order per band is m = -l..l,
+    # with the basis negated for odd m in band 1 and matching CS phase in others
+    lm = [(1, -1), (1, 0), (1, 1), (2, -2), (2, -1), (2, 0), (2, 1), (2, 2),
+          (3, -3), (3, -2), (3, -1), (3, 0), (3, 1), (3, 2), (3, 3)]
+    cam = O.camera_from_pose((0, 0, 0), (0, 0, -1), (0, 1, 0), math.pi / 2, 64, 64, 1)
+    for k, (l, m) in enumerate(lm):
+        vals = []
+        refs = []
+        for d in dirs:
+            # splat placed so that the direction splat -> camera is d, in front
+            # of the camera; use a tiny splat far away along -d to keep it visible
+            pos = -np.asarray(d) * 5.0
+            if pos[2] > -1.0:  # must lie in front (z < 0)
+                continue
+            s = synth.HostSplats(pos[None].astype(np.float32), np.array([[1, 0, 0, 0]], np.float32),
+                                 np.full((1, 3), -3.0, np.float32), np.array([0.0], np.float32),
+                                 np.zeros((1, 3), np.float32), np.zeros((1, 45), np.float32))
+            s.sh_rest[0, k] = 1.0  # red channel, coefficient k
+            s.sh_dc[0] = -0.5 / 0.2820948  # cancel the +0.5 offset
+            p = O.project(s, cam, np.float64)
+            if p[0, 7] == 0:
+                continue
+            theta = math.acos(np.clip(d[2], -1, 1))
+            phi = math.atan2(d[1], d[0])
+            vals.append(p[0, 8])
+            refs.append(_real_sh(l, m, theta, phi))
+        vals, refs = np.array(vals), np.array(refs)
+        assert len(vals) > 10
+        # the renderer clamps at 0: compare where the reference is positive
+        sgn = 1.0 if np.dot(np.maximum(refs, 0), vals) >= np.dot(np.maximum(-refs, 0), vals) else -1.0
+        pos = sgn * refs > 1e-3
+        assert pos.sum() > 3
+        assert np.allclose(vals[pos], sgn * refs[pos], rtol=2e-5, atol=2e-6), (k, l, m)
