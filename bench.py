@@ -31,7 +31,7 @@ WIDTH, HEIGHT = 1920, 1080
 SEED = 0x5EED0002
 
 
-def build_workload(ctx):
+def build_workload(ctx, backward_mode):
     from nanotracer_amd import synth
     from nanotracer_amd.api import GpuSplatBuffer
     from nanotracer_amd.train import Trainer, View
@@ -47,7 +47,7 @@ def build_workload(ctx):
     bins.free()
     tgt_scene.free()
     g = GpuSplatBuffer.upload(ctx, s)
-    tr = Trainer(ctx, g, [View(eye, WIDTH, HEIGHT, target)], 12.0)
+    tr = Trainer(ctx, g, [View(eye, WIDTH, HEIGHT, target)], 12.0, backward_mode=backward_mode)
     return s, eye, target, g, tr
 
 
@@ -65,6 +65,8 @@ def alg_bytes(name: str, N: int, Nv: int, K: int, P: int, T: int, kb: int = 4) -
         "blend_fwd": 8 * T + 40 * K + 16 * P,
         "loss": 48 * P,
         "blend_bwd": 8 * T + 40 * K + 32 * P + 36 * Nv,
+        "blend_bwd_recompute": 8 * T + 40 * K + 24 * P + 36 * Nv,
+        "project_backward_full": (236 + 48 + 236) * N,
         "project_backward": (236 + 48 + 236) * N,
         "adam_step": 7 * 236 * N + 8 * N,
     }
@@ -102,16 +104,20 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--backward", choices=("ref", "recompute"), default="ref",
+                    help="reference-parity backward or the recompute (final-T, last-index) backward")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(see the sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    from nanotracer_amd import _lib as L
     from nanotracer_amd.api import GpuCtx
 
     ctx = GpuCtx(0)
-    s, eye, target, g, tr = build_workload(ctx)
+    mode = L.BWD_RECOMPUTE if args.backward == "recompute" else L.BWD_REF_PARITY
+    s, eye, target, g, tr = build_workload(ctx, mode)
     for it in range(args.warmup):
         tr.step(it)
     ctx.sync()
@@ -166,7 +172,7 @@ def main():
         data="synthetic",
         config=dict(workload="plain single-view training step (project, bin, blend fwd, L1, blend bwd, "
                              "project_backward, fused Adam)", splats=N_SPLATS, width=WIDTH, height=HEIGHT,
-                    tile=16, pairs=pairs, visible=Nv),
+                    tile=16, pairs=pairs, visible=Nv, backward=args.backward),
         render_mpx_s=round(render_mpx_s, 2), render_p50_ms=round(float(np.percentile(lat, 50)), 3),
         render_p99_ms=round(float(np.percentile(lat, 99)), 3),
         roofline=roofline,

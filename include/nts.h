@@ -166,6 +166,12 @@ int nts_sort_pairs_u64(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_
  * the reference's kernel is fixed at 16. */
 int nts_composite(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,
                   const nts_tiling* tiling);
+/* As nts_composite, also saving per pixel the final transmittance (f32)
+ * and the slice-relative index + 1 of the last contributing splat (u32),
+ * which the recompute backward starts from.  Either may be NULL.  No
+ * reference counterpart: the reference stores only 1 - T (rasterize.wgsl:116). */
+int nts_composite_ex(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,
+                     const nts_tiling* tiling, nts_buffer* final_t, nts_buffer* last_idx);
 
 /* ---- loss (dL/dC contract, train.rs:180-196, loss.rs:163-179) ---------- */
 /* dl_dc: 16 B per pixel (w = 0).  *loss receives the scalar loss. */
@@ -179,9 +185,20 @@ int nts_loss_grad(nts_ctx* ctx, const nts_buffer* image, const nts_buffer* targe
 int nts_composite_backward(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins,
                            const nts_buffer* forward_out, const nts_buffer* dl_dc,
                            nts_buffer* projected_grad, const nts_tiling* tiling);
-/* Writes the scene's parameter gradients. */
+/* Either mode: REF_PARITY reads forward_out (T = 1 - w, walk from the
+ * slice end, bail at T > 1.0001); RECOMPUTE reads final_t / last_idx of
+ * nts_composite_ex (walk from each pixel's last contributor, no bail). */
+int nts_composite_backward_ex(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins,
+                              const nts_buffer* forward_out, const nts_buffer* final_t, const nts_buffer* last_idx,
+                              const nts_buffer* dl_dc, nts_buffer* projected_grad, const nts_tiling* tiling, int mode);
+/* Writes the scene's parameter gradients (reference chain rule). */
 int nts_project_backward(nts_ctx* ctx, nts_scene* scene, const nts_buffer* projected_grad,
                          const nts_camera* cam);
+/* mode NTS_BWD_RECOMPUTE: the exact chain rule, adding the colour clamp
+ * gate, d(view dir)/d(position) and dJ/dt that project_backward.wgsl:17-24
+ * drops. */
+int nts_project_backward_ex(nts_ctx* ctx, nts_scene* scene, const nts_buffer* projected_grad,
+                            const nts_camera* cam, int mode);
 
 /* ---- optimiser (AdamGpu, adam_gpu.rs:334-541) ------------------------- */
 /* AdamGpu::step on a flat slab of n floats (adam_step.wgsl). */

@@ -320,14 +320,26 @@ class Rasterizer:
                   params: TilingParams) -> None:
         L.check(L.lib().nts_composite(ctx.h, projected.h, bins.h, image.h, C.byref(params.c())))
 
+    def composite_ex(self, ctx: GpuCtx, projected: Buffer, bins: TileBinningResult, image: Buffer,
+                     params: TilingParams, final_t: Buffer | None, last_idx: Buffer | None) -> None:
+        """composite that also saves the final transmittance and last contributor per pixel."""
+        L.check(L.lib().nts_composite_ex(ctx.h, projected.h, bins.h, image.h, C.byref(params.c()),
+                                         final_t.h if final_t else None, last_idx.h if last_idx else None))
+
     def composite_backward(self, ctx: GpuCtx, projected: Buffer, bins: TileBinningResult, forward_out: Buffer,
                            dl_dc: Buffer, projected_grad: Buffer, params: TilingParams) -> None:
         L.check(L.lib().nts_composite_backward(ctx.h, projected.h, bins.h, forward_out.h, dl_dc.h,
                                                projected_grad.h, C.byref(params.c())))
 
+    def composite_backward_recompute(self, ctx: GpuCtx, projected: Buffer, bins: TileBinningResult,
+                                     final_t: Buffer, last_idx: Buffer, dl_dc: Buffer, projected_grad: Buffer,
+                                     params: TilingParams) -> None:
+        L.check(L.lib().nts_composite_backward_ex(ctx.h, projected.h, bins.h, None, final_t.h, last_idx.h, dl_dc.h,
+                                                  projected_grad.h, C.byref(params.c()), L.BWD_RECOMPUTE))
+
     def project_backward(self, ctx: GpuCtx, splats: GpuSplatBuffer, projected_grad: Buffer,
-                         camera: L.Camera) -> None:
-        L.check(L.lib().nts_project_backward(ctx.h, splats.h, projected_grad.h, C.byref(camera)))
+                         camera: L.Camera, mode: int = L.BWD_REF_PARITY) -> None:
+        L.check(L.lib().nts_project_backward_ex(ctx.h, splats.h, projected_grad.h, C.byref(camera), int(mode)))
 
     def tonemap(self, ctx: GpuCtx, composite: Buffer, target: Buffer, width: int, height: int) -> None:
         L.check(L.lib().nts_tonemap(ctx.h, composite.h, target.h, width, height))

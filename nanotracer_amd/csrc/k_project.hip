@@ -70,7 +70,11 @@ __global__ void __launch_bounds__(256) k_project(const float* __restrict__ P, ui
 }
 
 // project_backward.wgsl:79-304: chain the 2-D gradients back to the splat
-// parameters.  Writes every gradient plane of splat i.
+// parameters.  Writes every gradient plane of splat i.  FULL adds the terms
+// the reference drops (project_backward.wgsl:17-24): the colour clamp gate,
+// the view direction's dependence on the position and the Jacobian's
+// dependence on the camera-space mean.
+template <bool FULL>
 __global__ void __launch_bounds__(256) k_project_backward(const float* __restrict__ P, uint32_t cap, uint32_t n,
                                                           DevCamera cam, const ProjectedGrad* __restrict__ pgrad,
                                                           float* __restrict__ G) {
@@ -101,7 +105,7 @@ __global__ void __launch_bounds__(256) k_project_backward(const float* __restric
     const ProjectedGrad pg = pgrad[i];
     const float dmx = pg.a.x, dmy = pg.a.y, dop = pg.a.z;
     const float dca = pg.b.x, dcb = pg.b.y, dcc = pg.b.z;
-    const float dcol[3] = {pg.c.x, pg.c.y, pg.c.z};
+    float dcol[3] = {pg.c.x, pg.c.y, pg.c.z};
 
     const float os = nts_sigmoidf(P[P_LOGIT * (size_t)cap + i]);
     g[P_LOGIT] = dop * os * (1.0f - os);
@@ -111,10 +115,20 @@ __global__ void __launch_bounds__(256) k_project_backward(const float* __restric
     dx = dx * inv_len;
     dy = dy * inv_len;
     dz = dz * inv_len;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) g[P_DC + ch] = dcol[ch] * NTS_SH_C0;
     float b[15];
     sh_basis15(dx, dy, dz, b);
+    if (FULL) {  // max(SH + 0.5, 0) passes no gradient where it clamps
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 15; ++k) acc = acc + P[(P_REST + 15 * ch + k) * (size_t)cap + i] * b[k];
+        const float raw = (NTS_SH_C0 * P[(P_DC + ch) * (size_t)cap + i] + acc) + 0.5f;
+        if (!(raw > 0.0f)) dcol[ch] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) g[P_DC + ch] = dcol[ch] * NTS_SH_C0;
 #pragma unroll
     for (int k = 0; k < 15; ++k) {
       g[P_REST + k] = dcol[0] * b[k];
@@ -184,9 +198,42 @@ __global__ void __launch_bounds__(256) k_project_backward(const float* __restric
 
     const float3 dtx = make_float3(f * inv_z, 0.0f, -f * t.x * inv_z2);
     const float3 dty = make_float3(0.0f, -f * inv_z, f * t.y * inv_z2);
-    const float3 dLdt = make_float3(dtx.x * dmx + dty.x * dmy, dtx.y * dmx + dty.y * dmy, dtx.z * dmx + dty.z * dmy);
+    float3 dLdt = make_float3(dtx.x * dmx + dty.x * dmy, dtx.y * dmx + dty.y * dmy, dtx.z * dmx + dty.z * dmy);
+    if (FULL) {  // dL/dJ = 2 dSigma2d J Sigma_cam, then dJ/dt
+      const float h01 = ds01 * 0.5f;
+      const float3 r0 = cv.r0, r1 = cv.r1;
+      const float3 dJ0 = make_float3(2.0f * (ds00 * r0.x + h01 * r1.x), 2.0f * (ds00 * r0.y + h01 * r1.y),
+                                     2.0f * (ds00 * r0.z + h01 * r1.z));
+      const float3 dJ1 = make_float3(2.0f * (h01 * r0.x + ds11 * r1.x), 2.0f * (h01 * r0.y + ds11 * r1.y),
+                                     2.0f * (h01 * r0.z + ds11 * r1.z));
+      const float fz2 = f * inv_z2;
+      const float fz3 = fz2 * inv_z;
+      const float jtx = dJ0.z * -fz2;
+      const float jty = dJ1.z * fz2;
+      const float jtz = dJ0.x * -fz2 + dJ0.z * (2.0f * t.x * fz3) + dJ1.y * fz2 + dJ1.z * (-2.0f * t.y * fz3);
+      dLdt.x = dLdt.x + jtx;
+      dLdt.y = dLdt.y + jty;
+      dLdt.z = dLdt.z + jtz;
+    }
     const float3 dLdcam = make_float3(dLdt.x, dLdt.y, -dLdt.z);
-    const float3 dp = m3_mulv(m3_t(cv.W), dLdcam);
+    float3 dp = m3_mulv(m3_t(cv.W), dLdcam);
+    if (FULL) {  // colour through the view direction (cam - p) / |cam - p|
+      float db[15][3];
+      sh_basis_grad15(dx, dy, dz, db);
+      float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 15; ++k) {
+        const float wk = dcol[0] * P[(P_REST + k) * (size_t)cap + i] + dcol[1] * P[(P_REST + 15 + k) * (size_t)cap + i] +
+                         dcol[2] * P[(P_REST + 30 + k) * (size_t)cap + i];
+        vx = vx + wk * db[k][0];
+        vy = vy + wk * db[k][1];
+        vz = vz + wk * db[k][2];
+      }
+      const float pr = dx * vx + dy * vy + dz * vz;
+      dp.x = dp.x - (vx - dx * pr) * inv_len;
+      dp.y = dp.y - (vy - dy * pr) * inv_len;
+      dp.z = dp.z - (vz - dz * pr) * inv_len;
+    }
     g[P_POS + 0] = dp.x;
     g[P_POS + 1] = dp.y;
     g[P_POS + 2] = dp.z;
@@ -206,14 +253,24 @@ extern "C" int nts_project(nts_ctx* ctx, const nts_scene* scene, const nts_camer
   return NTS_OK;
 }
 
-extern "C" int nts_project_backward(nts_ctx* ctx, nts_scene* scene, const nts_buffer* projected_grad,
-                                    const nts_camera* cam) {
+extern "C" int nts_project_backward_ex(nts_ctx* ctx, nts_scene* scene, const nts_buffer* projected_grad,
+                                       const nts_camera* cam, int mode) {
   NTS_CHECK_ARG(ctx && scene && cam && projected_grad);
+  NTS_CHECK_ARG(mode == NTS_BWD_REF_PARITY || mode == NTS_BWD_RECOMPUTE);
   NTS_CHECK_ARG(projected_grad->bytes >= (uint64_t)scene->n * sizeof(ProjectedGrad));
   if (scene->n == 0) return NTS_OK;
   DevCamera dc;
   memcpy(&dc, cam, sizeof dc);
-  NTS_LAUNCH(ctx, "project_backward", k_project_backward, dim3(nts_div_up(scene->n, 256)), dim3(256), 0,
-             scene->params, scene->cap, scene->n, dc, (const ProjectedGrad*)projected_grad->ptr, scene->grads);
+  if (mode == NTS_BWD_RECOMPUTE)
+    NTS_LAUNCH(ctx, "project_backward_full", k_project_backward<true>, dim3(nts_div_up(scene->n, 256)), dim3(256), 0,
+               scene->params, scene->cap, scene->n, dc, (const ProjectedGrad*)projected_grad->ptr, scene->grads);
+  else
+    NTS_LAUNCH(ctx, "project_backward", k_project_backward<false>, dim3(nts_div_up(scene->n, 256)), dim3(256), 0,
+               scene->params, scene->cap, scene->n, dc, (const ProjectedGrad*)projected_grad->ptr, scene->grads);
   return NTS_OK;
+}
+
+extern "C" int nts_project_backward(nts_ctx* ctx, nts_scene* scene, const nts_buffer* projected_grad,
+                                    const nts_camera* cam) {
+  return nts_project_backward_ex(ctx, scene, projected_grad, cam, NTS_BWD_REF_PARITY);
 }

@@ -3,18 +3,101 @@
 // One workgroup per (view, tile).  Splat records of the tile's depth-sorted
 // slice are staged through LDS in batches; every thread blends its own
 // pixel(s) front to back with exactly the arithmetic of the host
-// restatement (bit-identical images).  A block-wide vote ends the tile as
-// soon as every pixel is saturated.  The backward walks the slice in
-// reverse, sums each splat's per-pixel gradient across the wave with
-// shuffles and commits one set of float atomics per wave.
+// restatement (bit-identical images).  A wave skips a splat whose
+// alpha >= 1/255 region misses the wave's pixels, a pixel skips the
+// exponential below the splat's power threshold, and a block-wide vote ends
+// the tile once every pixel is saturated.
 #include "nts_raster.h"
 
 template <int TS>
 struct Blend {
   static constexpr int NPIX = TS * TS;
   static constexpr int NT = NPIX >= 256 ? 256 : NPIX;
-  static constexpr int PPT = NPIX / NT;
+  static constexpr int PPT = NPIX / NT;  // 1, or 4 (a 2x2 quad per thread at TS = 32)
 };
+
+// Tile-local pixel j of thread t: one pixel per thread in row-major order,
+// or a 2x2 quad per thread so that each wave covers a compact 32x8 strip.
+template <int TS>
+__device__ __forceinline__ void tile_pixel(uint32_t t, int j, uint32_t& lx, uint32_t& ly) {
+  if (Blend<TS>::PPT == 1) {
+    lx = t % TS;
+    ly = t / TS;
+  } else {
+    const uint32_t qx = t % (TS / 2), qy = t / (TS / 2);
+    lx = 2 * qx + (j & 1);
+    ly = 2 * qy + (j >> 1);
+  }
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Pixel-centre rectangle of the wave (centre and half extents).
+struct WaveRect {
+  float cx, cy, hw, hh;
+};
+
+template <int TS>
+__device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
+  float x0 = 3.4e38f, x1 = -3.4e38f, y0 = 3.4e38f, y1 = -3.4e38f;
+#pragma unroll
+  for (int j = 0; j < Blend<TS>::PPT; ++j) {
+    uint32_t lx, ly;
+    tile_pixel<TS>(threadIdx.x, j, lx, ly);
+    const float fx = (float)(tx * TS + lx) + 0.5f, fy = (float)(ty * TS + ly) + 0.5f;
+    x0 = fminf(x0, fx);
+    x1 = fmaxf(x1, fx);
+    y0 = fminf(y0, fy);
+    y1 = fmaxf(y1, fy);
+  }
+  x0 = wave_min(x0);
+  x1 = wave_max(x1);
+  y0 = wave_min(y0);
+  y1 = wave_max(y1);
+  return WaveRect{0.5f * (x0 + x1), 0.5f * (y0 + y1), 0.5f * (x1 - x0), 0.5f * (y1 - y0)};
+}
+
+// Staged splat: the fields the blend reads plus a conservative bound of the
+// region where alpha >= 1/255:  -0.5 x^T Q x >= -ln(255 o)  is the ellipse
+// x^T Q x <= 2 ln(255 o), whose half extents are sqrt(2 ln(255 o) Sigma_xx)
+// and sqrt(2 ln(255 o) Sigma_yy) with Sigma = Q^-1.  Extents get a 1% + 0.05 px
+// margin and the power threshold a 1e-3 margin, so a culled evaluation is
+// one that would have failed the alpha >= 1/255 test: images and gradients
+// are unchanged.
+__device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c) {
+  a = make_float4(p.a.x, p.a.y, p.b.x, p.b.y);
+  b = make_float4(p.b.z, p.c.w, p.c.x, p.c.y);
+  const float o = p.c.w, qa = p.b.x, qb = p.b.y, qc = p.b.z;
+  const float det = qa * qc - qb * qb;
+  float ex, ey, pmin;
+  if (o * 255.0f < 0.999f) {  // alpha = o G < 1/255 everywhere
+    ex = ey = -3.4e38f;
+    pmin = 3.4e38f;
+  } else if (!(det > 0.0f)) {  // degenerate conic: no culling
+    ex = ey = 3.4e38f;
+    pmin = -3.4e38f;
+  } else {
+    const float l = logf(o * 255.0f);
+    const float r2 = fmaxf(2.0f * l, 0.0f);
+    ex = sqrtf(r2 * (qc / det)) * 1.01f + 0.05f;
+    ey = sqrtf(r2 * (qa / det)) * 1.01f + 0.05f;
+    pmin = -l - 1e-3f;
+  }
+  c = make_float4(p.c.z, ex, ey, pmin);
+}
+
+__device__ __forceinline__ bool wave_misses(const WaveRect& r, float4 a, float4 c) {
+  return fabsf(r.cx - a.x) > r.hw + c.y || fabsf(r.cy - a.y) > r.hh + c.z;
+}
 
 // ---------------------------------------------------------------------------
 // forward (rasterize.wgsl:48-118)
@@ -28,7 +111,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
   constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT;
   __shared__ float4 s_a[BATCH];  // mean.x, mean.y, conic.a, conic.b
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
-  __shared__ float s_c[BATCH];   // b
+  __shared__ float4 s_c[BATCH];  // b, cull half-extents x / y, power threshold
   const uint32_t gt = blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
   const ViewDesc vd = vt.v[vi];
@@ -36,15 +119,17 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
   const uint32_t tx = lt % vd.tiles_x, ty = lt / vd.tiles_x;
   const uint2 rg = ranges[gt];
   const uint32_t begin = rg.x, end = rg.y;
+  const WaveRect wr = wave_rect<TS>(tx, ty);
 
   float pxf[PPT], pyf[PPT], T[PPT], C0[PPT], C1[PPT], C2[PPT];
   uint32_t last[PPT], pix_x[PPT], pix_y[PPT];
   bool done[PPT], inb[PPT];
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
-    const uint32_t li = threadIdx.x + j * NT;
-    pix_x[j] = tx * TS + li % TS;
-    pix_y[j] = ty * TS + li / TS;
+    uint32_t lx, ly;
+    tile_pixel<TS>(threadIdx.x, j, lx, ly);
+    pix_x[j] = tx * TS + lx;
+    pix_y[j] = ty * TS + ly;
     inb[j] = pix_x[j] < vd.width && pix_y[j] < vd.height;
     pxf[j] = (float)pix_x[j] + 0.5f;
     pyf[j] = (float)pix_y[j] + 0.5f;
@@ -59,20 +144,18 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
     for (int j = 0; j < PPT; ++j) live |= !done[j];
     if (!__syncthreads_or(live)) break;
     const uint32_t nb = (end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH;
-    for (uint32_t k = threadIdx.x; k < nb; k += NT) {
-      const Projected p = proj[vd.proj_base + payloads[base + k]];
-      s_a[k] = make_float4(p.a.x, p.a.y, p.b.x, p.b.y);
-      s_b[k] = make_float4(p.b.z, p.c.w, p.c.x, p.c.y);
-      s_c[k] = p.c.z;
-    }
+    for (uint32_t k = threadIdx.x; k < nb; k += NT)
+      stage_splat(proj[vd.proj_base + payloads[base + k]], s_a[k], s_b[k], s_c[k]);
     __syncthreads();
     if (PPT == 1) {
       if (!done[0]) {
         for (uint32_t k = 0; k < nb; ++k) {
-          const float4 A = s_a[k], B = s_b[k];
+          const float4 A = s_a[k], Cc = s_c[k];
+          if (wave_misses(wr, A, Cc)) continue;
+          const float4 B = s_b[k];
           const float dx = pxf[0] - A.x, dy = pyf[0] - A.y;
           const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-          if (power > 0.0f) continue;
+          if (power > 0.0f || power < Cc.w) continue;
           const float g = nts_expf_nonpos(power);
           const float ar = B.y * g;
           const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
@@ -80,7 +163,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
           const float ta = T[0] * alpha;
           C0[0] = C0[0] + ta * B.z;
           C1[0] = C1[0] + ta * B.w;
-          C2[0] = C2[0] + ta * s_c[k];
+          C2[0] = C2[0] + ta * Cc.x;
           T[0] = T[0] * (1.0f - alpha);
           last[0] = base + k - begin + 1;
           if (T[0] < NTS_T_MIN) {
@@ -91,14 +174,15 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
       }
     } else {
       for (uint32_t k = 0; k < nb; ++k) {
-        const float4 A = s_a[k], B = s_b[k];
-        const float cb = s_c[k];
+        const float4 A = s_a[k], Cc = s_c[k];
+        if (wave_misses(wr, A, Cc)) continue;
+        const float4 B = s_b[k];
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
           if (done[j]) continue;
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
           const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-          if (power > 0.0f) continue;
+          if (power > 0.0f || power < Cc.w) continue;
           const float g = nts_expf_nonpos(power);
           const float ar = B.y * g;
           const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
@@ -106,7 +190,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
           const float ta = T[j] * alpha;
           C0[j] = C0[j] + ta * B.z;
           C1[j] = C1[j] + ta * B.w;
-          C2[j] = C2[j] + ta * cb;
+          C2[j] = C2[j] + ta * Cc.x;
           T[j] = T[j] * (1.0f - alpha);
           last[j] = base + k - begin + 1;
           if (T[j] < NTS_T_MIN) done[j] = true;
@@ -125,24 +209,75 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
 }
 
 // ---------------------------------------------------------------------------
-// backward, reference-parity mode (rasterize_backward.wgsl:73-210)
+// backward
+//   reference-parity mode (rasterize_backward.wgsl:73-210): T = 1 - out.w,
+//     walk from the slice end, T /= max(1 - a, 1e-4), bail once T > 1.0001;
+//   recompute mode: T = saved final T, walk from the pixel's last
+//     contributing splat, no bail (the exact chain rule of the forward).
+// Each splat's nine per-pixel terms are reduce-scattered across the wave,
+// the waves' partial sums meet in LDS, and one set of float atomics per
+// (tile, splat) commits them.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float wave_sum(float v) {
+
+// Reduce-scatter of the 9 per-lane gradient terms across a wave: after 13
+// shuffles the lane with lane & 3 == 0 and (lane >> 2) = q holds the wave sum
+// of term grad_term_of_quad(q) (or of nothing when that is >= 9).
+__device__ __forceinline__ float wave_reduce9(const float (&v)[9]) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool b5 = lane & 32u, b4 = lane & 16u, b3 = lane & 8u, b2 = lane & 4u;
+  float w[5];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  for (int j = 0; j < 5; ++j) {  // halves keep terms 0..4 / 5..8
+    const float hi = (j + 5 < 9) ? v[j + 5] : 0.0f;
+    const float keep = b5 ? hi : v[j];
+    const float send = b5 ? v[j] : hi;
+    w[j] = keep + __shfl_xor(send, 32, 64);
+  }
+  float x[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float hi = (j + 3 < 5) ? w[j + 3] : 0.0f;
+    const float keep = b4 ? hi : w[j];
+    const float send = b4 ? w[j] : hi;
+    x[j] = keep + __shfl_xor(send, 16, 64);
+  }
+  float y[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const float hi = (j + 2 < 3) ? x[j + 2] : 0.0f;
+    const float keep = b3 ? hi : x[j];
+    const float send = b3 ? x[j] : hi;
+    y[j] = keep + __shfl_xor(send, 8, 64);
+  }
+  float z = (b2 ? y[1] : y[0]) + __shfl_xor(b2 ? y[0] : y[1], 4, 64);
+  z += __shfl_xor(z, 2, 64);
+  z += __shfl_xor(z, 1, 64);
+  return z;
 }
 
-template <int TS, int BATCH>
-__global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd_ref(
+// Term index held by quad q = lane >> 2 after wave_reduce9 (9 = none).
+__device__ __forceinline__ uint32_t grad_term_of_quad(uint32_t q) {
+  const uint32_t b5 = (q >> 3) & 1u, b4 = (q >> 2) & 1u, b3 = (q >> 1) & 1u, b2 = q & 1u;
+  const uint32_t xs = (b3 ? 2u : 0u) + b2;
+  if (xs >= 3u) return 9u;
+  const uint32_t ws = (b4 ? 3u : 0u) + xs;
+  if (ws >= 5u) return 9u;
+  const uint32_t vi = (b5 ? 5u : 0u) + ws;
+  return vi < 9u ? vi : 9u;
+}
+
+template <int TS, int BATCH, bool RECOMP>
+__global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
-    ViewTable vt, const float4* __restrict__ fwd_out, const float4* __restrict__ dl_dc,
-    ProjectedGrad* __restrict__ pgrad) {
-  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT;
+    ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
+    const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
+  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT, NW = NT / 64;
   __shared__ float4 s_a[BATCH];
   __shared__ float4 s_b[BATCH];
-  __shared__ float s_c[BATCH];
+  __shared__ float4 s_c[BATCH];
   __shared__ uint32_t s_idx[BATCH];
+  __shared__ float s_part[NW][BATCH][9];  // per-wave sums of each splat's terms
+  __shared__ uint32_t s_max;
   const uint32_t gt = blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
   const ViewDesc vd = vt.v[vi];
@@ -150,72 +285,92 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd_ref(
   const uint32_t tx = lt % vd.tiles_x, ty = lt / vd.tiles_x;
   const uint2 rg = ranges[gt];
   const uint32_t begin = rg.x, end = rg.y;
-  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t my_term = grad_term_of_quad(lane >> 2);
+  const bool term_lane = (lane & 3u) == 0u && my_term < 9u;
+  const WaveRect wr = wave_rect<TS>(tx, ty);
 
   float pxf[PPT], pyf[PPT], T[PPT], G0[PPT], G1[PPT], G2[PPT], S0[PPT], S1[PPT], S2[PPT];
+  uint32_t lastj[PPT];
   bool done[PPT];
+  uint32_t my_max = 0;
+  if (threadIdx.x == 0) s_max = 0;
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
-    const uint32_t li = threadIdx.x + j * NT;
-    const uint32_t px = tx * TS + li % TS, py = ty * TS + li / TS;
+    uint32_t lx, ly;
+    tile_pixel<TS>(threadIdx.x, j, lx, ly);
+    const uint32_t px = tx * TS + lx, py = ty * TS + ly;
     const bool inb = px < vd.width && py < vd.height;
     pxf[j] = (float)px + 0.5f;
     pyf[j] = (float)py + 0.5f;
     T[j] = 1.0f;
     G0[j] = G1[j] = G2[j] = 0.0f;
+    lastj[j] = 0;
     if (inb) {
       const uint64_t pix = vd.pix_base + (uint64_t)py * vd.width + px;
-      T[j] = 1.0f - fwd_out[pix].w;
+      if (RECOMP) {
+        T[j] = final_t[pix];
+        lastj[j] = last_idx[pix];
+      } else {
+        T[j] = 1.0f - fwd_out[pix].w;
+        lastj[j] = end > begin ? end - begin : 0;
+      }
       const float4 g = dl_dc[pix];
       G0[j] = g.x;
       G1[j] = g.y;
       G2[j] = g.z;
     }
     S0[j] = S1[j] = S2[j] = 0.0f;
-    done[j] = !inb || end <= begin;
+    done[j] = lastj[j] == 0;
+    my_max = lastj[j] > my_max ? lastj[j] : my_max;
   }
-  uint32_t rem = end > begin ? end - begin : 0;
+  __syncthreads();
+  if (my_max) atomicMax(&s_max, my_max);
+  __syncthreads();
+  uint32_t rem = s_max;  // entries [begin, begin + rem) can matter to this tile
   while (rem > 0) {
     int live = 0;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) live |= !done[j];
     if (!__syncthreads_or(live)) break;
     const uint32_t take = rem < (uint32_t)BATCH ? rem : (uint32_t)BATCH;
-    const uint32_t cs = begin + rem - take;
+    const uint32_t rel0 = rem - take;  // slice-relative index of LDS slot 0
     for (uint32_t k = threadIdx.x; k < take; k += NT) {
-      const uint32_t sidx = payloads[cs + k];
-      const Projected p = proj[vd.proj_base + sidx];
-      s_a[k] = make_float4(p.a.x, p.a.y, p.b.x, p.b.y);
-      s_b[k] = make_float4(p.b.z, p.c.w, p.c.x, p.c.y);
-      s_c[k] = p.c.z;
+      const uint32_t sidx = payloads[begin + rel0 + k];
+      stage_splat(proj[vd.proj_base + sidx], s_a[k], s_b[k], s_c[k]);
       s_idx[k] = sidx;
     }
     __syncthreads();
     for (int k = (int)take - 1; k >= 0; --k) {
-      const float4 A = s_a[k], B = s_b[k];
-      const float cb = s_c[k];
-      float gmx = 0.f, gmy = 0.f, gsig = 0.f, ga_ = 0.f, gb_ = 0.f, gc_ = 0.f, gr = 0.f, gg = 0.f, gbl = 0.f;
+      const float4 A = s_a[k], Cc = s_c[k];
+      if (wave_misses(wr, A, Cc)) {
+        if (term_lane) s_part[wave][k][my_term] = 0.0f;
+        continue;
+      }
+      const float4 B = s_b[k];
+      float acc[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
       bool contrib = false;
 #pragma unroll
       for (int j = 0; j < PPT; ++j) {
-        if (done[j]) continue;
+        if (done[j] || rel0 + (uint32_t)k >= lastj[j]) continue;
         const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
         const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-        if (power > 0.0f) continue;
+        if (power > 0.0f || power < Cc.w) continue;
         const float g = nts_expf_nonpos(power);
         const float sigma = B.y;
         const float ar = sigma * g;
         const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
         if (alpha < NTS_ALPHA_MIN) continue;
-        const float oma = nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
+        const float oma = RECOMP ? 1.0f - alpha : nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
         T[j] = T[j] / oma;
-        if (T[j] > NTS_T_BAIL) {
+        if (!RECOMP && T[j] > NTS_T_BAIL) {
           done[j] = true;
           continue;
         }
         const float ta = T[j] * alpha;
-        const float c0 = B.z, c1 = B.w, c2 = cb;
-        const float ct0 = ta * c0, ct1 = ta * c1, ct2 = ta * c2;
+        const float c0 = B.z, c1 = B.w, c2 = Cc.x;
         const float d0 = T[j] * c0 - S0[j] / oma;
         const float d1 = T[j] * c1 - S1[j] / oma;
         const float d2 = T[j] * c2 - S2[j] / oma;
@@ -223,42 +378,37 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd_ref(
         const float sat = (ar < NTS_ALPHA_MAX) ? 1.0f : 0.0f;
         const float gar = gal * sat;
         const float gpow = gar * sigma * g;
-        gr += G0[j] * ta;
-        gg += G1[j] * ta;
-        gbl += G2[j] * ta;
-        gsig += gar * g;
-        ga_ += gpow * -0.5f * (dx * dx);
-        gb_ += gpow * -0.5f * (2.0f * dx * dy);
-        gc_ += gpow * -0.5f * (dy * dy);
-        gmx += gpow * (A.z * dx + A.w * dy);
-        gmy += gpow * (A.w * dx + B.x * dy);
-        S0[j] = S0[j] + ct0;
-        S1[j] = S1[j] + ct1;
-        S2[j] = S2[j] + ct2;
+        acc[0] += gpow * (A.z * dx + A.w * dy);
+        acc[1] += gpow * (A.w * dx + B.x * dy);
+        acc[2] += gar * g;
+        acc[3] += gpow * -0.5f * (dx * dx);
+        acc[4] += gpow * -0.5f * (2.0f * dx * dy);
+        acc[5] += gpow * -0.5f * (dy * dy);
+        acc[6] += G0[j] * ta;
+        acc[7] += G1[j] * ta;
+        acc[8] += G2[j] * ta;
+        S0[j] = S0[j] + ta * c0;
+        S1[j] = S1[j] + ta * c1;
+        S2[j] = S2[j] + ta * c2;
         contrib = true;
       }
       if (__ballot(contrib)) {
-        gmx = wave_sum(gmx);
-        gmy = wave_sum(gmy);
-        gsig = wave_sum(gsig);
-        ga_ = wave_sum(ga_);
-        gb_ = wave_sum(gb_);
-        gc_ = wave_sum(gc_);
-        gr = wave_sum(gr);
-        gg = wave_sum(gg);
-        gbl = wave_sum(gbl);
-        if (lane == 0) {
-          float* o = reinterpret_cast<float*>(pgrad + vd.proj_base + s_idx[k]);
-          unsafeAtomicAdd(o + 0, gmx);
-          unsafeAtomicAdd(o + 1, gmy);
-          unsafeAtomicAdd(o + 2, gsig);
-          unsafeAtomicAdd(o + 4, ga_);
-          unsafeAtomicAdd(o + 5, gb_);
-          unsafeAtomicAdd(o + 6, gc_);
-          unsafeAtomicAdd(o + 8, gr);
-          unsafeAtomicAdd(o + 9, gg);
-          unsafeAtomicAdd(o + 10, gbl);
-        }
+        const float r = wave_reduce9(acc);
+        if (term_lane) s_part[wave][k][my_term] = r;
+      } else if (term_lane) {
+        s_part[wave][k][my_term] = 0.0f;
+      }
+    }
+    __syncthreads();
+    // one set of float atomics per (tile, splat): sum the waves' partials
+    for (uint32_t k = threadIdx.x; k < take; k += NT) {
+      float* o = reinterpret_cast<float*>(pgrad + vd.proj_base + s_idx[k]);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sum += s_part[w][k][q];
+        if (sum != 0.0f) unsafeAtomicAdd(o + (q < 3 ? q : (q < 6 ? q + 1 : q + 2)), sum);
       }
     }
     rem -= take;
@@ -310,54 +460,78 @@ int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, const Projected* proj, c
   return NTS_OK;
 }
 
-int nts_blend_backward_ref(nts_ctx* ctx, uint32_t tile_size, const Projected* proj, const uint32_t* payloads,
-                           const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float4* dl_dc,
-                           ProjectedGrad* pgrad) {
+int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, int mode, const Projected* proj, const uint32_t* payloads,
+                       const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
+                       const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
   if (vt.total_tiles == 0) return NTS_OK;
   const dim3 grid(vt.total_tiles);
+#define NTS_BWD_CASE(TSZ)                                                                                     \
+  case TSZ:                                                                                                   \
+    if (mode == NTS_BWD_RECOMPUTE)                                                                            \
+      NTS_LAUNCH(ctx, "blend_bwd_recompute", (k_blend_bwd<TSZ, NTS_BATCH, true>), grid, dim3(Blend<TSZ>::NT), \
+                 0, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);                    \
+    else                                                                                                      \
+      NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd<TSZ, NTS_BATCH, false>), grid, dim3(Blend<TSZ>::NT), 0, proj, \
+                 payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);                             \
+    break;
   switch (tile_size) {
-    case 8:
-      NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd_ref<8, NTS_BATCH>), grid, dim3(Blend<8>::NT), 0, proj, payloads,
-                 ranges, vt, fwd_out, dl_dc, pgrad);
-      break;
-    case 16:
-      NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd_ref<16, NTS_BATCH>), grid, dim3(Blend<16>::NT), 0, proj, payloads,
-                 ranges, vt, fwd_out, dl_dc, pgrad);
-      break;
-    case 32:
-      NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd_ref<32, NTS_BATCH>), grid, dim3(Blend<32>::NT), 0, proj, payloads,
-                 ranges, vt, fwd_out, dl_dc, pgrad);
-      break;
+    NTS_BWD_CASE(8)
+    NTS_BWD_CASE(16)
+    NTS_BWD_CASE(32)
     default:
       return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
   }
+#undef NTS_BWD_CASE
   return NTS_OK;
 }
 
-extern "C" int nts_composite(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,
-                             const nts_tiling* tiling) {
+extern "C" int nts_composite_ex(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,
+                                const nts_tiling* tiling, nts_buffer* final_t, nts_buffer* last_idx) {
   NTS_CHECK_ARG(ctx && projected && bins && image && tiling);
-  NTS_CHECK_ARG(image->bytes >= (uint64_t)tiling->width * tiling->height * 16);
+  const uint64_t np = (uint64_t)tiling->width * tiling->height;
+  NTS_CHECK_ARG(image->bytes >= np * 16);
+  NTS_CHECK_ARG(!final_t || final_t->bytes >= np * 4);
+  NTS_CHECK_ARG(!last_idx || last_idx->bytes >= np * 4);
   ViewTable vt;
   int rc = fill_single_view(bins, tiling, vt);
   if (rc) return rc;
   return nts_blend_forward(ctx, tiling->tile_size, (const Projected*)projected->ptr, bins->payloads, bins->ranges, vt,
-                           (float4*)image->ptr, nullptr, nullptr);
+                           (float4*)image->ptr, final_t ? (float*)final_t->ptr : nullptr,
+                           last_idx ? (uint32_t*)last_idx->ptr : nullptr);
+}
+
+extern "C" int nts_composite(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,
+                             const nts_tiling* tiling) {
+  return nts_composite_ex(ctx, projected, bins, image, tiling, nullptr, nullptr);
+}
+
+extern "C" int nts_composite_backward_ex(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins,
+                                         const nts_buffer* forward_out, const nts_buffer* final_t,
+                                         const nts_buffer* last_idx, const nts_buffer* dl_dc,
+                                         nts_buffer* projected_grad, const nts_tiling* tiling, int mode) {
+  NTS_CHECK_ARG(ctx && projected && bins && dl_dc && projected_grad && tiling);
+  NTS_CHECK_ARG(mode == NTS_BWD_REF_PARITY || mode == NTS_BWD_RECOMPUTE);
+  const uint64_t np = (uint64_t)tiling->width * tiling->height;
+  NTS_CHECK_ARG(dl_dc->bytes >= np * 16);
+  if (mode == NTS_BWD_REF_PARITY) NTS_CHECK_ARG(forward_out && forward_out->bytes >= np * 16);
+  if (mode == NTS_BWD_RECOMPUTE)
+    NTS_CHECK_ARG(final_t && last_idx && final_t->bytes >= np * 4 && last_idx->bytes >= np * 4);
+  NTS_CHECK_ARG(projected_grad->bytes >= projected->bytes);
+  ViewTable vt;
+  int rc = fill_single_view(bins, tiling, vt);
+  if (rc) return rc;
+  return nts_blend_backward(ctx, tiling->tile_size, mode, (const Projected*)projected->ptr, bins->payloads,
+                            bins->ranges, vt, forward_out ? (const float4*)forward_out->ptr : nullptr,
+                            final_t ? (const float*)final_t->ptr : nullptr,
+                            last_idx ? (const uint32_t*)last_idx->ptr : nullptr, (const float4*)dl_dc->ptr,
+                            (ProjectedGrad*)projected_grad->ptr);
 }
 
 extern "C" int nts_composite_backward(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins,
                                       const nts_buffer* forward_out, const nts_buffer* dl_dc,
                                       nts_buffer* projected_grad, const nts_tiling* tiling) {
-  NTS_CHECK_ARG(ctx && projected && bins && forward_out && dl_dc && projected_grad && tiling);
-  const uint64_t np = (uint64_t)tiling->width * tiling->height;
-  NTS_CHECK_ARG(forward_out->bytes >= np * 16 && dl_dc->bytes >= np * 16);
-  NTS_CHECK_ARG(projected_grad->bytes * sizeof(Projected) >= projected->bytes * sizeof(ProjectedGrad));
-  ViewTable vt;
-  int rc = fill_single_view(bins, tiling, vt);
-  if (rc) return rc;
-  return nts_blend_backward_ref(ctx, tiling->tile_size, (const Projected*)projected->ptr, bins->payloads,
-                                bins->ranges, vt, (const float4*)forward_out->ptr, (const float4*)dl_dc->ptr,
-                                (ProjectedGrad*)projected_grad->ptr);
+  return nts_composite_backward_ex(ctx, projected, bins, forward_out, nullptr, nullptr, dl_dc, projected_grad, tiling,
+                                   NTS_BWD_REF_PARITY);
 }
 
 // ---------------------------------------------------------------------------

@@ -72,6 +72,30 @@ __device__ __forceinline__ void sh_basis15(float x, float y, float z, float b[15
   b[14] = NTS_SH_C3_6 * x * (xx - 3.0f * yy);
 }
 
+// d b_k / d (x, y, z) of the 15 basis functions of sh_basis15.
+__device__ __forceinline__ void sh_basis_grad15(float x, float y, float z, float g[15][3]) {
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float C1 = NTS_SH_C1;
+  g[0][0] = 0.0f; g[0][1] = -C1; g[0][2] = 0.0f;
+  g[1][0] = 0.0f; g[1][1] = 0.0f; g[1][2] = C1;
+  g[2][0] = -C1; g[2][1] = 0.0f; g[2][2] = 0.0f;
+  g[3][0] = NTS_SH_C2_0 * y; g[3][1] = NTS_SH_C2_0 * x; g[3][2] = 0.0f;
+  g[4][0] = 0.0f; g[4][1] = NTS_SH_C2_1 * z; g[4][2] = NTS_SH_C2_1 * y;
+  g[5][0] = -2.0f * NTS_SH_C2_2 * x; g[5][1] = -2.0f * NTS_SH_C2_2 * y; g[5][2] = 4.0f * NTS_SH_C2_2 * z;
+  g[6][0] = NTS_SH_C2_3 * z; g[6][1] = 0.0f; g[6][2] = NTS_SH_C2_3 * x;
+  g[7][0] = 2.0f * NTS_SH_C2_4 * x; g[7][1] = -2.0f * NTS_SH_C2_4 * y; g[7][2] = 0.0f;
+  g[8][0] = 6.0f * NTS_SH_C3_0 * x * y; g[8][1] = NTS_SH_C3_0 * (3.0f * xx - 3.0f * yy); g[8][2] = 0.0f;
+  g[9][0] = NTS_SH_C3_1 * y * z; g[9][1] = NTS_SH_C3_1 * x * z; g[9][2] = NTS_SH_C3_1 * x * y;
+  g[10][0] = -2.0f * NTS_SH_C3_2 * x * y; g[10][1] = NTS_SH_C3_2 * (4.0f * zz - xx - 3.0f * yy);
+  g[10][2] = 8.0f * NTS_SH_C3_2 * y * z;
+  g[11][0] = -6.0f * NTS_SH_C3_3 * x * z; g[11][1] = -6.0f * NTS_SH_C3_3 * y * z;
+  g[11][2] = NTS_SH_C3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+  g[12][0] = NTS_SH_C3_4 * (4.0f * zz - 3.0f * xx - yy); g[12][1] = -2.0f * NTS_SH_C3_4 * x * y;
+  g[12][2] = 8.0f * NTS_SH_C3_4 * x * z;
+  g[13][0] = 2.0f * NTS_SH_C3_5 * x * z; g[13][1] = -2.0f * NTS_SH_C3_5 * y * z; g[13][2] = NTS_SH_C3_5 * (xx - yy);
+  g[14][0] = NTS_SH_C3_6 * (3.0f * xx - 3.0f * yy); g[14][1] = -6.0f * NTS_SH_C3_6 * x * y; g[14][2] = 0.0f;
+}
+
 __device__ __forceinline__ M3f cam_rot(const DevCamera& cam) {
   M3f w;
 #pragma unroll
@@ -96,6 +120,7 @@ __device__ __forceinline__ float3 to_cam(const DevCamera& cam, float px, float p
 struct Cov2D {
   float s00, s01, s11, det;
   float3 j0, j1;
+  float3 r0, r1;  // rows of J * Sigma_cam
   M3f R, W;
   float s2[3];
 };
@@ -120,7 +145,8 @@ __device__ __forceinline__ bool screen_cov(const DevCamera& cam, float3 t, float
   const float inv_z2 = inv_z * inv_z;
   o.j0 = make_float3(f * inv_z, 0.0f, -f * t.x * inv_z2);
   o.j1 = make_float3(0.0f, -f * inv_z, f * t.y * inv_z2);
-  float3 r0, r1;
+  float3& r0 = o.r0;
+  float3& r1 = o.r1;
   r0.x = cov_c.c[0][0] * o.j0.x + cov_c.c[1][0] * o.j0.y + cov_c.c[2][0] * o.j0.z;
   r0.y = cov_c.c[0][1] * o.j0.x + cov_c.c[1][1] * o.j0.y + cov_c.c[2][1] * o.j0.z;
   r0.z = cov_c.c[0][2] * o.j0.x + cov_c.c[1][2] * o.j0.y + cov_c.c[2][2] * o.j0.z;

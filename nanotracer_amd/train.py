@@ -40,6 +40,7 @@ class Trainer:
     depth_max: float
     tile_size: int = 16
     loss_kind: int = L.LOSS_L1
+    backward_mode: int = L.BWD_REF_PARITY
     adam_pos: AdamConfig = field(default_factory=AdamConfig)
     adam_attr: AdamConfig = field(default_factory=AdamConfig)
     t: int = 0
@@ -70,7 +71,12 @@ class Trainer:
         params = TilingParams(v.width, v.height, self.tile_size, self.depth_max)
         bins = self.binner.bin(self.ctx, proj, n, params)
         img = self._buf(("img", v.width, v.height), v.width * v.height * 16)
-        self.r.composite(self.ctx, proj, bins, img, params)
+        if self.backward_mode == L.BWD_RECOMPUTE:
+            ft = self._buf(("ft", v.width, v.height), v.width * v.height * 4)
+            li = self._buf(("li", v.width, v.height), v.width * v.height * 4)
+            self.r.composite_ex(self.ctx, proj, bins, img, params, ft, li)
+        else:
+            self.r.composite(self.ctx, proj, bins, img, params)
         return proj, bins, img, cam, params
 
     def step(self, iteration: int) -> StepResult:
@@ -86,8 +92,12 @@ class Trainer:
         loss = loss_grad(self.ctx, img, tgt, dl, v.width, v.height, self.loss_kind)
         pg = self._buf("pgrad", max(n, 1) * 48)
         pg.zero()
-        self.r.composite_backward(self.ctx, proj, bins, img, dl, pg, params)
-        self.r.project_backward(self.ctx, self.splats, pg, cam)
+        if self.backward_mode == L.BWD_RECOMPUTE:
+            self.r.composite_backward_recompute(self.ctx, proj, bins, self._bufs[("ft", v.width, v.height)],
+                                                self._bufs[("li", v.width, v.height)], dl, pg, params)
+        else:
+            self.r.composite_backward(self.ctx, proj, bins, img, dl, pg, params)
+        self.r.project_backward(self.ctx, self.splats, pg, cam, self.backward_mode)
         self.t += 1
         self.adam.step_all_fused(self.ctx, self.splats, self.t, self.adam_pos, self.adam_attr)
         total = bins.total_pairs

@@ -102,6 +102,30 @@ template <typename T> void sh_basis(T x, T y, T z, T b[15]) {
   b[14] = T(NTS_SH_C3_6) * x * (xx - T(3) * yy);
 }
 
+// d b_k / d (x, y, z) of the 15 basis functions above.
+static void sh_basis_grad(float x, float y, float z, float g[15][3]) {
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float C1 = NTS_SH_C1;
+  g[0][0] = 0.0f; g[0][1] = -C1; g[0][2] = 0.0f;
+  g[1][0] = 0.0f; g[1][1] = 0.0f; g[1][2] = C1;
+  g[2][0] = -C1; g[2][1] = 0.0f; g[2][2] = 0.0f;
+  g[3][0] = NTS_SH_C2_0 * y; g[3][1] = NTS_SH_C2_0 * x; g[3][2] = 0.0f;
+  g[4][0] = 0.0f; g[4][1] = NTS_SH_C2_1 * z; g[4][2] = NTS_SH_C2_1 * y;
+  g[5][0] = -2.0f * NTS_SH_C2_2 * x; g[5][1] = -2.0f * NTS_SH_C2_2 * y; g[5][2] = 4.0f * NTS_SH_C2_2 * z;
+  g[6][0] = NTS_SH_C2_3 * z; g[6][1] = 0.0f; g[6][2] = NTS_SH_C2_3 * x;
+  g[7][0] = 2.0f * NTS_SH_C2_4 * x; g[7][1] = -2.0f * NTS_SH_C2_4 * y; g[7][2] = 0.0f;
+  g[8][0] = 6.0f * NTS_SH_C3_0 * x * y; g[8][1] = NTS_SH_C3_0 * (3.0f * xx - 3.0f * yy); g[8][2] = 0.0f;
+  g[9][0] = NTS_SH_C3_1 * y * z; g[9][1] = NTS_SH_C3_1 * x * z; g[9][2] = NTS_SH_C3_1 * x * y;
+  g[10][0] = -2.0f * NTS_SH_C3_2 * x * y; g[10][1] = NTS_SH_C3_2 * (4.0f * zz - xx - 3.0f * yy);
+  g[10][2] = 8.0f * NTS_SH_C3_2 * y * z;
+  g[11][0] = -6.0f * NTS_SH_C3_3 * x * z; g[11][1] = -6.0f * NTS_SH_C3_3 * y * z;
+  g[11][2] = NTS_SH_C3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+  g[12][0] = NTS_SH_C3_4 * (4.0f * zz - 3.0f * xx - yy); g[12][1] = -2.0f * NTS_SH_C3_4 * x * y;
+  g[12][2] = 8.0f * NTS_SH_C3_4 * x * z;
+  g[13][0] = 2.0f * NTS_SH_C3_5 * x * z; g[13][1] = -2.0f * NTS_SH_C3_5 * y * z; g[13][2] = NTS_SH_C3_5 * (xx - yy);
+  g[14][0] = NTS_SH_C3_6 * (3.0f * xx - 3.0f * yy); g[14][1] = -6.0f * NTS_SH_C3_6 * x * y; g[14][2] = 0.0f;
+}
+
 template <typename T> struct Splats {
   const T *pos, *rot, *log_scale, *logit, *sh_dc, *sh_rest;
 };
@@ -227,7 +251,7 @@ bool tile_bbox(const T* p, const orc_tiling* t, int32_t* x0, int32_t* y0, int32_
 
 template <typename T>
 void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* ranges,
-                 const orc_tiling* t, T* out, float* final_t, uint32_t* last_idx) {
+                 const orc_tiling* t, T* out, float* final_t, uint32_t* last_idx, uint32_t* n_used = nullptr) {
   const uint32_t ts = t->tile_size;
   const uint32_t tiles_x = (t->width + ts - 1) / ts;
   const uint32_t tiles_y = (t->height + ts - 1) / ts;
@@ -243,7 +267,7 @@ void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* r
         const T pxf = T((float)px + 0.5f), pyf = T((float)py + 0.5f);
         T C[3] = {T(0), T(0), T(0)};
         T Tr = T(1);
-        uint32_t last = 0;
+        uint32_t last = 0, used = 0;
         for (uint32_t i = begin; i < end; ++i) {
           const T* p = projected + 12 * (size_t)payloads[i];
           const T dx = pxf - p[0], dy = pyf - p[1];
@@ -259,6 +283,7 @@ void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* r
           C[2] = C[2] + ta * p[10];
           Tr = Tr * (T(1) - alpha);
           last = i - begin + 1;
+          ++used;
           if (Tr < T(NTS_T_MIN)) break;
         }
         const size_t pix = (size_t)py * t->width + px;
@@ -268,6 +293,7 @@ void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* r
         out[4 * pix + 3] = T(1) - Tr;
         if (final_t) final_t[pix] = (float)Tr;
         if (last_idx) last_idx[pix] = last;
+        if (n_used) n_used[pix] = used;
       }
   }
 }
@@ -458,8 +484,8 @@ void orc_composite(const float* projected, const uint32_t* payloads, const uint3
 }
 
 void orc_composite_f64(const double* projected, const uint32_t* payloads, const uint32_t* ranges,
-                       const orc_tiling* t, double* out) {
-  composite_t<double>(projected, payloads, ranges, t, out, nullptr, nullptr);
+                       const orc_tiling* t, double* out, uint32_t* last_idx, uint32_t* n_used) {
+  composite_t<double>(projected, payloads, ranges, t, out, nullptr, last_idx, n_used);
 }
 
 // rasterize_backward.wgsl:73-210.  Per-pixel terms are computed exactly as
@@ -552,9 +578,100 @@ void orc_composite_backward_ref(const float* projected, const uint32_t* payloads
   }
 }
 
-// project_backward.wgsl:79-304.
-void orc_project_backward(const orc_splats_f32* sp, uint32_t n, const orc_camera* cam,
-                          const float* projected_grad, orc_grads_f32* g) {
+// Recompute-mode backward (the Differentiation form): each pixel starts
+// at its last contributing splat with the final transmittance saved by the
+// forward, walks back to the slice start recovering T_i = T_{i+1}/(1 - a_i),
+// and has no 1.0001 bail.  Per-pixel terms otherwise as
+// rasterize_backward.wgsl:130-200; sums accumulated in float64.
+void orc_composite_backward_recompute(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
+                                      const orc_tiling* t, uint32_t n, const float* final_t,
+                                      const uint32_t* last_idx, const float* dl_dc, float* projected_grad) {
+  const uint32_t ts = t->tile_size;
+  const uint32_t tiles_x = (t->width + ts - 1) / ts;
+  const uint32_t tiles_y = (t->height + ts - 1) / ts;
+  const int64_t ntiles = (int64_t)tiles_x * tiles_y;
+  int nthreads = orc_num_threads();
+  std::vector<std::vector<double>> partial(nthreads);
+#pragma omp parallel
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+    std::vector<double>& mine = partial[tid];
+    mine.assign((size_t)n * 9, 0.0);
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t tile = 0; tile < ntiles; ++tile) {
+      const uint32_t tx = (uint32_t)(tile % tiles_x), ty = (uint32_t)(tile / tiles_x);
+      const uint32_t begin = ranges[2 * tile];
+      for (uint32_t ly = 0; ly < ts; ++ly)
+        for (uint32_t lx = 0; lx < ts; ++lx) {
+          const uint32_t px = tx * ts + lx, py = ty * ts + ly;
+          if (px >= t->width || py >= t->height) continue;
+          const size_t pix = (size_t)py * t->width + px;
+          const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+          float T = final_t[pix];
+          const float gC[3] = {dl_dc[4 * pix + 0], dl_dc[4 * pix + 1], dl_dc[4 * pix + 2]};
+          float S[3] = {0.0f, 0.0f, 0.0f};
+          for (int64_t i = (int64_t)begin + (int64_t)last_idx[pix] - 1; i >= (int64_t)begin; --i) {
+            const uint32_t sidx = payloads[i];
+            const float* p = projected + 12 * (size_t)sidx;
+            const float dx = pxf - p[0], dy = pyf - p[1];
+            const float ca = p[4], cb = p[5], cc = p[6];
+            const float power = -0.5f * (ca * dx * dx + 2.0f * cb * dx * dy + cc * dy * dy);
+            if (power > 0.0f) continue;
+            const float g = nts_expf(power);
+            const float sigma = p[11];
+            const float ar = sigma * g;
+            const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
+            if (alpha < NTS_ALPHA_MIN) continue;
+            const float oma = 1.0f - alpha;
+            T = T / oma;
+            const float ta = T * alpha;
+            const float col[3] = {p[8], p[9], p[10]};
+            const float contrib[3] = {ta * col[0], ta * col[1], ta * col[2]};
+            const float dCa[3] = {T * col[0] - S[0] / oma, T * col[1] - S[1] / oma, T * col[2] - S[2] / oma};
+            const float ga = gC[0] * dCa[0] + gC[1] * dCa[1] + gC[2] * dCa[2];
+            const float sat = (ar < NTS_ALPHA_MAX) ? 1.0f : 0.0f;
+            const float gar = ga * sat;
+            const float gpow = gar * sigma * g;
+            double* a = &mine[(size_t)sidx * 9];
+            a[0] += gpow * (ca * dx + cb * dy);
+            a[1] += gpow * (cb * dx + cc * dy);
+            a[2] += gar * g;
+            a[3] += gpow * -0.5f * (dx * dx);
+            a[4] += gpow * -0.5f * (2.0f * dx * dy);
+            a[5] += gpow * -0.5f * (dy * dy);
+            a[6] += gC[0] * ta;
+            a[7] += gC[1] * ta;
+            a[8] += gC[2] * ta;
+            S[0] = S[0] + contrib[0];
+            S[1] = S[1] + contrib[1];
+            S[2] = S[2] + contrib[2];
+          }
+        }
+    }
+  }
+  std::vector<double> acc((size_t)n * 9, 0.0);
+  for (int th = 0; th < nthreads; ++th)
+    if (!partial[th].empty())
+      for (size_t j = 0; j < acc.size(); ++j) acc[j] += partial[th][j];
+  for (uint32_t i = 0; i < n; ++i) {
+    float* o = projected_grad + 12 * (size_t)i;
+    const double* a = &acc[(size_t)i * 9];
+    o[0] = (float)a[0]; o[1] = (float)a[1]; o[2] = (float)a[2]; o[3] = 0.0f;
+    o[4] = (float)a[3]; o[5] = (float)a[4]; o[6] = (float)a[5]; o[7] = 0.0f;
+    o[8] = (float)a[6]; o[9] = (float)a[7]; o[10] = (float)a[8]; o[11] = 0.0f;
+  }
+}
+
+// project_backward.wgsl:79-304.  With `full` set, the exact chain rule of
+// the forward: the colour clamp gate, the SH direction's dependence on the
+// position (d dir / d p) and the Jacobian's dependence on the camera-space
+// mean (d J / d t), which the reference drops (project_backward.wgsl:17-24).
+static void project_backward_impl(const orc_splats_f32* sp, uint32_t n, const orc_camera* cam,
+                                  const float* projected_grad, orc_grads_f32* g, int full_chain) {
 #pragma omp parallel for schedule(static)
   for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
     const uint32_t idx = (uint32_t)ii;
@@ -616,13 +733,24 @@ void orc_project_backward(const orc_splats_f32* sp, uint32_t n, const orc_camera
     V3<float> d{cam->cam_pos[0] - pw[0], cam->cam_pos[1] - pw[1], cam->cam_pos[2] - pw[2]};
     const float inv_len = 1.0f / __builtin_sqrtf(dot3(d, d));
     d.x = d.x * inv_len; d.y = d.y * inv_len; d.z = d.z * inv_len;
-    for (int ch = 0; ch < 3; ++ch) gd[ch] = dcol[ch] * NTS_SH_C0;
     float b[15];
     sh_basis<float>(d.x, d.y, d.z, b);
+    const float* dcp = sp->sh_dc + 3 * (size_t)idx;
+    const float* rest = sp->sh_rest + 45 * (size_t)idx;
+    float dcu[3] = {dcol[0], dcol[1], dcol[2]};
+    if (full_chain) {  // max(SH + 0.5, 0) passes no gradient where it clamps
+      for (int ch = 0; ch < 3; ++ch) {
+        float acc = 0.0f;
+        for (int k = 0; k < 15; ++k) acc = acc + rest[15 * ch + k] * b[k];
+        const float raw = (NTS_SH_C0 * dcp[ch] + acc) + 0.5f;
+        if (!(raw > 0.0f)) dcu[ch] = 0.0f;
+      }
+    }
+    for (int ch = 0; ch < 3; ++ch) gd[ch] = dcu[ch] * NTS_SH_C0;
     for (int k = 0; k < 15; ++k) {
-      grest[k] = dcol[0] * b[k];
-      grest[15 + k] = dcol[1] * b[k];
-      grest[30 + k] = dcol[2] * b[k];
+      grest[k] = dcu[0] * b[k];
+      grest[15 + k] = dcu[1] * b[k];
+      grest[30 + k] = dcu[2] * b[k];
     }
 
     // dSigma2d = -(cm * dconic_mat * cm), symmetric halving (203-217).
@@ -680,11 +808,53 @@ void orc_project_backward(const orc_splats_f32* sp, uint32_t n, const orc_camera
 
     const V3<float> dtx{f * inv_z, 0.0f, -f * t.x * inv_z2};
     const V3<float> dty{0.0f, -f * inv_z, f * t.y * inv_z2};
-    const V3<float> dLdt{dtx.x * dmx + dty.x * dmy, dtx.y * dmx + dty.y * dmy, dtx.z * dmx + dty.z * dmy};
+    V3<float> dLdt{dtx.x * dmx + dty.x * dmy, dtx.y * dmx + dty.y * dmy, dtx.z * dmx + dty.z * dmy};
+    if (full_chain) {
+      // dL/dJ = 2 dSigma2d J Sigma_cam, with (J Sigma_cam) rows r0, r1
+      const float h01 = ds01 * 0.5f;
+      const V3<float> dJ0{2.0f * (ds00 * r0.x + h01 * r1.x), 2.0f * (ds00 * r0.y + h01 * r1.y),
+                          2.0f * (ds00 * r0.z + h01 * r1.z)};
+      const V3<float> dJ1{2.0f * (h01 * r0.x + ds11 * r1.x), 2.0f * (h01 * r0.y + ds11 * r1.y),
+                          2.0f * (h01 * r0.z + ds11 * r1.z)};
+      const float fz2 = f * inv_z2;
+      const float fz3 = fz2 * inv_z;
+      const float jtx = dJ0.z * -fz2;
+      const float jty = dJ1.z * fz2;
+      const float jtz = dJ0.x * -fz2 + dJ0.z * (2.0f * t.x * fz3) + dJ1.y * fz2 + dJ1.z * (-2.0f * t.y * fz3);
+      dLdt.x = dLdt.x + jtx;
+      dLdt.y = dLdt.y + jty;
+      dLdt.z = dLdt.z + jtz;
+    }
     const V3<float> dLdcam{dLdt.x, dLdt.y, -dLdt.z};
-    const V3<float> dp = mulv(transpose(Wm), dLdcam);
+    V3<float> dp = mulv(transpose(Wm), dLdcam);
+    if (full_chain) {
+      // colour through the view direction: dir = (cam - p) / |cam - p|
+      float db[15][3];
+      sh_basis_grad(d.x, d.y, d.z, db);
+      V3<float> v{0.0f, 0.0f, 0.0f};
+      for (int k = 0; k < 15; ++k) {
+        const float wk = dcu[0] * rest[k] + dcu[1] * rest[15 + k] + dcu[2] * rest[30 + k];
+        v.x = v.x + wk * db[k][0];
+        v.y = v.y + wk * db[k][1];
+        v.z = v.z + wk * db[k][2];
+      }
+      const float pr = dot3(d, v);
+      dp.x = dp.x - (v.x - d.x * pr) * inv_len;
+      dp.y = dp.y - (v.y - d.y * pr) * inv_len;
+      dp.z = dp.z - (v.z - d.z * pr) * inv_len;
+    }
     gp[0] = dp.x; gp[1] = dp.y; gp[2] = dp.z;
   }
+}
+
+void orc_project_backward(const orc_splats_f32* sp, uint32_t n, const orc_camera* cam,
+                          const float* projected_grad, orc_grads_f32* g) {
+  project_backward_impl(sp, n, cam, projected_grad, g, 0);
+}
+
+void orc_project_backward_full(const orc_splats_f32* sp, uint32_t n, const orc_camera* cam,
+                               const float* projected_grad, orc_grads_f32* g) {
+  project_backward_impl(sp, n, cam, projected_grad, g, 1);
 }
 
 float orc_powi_f32(float a, int32_t b) { return powi_f32(a, b); }

@@ -86,8 +86,9 @@ uint64_t orc_bin(const float* projected, uint32_t n, const orc_tiling* t,
 // contributed to the pixel.
 void orc_composite(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
                    const orc_tiling* t, float* out, float* final_t, uint32_t* last_idx);
+// float64 blend; last_idx / n_used (contributing splats per pixel) may be NULL.
 void orc_composite_f64(const double* projected, const uint32_t* payloads, const uint32_t* ranges,
-                       const orc_tiling* t, double* out);
+                       const orc_tiling* t, double* out, uint32_t* last_idx, uint32_t* n_used);
 
 // Reference-parity backward (rasterize_backward.wgsl:73-210).  Accumulates
 // (in float64, then rounds) into projected_grad[12n], which is overwritten.
@@ -95,9 +96,19 @@ void orc_composite_backward_ref(const float* projected, const uint32_t* payloads
                                 const uint32_t* ranges, const orc_tiling* t, uint32_t n,
                                 const float* forward_out, const float* dl_dc, float* projected_grad);
 
+// Recompute-mode backward: starts at each pixel's last contributing splat
+// with the saved final T (outputs of orc_composite), no 1.0001 bail.
+void orc_composite_backward_recompute(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
+                                      const orc_tiling* t, uint32_t n, const float* final_t,
+                                      const uint32_t* last_idx, const float* dl_dc, float* projected_grad);
+
 // project_backward.wgsl:79-304.  Writes every gradient of every splat.
 void orc_project_backward(const orc_splats_f32* s, uint32_t n, const orc_camera* cam,
                           const float* projected_grad, orc_grads_f32* g);
+// The exact chain rule of the forward (adds the colour clamp gate, the SH
+// direction term and the Jacobian term that the reference drops).
+void orc_project_backward_full(const orc_splats_f32* s, uint32_t n, const orc_camera* cam,
+                               const float* projected_grad, orc_grads_f32* g);
 
 // Adam (adam.rs:70-90) on a flat slab; bias corrections as adam_gpu.rs:275-276.
 float orc_powi_f32(float a, int32_t b);

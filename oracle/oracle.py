@@ -176,15 +176,19 @@ def composite(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t
     return (img, ft, li) if with_state else img
 
 
-def composite_f64(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t: Tiling) -> np.ndarray:
+def composite_f64(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t: Tiling,
+                  with_state: bool = False):
+    """float64 blend; with_state also returns (last_idx, n_used) per pixel."""
     projected = np.ascontiguousarray(projected, np.float64)
     payloads = np.ascontiguousarray(payloads, np.uint32)
     if payloads.size == 0:
         payloads = np.zeros(1, np.uint32)
     img = np.zeros((t.height, t.width, 4), np.float64)
+    li = np.zeros((t.height, t.width), np.uint32)
+    nu = np.zeros((t.height, t.width), np.uint32)
     rg = np.ascontiguousarray(ranges, np.uint32)
-    lib().orc_composite_f64(_p(projected), _p(payloads), _p(rg), C.byref(t), _p(img))
-    return img
+    lib().orc_composite_f64(_p(projected), _p(payloads), _p(rg), C.byref(t), _p(img), _p(li), _p(nu))
+    return (img, li, nu) if with_state else img
 
 
 def composite_backward_ref(projected, payloads, ranges, t: Tiling, forward_out, dl_dc) -> np.ndarray:
@@ -202,8 +206,25 @@ def composite_backward_ref(projected, payloads, ranges, t: Tiling, forward_out, 
     return out
 
 
-def project_backward(s, cam: Camera, projected_grad: np.ndarray):
-    """Returns gradients in the host SoA layout (HostSplats-like tuple of arrays)."""
+def composite_backward_recompute(projected, payloads, ranges, t: Tiling, final_t, last_idx, dl_dc) -> np.ndarray:
+    projected = np.ascontiguousarray(projected, np.float32)
+    n = projected.shape[0]
+    payloads = np.ascontiguousarray(payloads, np.uint32)
+    if payloads.size == 0:
+        payloads = np.zeros(1, np.uint32)
+    out = np.zeros((n, 12), np.float32)
+    rg = np.ascontiguousarray(ranges, np.uint32)
+    ft = np.ascontiguousarray(final_t, np.float32)
+    li = np.ascontiguousarray(last_idx, np.uint32)
+    dl = np.ascontiguousarray(dl_dc, np.float32)
+    lib().orc_composite_backward_recompute(_p(projected), _p(payloads), _p(rg), C.byref(t), C.c_uint32(n), _p(ft),
+                                           _p(li), _p(dl), _p(out))
+    return out
+
+
+def project_backward(s, cam: Camera, projected_grad: np.ndarray, full: bool = False):
+    """Returns gradients in the host SoA layout (HostSplats-like tuple of
+    arrays).  full=True: the exact chain rule (recompute mode)."""
     from nanotracer_amd.synth import HostSplats  # data container only
 
     st, keep = _splats(s, np.float32)
@@ -211,7 +232,8 @@ def project_backward(s, cam: Camera, projected_grad: np.ndarray):
     g = HostSplats.zeros(n)
     gs = _Splats32(*[a.ctypes.data for a in g.arrays()])
     pg = np.ascontiguousarray(projected_grad, np.float32)
-    lib().orc_project_backward(C.byref(st), C.c_uint32(n), C.byref(cam), _p(pg), C.byref(gs))
+    fn = lib().orc_project_backward_full if full else lib().orc_project_backward
+    fn(C.byref(st), C.c_uint32(n), C.byref(cam), _p(pg), C.byref(gs))
     return g
 
 

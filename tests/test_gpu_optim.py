@@ -155,3 +155,50 @@ def test_config_a_training_per_step_parity(ctx):
                 assert d.max() <= 2.0 * lr + 1e-6
                 assert (d > 1e-6).mean() < 2e-3
     assert losses[-1] < losses[0]
+
+
+def test_recompute_mode_training_per_step_parity(ctx):
+    """Config A in the Differentiation form (recompute backward, exact chain
+    rule): each checked step matches the oracle's recompute path."""
+    from nanotracer_amd import _lib as L
+
+    s = config_a_scene()
+    c = config_a_camera()
+    ocam = O.camera_from_pose(c["eye"], c["target"], c["up"], c["fov_y"], c["width"], c["height"], s.n)
+    t = O.tiling(c["width"], c["height"], 16, c["depth_max"])
+    tp = O.project(synth.perturb_means(s, 0x7A46E7), ocam)
+    tk, tpay, trg = O.bin(tp, t)
+    target = O.composite(tp, tpay, trg, t)
+    gpu = GpuSplatBuffer.upload(ctx, s)
+    tr = Trainer(ctx, gpu, [View(c["eye"], c["width"], c["height"], target)], c["depth_max"],
+                 backward_mode=L.BWD_RECOMPUTE)
+    cfg = AdamConfig()
+    losses = []
+    for it in range(30):
+        check = it in (0, 1, 2, 10, 29)
+        if check:
+            before = gpu.readback()
+            mg, vg = gpu.moments()
+            m = [a.reshape(-1).copy() for a in mg.arrays()]
+            v = [a.reshape(-1).copy() for a in vg.arrays()]
+            acc = gpu.grad_acc()
+        res = tr.step(it)
+        losses.append(res.loss)
+        if check:
+            op = O.project(before, ocam)
+            keys, pay, rg = O.bin(op, t)
+            img, ft, li = O.composite(op, pay, rg, t, with_state=True)
+            loss, dl = O.loss_grad(img, target, 0)
+            pg = O.composite_backward_recompute(op, pay, rg, t, ft, li, dl)
+            ref_g = O.project_backward(before, ocam, pg, full=True)
+            ref_s = _oracle_fused_step(before, ref_g, m, v, acc, it + 1, cfg, cfg)
+            assert abs(res.loss - loss) <= 1e-9
+            gg = gpu.grads()
+            for name, a, b in zip(("pos", "rot", "scale", "logit", "dc", "rest"), gg.arrays(), ref_g.arrays()):
+                assert rel_norm_err(a, b) < 1e-4, (it, name, rel_norm_err(a, b))
+            after = gpu.readback()
+            for a, b in zip(after.arrays(), ref_s.arrays()):
+                d = np.abs(a.astype(np.float64) - b)
+                assert d.max() <= 2.0 * cfg.lr + 1e-6
+                assert (d > 1e-6).mean() < 2e-3
+    assert losses[-1] < losses[0]

@@ -195,3 +195,90 @@ def test_tonemap_matches_oracle(ctx):
     # powf may differ by an ulp between libm and the device: allow one level
     assert np.abs(g.astype(int) - o.astype(int)).max() <= 1
     assert np.array_equal(g[..., 3], o[..., 3])
+
+
+def _forward_ex(ctx, s, c, tile=16):
+    f = _forward(ctx, s, c, tile)
+    W, H = c["width"], c["height"]
+    ft = ctx.storage_buffer_zeroed(W * H * 4)
+    li = ctx.storage_buffer_zeroed(W * H * 4)
+    f["r"].composite_ex(ctx, f["proj"], f["bins"], f["img"], f["params"], ft, li)
+    f["ft"], f["li"] = ft, li
+    return f
+
+
+@pytest.mark.parametrize("which,tile", [("A", 16), ("A", 32), ("B", 16)])
+def test_composite_ex_state_bit_exact(ctx, which, tile):
+    s, c = (config_a_scene(), config_a_camera()) if which == "A" else (config_b_scene(), config_b_camera())
+    f = _forward_ex(ctx, s, c, tile)
+    W, H = c["width"], c["height"]
+    op = O.project(s, f["ocam"])
+    t = O.tiling(W, H, tile, c["depth_max"])
+    ok, opay, orr = O.bin(op, t)
+    oimg, oft, oli = O.composite(op, opay, orr, t, with_state=True)
+    assert np.array_equal(f["ft"].download(np.float32, W * H).reshape(H, W).view(np.uint32), oft.view(np.uint32))
+    assert np.array_equal(f["li"].download(np.uint32, W * H).reshape(H, W), oli)
+    gimg = f["img"].download(np.float32, W * H * 4).reshape(H, W, 4)
+    assert np.array_equal(gimg.view(np.uint32), oimg.view(np.uint32))
+
+
+@pytest.mark.parametrize("which,tile", [("A", 16), ("A", 8), ("A", 32), ("B", 16)])
+def test_recompute_backward_matches_oracle(ctx, which, tile):
+    s, c = (config_a_scene(), config_a_camera()) if which == "A" else (config_b_scene(), config_b_camera())
+    f = _forward_ex(ctx, s, c, tile)
+    W, H = c["width"], c["height"]
+    rng = np.random.default_rng(13)
+    dl = rng.normal(0, 1e-3, (H, W, 4)).astype(np.float32)
+    dl[..., 3] = 0
+    dlb = ctx.storage_buffer(dl)
+    pg = f["r"].alloc_projected_grad(ctx, s.n)
+    f["r"].composite_backward_recompute(ctx, f["proj"], f["bins"], f["ft"], f["li"], dlb, pg, f["params"])
+    g = pg.download(np.float32, s.n * 12).reshape(s.n, 12)
+    op = O.project(s, f["ocam"])
+    t = O.tiling(W, H, tile, c["depth_max"])
+    ok, opay, orr = O.bin(op, t)
+    oimg, oft, oli = O.composite(op, opay, orr, t, with_state=True)
+    og = O.composite_backward_recompute(op, opay, orr, t, oft, oli, dl)
+    for name, cols in (("dmean", [0, 1]), ("dopacity", [2]), ("dconic", [4, 5, 6]), ("dcolor", [8, 9, 10])):
+        err = rel_norm_err(g[:, cols], og[:, cols])
+        assert err < GRAD_TOL, f"{name}: relative error {err:.2e}"
+    scale = np.abs(og).max(axis=0) + 1e-30
+    assert (np.abs(g - og) / scale).max() < 1e-3
+
+
+@pytest.mark.parametrize("which", ["A", "B"])
+def test_project_backward_full_chain_bit_exact(ctx, which):
+    from nanotracer_amd import _lib as L
+
+    s, c = (config_a_scene(), config_a_camera()) if which == "A" else (config_b_scene(), config_b_camera())
+    f = _forward(ctx, s, c)
+    n = s.n
+    rng = np.random.default_rng(17)
+    pg_host = rng.normal(0, 1e-2, (n, 12)).astype(np.float32)
+    pg_host[:, [3, 7, 11]] = 0
+    pg = ctx.storage_buffer(pg_host)
+    f["r"].project_backward(ctx, f["g"], pg, f["cam"], L.BWD_RECOMPUTE)
+    gg = f["g"].grads()
+    og = O.project_backward(s, f["ocam"], pg_host, full=True)
+    for a, b in zip(gg.arrays(), og.arrays()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_recompute_and_parity_agree_without_saturation(ctx):
+    """Where no pixel saturates, both backward modes are the same chain rule
+    (they differ only in T = 1 - (1 - T) and where the walk starts)."""
+    s = config_a_scene()
+    s.logit[:] = -2.0  # low opacity: T never drops below 1e-4
+    c = config_a_camera()
+    f = _forward_ex(ctx, s, c)
+    W, H = c["width"], c["height"]
+    assert f["ft"].download(np.float32, W * H).min() > 1e-4
+    dl = np.random.default_rng(19).normal(0, 1e-3, (H, W, 4)).astype(np.float32)
+    dlb = ctx.storage_buffer(dl)
+    a = f["r"].alloc_projected_grad(ctx, s.n)
+    b = f["r"].alloc_projected_grad(ctx, s.n)
+    f["r"].composite_backward(ctx, f["proj"], f["bins"], f["img"], dlb, a, f["params"])
+    f["r"].composite_backward_recompute(ctx, f["proj"], f["bins"], f["ft"], f["li"], dlb, b, f["params"])
+    ga = a.download(np.float32, s.n * 12).reshape(s.n, 12)
+    gb = b.download(np.float32, s.n * 12).reshape(s.n, 12)
+    assert rel_norm_err(ga, gb) < 1e-4
