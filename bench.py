@@ -1,14 +1,23 @@
-"""Benchmark: differentiable splat rasteriser training step + render on one MI355X.
+"""Benchmark: differentiable splat rasteriser on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched|plain]
 
-Prints ONE JSON line.  The timed region holds K full training steps
-(project, bin, forward blend, device loss, backward blend, project_backward,
-fused Adam) with every input resident in HBM; it is bracketed by device
-synchronisation.  Afterwards a few steps run with per-kernel HIP-event timing
-(on the library's own stream) to report the dominant kernel against the HBM
-roofline, and the host restatement of the reference path (the checker) is
-timed on a bounded sample as the CPU baseline.
+Prints ONE JSON line.
+
+batched (default): one scene of 1,000,000 splats (SH order 3, fp32); each
+  training iteration renders 16 cameras on a seeded radius-6 orbit with
+  resolutions drawn from {640x480, 1280x720, 1920x1080, 2560x1440}, sums the
+  per-view L1 losses and runs backward + fused Adam: one launch per stage
+  for all views, the whole iteration captured once in a hipGraph and
+  replayed.  `value` = training iterations per second.
+plain: the single-view path (one 1920x1080 view per iteration, reference
+  call sequence).
+
+Also reported: single-view 1080p render throughput and p50/p99 latency at
+1M splats, per-kernel HIP-event timings (library stream) with the dominant
+kernel against the HBM roofline, and the CPU baseline (the host
+restatement of the reference path, run on this host's cores on the same
+workload).
 """
 from __future__ import annotations
 
@@ -27,97 +36,206 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 N_SPLATS = 1_000_000
-WIDTH, HEIGHT = 1920, 1080
+N_VIEWS = 16
 SEED = 0x5EED0002
+TILE = 16
 
 
-def build_workload(ctx, backward_mode):
+def make_views(n):
     from nanotracer_amd import synth
-    from nanotracer_amd.api import GpuSplatBuffer
-    from nanotracer_amd.train import Trainer, View
+    from nanotracer_amd.api import CameraUniform, ViewSpec
 
-    s = synth.make_scene(N_SPLATS, SEED, 3.0)
-    u = synth.uniform((SEED << 8) + 31, 1)
-    eye = synth.orbit_eye(6.0, 2.0 * math.pi * float(u[0]), 0.3)
-    # target: the same scene with perturbed means, rendered on the device
-    tgt_scene = GpuSplatBuffer.upload(ctx, synth.perturb_means(s, SEED + 1))
-    tmp = Trainer(ctx, tgt_scene, [View(eye, WIDTH, HEIGHT)], 12.0)
-    _, bins, img, _, _ = tmp.render(tmp.views[0])
-    target = img.download(np.float32, WIDTH * HEIGHT * 4).reshape(HEIGHT, WIDTH, 4)
-    bins.free()
-    tgt_scene.free()
-    g = GpuSplatBuffer.upload(ctx, s)
-    tr = Trainer(ctx, g, [View(eye, WIDTH, HEIGHT, target)], 12.0, backward_mode=backward_mode)
-    return s, eye, target, g, tr
+    views = []
+    for eye, w, h in synth.orbit_views(N_VIEWS, SEED, 6.0):
+        cam = CameraUniform.from_pose(eye, (0, 0, 0), (0, 1, 0), math.pi / 3, w, h, n)
+        views.append((ViewSpec(cam, w, h, 12.0), eye))
+    return views
 
 
-def alg_bytes(name: str, N: int, Nv: int, K: int, P: int, T: int, kb: int = 4) -> float:
-    """Algorithmic (compulsory) HBM bytes of one launch of each kernel."""
+def alg_bytes(name: str, N: int, Nv: int, K: int, P: int, T: int, V: int, kb: int = 4) -> float:
+    """Algorithmic (compulsory) HBM bytes of one launch of each kernel.
+    N splats, Nv visible (splat, view) records, K pairs, P pixels, T tiles,
+    V views (all summed over the launch's views)."""
     table = {
         "project": 236 * N + 48 * N,
+        "project_views": 236 * N + 52 * N * V,
         "tile_count": 20 * N + 4 * N,
-        "scan_reduce": 4 * N,
-        "scan_down": 8 * N,
+        "scan_reduce": 4 * N * V,
+        "scan_down": 8 * N * V,
         "tile_emit": 24 * N + (kb + 4) * K,
+        "emit_views": 24 * N * V + (kb + 4) * K,
         "sort_hist": kb * K,
         "sort_onesweep": 2 * (kb + 4) * K,
         "tile_ranges": kb * K + 8 * T,
         "blend_fwd": 8 * T + 40 * K + 16 * P,
+        "blend_fwd_loss": 8 * T + 40 * K + 16 * P + 16 * P + 16 * P + 8 * P,
         "loss": 48 * P,
         "blend_bwd": 8 * T + 40 * K + 32 * P + 36 * Nv,
         "blend_bwd_recompute": 8 * T + 40 * K + 24 * P + 36 * Nv,
-        "project_backward_full": (236 + 48 + 236) * N,
         "project_backward": (236 + 48 + 236) * N,
+        "project_backward_full": (236 + 48 + 236) * N,
         "adam_step": 7 * 236 * N + 8 * N,
+        "backward_adam_views": 236 * N + 48 * N * V + 6 * 236 * N + 8 * N,
     }
     return float(table.get(name, 0))
 
 
-def cpu_baseline(s, eye, target):
-    """The checker (host restatement of the reference path) on one full
-    training step of the same workload; returns iterations/s."""
+def kernel_table(rep, N, Nv, K, P, T, V):
+    kernels = {}
+    for name, (launches, ms) in rep.items():
+        per = alg_bytes(name, N, Nv, K, P, T, V)
+        kernels[name] = dict(launches=launches, avg_ms=ms / launches, alg_bytes=per,
+                             gbs=(per / (ms / launches * 1e-3) / 1e9) if per else None)
+    return kernels
+
+
+def roofline_of(kernels):
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    dk = kernels[dom]
+    achieved = dk["gbs"] or 0.0
+    return dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, avg_ms=round(dk["avg_ms"], 4),
+                alg_bytes_per_launch=dk["alg_bytes"])
+
+
+def render_latency(ctx, g, n):
+    """Single-view 1920x1080 forward (project, bin, blend) latency."""
+    from nanotracer_amd import synth
+    from nanotracer_amd.api import CameraUniform, ViewBatch, ViewSpec
+
+    u = synth.uniform((SEED << 8) + 31, 1)
+    eye = synth.orbit_eye(6.0, 2.0 * math.pi * float(u[0]), 0.3)
+    spec = ViewSpec(CameraUniform.from_pose(eye, (0, 0, 0), (0, 1, 0), math.pi / 3, 1920, 1080, n), 1920, 1080, 12.0)
+    b = ViewBatch(ctx, 1, n, 1920 * 1080, 20_000_000)
+    lat = []
+    for _ in range(40):
+        ctx.sync()
+        a = time.perf_counter()
+        b.render(g, [spec], TILE)
+        ctx.sync()
+        lat.append((time.perf_counter() - a) * 1e3)
+    b.free()
+    lat = np.array(lat[10:])
+    return dict(render_mpx_s=round(1920 * 1080 / (lat.mean() * 1e-3) / 1e6, 2),
+                render_p50_ms=round(float(np.percentile(lat, 50)), 3),
+                render_p99_ms=round(float(np.percentile(lat, 99)), 3))
+
+
+def cpu_baseline_batched(s, views, targets):
+    """The host restatement of the reference path on one full iteration of
+    the same 16-view workload (recompute backward as on the GPU)."""
+    from nanotracer_amd import synth
     from oracle import oracle as O
 
     t0 = time.perf_counter()
-    cam = O.camera_from_pose(eye, (0, 0, 0), (0, 1, 0), math.pi / 3, WIDTH, HEIGHT, s.n)
-    t = O.tiling(WIDTH, HEIGHT, 16, 12.0)
-    op = O.project(s, cam)
-    keys, pay, rg = O.bin(op, t)
-    img = O.composite(op, pay, rg, t)
-    loss, dl = O.loss_grad(img, target, 0)
-    pg = O.composite_backward_ref(op, pay, rg, t, img, dl)
-    g = O.project_backward(s, cam, pg)
-    flat_p = [a.reshape(-1).copy() for a in s.arrays()]
-    for p, gr in zip(flat_p, g.arrays()):
+    total = None
+    for (spec, eye), tgt in zip(views, targets):
+        cam = O.camera_like(spec.camera)
+        t = O.tiling(spec.width, spec.height, TILE, spec.depth_max)
+        op = O.project(s, cam)
+        keys, pay, rg = O.bin(op, t)
+        img, ft, li = O.composite(op, pay, rg, t, with_state=True)
+        loss, dl = O.loss_grad(img, tgt, 0)
+        pg = O.composite_backward_recompute(op, pay, rg, t, ft, li, dl)
+        gv = O.project_backward(s, cam, pg, full=True)
+        total = gv if total is None else synth.HostSplats(*[a + b for a, b in zip(total.arrays(), gv.arrays())])
+    for p, gr in zip([a.reshape(-1).copy() for a in s.arrays()], total.arrays()):
         m = np.zeros_like(p)
         v = np.zeros_like(p)
         O.adam_step(p, gr.reshape(-1), m, v, 1.6e-4, 0.9, 0.999, 1e-15, 1)
     dt = time.perf_counter() - t0
-    return dict(value=1.0 / dt, unit="iters/s", cores=O.num_threads(), kind="port",
-                sample=f"one full training step (project, bin, blend fwd, L1, blend bwd, project_backward, Adam) "
-                       f"of the {N_SPLATS}-splat {WIDTH}x{HEIGHT} workload, {dt:.2f} s")
+    return dict(value=round(1.0 / dt, 5), unit="iters/s", cores=O.num_threads(), kind="port",
+                sample=f"one full 16-view training iteration of the same workload on the host restatement "
+                       f"(project, bin, blend, L1, recompute backward, project_backward, Adam): {dt:.2f} s")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--backward", choices=("ref", "recompute"), default="ref",
-                    help="reference-parity backward or the recompute (final-T, last-index) backward")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see the sharding design); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def run_batched(args, ctx):
     from nanotracer_amd import _lib as L
-    from nanotracer_amd.api import GpuCtx
+    from nanotracer_amd import synth
+    from nanotracer_amd.api import GpuSplatBuffer, ViewBatch
 
-    ctx = GpuCtx(0)
+    s = synth.make_scene(N_SPLATS, SEED, 3.0)
+    views = make_views(s.n)
+    specs = [v[0] for v in views]
+    pixels = sum(v.width * v.height for v in specs)
+    # targets: the scene with perturbed means, rendered on the device
+    tgt_scene = GpuSplatBuffer.upload(ctx, synth.perturb_means(s, SEED + 1))
+    probe = ViewBatch(ctx, N_VIEWS, s.n, pixels, 400_000_000)
+    probe.render(tgt_scene, specs, TILE)
+    targets = [probe.image(k) for k in range(N_VIEWS)]
+    pairs_t, _, _ = probe.stats()
+    probe.free()
+    tgt_scene.free()
+    tbuf = ctx.storage_buffer(np.concatenate([x.reshape(-1) for x in targets]))
+    g = GpuSplatBuffer.upload(ctx, s)
+    cap = int(max(pairs_t, 1) * 1.3) + 1_000_000
+    b = ViewBatch(ctx, N_VIEWS, s.n, pixels, cap)
     mode = L.BWD_RECOMPUTE if args.backward == "recompute" else L.BWD_REF_PARITY
-    s, eye, target, g, tr = build_workload(ctx, mode)
+    t = 0
+    for _ in range(max(args.warmup, 1)):
+        t += 1
+        b.train_step(g, specs, tbuf, t, TILE, L.LOSS_L1, mode, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t += 1
+        b.train_step(g, specs, tbuf, t, TILE, L.LOSS_L1, mode, graph=True)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    pairs, visible, overflow = b.stats()
+    if overflow:
+        raise RuntimeError("pair capacity overflowed during the timed run")
+    losses = b.losses()
+    iters_s = args.steps / dt
+    # per-kernel timing (eager launches with HIP events on the library stream)
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for _ in range(3):
+        t += 1
+        b.train_step(g, specs, tbuf, t, TILE, L.LOSS_L1, mode, graph=False)
+    ctx.prof_enable(False)
+    kernels = kernel_table(ctx.prof_report(), s.n, visible, pairs, pixels,
+                           sum((-(-v.width // TILE)) * (-(-v.height // TILE)) for v in specs), N_VIEWS)
+    b.free()
+    out = dict(
+        metric="train iters/s + render Mpx/s at 1M splats 1080p; p50/p99 render latency",
+        value=round(iters_s, 3), unit="iters/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=round(dt * 1e3 / args.steps, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32", data="synthetic",
+        config=dict(workload="batched training iteration: 1M splats, 16 views of mixed resolution per launch, "
+                             "hipGraph replay", splats=s.n, views=N_VIEWS, pixels=pixels, tile=TILE, pairs=pairs,
+                    visible_records=visible, backward=args.backward, loss="L1 summed over views"),
+        views_per_s=round(iters_s * N_VIEWS, 2), train_mpx_s=round(iters_s * pixels / 1e6, 2),
+        final_loss=float(losses.sum()),
+    )
+    out.update(render_latency(ctx, g, s.n))
+    out["roofline"] = roofline_of(kernels)
+    out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
+                              gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batched(s, views, targets)
+    return out
+
+
+def run_plain(args, ctx):
+    from nanotracer_amd import _lib as L
+    from nanotracer_amd import synth
+    from nanotracer_amd.api import GpuSplatBuffer
+    from nanotracer_amd.train import Trainer, View
+
+    W, H = 1920, 1080
+    s = synth.make_scene(N_SPLATS, SEED, 3.0)
+    u = synth.uniform((SEED << 8) + 31, 1)
+    eye = synth.orbit_eye(6.0, 2.0 * math.pi * float(u[0]), 0.3)
+    tgt_scene = GpuSplatBuffer.upload(ctx, synth.perturb_means(s, SEED + 1))
+    tmp = Trainer(ctx, tgt_scene, [View(eye, W, H)], 12.0)
+    _, bins, img, _, _ = tmp.render(tmp.views[0])
+    target = img.download(np.float32, W * H * 4).reshape(H, W, 4)
+    bins.free()
+    tgt_scene.free()
+    g = GpuSplatBuffer.upload(ctx, s)
+    mode = L.BWD_RECOMPUTE if args.backward == "recompute" else L.BWD_REF_PARITY
+    tr = Trainer(ctx, g, [View(eye, W, H, target)], 12.0, backward_mode=mode)
     for it in range(args.warmup):
         tr.step(it)
     ctx.sync()
@@ -127,60 +245,51 @@ def main():
         pairs = tr.step(args.warmup + it).total_pairs
     ctx.sync()
     dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-    iters_s = args.steps / dt
-
-    # render-only latency (forward: project, bin, blend) at the same size
-    lat = []
-    for _ in range(30):
-        ctx.sync()
-        a = time.perf_counter()
-        _, bins, _, _, _ = tr.render(tr.views[0])
-        ctx.sync()
-        lat.append((time.perf_counter() - a) * 1e3)
-        bins.free()
-    lat = np.array(lat[5:])
-    render_mpx_s = WIDTH * HEIGHT / (lat.mean() * 1e-3) / 1e6
-
-    # per-kernel timing on the library stream
     ctx.prof_reset()
     ctx.prof_enable(True)
-    nprof = 3
-    for it in range(nprof):
+    for it in range(3):
         tr.step(1000 + it)
     ctx.prof_enable(False)
-    rep = ctx.prof_report()
     proj = tr._bufs["proj"].download(np.float32, N_SPLATS * 12).reshape(-1, 12)
     Nv = int((proj[:, 7] != 0).sum())
-    T = ((WIDTH + 15) // 16) * ((HEIGHT + 15) // 16)
-    kernels = {}
-    for name, (launches, ms) in rep.items():
-        per = alg_bytes(name, N_SPLATS, Nv, pairs, WIDTH * HEIGHT, T)
-        kernels[name] = dict(launches=launches, avg_ms=ms / launches, alg_bytes=per,
-                             gbs=(per / (ms / launches * 1e-3) / 1e9) if per else None)
-    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
-    dk = kernels[dom]
-    achieved = dk["gbs"] or 0.0
-    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, avg_ms=round(dk["avg_ms"], 4),
-                    alg_bytes_per_launch=dk["alg_bytes"])
-
+    kernels = kernel_table(ctx.prof_report(), N_SPLATS, Nv, pairs, W * H, ((W + 15) // 16) * ((H + 15) // 16), 1)
     out = dict(
         metric="train iters/s + render Mpx/s at 1M splats 1080p; p50/p99 render latency",
-        value=round(iters_s, 3), unit="iters/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
-        ms_per_step=round(ms_per_step, 3), higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
-        data="synthetic",
-        config=dict(workload="plain single-view training step (project, bin, blend fwd, L1, blend bwd, "
-                             "project_backward, fused Adam)", splats=N_SPLATS, width=WIDTH, height=HEIGHT,
-                    tile=16, pairs=pairs, visible=Nv, backward=args.backward),
-        render_mpx_s=round(render_mpx_s, 2), render_p50_ms=round(float(np.percentile(lat, 50)), 3),
-        render_p99_ms=round(float(np.percentile(lat, 99)), 3),
-        roofline=roofline,
-        kernels={k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
-                         gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()},
+        value=round(args.steps / dt, 3), unit="iters/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=round(dt * 1e3 / args.steps, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32", data="synthetic",
+        config=dict(workload="plain single-view training step (reference call sequence)", splats=N_SPLATS,
+                    width=W, height=H, tile=16, pairs=pairs, visible=Nv, backward=args.backward),
     )
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(s, eye, target)
+    out.update(render_latency(ctx, g, s.n))
+    out["roofline"] = roofline_of(kernels)
+    out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
+                              gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("batched", "plain"), default="batched")
+    ap.add_argument("--backward", choices=("ref", "recompute"), default="recompute",
+                    help="reference-parity backward or the recompute (final-T, last-index) backward")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile", type=int, choices=(8, 16, 32), default=16)
+    args = ap.parse_args()
+    global TILE
+    TILE = args.tile
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is view-parallel with one gradient reduce-scatter); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    from nanotracer_amd.api import GpuCtx
+
+    ctx = GpuCtx(0)
+    out = run_batched(args, ctx) if args.workload == "batched" else run_plain(args, ctx)
     print(json.dumps(out))
     ctx.close()
 

@@ -211,6 +211,58 @@ int nts_adam_step_flat(nts_ctx* ctx, nts_buffer* params, const nts_buffer* grads
 int nts_adam_step(nts_ctx* ctx, nts_scene* scene, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr,
                   uint64_t t, const nts_constraints* cons);
 
+/* ---- batched multi-view step and render (no reference counterpart: the
+ *      reference renders one view per dispatch chain, train.rs:155-250) -- */
+typedef struct nts_batch nts_batch;
+
+/* One view of a batch: camera (CameraUniform layout), depth range of the
+ * 16-bit depth keys, image size (must equal cam.width / cam.height). */
+typedef struct nts_view_desc {
+  nts_camera cam;
+  float depth_max;
+  uint32_t width, height;
+  uint32_t pad;
+} nts_view_desc;
+
+#define NTS_STEP_GRAPH 1        /* capture the step once, replay it (same views) */
+#define NTS_STEP_WRITE_GRADS 2  /* also store the summed parameter gradients */
+
+/* Workspace for up to max_views views (<= 32) of a scene of <= max_splats
+ * splats, <= max_pixels pixels in total and <= pair_capacity (splat, tile)
+ * pairs.  All per-step memory comes from it: a step allocates nothing. */
+int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_splats, uint64_t max_pixels,
+                     uint64_t pair_capacity, nts_batch** out);
+int nts_batch_destroy(nts_ctx* ctx, nts_batch* batch);
+/* One training iteration over n_views views in single launches per stage:
+ * project + count, scan, emit, view-segmented sort, ranges, blend forward
+ * with the loss fused (targets: float4 per pixel, views concatenated),
+ * blend backward, then per-splat project_backward summed over views fused
+ * with accumulate_grad + Adam (step t) + constraints.  Per-view losses
+ * (mean over 3WH) are summed into the loss reported by nts_batch_losses.
+ * Nothing synchronises with the host; with NTS_STEP_GRAPH the step is
+ * captured in a hipGraph on first use and replayed while the views and
+ * settings stay the same and t advances by one.  Pairs beyond
+ * pair_capacity are dropped and flagged (nts_batch_stats). */
+int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* batch, const nts_view_desc* views,
+                         uint32_t n_views, const nts_buffer* targets, uint32_t tile_size, int loss_kind,
+                         int bwd_mode, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr, uint64_t t,
+                         const nts_constraints* cons, int flags);
+/* Forward only (project, bin, blend) of n_views views into the batch image. */
+int nts_render_views(nts_ctx* ctx, const nts_scene* scene, nts_batch* batch, const nts_view_desc* views,
+                     uint32_t n_views, uint32_t tile_size);
+int nts_batch_losses(nts_ctx* ctx, const nts_batch* batch, double* losses);
+int nts_batch_stats(nts_ctx* ctx, const nts_batch* batch, uint64_t* total_pairs, uint32_t* visible,
+                    uint32_t* overflow);
+int nts_batch_download_image(nts_ctx* ctx, const nts_batch* batch, uint32_t view, float* host);
+int nts_batch_download_state(nts_ctx* ctx, const nts_batch* batch, uint32_t view, float* final_t,
+                             uint32_t* last_idx);
+int nts_batch_download_projected_grad(nts_ctx* ctx, const nts_batch* batch, uint32_t view, uint32_t n,
+                                      float* host);
+/* Keys (reference u32 format, view-local tile id), payloads and ranges of
+ * one view's segment; *count receives the view's pair count. */
+int nts_batch_download_bins(nts_ctx* ctx, const nts_batch* batch, uint32_t view, uint64_t* count, uint32_t* keys,
+                            uint32_t* payloads, uint32_t* ranges);
+
 /* ---- viewer (Rasterizer::tonemap, raster.rs:405-455) ------------------- */
 int nts_tonemap(nts_ctx* ctx, const nts_buffer* image, nts_buffer* rgba8, uint32_t width, uint32_t height);
 

@@ -67,7 +67,17 @@ class Constraints(C.Structure):
     _fields_ = [("log_scale_min", C.c_float), ("log_scale_max", C.c_float), ("logit_max", C.c_float)]
 
 
+class ViewDesc(C.Structure):
+    """nts_view_desc: camera, depth range, image size."""
+
+    _fields_ = [("cam", Camera), ("depth_max", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+STEP_GRAPH, STEP_WRITE_GRADS = 1, 2
+
 assert C.sizeof(Camera) == 96
+assert C.sizeof(ViewDesc) == 112
 
 
 _lib = None
@@ -129,6 +139,22 @@ _SIGS = {
     "nts_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AdamCfg), C.POINTER(AdamCfg), C.c_uint64,
                                 C.POINTER(Constraints)]),
     "nts_tonemap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "nts_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
+                                   C.POINTER(C.c_void_p)]),
+    "nts_batch_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "nts_train_step_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ViewDesc), C.c_uint32,
+                                       C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.POINTER(AdamCfg),
+                                       C.POINTER(AdamCfg), C.c_uint64, C.POINTER(Constraints), C.c_int]),
+    "nts_render_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ViewDesc), C.c_uint32,
+                                   C.c_uint32]),
+    "nts_batch_losses": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nts_batch_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint32)]),
+    "nts_batch_download_image": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "nts_batch_download_state": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "nts_batch_download_projected_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "nts_batch_download_bins": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
 }
 
 # symbols of include/nts.h, checked by the CPU test suite

@@ -387,3 +387,103 @@ class AdamGpu:
         cons = L.Constraints(log_scale_min, log_scale_max, opacity_logit_max)
         L.check(L.lib().nts_adam_step(ctx.h, splats.h, C.byref(cfg_pos.c()), C.byref(cfg_attr.c()), int(t),
                                       C.byref(cons)))
+
+
+@dataclass
+class ViewSpec:
+    """One view of a batch (nts_view_desc)."""
+
+    camera: L.Camera
+    width: int
+    height: int
+    depth_max: float
+
+    def c(self) -> L.ViewDesc:
+        return L.ViewDesc(self.camera, float(self.depth_max), int(self.width), int(self.height), 0)
+
+
+def views_array(views):
+    arr = (L.ViewDesc * len(views))()
+    for k, v in enumerate(views):
+        arr[k] = v.c()
+    return arr
+
+
+class ViewBatch:
+    """Workspace of the batched multi-view step (nts_batch): all views of an
+    iteration go through each stage in one launch; optional hipGraph replay."""
+
+    def __init__(self, ctx: GpuCtx, max_views: int, max_splats: int, max_pixels: int, pair_capacity: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        L.check(L.lib().nts_batch_create(ctx.h, int(max_views), int(max_splats), int(max_pixels), int(pair_capacity),
+                                         C.byref(h)))
+        self.h = h
+        self.sizes = []
+
+    def free(self) -> None:
+        if self.h:
+            L.lib().nts_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def train_step(self, splats: GpuSplatBuffer, views, targets: Buffer, t: int, tile_size: int = 16,
+                   loss_kind: int = L.LOSS_L1, backward_mode: int = L.BWD_RECOMPUTE,
+                   cfg_pos: AdamConfig | None = None, cfg_attr: AdamConfig | None = None, graph: bool = False,
+                   write_grads: bool = False, log_scale_min: float = LOG_SCALE_MIN,
+                   log_scale_max: float = LOG_SCALE_MAX, opacity_logit_max: float = OPACITY_LOGIT_MAX) -> None:
+        cfg_pos = cfg_pos or AdamConfig()
+        cfg_attr = cfg_attr or AdamConfig()
+        arr = views_array(views)
+        self.sizes = [(v.width, v.height) for v in views]
+        flags = (L.STEP_GRAPH if graph else 0) | (L.STEP_WRITE_GRADS if write_grads else 0)
+        cons = L.Constraints(log_scale_min, log_scale_max, opacity_logit_max)
+        L.check(L.lib().nts_train_step_views(self.ctx.h, splats.h, self.h, arr, len(views), targets.h,
+                                             int(tile_size), int(loss_kind), int(backward_mode),
+                                             C.byref(cfg_pos.c()), C.byref(cfg_attr.c()), int(t), C.byref(cons),
+                                             flags))
+
+    def render(self, splats: GpuSplatBuffer, views, tile_size: int = 16) -> None:
+        arr = views_array(views)
+        self.sizes = [(v.width, v.height) for v in views]
+        L.check(L.lib().nts_render_views(self.ctx.h, splats.h, self.h, arr, len(views), int(tile_size)))
+
+    def losses(self) -> np.ndarray:
+        out = np.zeros(len(self.sizes), np.float64)
+        L.check(L.lib().nts_batch_losses(self.ctx.h, self.h, L.vptr(out)))
+        return out
+
+    def stats(self):
+        tp, vis, ov = C.c_uint64(), C.c_uint32(), C.c_uint32()
+        L.check(L.lib().nts_batch_stats(self.ctx.h, self.h, C.byref(tp), C.byref(vis), C.byref(ov)))
+        return tp.value, vis.value, ov.value
+
+    def image(self, v: int) -> np.ndarray:
+        w, h = self.sizes[v]
+        out = np.zeros((h, w, 4), np.float32)
+        L.check(L.lib().nts_batch_download_image(self.ctx.h, self.h, v, L.vptr(out)))
+        return out
+
+    def state(self, v: int):
+        w, h = self.sizes[v]
+        ft = np.zeros((h, w), np.float32)
+        li = np.zeros((h, w), np.uint32)
+        L.check(L.lib().nts_batch_download_state(self.ctx.h, self.h, v, L.vptr(ft), L.vptr(li)))
+        return ft, li
+
+    def projected_grad(self, v: int, n: int) -> np.ndarray:
+        out = np.zeros((n, 12), np.float32)
+        L.check(L.lib().nts_batch_download_projected_grad(self.ctx.h, self.h, v, n, L.vptr(out)))
+        return out
+
+    def bins(self, v: int, tile_size: int):
+        cnt = C.c_uint64()
+        L.check(L.lib().nts_batch_download_bins(self.ctx.h, self.h, v, C.byref(cnt), None, None, None))
+        k = cnt.value
+        w, h = self.sizes[v]
+        ntiles = (-(-w // tile_size)) * (-(-h // tile_size))
+        keys = np.zeros(max(k, 1), np.uint32)
+        pay = np.zeros(max(k, 1), np.uint32)
+        rng = np.zeros(2 * ntiles, np.uint32)
+        L.check(L.lib().nts_batch_download_bins(self.ctx.h, self.h, v, C.byref(cnt), L.vptr(keys), L.vptr(pay),
+                                                L.vptr(rng)))
+        return keys[:k], pay[:k], rng

@@ -1,0 +1,690 @@
+// Multi-view training step and multi-view render over one scene.
+//
+// Every stage is one launch across all views of the batch:
+//   project + tile count   one thread per splat, parameters read once, every
+//                          view projected from registers;
+//   scan                   over the (view, splat) counts;
+//   emit                   (view-local tile << 16 | depth16) keys, one segment
+//                          per view;
+//   onesweep sort          segmented by view;
+//   ranges                 global tile ids (view tile base + local tile);
+//   blend forward          one workgroup per (view, tile), loss fused in the
+//                          epilogue (dL/dC and per-view loss);
+//   blend backward         per (view, tile), per-view projected gradients;
+//   project_backward+Adam  one thread per splat: the views' parameter
+//                          gradients are summed in registers and fed straight
+//                          into Adam + constraints + grad_acc.
+// All buffers live in a preallocated workspace and nothing synchronises with
+// the host, so the whole step can be captured once in a hipGraph and replayed.
+#include "nts_adam.h"
+#include "nts_bin.h"
+#include "nts_project.h"
+#include "nts_raster.h"
+
+struct TileTable {
+  uint32_t n;
+  uint32_t pad[3];
+  ViewTiles t[NTS_MAX_VIEWS];
+};
+
+struct nts_batch {
+  uint32_t max_views = 0, max_splats = 0;
+  uint64_t max_pixels = 0, pair_cap = 0, max_tiles = 0;
+  void* base = nullptr;
+  uint64_t bytes = 0;
+  DevCamera* d_cams = nullptr;
+  Projected* proj = nullptr;
+  ProjectedGrad* pgrad = nullptr;
+  uint32_t* counts = nullptr;  // exclusive offsets after the scan
+  uint64_t* d_total = nullptr;
+  uint64_t* seg_begin = nullptr;
+  uint32_t* tile_base = nullptr;
+  uint32_t* flags = nullptr;    // [0] overflow, [1] visible count, [2..3] total copy
+  uint64_t* d_step = nullptr;   // Adam step counter for captured replays
+  void* keys_a = nullptr;
+  void* keys_b = nullptr;
+  uint32_t* vals_a = nullptr;
+  uint32_t* vals_b = nullptr;
+  uint2* ranges = nullptr;
+  float4* image = nullptr;
+  float4* dl_dc = nullptr;
+  float* final_t = nullptr;
+  uint32_t* last_idx = nullptr;
+  double* losses = nullptr;
+  void* scan_ws = nullptr;
+  void* sort_ws = nullptr;
+  uint32_t* sort_err = nullptr;
+  DevCamera* h_cams = nullptr;  // pinned staging
+  // last launch shape (for downloads)
+  uint32_t n_views = 0;
+  uint64_t pix_base[NTS_MAX_VIEWS + 1] = {};
+  uint32_t width[NTS_MAX_VIEWS] = {}, height[NTS_MAX_VIEWS] = {};
+  // captured step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint8_t> graph_key;
+  uint64_t graph_last_t = 0;
+  bool last_alt = false;
+  uint32_t last_key_bytes = 4;
+  uint32_t last_tile_base[NTS_MAX_VIEWS + 1] = {};
+};
+
+struct U32Table {
+  uint32_t n;
+  uint32_t v[NTS_MAX_VIEWS + 1];
+};
+
+__global__ void k_set_u32s(uint32_t* __restrict__ dst, U32Table t) {
+  const uint32_t i = threadIdx.x;
+  if (blockIdx.x == 0 && i < t.n) dst[i] = t.v[i];
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_project_views(const float* __restrict__ P, uint32_t cap, uint32_t n,
+                                                       const DevCamera* __restrict__ cams, TileTable tt,
+                                                       Projected* __restrict__ proj, uint32_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ visible) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t vis = 0;
+  if (i < n) {
+    SplatP s;
+    load_splat(P, cap, i, s);
+    for (uint32_t v = 0; v < tt.n; ++v) {
+      const DevCamera cam = cams[v];
+      const Projected p = project_p(s, cam);
+      proj[(size_t)v * n + i] = p;
+      uint32_t c = 0;
+      int32_t x0, y0, x1, y1;
+      const ViewTiles vt = tt.t[v];
+      if (p.b.w != 0.0f && tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x,
+                                         (int32_t)vt.tiles_y, x0, y0, x1, y1)) {
+        c = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1));
+        ++vis;
+      }
+      counts[(size_t)v * n + i] = c;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) vis += __shfl_xor(vis, off, 64);
+  if ((threadIdx.x & 63u) == 0 && vis) atomicAdd(visible, vis);
+}
+
+// Segment bounds per view from the scanned counts, clamped to the pair
+// capacity (overflow is flagged, never written past).
+__global__ void k_view_segments(const uint32_t* __restrict__ offsets, uint32_t n, uint32_t n_views,
+                                const uint64_t* __restrict__ d_total, uint64_t cap, uint64_t* __restrict__ seg_begin,
+                                uint32_t* __restrict__ flags) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint64_t total = *d_total;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    const uint64_t b = (n == 0) ? 0 : (uint64_t)offsets[(size_t)v * n];
+    seg_begin[v] = b < cap ? b : cap;
+  }
+  seg_begin[n_views] = total < cap ? total : cap;
+  flags[0] = total > cap ? 1u : 0u;
+  flags[2] = (uint32_t)total;
+  flags[3] = (uint32_t)(total >> 32);
+}
+
+template <typename K>
+__global__ void __launch_bounds__(256) k_emit_views(const Projected* __restrict__ proj, uint32_t n, TileTable tt,
+                                                    const uint32_t* __restrict__ offsets, uint64_t capacity,
+                                                    K* __restrict__ keys, uint32_t* __restrict__ payloads) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = blockIdx.y;
+  if (i >= n) return;
+  const Projected p = proj[(size_t)v * n + i];
+  if (p.b.w == 0.0f) return;
+  const ViewTiles vt = tt.t[v];
+  int32_t x0, y0, x1, y1;
+  if (!tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1,
+                     y1))
+    return;
+  const K d16 = (K)depth16(p.a.z, vt.depth_max);
+  uint32_t w = offsets[(size_t)v * n + i];
+  for (int32_t ty = y0; ty <= y1; ++ty)
+    for (int32_t tx = x0; tx <= x1; ++tx) {
+      const K tile = (K)((uint32_t)ty * vt.tiles_x + (uint32_t)tx);
+      if ((uint64_t)w < capacity) {
+        keys[w] = (tile << 16) | d16;
+        payloads[w] = i;
+      }
+      ++w;
+    }
+}
+
+__global__ void k_set_step(uint64_t* d_step, uint64_t t) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d_step = t;
+}
+__global__ void k_inc_step(uint64_t* d_step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d_step += 1;
+}
+
+// Bias correction 1 - beta^t with the host powi arithmetic (device side so
+// that captured replays advance t without new kernel arguments).
+__device__ __forceinline__ float dev_bias(float beta, uint64_t t) {
+  int32_t b = (int32_t)t;
+  float a = beta, r = 1.0f;
+  while (1) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return 1.0f - r;
+}
+
+struct AdamCfgs {
+  float lr_pos, lr_attr, b1_pos, b2_pos, b1_attr, b2_attr, eps_pos, eps_attr;
+  float smin, smax, omax;
+};
+
+template <bool FULL>
+__global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__ P, float* __restrict__ G,
+                                                             float* __restrict__ M, float* __restrict__ V,
+                                                             float* __restrict__ grad_acc, uint32_t cap, uint32_t n,
+                                                             uint32_t n_views, const DevCamera* __restrict__ cams,
+                                                             const ProjectedGrad* __restrict__ pgrad,
+                                                             const uint64_t* __restrict__ d_step, AdamCfgs cfg,
+                                                             int write_grads) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  SplatP s;
+  load_splat(P, cap, i, s);
+  float gs[P_COUNT];
+#pragma unroll
+  for (int k = 0; k < P_COUNT; ++k) gs[k] = 0.0f;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    const ProjectedGrad pg = pgrad[(size_t)v * n + i];
+    if (pg.a.x == 0.0f && pg.a.y == 0.0f && pg.a.z == 0.0f && pg.b.x == 0.0f && pg.b.y == 0.0f && pg.b.z == 0.0f &&
+        pg.c.x == 0.0f && pg.c.y == 0.0f && pg.c.z == 0.0f)
+      continue;  // the view gave this splat no gradient: its terms are all zero
+    float g[P_COUNT];
+    project_backward_p<FULL>(s, cams[v], pg, g);
+#pragma unroll
+    for (int k = 0; k < P_COUNT; ++k) gs[k] = gs[k] + g[k];
+  }
+  const size_t c = cap;
+  if (write_grads) {
+#pragma unroll
+    for (int k = 0; k < P_COUNT; ++k) G[(size_t)k * c + i] = gs[k];
+  }
+  grad_acc[i] = grad_acc[i] + fabsf(gs[P_POS + 0]) + fabsf(gs[P_POS + 1]) + fabsf(gs[P_POS + 2]);
+  const uint64_t t = *d_step;
+  AdamK kp, ka;
+  kp.lr = cfg.lr_pos; kp.b1 = cfg.b1_pos; kp.b2 = cfg.b2_pos; kp.eps = cfg.eps_pos;
+  kp.bc1 = dev_bias(cfg.b1_pos, t); kp.bc2 = dev_bias(cfg.b2_pos, t);
+  ka.lr = cfg.lr_attr; ka.b1 = cfg.b1_attr; ka.b2 = cfg.b2_attr; ka.eps = cfg.eps_attr;
+  ka.bc1 = dev_bias(cfg.b1_attr, t); ka.bc2 = dev_bias(cfg.b2_attr, t);
+  float np[P_COUNT];
+#pragma unroll
+  for (int k = 0; k < P_COUNT; ++k) {
+    const size_t j = (size_t)k * c + i;
+    float mm = M[j], vv = V[j];
+    np[k] = adam_lane(s.v[k], gs[k], mm, vv, k < P_ROT ? kp : ka);
+    M[j] = mm;
+    V[j] = vv;
+  }
+  const float n2 = np[P_ROT] * np[P_ROT] + np[P_ROT + 1] * np[P_ROT + 1] + np[P_ROT + 2] * np[P_ROT + 2] +
+                   np[P_ROT + 3] * np[P_ROT + 3];
+  const float nrm = nts_maxf(sqrtf(n2), 1e-8f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) np[P_ROT + k] = np[P_ROT + k] / nrm;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) np[P_SCALE + k] = nts_minf(nts_maxf(np[P_SCALE + k], cfg.smin), cfg.smax);
+  np[P_LOGIT] = nts_minf(nts_maxf(np[P_LOGIT], -cfg.omax), cfg.omax);
+#pragma unroll
+  for (int k = 0; k < P_COUNT; ++k) P[(size_t)k * c + i] = np[k];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
+
+extern "C" int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_splats, uint64_t max_pixels,
+                                uint64_t pair_capacity, nts_batch** out) {
+  NTS_CHECK_ARG(ctx && out && max_views > 0 && max_views <= NTS_MAX_VIEWS && max_splats > 0 && max_pixels > 0);
+  NTS_CHECK_ARG(pair_capacity > 0 && pair_capacity < (1ull << 30));
+  *out = nullptr;
+  nts_batch* b = new nts_batch();
+  b->max_views = max_views;
+  b->max_splats = max_splats;
+  b->max_pixels = max_pixels;
+  b->pair_cap = pair_capacity;
+  // tiles at the smallest tile size (8): ceil per axis adds at most one row and column per view
+  b->max_tiles = max_pixels / 64 + (uint64_t)max_views * 1024 + 64;
+  const uint64_t vn = (uint64_t)max_views * max_splats;
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) { const uint64_t o = off; off += al256(bytes); return o; };
+  const uint64_t o_cams = take(sizeof(DevCamera) * max_views);
+  const uint64_t o_proj = take(vn * sizeof(Projected));
+  const uint64_t o_pgrad = take(vn * sizeof(ProjectedGrad));
+  const uint64_t o_counts = take(vn * 4);
+  const uint64_t o_total = take(8);
+  const uint64_t o_seg = take(8 * (max_views + 1));
+  const uint64_t o_tb = take(4 * (max_views + 1));
+  const uint64_t o_flags = take(64);
+  const uint64_t o_step = take(8);
+  const uint64_t o_ka = take(pair_capacity * 8);
+  const uint64_t o_kb = take(pair_capacity * 8);
+  const uint64_t o_va = take(pair_capacity * 4);
+  const uint64_t o_vb = take(pair_capacity * 4);
+  const uint64_t o_rg = take(b->max_tiles * 8);
+  const uint64_t o_img = take(max_pixels * 16);
+  const uint64_t o_dl = take(max_pixels * 16);
+  const uint64_t o_ft = take(max_pixels * 4);
+  const uint64_t o_li = take(max_pixels * 4);
+  const uint64_t o_loss = take(8 * max_views);
+  const uint64_t o_scan = take(nts_scan_ws_bytes(vn));
+  const uint64_t o_sort = take(nts_sort_ws_bytes(max_views, pair_capacity, 8));
+  b->bytes = off;
+  int rc = nts_dev_alloc(ctx, &b->base, b->bytes);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  char* p = (char*)b->base;
+  b->d_cams = (DevCamera*)(p + o_cams);
+  b->proj = (Projected*)(p + o_proj);
+  b->pgrad = (ProjectedGrad*)(p + o_pgrad);
+  b->counts = (uint32_t*)(p + o_counts);
+  b->d_total = (uint64_t*)(p + o_total);
+  b->seg_begin = (uint64_t*)(p + o_seg);
+  b->tile_base = (uint32_t*)(p + o_tb);
+  b->flags = (uint32_t*)(p + o_flags);
+  b->d_step = (uint64_t*)(p + o_step);
+  b->keys_a = p + o_ka;
+  b->keys_b = p + o_kb;
+  b->vals_a = (uint32_t*)(p + o_va);
+  b->vals_b = (uint32_t*)(p + o_vb);
+  b->ranges = (uint2*)(p + o_rg);
+  b->image = (float4*)(p + o_img);
+  b->dl_dc = (float4*)(p + o_dl);
+  b->final_t = (float*)(p + o_ft);
+  b->last_idx = (uint32_t*)(p + o_li);
+  b->losses = (double*)(p + o_loss);
+  b->scan_ws = p + o_scan;
+  b->sort_ws = p + o_sort;
+  hipError_t e = hipHostMalloc((void**)&b->h_cams, sizeof(DevCamera) * NTS_MAX_VIEWS, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    nts_dev_free(ctx, b->base, b->bytes);
+    delete b;
+    return nts_fail(NTS_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e));
+  }
+  rc = nts_fill_u32(ctx, b->base, 0, b->bytes / 4);
+  if (rc) return rc;
+  *out = b;
+  return NTS_OK;
+}
+
+static void drop_graph(nts_batch* b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key.clear();
+}
+
+extern "C" int nts_batch_destroy(nts_ctx* ctx, nts_batch* b) {
+  if (!ctx || !b) return NTS_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(b);
+  if (b->h_cams) (void)hipHostFree(b->h_cams);
+  nts_dev_free(ctx, b->base, b->bytes);
+  delete b;
+  return NTS_OK;
+}
+
+struct BatchPlan {
+  uint32_t n_views, tile_size, n;
+  ViewTable vt;
+  TileTable tt;
+  uint32_t key_bits, key_bytes;
+  uint64_t pixels;
+};
+
+static int plan_batch(nts_batch* b, const nts_scene* scene, const nts_view_desc* views, uint32_t n_views,
+                      uint32_t tile_size, int loss_kind, BatchPlan& pl) {
+  if (n_views == 0 || n_views > b->max_views) return nts_fail(NTS_E_INVALID, "n_views %u (max %u)", n_views, b->max_views);
+  if (tile_size != 8 && tile_size != 16 && tile_size != 32)
+    return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
+  if (scene->n > b->max_splats) return nts_fail(NTS_E_CAPACITY, "scene has %u splats, batch holds %u", scene->n,
+                                                b->max_splats);
+  memset(&pl, 0, sizeof pl);
+  pl.n_views = n_views;
+  pl.tile_size = tile_size;
+  pl.n = scene->n;
+  pl.vt.n = n_views;
+  pl.tt.n = n_views;
+  uint64_t pix = 0, tiles = 0;
+  uint32_t max_tiles_view = 1;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    const uint32_t w = views[v].width, h = views[v].height;
+    if (w == 0 || h == 0) return nts_fail(NTS_E_INVALID, "view %u has an empty image", v);
+    if ((float)w != views[v].cam.width || (float)h != views[v].cam.height)
+      return nts_fail(NTS_E_INVALID, "view %u: camera size does not match %ux%u", v, w, h);
+    const uint32_t tx = (w + tile_size - 1) / tile_size, ty = (h + tile_size - 1) / tile_size;
+    ViewDesc& d = pl.vt.v[v];
+    d.width = w;
+    d.height = h;
+    d.tiles_x = tx;
+    d.tiles_y = ty;
+    d.tile_base = (uint32_t)tiles;
+    d.proj_base = v * scene->n;
+    d.pix_base = pix;
+    const float n_scalars = (float)w * (float)h * 3.0f;
+    d.inv_n = loss_kind == NTS_LOSS_MSE ? 2.0f / n_scalars : 1.0f / n_scalars;
+    pl.tt.t[v] = ViewTiles{tx, ty, tile_size, views[v].depth_max};
+    pix += (uint64_t)w * h;
+    tiles += (uint64_t)tx * ty;
+    if (tx * ty > max_tiles_view) max_tiles_view = tx * ty;
+  }
+  if (pix > b->max_pixels) return nts_fail(NTS_E_CAPACITY, "%llu pixels exceed the batch's %llu",
+                                           (unsigned long long)pix, (unsigned long long)b->max_pixels);
+  if (tiles > b->max_tiles) return nts_fail(NTS_E_CAPACITY, "%llu tiles exceed the batch's %llu",
+                                            (unsigned long long)tiles, (unsigned long long)b->max_tiles);
+  if ((uint64_t)n_views * scene->n >= (1ull << 32)) return nts_fail(NTS_E_CAPACITY, "views x splats >= 2^32");
+  pl.vt.total_tiles = (uint32_t)tiles;
+  pl.pixels = pix;
+  uint32_t tb = 1;
+  while (tb < 32 && (1ull << tb) < max_tiles_view) ++tb;
+  pl.key_bits = 16 + tb;
+  pl.key_bytes = pl.key_bits <= 32 ? 4 : 8;
+  return NTS_OK;
+}
+
+// Forward half of a step: project, bin (segmented by view), blend.
+static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, const BatchPlan& pl,
+                          const LossArgs* loss, bool keep_state, bool** alt_out) {
+  const uint32_t n = pl.n, V = pl.n_views;
+  const uint64_t vn = (uint64_t)V * n;
+  int rc = nts_fill_u32(ctx, b->flags, 0, 16);
+  if (rc) return rc;
+  if (n > 0)
+    NTS_LAUNCH(ctx, "project_views", k_project_views, dim3(nts_div_up(n, 256)), dim3(256), 0, scene->params,
+               scene->cap, n, (const DevCamera*)b->d_cams, pl.tt, b->proj, b->counts, b->flags + 1);
+  rc = nts_scan_u32_ws(ctx, b->counts, b->counts, vn, b->d_total, b->scan_ws);
+  if (rc) return rc;
+  NTS_LAUNCH(ctx, "view_segments", k_view_segments, dim3(1), dim3(64), 0, (const uint32_t*)b->counts, n, V,
+             (const uint64_t*)b->d_total, b->pair_cap, b->seg_begin, b->flags);
+  bool alt = false;
+  if (n > 0) {
+    if (pl.key_bytes == 4)
+      NTS_LAUNCH(ctx, "emit_views", k_emit_views<uint32_t>, dim3(nts_div_up(n, 256), V), dim3(256), 0,
+                 (const Projected*)b->proj, n, pl.tt, (const uint32_t*)b->counts, b->pair_cap, (uint32_t*)b->keys_a,
+                 b->vals_a);
+    else
+      NTS_LAUNCH(ctx, "emit_views", k_emit_views<uint64_t>, dim3(nts_div_up(n, 256), V), dim3(256), 0,
+                 (const Projected*)b->proj, n, pl.tt, (const uint32_t*)b->counts, b->pair_cap, (uint64_t*)b->keys_a,
+                 b->vals_a);
+  }
+  if (pl.key_bytes == 4)
+    rc = nts_sort_ws_launch32(ctx, (uint32_t*)b->keys_a, b->vals_a, (uint32_t*)b->keys_b, b->vals_b, b->seg_begin, V,
+                              b->pair_cap, pl.key_bits, b->sort_ws, &alt, &b->sort_err);
+  else
+    rc = nts_sort_ws_launch64(ctx, (uint64_t*)b->keys_a, b->vals_a, (uint64_t*)b->keys_b, b->vals_b, b->seg_begin, V,
+                              b->pair_cap, pl.key_bits, b->sort_ws, &alt, &b->sort_err);
+  if (rc) return rc;
+  void* keys = alt ? b->keys_b : b->keys_a;
+  uint32_t* pay = alt ? b->vals_b : b->vals_a;
+  rc = nts_fill_u32(ctx, b->ranges, 0, (uint64_t)pl.vt.total_tiles * 2);
+  if (rc) return rc;
+  {
+    U32Table tb;
+    memset(&tb, 0, sizeof tb);
+    tb.n = V + 1;
+    for (uint32_t v = 0; v <= V; ++v) tb.v[v] = v < V ? pl.vt.v[v].tile_base : pl.vt.total_tiles;
+    NTS_LAUNCH(ctx, "set_tile_base", k_set_u32s, dim3(1), dim3(64), 0, b->tile_base, tb);
+  }
+  uint32_t grid = nts_div_up(b->pair_cap, 256);
+  if (grid > 8192) grid = 8192;
+  if (pl.key_bytes == 4)
+    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint32_t>, dim3(grid), dim3(256), 0, (const uint32_t*)keys,
+               (const uint64_t*)b->seg_begin, V, (const uint32_t*)b->tile_base, b->ranges);
+  else
+    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)keys,
+               (const uint64_t*)b->seg_begin, V, (const uint32_t*)b->tile_base, b->ranges);
+  rc = nts_blend_forward(ctx, pl.tile_size, b->proj, pay, b->ranges, pl.vt, b->image,
+                         keep_state ? b->final_t : nullptr, keep_state ? b->last_idx : nullptr, loss);
+  if (rc) return rc;
+  if (alt_out) **alt_out = alt;
+  b->last_alt = alt;
+  b->last_key_bytes = pl.key_bytes;
+  for (uint32_t v = 0; v < V; ++v) b->last_tile_base[v] = pl.vt.v[v].tile_base;
+  b->last_tile_base[V] = pl.vt.total_tiles;
+  return NTS_OK;
+}
+
+static void remember_shape(nts_batch* b, const BatchPlan& pl) {
+  b->n_views = pl.n_views;
+  for (uint32_t v = 0; v < pl.n_views; ++v) {
+    b->pix_base[v] = pl.vt.v[v].pix_base;
+    b->width[v] = pl.vt.v[v].width;
+    b->height[v] = pl.vt.v[v].height;
+  }
+  b->pix_base[pl.n_views] = pl.pixels;
+}
+
+static int upload_cams(nts_ctx* ctx, nts_batch* b, const nts_view_desc* views, uint32_t V) {
+  for (uint32_t v = 0; v < V; ++v) memcpy(&b->h_cams[v], &views[v].cam, sizeof(DevCamera));
+  NTS_HIP_TRY(hipMemcpyAsync(b->d_cams, b->h_cams, sizeof(DevCamera) * V, hipMemcpyHostToDevice, ctx->stream));
+  return NTS_OK;
+}
+
+static int record_step(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const BatchPlan& pl, const nts_buffer* targets,
+                       int loss_kind, int mode, const nts_adam_cfg* cp, const nts_adam_cfg* ca,
+                       const nts_constraints* cons, int write_grads) {
+  const uint32_t n = pl.n, V = pl.n_views;
+  int rc = nts_fill_u32(ctx, b->losses, 0, 2 * (uint64_t)V);
+  if (rc) return rc;
+  rc = nts_fill_u32(ctx, b->pgrad, 0, (uint64_t)V * n * (sizeof(ProjectedGrad) / 4));
+  if (rc) return rc;
+  LossArgs la{(const float4*)targets->ptr, b->dl_dc, b->losses, loss_kind};
+  bool alt = false;
+  bool* altp = &alt;
+  rc = launch_forward(ctx, scene, b, pl, &la, mode == NTS_BWD_RECOMPUTE, &altp);
+  if (rc) return rc;
+  const uint32_t* pay = alt ? b->vals_b : b->vals_a;
+  rc = nts_blend_backward(ctx, pl.tile_size, mode, b->proj, pay, b->ranges, pl.vt, b->image, b->final_t, b->last_idx,
+                          b->dl_dc, b->pgrad);
+  if (rc) return rc;
+  AdamCfgs cfg;
+  cfg.lr_pos = cp->lr; cfg.b1_pos = cp->beta1; cfg.b2_pos = cp->beta2; cfg.eps_pos = cp->eps;
+  cfg.lr_attr = ca->lr; cfg.b1_attr = ca->beta1; cfg.b2_attr = ca->beta2; cfg.eps_attr = ca->eps;
+  cfg.smin = cons->log_scale_min; cfg.smax = cons->log_scale_max; cfg.omax = cons->logit_max;
+  if (n > 0) {
+    if (mode == NTS_BWD_RECOMPUTE)
+      NTS_LAUNCH(ctx, "backward_adam_views", k_backward_adam_views<true>, dim3(nts_div_up(n, 256)), dim3(256), 0,
+                 scene->params, scene->grads, scene->m, scene->v, scene->grad_acc, scene->cap, n, V,
+                 (const DevCamera*)b->d_cams, (const ProjectedGrad*)b->pgrad, (const uint64_t*)b->d_step, cfg,
+                 write_grads);
+    else
+      NTS_LAUNCH(ctx, "backward_adam_views", k_backward_adam_views<false>, dim3(nts_div_up(n, 256)), dim3(256), 0,
+                 scene->params, scene->grads, scene->m, scene->v, scene->grad_acc, scene->cap, n, V,
+                 (const DevCamera*)b->d_cams, (const ProjectedGrad*)b->pgrad, (const uint64_t*)b->d_step, cfg,
+                 write_grads);
+  }
+  return NTS_OK;
+}
+
+extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const nts_view_desc* views,
+                                    uint32_t n_views, const nts_buffer* targets, uint32_t tile_size, int loss_kind,
+                                    int bwd_mode, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr,
+                                    uint64_t t, const nts_constraints* cons, int flags) {
+  NTS_CHECK_ARG(ctx && scene && b && views && targets && cfg_pos && cfg_attr && cons && t > 0);
+  NTS_CHECK_ARG(loss_kind == NTS_LOSS_L1 || loss_kind == NTS_LOSS_MSE);
+  NTS_CHECK_ARG(bwd_mode == NTS_BWD_REF_PARITY || bwd_mode == NTS_BWD_RECOMPUTE);
+  BatchPlan pl;
+  int rc = plan_batch(b, scene, views, n_views, tile_size, loss_kind, pl);
+  if (rc) return rc;
+  NTS_CHECK_ARG(targets->bytes >= pl.pixels * 16);
+  remember_shape(b, pl);
+  const int write_grads = (flags & NTS_STEP_WRITE_GRADS) ? 1 : 0;
+  const bool use_graph = (flags & NTS_STEP_GRAPH) != 0 && !ctx->prof;
+  if (!use_graph) {
+    b->graph_key.clear();  // cameras and step counter change under a captured graph
+    rc = upload_cams(ctx, b, views, n_views);
+    if (rc) return rc;
+    NTS_LAUNCH(ctx, "set_step", k_set_step, dim3(1), dim3(64), 0, b->d_step, t);
+    return record_step(ctx, scene, b, pl, targets, loss_kind, bwd_mode, cfg_pos, cfg_attr, cons, write_grads);
+  }
+  // Captured replay: the key holds everything baked into the graph.
+  std::vector<uint8_t> key;
+  auto put = [&](const void* p, size_t nbytes) {
+    const uint8_t* c = (const uint8_t*)p;
+    key.insert(key.end(), c, c + nbytes);
+  };
+  const void* ids[3] = {scene, targets->ptr, scene->params};
+  put(ids, sizeof ids);
+  put(&scene->n, 4);
+  put(views, sizeof(nts_view_desc) * n_views);
+  const int meta[4] = {(int)tile_size, loss_kind, bwd_mode, write_grads};
+  put(meta, sizeof meta);
+  put(cfg_pos, sizeof *cfg_pos);
+  put(cfg_attr, sizeof *cfg_attr);
+  put(cons, sizeof *cons);
+  if (key != b->graph_key || t != b->graph_last_t + 1) {
+    // (re)capture: cameras and the step counter are set outside the graph
+    drop_graph(b);
+    rc = upload_cams(ctx, b, views, n_views);
+    if (rc) return rc;
+    NTS_LAUNCH(ctx, "set_step", k_set_step, dim3(1), dim3(64), 0, b->d_step, t - 1);
+    NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    NTS_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc2 = NTS_OK;
+    {
+      PendingEvent pe;
+      (void)pe;
+      hipLaunchKernelGGL(k_inc_step, dim3(1), dim3(64), 0, ctx->stream, b->d_step);
+      rc2 = record_step(ctx, scene, b, pl, targets, loss_kind, bwd_mode, cfg_pos, cfg_attr, cons, write_grads);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc2) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc2;
+    }
+    if (e != hipSuccess) return nts_fail(NTS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(b);
+      return nts_fail(NTS_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    b->graph_key = key;
+  }
+  NTS_HIP_TRY(hipGraphLaunch(b->graph_exec, ctx->stream));
+  b->graph_last_t = t;
+  return NTS_OK;
+}
+
+extern "C" int nts_render_views(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, const nts_view_desc* views,
+                                uint32_t n_views, uint32_t tile_size) {
+  NTS_CHECK_ARG(ctx && scene && b && views);
+  BatchPlan pl;
+  int rc = plan_batch(b, scene, views, n_views, tile_size, NTS_LOSS_L1, pl);
+  if (rc) return rc;
+  remember_shape(b, pl);
+  b->graph_key.clear();
+  rc = upload_cams(ctx, b, views, n_views);
+  if (rc) return rc;
+  return launch_forward(ctx, scene, b, pl, nullptr, true, nullptr);
+}
+
+extern "C" int nts_batch_losses(nts_ctx* ctx, const nts_batch* b, double* losses) {
+  NTS_CHECK_ARG(ctx && b && losses);
+  NTS_HIP_TRY(hipMemcpyAsync(losses, b->losses, 8 * (size_t)b->n_views, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (uint32_t v = 0; v < b->n_views; ++v) {
+    const uint64_t np = b->pix_base[v + 1] - b->pix_base[v];
+    losses[v] = losses[v] / ((double)np * 3.0);
+  }
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_stats(nts_ctx* ctx, const nts_batch* b, uint64_t* total_pairs, uint32_t* visible,
+                               uint32_t* overflow) {
+  NTS_CHECK_ARG(ctx && b);
+  uint32_t f[4] = {0, 0, 0, 0};
+  NTS_HIP_TRY(hipMemcpyAsync(f, b->flags, 16, hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t err = 0;
+  if (b->sort_err) NTS_HIP_TRY(hipMemcpyAsync(&err, b->sort_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (err) return nts_fail(NTS_E_HIP, "radix sort look-back timed out");
+  if (total_pairs) *total_pairs = (uint64_t)f[2] | ((uint64_t)f[3] << 32);
+  if (visible) *visible = f[1];
+  if (overflow) *overflow = f[0];
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_download_image(nts_ctx* ctx, const nts_batch* b, uint32_t view, float* host) {
+  NTS_CHECK_ARG(ctx && b && host && view < b->n_views);
+  const uint64_t np = b->pix_base[view + 1] - b->pix_base[view];
+  NTS_HIP_TRY(hipMemcpyAsync(host, b->image + b->pix_base[view], np * 16, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_download_state(nts_ctx* ctx, const nts_batch* b, uint32_t view, float* final_t,
+                                        uint32_t* last_idx) {
+  NTS_CHECK_ARG(ctx && b && view < b->n_views);
+  const uint64_t np = b->pix_base[view + 1] - b->pix_base[view];
+  if (final_t)
+    NTS_HIP_TRY(hipMemcpyAsync(final_t, b->final_t + b->pix_base[view], np * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (last_idx)
+    NTS_HIP_TRY(
+        hipMemcpyAsync(last_idx, b->last_idx + b->pix_base[view], np * 4, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_download_projected_grad(nts_ctx* ctx, const nts_batch* b, uint32_t view, uint32_t n,
+                                                 float* host) {
+  NTS_CHECK_ARG(ctx && b && host && view < b->n_views);
+  NTS_HIP_TRY(hipMemcpyAsync(host, b->pgrad + (size_t)view * n, (size_t)n * sizeof(ProjectedGrad),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_download_bins(nts_ctx* ctx, const nts_batch* b, uint32_t view, uint64_t* count,
+                                       uint32_t* keys, uint32_t* payloads, uint32_t* ranges) {
+  NTS_CHECK_ARG(ctx && b && count && view < b->n_views);
+  uint64_t sb[2];
+  NTS_HIP_TRY(hipMemcpyAsync(sb, b->seg_begin + view, 16, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const uint64_t k = sb[1] - sb[0];
+  *count = k;
+  const void* kbase = b->last_alt ? b->keys_b : b->keys_a;
+  const uint32_t* pbase = b->last_alt ? b->vals_b : b->vals_a;
+  if (keys && k) {
+    if (b->last_key_bytes == 4) {
+      NTS_HIP_TRY(hipMemcpyAsync(keys, (const uint32_t*)kbase + sb[0], k * 4, hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+      std::vector<uint64_t> tmp(k);
+      NTS_HIP_TRY(hipMemcpyAsync(tmp.data(), (const uint64_t*)kbase + sb[0], k * 8, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+      NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+      for (uint64_t i = 0; i < k; ++i) {
+        if (tmp[i] >> 32) return nts_fail(NTS_E_CAPACITY, "tile id does not fit the 16-bit reference key");
+        keys[i] = (uint32_t)tmp[i];
+      }
+    }
+  }
+  if (payloads && k)
+    NTS_HIP_TRY(hipMemcpyAsync(payloads, pbase + sb[0], k * 4, hipMemcpyDeviceToHost, ctx->stream));
+  const uint32_t t0 = b->last_tile_base[view], t1 = b->last_tile_base[view + 1];
+  if (ranges)
+    NTS_HIP_TRY(hipMemcpyAsync(ranges, b->ranges + t0, (uint64_t)(t1 - t0) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ranges)  // relative to the view's segment, as a single-view bin
+    for (uint32_t t = 0; t < t1 - t0; ++t)
+      if (ranges[2 * t + 1] > ranges[2 * t]) {
+        ranges[2 * t] -= (uint32_t)sb[0];
+        ranges[2 * t + 1] -= (uint32_t)sb[0];
+      }
+  return NTS_OK;
+}

@@ -102,12 +102,25 @@ __device__ __forceinline__ bool wave_misses(const WaveRect& r, float4 a, float4 
 // ---------------------------------------------------------------------------
 // forward (rasterize.wgsl:48-118)
 // ---------------------------------------------------------------------------
-template <int TS, int BATCH>
+// dL/dC of one pixel (L1 with sign(0) = 0, or MSE as loss.rs:163-179) and
+// its loss term, with exactly the arithmetic of the standalone loss kernel.
+__device__ __forceinline__ float4 pixel_loss_grad(float4 p, float4 t, float inv_n, int kind, double& acc) {
+  const float d0 = p.x - t.x, d1 = p.y - t.y, d2 = p.z - t.z;
+  if (kind == NTS_LOSS_MSE) {
+    acc += (double)(d0 * d0 + d1 * d1 + d2 * d2);
+    return make_float4(d0 * inv_n, d1 * inv_n, d2 * inv_n, 0.0f);
+  }
+  acc += (double)fabsf(d0) + (double)fabsf(d1) + (double)fabsf(d2);
+  return make_float4(d0 > 0.f ? inv_n : (d0 < 0.f ? -inv_n : 0.f), d1 > 0.f ? inv_n : (d1 < 0.f ? -inv_n : 0.f),
+                     d2 > 0.f ? inv_n : (d2 < 0.f ? -inv_n : 0.f), 0.0f);
+}
+
+template <int TS, int BATCH, bool LOSS>
 __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __restrict__ proj,
                                                              const uint32_t* __restrict__ payloads,
                                                              const uint2* __restrict__ ranges, ViewTable vt,
                                                              float4* __restrict__ out, float* __restrict__ final_t,
-                                                             uint32_t* __restrict__ last_idx) {
+                                                             uint32_t* __restrict__ last_idx, LossArgs la) {
   constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT;
   __shared__ float4 s_a[BATCH];  // mean.x, mean.y, conic.a, conic.b
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
@@ -198,13 +211,29 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
       }
     }
   }
+  double lacc = 0.0;
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     if (!inb[j]) continue;
     const uint64_t pix = vd.pix_base + (uint64_t)pix_y[j] * vd.width + pix_x[j];
-    out[pix] = make_float4(C0[j], C1[j], C2[j], 1.0f - T[j]);
+    const float4 px = make_float4(C0[j], C1[j], C2[j], 1.0f - T[j]);
+    if (out) out[pix] = px;
     if (final_t) final_t[pix] = T[j];
     if (last_idx) last_idx[pix] = last[j];
+    if (LOSS) la.dl_dc[pix] = pixel_loss_grad(px, la.target[pix], vd.inv_n, la.kind, lacc);
+  }
+  if (LOSS) {
+    __shared__ double s_loss[NT / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) lacc += __shfl_xor(lacc, off, 64);
+    if ((threadIdx.x & 63u) == 0) s_loss[threadIdx.x >> 6] = lacc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) sum += s_loss[w];
+      unsafeAtomicAdd(la.losses + vi, sum);
+    }
   }
 }
 
@@ -438,25 +467,29 @@ static int fill_single_view(const nts_bins* bins, const nts_tiling* t, ViewTable
 }
 
 int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, const Projected* proj, const uint32_t* payloads,
-                      const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx) {
+                      const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx,
+                      const LossArgs* loss) {
   if (vt.total_tiles == 0) return NTS_OK;
   const dim3 grid(vt.total_tiles);
+  LossArgs la{};
+  if (loss) la = *loss;
+#define NTS_FWD_CASE(TSZ)                                                                                        \
+  case TSZ:                                                                                                      \
+    if (loss)                                                                                                    \
+      NTS_LAUNCH(ctx, "blend_fwd_loss", (k_blend_fwd<TSZ, NTS_BATCH, true>), grid, dim3(Blend<TSZ>::NT), 0, proj, \
+                 payloads, ranges, vt, out, final_t, last_idx, la);                                              \
+    else                                                                                                         \
+      NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<TSZ, NTS_BATCH, false>), grid, dim3(Blend<TSZ>::NT), 0, proj,     \
+                 payloads, ranges, vt, out, final_t, last_idx, la);                                              \
+    break;
   switch (tile_size) {
-    case 8:
-      NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<8, NTS_BATCH>), grid, dim3(Blend<8>::NT), 0, proj, payloads, ranges,
-                 vt, out, final_t, last_idx);
-      break;
-    case 16:
-      NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<16, NTS_BATCH>), grid, dim3(Blend<16>::NT), 0, proj, payloads,
-                 ranges, vt, out, final_t, last_idx);
-      break;
-    case 32:
-      NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<32, NTS_BATCH>), grid, dim3(Blend<32>::NT), 0, proj, payloads,
-                 ranges, vt, out, final_t, last_idx);
-      break;
+    NTS_FWD_CASE(8)
+    NTS_FWD_CASE(16)
+    NTS_FWD_CASE(32)
     default:
       return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
   }
+#undef NTS_FWD_CASE
   return NTS_OK;
 }
 
@@ -497,7 +530,7 @@ extern "C" int nts_composite_ex(nts_ctx* ctx, const nts_buffer* projected, const
   if (rc) return rc;
   return nts_blend_forward(ctx, tiling->tile_size, (const Projected*)projected->ptr, bins->payloads, bins->ranges, vt,
                            (float4*)image->ptr, final_t ? (float*)final_t->ptr : nullptr,
-                           last_idx ? (uint32_t*)last_idx->ptr : nullptr);
+                           last_idx ? (uint32_t*)last_idx->ptr : nullptr, nullptr);
 }
 
 extern "C" int nts_composite(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,

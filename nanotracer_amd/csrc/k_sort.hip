@@ -89,23 +89,32 @@ __global__ void __launch_bounds__(256) k_scan_down(const uint32_t* __restrict__ 
   }
 }
 
-int nts_scan_u32_dev(nts_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total) {
+uint64_t nts_scan_ws_bytes(uint64_t n) {
+  const uint64_t nblocks = (n + SC_TILE - 1) / SC_TILE;
+  return ((nblocks * 4 + 255) & ~255ull) + nblocks * 8 + 256;
+}
+
+int nts_scan_u32_ws(nts_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, void* ws) {
   if (n == 0) {
     if (d_total) return nts_fill_u32(ctx, d_total, 0, 2);
     return NTS_OK;
   }
   const uint32_t nblocks = nts_div_up(n, SC_TILE);
-  void* scr;
   const uint64_t sums_bytes = ((uint64_t)nblocks * 4 + 255) & ~255ull;
-  int rc = nts_scratch(ctx, sums_bytes + (uint64_t)nblocks * 8 + 256, &scr);
-  if (rc) return rc;
-  uint32_t* sums = (uint32_t*)scr;
-  unsigned long long* tot64 = (unsigned long long*)((char*)scr + sums_bytes);
+  uint32_t* sums = (uint32_t*)ws;
+  unsigned long long* tot64 = (unsigned long long*)((char*)ws + sums_bytes);
   NTS_LAUNCH(ctx, "scan_reduce", k_scan_reduce, dim3(nblocks), dim3(256), 0, in, n, sums, tot64);
   NTS_LAUNCH(ctx, "scan_top", k_scan_top, dim3(1), dim3(1024), 0, sums, nblocks,
              (const unsigned long long*)tot64, d_total);
   NTS_LAUNCH(ctx, "scan_down", k_scan_down, dim3(nblocks), dim3(256), 0, in, out, n, (const uint32_t*)sums);
   return NTS_OK;
+}
+
+int nts_scan_u32_dev(nts_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total) {
+  void* scr;
+  int rc = nts_scratch(ctx, nts_scan_ws_bytes(n), &scr);
+  if (rc) return rc;
+  return nts_scan_u32_ws(ctx, in, out, n, d_total, scr);
 }
 
 extern "C" int nts_scan_u32(nts_ctx* ctx, nts_buffer* data, uint64_t n) {
@@ -308,42 +317,43 @@ __global__ void __launch_bounds__(RS_BLOCK) k_onesweep(const K* __restrict__ key
   }
 }
 
-// Sort pairs of segments in place (result in keys/vals, or in the alt buffers
-// when *result_in_alt is set).  seg_begin (device, num_segs + 1 entries)
-// bounds the segments; max_pairs bounds seg_begin[num_segs].  Only the low
-// ceil(key_bits / 8) * 8 bits are sorted.
+// Workspace of the segmented sort, sized for max_pairs pairs in num_segs
+// segments and keys of key_bytes bytes (so for up to key_bytes passes).
+uint64_t nts_sort_ws_bytes(uint32_t num_segs, uint64_t max_pairs, uint32_t key_bytes) {
+  const uint64_t max_parts = (max_pairs + RS_TILE - 1) / RS_TILE + num_segs;
+  const uint64_t n_hist = (uint64_t)num_segs * key_bytes * RS_RADIX;
+  return 4 * (256 + 2 * n_hist + 64 + (uint64_t)key_bytes * max_parts * RS_RADIX);
+}
+
+// Launch the sort of pairs of segments (result in keys/vals, or in the alt
+// buffers when *result_in_alt is set) with no host synchronisation.
+// seg_begin (device, num_segs + 1 entries) bounds the segments; max_pairs
+// bounds seg_begin[num_segs].  Only the low ceil(key_bits / 8) * 8 bits are
+// sorted.  A look-back that never completes sets *err (checked by callers).
 template <typename K>
-static int sort_segmented(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
-                          const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
-                          bool* result_in_alt) {
+static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                             const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
+                             void* ws, bool* result_in_alt, uint32_t** err_out) {
   *result_in_alt = false;
-  if (max_pairs == 0) return NTS_OK;
   if (num_segs == 0 || num_segs > RS_MAX_SEGS) return nts_fail(NTS_E_INVALID, "segments %u", num_segs);
   if (max_pairs >= (1ull << 30)) return nts_fail(NTS_E_CAPACITY, "sort supports < 2^30 pairs (%llu)",
                                                  (unsigned long long)max_pairs);
   uint32_t passes = (key_bits + 7) / 8;
-  if (passes == 0) return NTS_OK;
   if (passes > sizeof(K)) passes = sizeof(K);
   const uint32_t max_parts = nts_div_up(max_pairs, RS_TILE) + num_segs;
-  // scratch: part_begin | hist | offsets | counters + err | status
-  const uint64_t n_pb = 256;
-  const uint64_t n_hist = (uint64_t)num_segs * passes * RS_RADIX;
-  const uint64_t n_cnt = 64;
-  const uint64_t n_status = (uint64_t)passes * max_parts * RS_RADIX;
-  const uint64_t words = n_pb + 2 * n_hist + n_cnt + n_status;
-  void* scr;
-  int rc = nts_scratch(ctx, words * 4, &scr);
-  if (rc) return rc;
-  uint32_t* part_begin = (uint32_t*)scr;
-  uint32_t* hist = part_begin + n_pb;
+  const uint64_t n_hist = (uint64_t)num_segs * sizeof(K) * RS_RADIX;
+  uint32_t* part_begin = (uint32_t*)ws;
+  uint32_t* hist = part_begin + 256;
   uint32_t* offsets = hist + n_hist;
   uint32_t* counters = offsets + n_hist;
   uint32_t* err = counters + 32;
-  uint32_t* status = counters + n_cnt;
-  rc = nts_fill_u32(ctx, hist, 0, n_hist);
+  uint32_t* status = counters + 64;
+  if (err_out) *err_out = err;
+  int rc = nts_fill_u32(ctx, hist, 0, n_hist);
   if (rc) return rc;
-  rc = nts_fill_u32(ctx, counters, 0, n_cnt + n_status);
+  rc = nts_fill_u32(ctx, counters, 0, 64 + (uint64_t)passes * max_parts * RS_RADIX);
   if (rc) return rc;
+  if (passes == 0 || max_pairs == 0) return NTS_OK;
   NTS_LAUNCH(ctx, "sort_parts", k_sort_parts, dim3(1), dim3(64), 0, d_seg_begin, num_segs, part_begin);
   NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(max_parts), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
              (const uint32_t*)part_begin, num_segs, passes, hist);
@@ -361,7 +371,22 @@ static int sort_segmented(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, ui
     uint32_t* tv = src_v; src_v = dst_v; dst_v = tv;
   }
   *result_in_alt = (passes & 1u) != 0;
-  // look-back timeout flag (never expected): read it back only in checked mode
+  return NTS_OK;
+}
+
+template <typename K>
+static int sort_segmented(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                          const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
+                          bool* result_in_alt) {
+  *result_in_alt = false;
+  if (max_pairs == 0) return NTS_OK;
+  void* scr;
+  int rc = nts_scratch(ctx, nts_sort_ws_bytes(num_segs, max_pairs, sizeof(K)), &scr);
+  if (rc) return rc;
+  uint32_t* err = nullptr;
+  rc = sort_segmented_ws<K>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, key_bits, scr,
+                            result_in_alt, &err);
+  if (rc) return rc;
   uint32_t* hw = ctx->host_word;
   NTS_HIP_TRY(hipMemcpyAsync(hw, err, 4, hipMemcpyDeviceToHost, ctx->stream));
   NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -381,6 +406,20 @@ int nts_sort_pairs32_dev(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t*
                          bool* result_in_alt) {
   return sort_segmented<uint32_t>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, key_bits,
                                   result_in_alt);
+}
+
+int nts_sort_ws_launch32(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                         const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
+                         void* ws, bool* result_in_alt, uint32_t** err) {
+  return sort_segmented_ws<uint32_t>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, key_bits,
+                                     ws, result_in_alt, err);
+}
+
+int nts_sort_ws_launch64(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                         const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
+                         void* ws, bool* result_in_alt, uint32_t** err) {
+  return sort_segmented_ws<uint64_t>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, key_bits,
+                                     ws, result_in_alt, err);
 }
 
 template <typename K>

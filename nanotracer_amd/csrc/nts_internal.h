@@ -48,6 +48,7 @@ struct DevCamera {  // the 96-byte camera, passed by value to kernels
 };
 static_assert(sizeof(DevCamera) == 96, "camera layout");
 static_assert(sizeof(nts_camera) == 96, "camera layout");
+static_assert(sizeof(nts_view_desc) == 112, "view layout");
 
 struct ProfEntry {
   std::string name;
@@ -160,3 +161,13 @@ int nts_sort_pairs_dev(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* k
 int nts_sort_pairs32_dev(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
                          const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
                          bool* result_in_alt);
+// Workspace variants: no allocation and no host synchronisation (capturable).
+uint64_t nts_scan_ws_bytes(uint64_t n);
+int nts_scan_u32_ws(nts_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* d_total, void* ws);
+uint64_t nts_sort_ws_bytes(uint32_t num_segs, uint64_t max_pairs, uint32_t key_bytes);
+int nts_sort_ws_launch32(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                         const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
+                         void* ws, bool* result_in_alt, uint32_t** err);
+int nts_sort_ws_launch64(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                         const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
+                         void* ws, bool* result_in_alt, uint32_t** err);

@@ -12,6 +12,17 @@ struct ViewDesc {
   uint32_t tile_base;   // first global tile id of the view
   uint32_t proj_base;   // offset of the view's records in projected[] / projected_grad[]
   uint64_t pix_base;    // offset of the view's first pixel in the image arrays
+  float inv_n;          // loss scale: 1/(3WH) for L1, 2/(3WH) for MSE
+  uint32_t pad;
+};
+
+// Optional loss epilogue of the forward blend: dL/dC and the per-view loss
+// sum (the reference computes these on the host, train.rs:180-196).
+struct LossArgs {
+  const float4* target;  // float4 per pixel (w ignored); nullptr = no loss
+  float4* dl_dc;
+  double* losses;        // per view, accumulated
+  int kind;              // NTS_LOSS_L1 or NTS_LOSS_MSE
 };
 
 // Passed by value as a kernel argument (kernarg segment), so launches need
@@ -55,3 +66,11 @@ __device__ __forceinline__ float nts_expf_nonpos(float x) {
   }
   return p * nts_pow2i(k);
 }
+
+// Host-side launchers of the blend kernels (k_raster.hip).
+int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, const Projected* proj, const uint32_t* payloads,
+                      const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx,
+                      const LossArgs* loss);
+int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, int mode, const Projected* proj, const uint32_t* payloads,
+                       const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
+                       const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad);
