@@ -39,6 +39,7 @@ N_SPLATS = 1_000_000
 N_VIEWS = 16
 SEED = 0x5EED0002
 TILE = 16
+LDS_BATCH = (256, 256)
 
 
 def make_views(n):
@@ -170,6 +171,7 @@ def run_batched(args, ctx):
     g = GpuSplatBuffer.upload(ctx, s)
     cap = int(max(pairs_t, 1) * 1.3) + 1_000_000
     b = ViewBatch(ctx, N_VIEWS, s.n, pixels, cap)
+    b.set_lds_batch(*LDS_BATCH)
     mode = L.BWD_RECOMPUTE if args.backward == "recompute" else L.BWD_REF_PARITY
     t = 0
     for _ in range(max(args.warmup, 1)):
@@ -203,7 +205,8 @@ def run_batched(args, ctx):
         ms_per_step=round(dt * 1e3 / args.steps, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32", data="synthetic",
         config=dict(workload="batched training iteration: 1M splats, 16 views of mixed resolution per launch, "
-                             "hipGraph replay", splats=s.n, views=N_VIEWS, pixels=pixels, tile=TILE, pairs=pairs,
+                             "hipGraph replay", splats=s.n, views=N_VIEWS, pixels=pixels, tile=TILE,
+                    lds_batch=dict(forward=LDS_BATCH[0], backward=LDS_BATCH[1]), pairs=pairs,
                     visible_records=visible, backward=args.backward, loss="L1 summed over views"),
         views_per_s=round(iters_s * N_VIEWS, 2), train_mpx_s=round(iters_s * pixels / 1e6, 2),
         final_loss=float(losses.sum()),
@@ -278,9 +281,11 @@ def main():
                     help="reference-parity backward or the recompute (final-T, last-index) backward")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, choices=(8, 16, 32), default=16)
+    ap.add_argument("--lds-batch", default="256,256", help="blend forward,backward LDS batch")
     args = ap.parse_args()
-    global TILE
+    global TILE, LDS_BATCH
     TILE = args.tile
+    LDS_BATCH = tuple(int(x) for x in args.lds_batch.split(","))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is view-parallel with one gradient reduce-scatter); run with --gpus 1",

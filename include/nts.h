@@ -250,6 +250,11 @@ int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* batch, const
 /* Forward only (project, bin, blend) of n_views views into the batch image. */
 int nts_render_views(nts_ctx* ctx, const nts_scene* scene, nts_batch* batch, const nts_view_desc* views,
                      uint32_t n_views, uint32_t tile_size);
+/* Splats staged per LDS batch by the blend forward (64, 128, 256 or 512)
+ * and backward (64, 128 or 256) of this batch's steps; 256 / 256 by
+ * default.  No reference counterpart: the reference stages 256
+ * (rasterize.wgsl:40). */
+int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* batch, uint32_t fwd_batch, uint32_t bwd_batch);
 int nts_batch_losses(nts_ctx* ctx, const nts_batch* batch, double* losses);
 int nts_batch_stats(nts_ctx* ctx, const nts_batch* batch, uint64_t* total_pairs, uint32_t* visible,
                     uint32_t* overflow);

@@ -67,6 +67,8 @@ struct nts_batch {
   bool last_alt = false;
   uint32_t last_key_bytes = 4;
   uint32_t last_tile_base[NTS_MAX_VIEWS + 1] = {};
+  // splats staged per LDS batch by the blend kernels
+  uint32_t fwd_batch = NTS_DEFAULT_LDS_BATCH, bwd_batch = NTS_DEFAULT_LDS_BATCH;
 };
 
 struct U32Table {
@@ -447,7 +449,7 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
   else
     NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)keys,
                (const uint64_t*)b->seg_begin, V, (const uint32_t*)b->tile_base, b->ranges);
-  rc = nts_blend_forward(ctx, pl.tile_size, b->proj, pay, b->ranges, pl.vt, b->image,
+  rc = nts_blend_forward(ctx, pl.tile_size, b->fwd_batch, b->proj, pay, b->ranges, pl.vt, b->image,
                          keep_state ? b->final_t : nullptr, keep_state ? b->last_idx : nullptr, loss);
   if (rc) return rc;
   if (alt_out) **alt_out = alt;
@@ -488,8 +490,8 @@ static int record_step(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const Batch
   rc = launch_forward(ctx, scene, b, pl, &la, mode == NTS_BWD_RECOMPUTE, &altp);
   if (rc) return rc;
   const uint32_t* pay = alt ? b->vals_b : b->vals_a;
-  rc = nts_blend_backward(ctx, pl.tile_size, mode, b->proj, pay, b->ranges, pl.vt, b->image, b->final_t, b->last_idx,
-                          b->dl_dc, b->pgrad);
+  rc = nts_blend_backward(ctx, pl.tile_size, b->bwd_batch, mode, b->proj, pay, b->ranges, pl.vt, b->image,
+                          b->final_t, b->last_idx, b->dl_dc, b->pgrad);
   if (rc) return rc;
   AdamCfgs cfg;
   cfg.lr_pos = cp->lr; cfg.b1_pos = cp->beta1; cfg.b2_pos = cp->beta2; cfg.eps_pos = cp->eps;
@@ -541,7 +543,7 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
   put(ids, sizeof ids);
   put(&scene->n, 4);
   put(views, sizeof(nts_view_desc) * n_views);
-  const int meta[4] = {(int)tile_size, loss_kind, bwd_mode, write_grads};
+  const int meta[6] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch};
   put(meta, sizeof meta);
   put(cfg_pos, sizeof *cfg_pos);
   put(cfg_attr, sizeof *cfg_attr);
@@ -592,6 +594,16 @@ extern "C" int nts_render_views(nts_ctx* ctx, const nts_scene* scene, nts_batch*
   rc = upload_cams(ctx, b, views, n_views);
   if (rc) return rc;
   return launch_forward(ctx, scene, b, pl, nullptr, true, nullptr);
+}
+
+extern "C" int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* b, uint32_t fwd_batch, uint32_t bwd_batch) {
+  NTS_CHECK_ARG(ctx && b);
+  if (!nts_blend_config_ok(16, fwd_batch, bwd_batch))
+    return nts_fail(NTS_E_INVALID, "LDS batch %u / %u: forward in {64, 128, 256, 512}, backward in {64, 128, 256}",
+                    fwd_batch, bwd_batch);
+  b->fwd_batch = fwd_batch;
+  b->bwd_batch = bwd_batch;
+  return NTS_OK;
 }
 
 extern "C" int nts_batch_losses(nts_ctx* ctx, const nts_batch* b, double* losses) {

@@ -41,8 +41,9 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Pixel-centre rectangle of the wave (centre and half extents).
+// Pixel-centre rectangle of the wave: bounds, centre and half extents.
 struct WaveRect {
+  float x0, x1, y0, y1;
   float cx, cy, hw, hh;
 };
 
@@ -63,7 +64,7 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
   x1 = wave_max(x1);
   y0 = wave_min(y0);
   y1 = wave_max(y1);
-  return WaveRect{0.5f * (x0 + x1), 0.5f * (y0 + y1), 0.5f * (x1 - x0), 0.5f * (y1 - y0)};
+  return WaveRect{x0, x1, y0, y1, 0.5f * (x0 + x1), 0.5f * (y0 + y1), 0.5f * (x1 - x0), 0.5f * (y1 - y0)};
 }
 
 // Staged splat: the fields the blend reads plus a conservative bound of the
@@ -95,8 +96,66 @@ __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float
   c = make_float4(p.c.z, ex, ey, pmin);
 }
 
-__device__ __forceinline__ bool wave_misses(const WaveRect& r, float4 a, float4 c) {
-  return fabsf(r.cx - a.x) > r.hw + c.y || fabsf(r.cy - a.y) > r.hh + c.z;
+// Whether any pixel centre of the wave's rectangle can pass the blend's
+// power test  q(d) = a dx^2 + 2b dx dy + c dy^2 <= R  (R = -2 pmin) for this
+// splat.  First the bounding-box test of the alpha >= 1/255 ellipse, then
+// the exact minimum of the convex q over the rectangle: it is 0 when the
+// centre lies inside, otherwise it lies on an edge that faces the centre,
+// where it is a clamped 1-D minimum.  Every pixel's float offset lies in the
+// float rectangle [x0 - mx, x1 - mx] x [y0 - my, y1 - my], and the float
+// evaluation of q at any point is within 6u * kappa * q of the exact value,
+// kappa = (sqrt(ac) + |b|)^2 / det (u = 2^-24); the threshold carries four
+// times that margin, and splats with kappa >= 6e4 skip the exact test.  A
+// rejected splat therefore fails every pixel's power test: results are
+// unchanged.
+__device__ __forceinline__ bool wave_hits(const WaveRect& r, float4 A, float4 B, float4 Cc) {
+  if (fabsf(r.cx - A.x) > r.hw + Cc.y || fabsf(r.cy - A.y) > r.hh + Cc.z) return false;
+  const float a = A.z, b = A.w, c = B.x;
+  const float det = a * c - b * b;
+  if (!(Cc.w > -1e30f) || !(a > 0.0f) || !(c > 0.0f) || !(det > 0.0f)) return true;
+  const float s = sqrtf(a * c) + fabsf(b);
+  const float kap = s * s / det;
+  if (!(kap < 6.0e4f)) return true;
+  const float u0 = r.x0 - A.x, u1 = r.x1 - A.x, v0 = r.y0 - A.y, v1 = r.y1 - A.y;
+  const bool out_x = u0 > 0.0f || u1 < 0.0f, out_y = v0 > 0.0f || v1 < 0.0f;
+  if (!out_x && !out_y) return true;
+  float q = 3.4e38f;
+  if (out_x) {
+    const float U = u0 > 0.0f ? u0 : u1;
+    const float dy = fminf(fmaxf(-b * U / c, v0), v1);
+    q = fminf(q, a * U * U + 2.0f * b * U * dy + c * dy * dy);
+  }
+  if (out_y) {
+    const float V = v0 > 0.0f ? v0 : v1;
+    const float dx = fminf(fmaxf(-b * V / a, u0), u1);
+    q = fminf(q, a * dx * dx + 2.0f * b * dx * V + c * V * V);
+  }
+  return q <= -2.0f * Cc.w * (1.0f + 1e-5f + 4e-6f * kap);
+}
+
+// Per-wave hit masks of the staged batch: bit b of mk[i] is set when slot
+// 64 i + b (< klim) can reach one of the wave's pixels.  Each lane tests
+// NM slots; the masks are wave-uniform.
+template <int NM>
+__device__ __forceinline__ void wave_masks(const WaveRect& wr, const float4* s_a, const float4* s_b,
+                                           const float4* s_c, uint32_t klim, uint64_t (&mk)[NM]) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    const uint32_t k = (uint32_t)i * 64u + lane;
+    bool h = false;
+    if (k < klim) h = wave_hits(wr, s_a[k], s_b[k], s_c[k]);
+    mk[i] = __ballot(h);
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -121,7 +180,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
                                                              const uint2* __restrict__ ranges, ViewTable vt,
                                                              float4* __restrict__ out, float* __restrict__ final_t,
                                                              uint32_t* __restrict__ last_idx, LossArgs la) {
-  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT;
+  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT, NM = BATCH / 64;
   __shared__ float4 s_a[BATCH];  // mean.x, mean.y, conic.a, conic.b
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
   __shared__ float4 s_c[BATCH];  // b, cull half-extents x / y, power threshold
@@ -160,36 +219,19 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
     for (uint32_t k = threadIdx.x; k < nb; k += NT)
       stage_splat(proj[vd.proj_base + payloads[base + k]], s_a[k], s_b[k], s_c[k]);
     __syncthreads();
-    if (PPT == 1) {
-      if (!done[0]) {
-        for (uint32_t k = 0; k < nb; ++k) {
-          const float4 A = s_a[k], Cc = s_c[k];
-          if (wave_misses(wr, A, Cc)) continue;
-          const float4 B = s_b[k];
-          const float dx = pxf[0] - A.x, dy = pyf[0] - A.y;
-          const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-          if (power > 0.0f || power < Cc.w) continue;
-          const float g = nts_expf_nonpos(power);
-          const float ar = B.y * g;
-          const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
-          if (alpha < NTS_ALPHA_MIN) continue;
-          const float ta = T[0] * alpha;
-          C0[0] = C0[0] + ta * B.z;
-          C1[0] = C1[0] + ta * B.w;
-          C2[0] = C2[0] + ta * Cc.x;
-          T[0] = T[0] * (1.0f - alpha);
-          last[0] = base + k - begin + 1;
-          if (T[0] < NTS_T_MIN) {
-            done[0] = true;
-            break;
-          }
-        }
-      }
-    } else {
-      for (uint32_t k = 0; k < nb; ++k) {
-        const float4 A = s_a[k], Cc = s_c[k];
-        if (wave_misses(wr, A, Cc)) continue;
-        const float4 B = s_b[k];
+    uint64_t mk[NM];
+    wave_masks<NM>(wr, s_a, s_b, s_c, __ballot(live) ? nb : 0u, mk);
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+      uint64_t m = mk[i];
+      while (m) {
+        int lv = 0;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) lv |= !done[j];
+        if (!__ballot(lv)) break;  // every pixel of the wave is saturated
+        const uint32_t k = (uint32_t)i * 64u + (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
           if (done[j]) continue;
@@ -242,56 +284,62 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
 //   reference-parity mode (rasterize_backward.wgsl:73-210): T = 1 - out.w,
 //     walk from the slice end, T /= max(1 - a, 1e-4), bail once T > 1.0001;
 //   recompute mode: T = saved final T, walk from the pixel's last
-//     contributing splat, no bail (the exact chain rule of the forward).
+//     contributing splat, no bail (the exact chain rule of the forward);
+//     the three divisions by 1 - a become one hardware reciprocal (1 ulp).
 // Each splat's nine per-pixel terms are reduce-scattered across the wave,
 // the waves' partial sums meet in LDS, and one set of float atomics per
 // (tile, splat) commits them.
 // ---------------------------------------------------------------------------
 
-// Reduce-scatter of the 9 per-lane gradient terms across a wave: after 13
-// shuffles the lane with lane & 3 == 0 and (lane >> 2) = q holds the wave sum
-// of term grad_term_of_quad(q) (or of nothing when that is >= 9).
-__device__ __forceinline__ float wave_reduce9(const float (&v)[9]) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const bool b5 = lane & 32u, b4 = lane & 16u, b3 = lane & 8u, b2 = lane & 4u;
-  float w[5];
+// Lane exchanges within a row of 16 lanes (DPP): the mirror partners differ
+// from the lane in bit 3 (row mirror) or bit 2 (half-row mirror); quad
+// permutes give the xor-2 and xor-1 partners.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+#define NTS_DPP_ROW_MIRROR 0x140
+#define NTS_DPP_ROW_HALF_MIRROR 0x141
+#define NTS_DPP_QUAD_XOR2 0x4E  // quad_perm [2, 3, 0, 1]
+#define NTS_DPP_QUAD_XOR1 0xB1  // quad_perm [1, 0, 3, 2]
+
+// One halving stage of the reduce-scatter: lanes whose bit is clear keep
+// terms [0, H) of the N, the others terms [H, N), each adding the partner's
+// copy of the terms it keeps.
+template <int N, int H, int CTRL>
+__device__ __forceinline__ void rs_stage(const float (&v)[N], float (&w)[H], bool hi_half) {
 #pragma unroll
-  for (int j = 0; j < 5; ++j) {  // halves keep terms 0..4 / 5..8
-    const float hi = (j + 5 < 9) ? v[j + 5] : 0.0f;
-    const float keep = b5 ? hi : v[j];
-    const float send = b5 ? v[j] : hi;
-    w[j] = keep + __shfl_xor(send, 32, 64);
+  for (int j = 0; j < H; ++j) {
+    const float hi = (j + H < N) ? v[j + H] : 0.0f;
+    const float keep = hi_half ? hi : v[j];
+    const float send = hi_half ? v[j] : hi;
+    w[j] = keep + dpp_f<CTRL>(send);
   }
-  float x[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const float hi = (j + 3 < 5) ? w[j + 3] : 0.0f;
-    const float keep = b4 ? hi : w[j];
-    const float send = b4 ? w[j] : hi;
-    x[j] = keep + __shfl_xor(send, 16, 64);
-  }
-  float y[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const float hi = (j + 2 < 3) ? x[j + 2] : 0.0f;
-    const float keep = b3 ? hi : x[j];
-    const float send = b3 ? x[j] : hi;
-    y[j] = keep + __shfl_xor(send, 8, 64);
-  }
-  float z = (b2 ? y[1] : y[0]) + __shfl_xor(b2 ? y[0] : y[1], 4, 64);
-  z += __shfl_xor(z, 2, 64);
-  z += __shfl_xor(z, 1, 64);
-  return z;
 }
 
-// Term index held by quad q = lane >> 2 after wave_reduce9 (9 = none).
-__device__ __forceinline__ uint32_t grad_term_of_quad(uint32_t q) {
-  const uint32_t b5 = (q >> 3) & 1u, b4 = (q >> 2) & 1u, b3 = (q >> 1) & 1u, b2 = q & 1u;
-  const uint32_t xs = (b3 ? 2u : 0u) + b2;
+// Reduce-scatter of the 9 per-lane gradient terms across the wave: four DPP
+// stages inside each row of 16 lanes, then two cross-row sums.  Afterwards
+// lane l < 16 holds the wave sum of term grad_term_of_lane(l) (none when
+// that is >= 9); rows 1-3 hold copies.
+__device__ __forceinline__ float wave_reduce9(const float (&v)[9]) {
+  const uint32_t lane = threadIdx.x & 63u;
+  float w[5], x[3], y[2], z[1];
+  rs_stage<9, 5, NTS_DPP_ROW_MIRROR>(v, w, lane & 8u);
+  rs_stage<5, 3, NTS_DPP_ROW_HALF_MIRROR>(w, x, lane & 4u);
+  rs_stage<3, 2, NTS_DPP_QUAD_XOR2>(x, y, lane & 2u);
+  rs_stage<2, 1, NTS_DPP_QUAD_XOR1>(y, z, lane & 1u);
+  float r = z[0] + __shfl_xor(z[0], 16, 64);
+  r += __shfl_xor(r, 32, 64);
+  return r;
+}
+
+// Term index held by lane l (< 16) after wave_reduce9 (9 = none).
+__device__ __forceinline__ uint32_t grad_term_of_lane(uint32_t l) {
+  const uint32_t xs = ((l & 2u) ? 2u : 0u) + (l & 1u);
   if (xs >= 3u) return 9u;
-  const uint32_t ws = (b4 ? 3u : 0u) + xs;
+  const uint32_t ws = ((l & 4u) ? 3u : 0u) + xs;
   if (ws >= 5u) return 9u;
-  const uint32_t vi = (b5 ? 5u : 0u) + ws;
+  const uint32_t vi = ((l & 8u) ? 5u : 0u) + ws;
   return vi < 9u ? vi : 9u;
 }
 
@@ -300,12 +348,13 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
     ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
     const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
-  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT, NW = NT / 64;
+  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT, NW = NT / 64, NM = BATCH / 64;
   __shared__ float4 s_a[BATCH];
   __shared__ float4 s_b[BATCH];
   __shared__ float4 s_c[BATCH];
   __shared__ uint32_t s_idx[BATCH];
   __shared__ float s_part[NW][BATCH][9];  // per-wave sums of each splat's terms
+  __shared__ uint64_t s_cm[NW][NM];      // which waves wrote s_part for a slot
   __shared__ uint32_t s_max;
   const uint32_t gt = blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
@@ -315,8 +364,8 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
   const uint2 rg = ranges[gt];
   const uint32_t begin = rg.x, end = rg.y;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t my_term = grad_term_of_quad(lane >> 2);
-  const bool term_lane = (lane & 3u) == 0u && my_term < 9u;
+  const uint32_t my_term = lane < 16u ? grad_term_of_lane(lane) : 9u;
+  const bool term_lane = my_term < 9u;
   const WaveRect wr = wave_rect<TS>(tx, ty);
 
   float pxf[PPT], pyf[PPT], T[PPT], G0[PPT], G1[PPT], G2[PPT], S0[PPT], S1[PPT], S2[PPT];
@@ -353,6 +402,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
     done[j] = lastj[j] == 0;
     my_max = lastj[j] > my_max ? lastj[j] : my_max;
   }
+  const uint32_t wave_last = wave_max_u32(my_max);  // slice entries this wave can need
   __syncthreads();
   if (my_max) atomicMax(&s_max, my_max);
   __syncthreads();
@@ -370,75 +420,107 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
       s_idx[k] = sidx;
     }
     __syncthreads();
-    for (int k = (int)take - 1; k >= 0; --k) {
-      const float4 A = s_a[k], Cc = s_c[k];
-      if (wave_misses(wr, A, Cc)) {
-        if (term_lane) s_part[wave][k][my_term] = 0.0f;
-        continue;
-      }
-      const float4 B = s_b[k];
-      float acc[9];
+    uint32_t klim = 0;
+    if (__ballot(live) && wave_last > rel0) klim = wave_last - rel0 < take ? wave_last - rel0 : take;
+    uint64_t mk[NM], cm[NM];
+    wave_masks<NM>(wr, s_a, s_b, s_c, klim, mk);
 #pragma unroll
-      for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
-      bool contrib = false;
+    for (int i = NM - 1; i >= 0; --i) {
+      uint64_t m = mk[i];
+      cm[i] = 0;
+      while (m) {
+        const int bt = 63 - __builtin_clzll(m);
+        m &= ~(1ull << bt);
+        const uint32_t k = (uint32_t)i * 64u + (uint32_t)bt;
+        const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+        float acc[9];
 #pragma unroll
-      for (int j = 0; j < PPT; ++j) {
-        if (done[j] || rel0 + (uint32_t)k >= lastj[j]) continue;
-        const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
-        const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-        if (power > 0.0f || power < Cc.w) continue;
-        const float g = nts_expf_nonpos(power);
-        const float sigma = B.y;
-        const float ar = sigma * g;
-        const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
-        if (alpha < NTS_ALPHA_MIN) continue;
-        const float oma = RECOMP ? 1.0f - alpha : nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
-        T[j] = T[j] / oma;
-        if (!RECOMP && T[j] > NTS_T_BAIL) {
-          done[j] = true;
-          continue;
+        for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
+        bool contrib = false;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+          if (done[j] || rel0 + k >= lastj[j]) continue;
+          const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
+          const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
+          if (power > 0.0f || power < Cc.w) continue;
+          const float g = nts_expf_nonpos(power);
+          const float sigma = B.y;
+          const float ar = sigma * g;
+          const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
+          if (alpha < NTS_ALPHA_MIN) continue;
+          float s0, s1, s2;
+          if (RECOMP) {
+            const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
+            T[j] = T[j] * r;
+            s0 = S0[j] * r;
+            s1 = S1[j] * r;
+            s2 = S2[j] * r;
+          } else {
+            const float oma = nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
+            T[j] = T[j] / oma;
+            if (T[j] > NTS_T_BAIL) {
+              done[j] = true;
+              continue;
+            }
+            s0 = S0[j] / oma;
+            s1 = S1[j] / oma;
+            s2 = S2[j] / oma;
+          }
+          const float ta = T[j] * alpha;
+          const float c0 = B.z, c1 = B.w, c2 = Cc.x;
+          const float d0 = T[j] * c0 - s0;
+          const float d1 = T[j] * c1 - s1;
+          const float d2 = T[j] * c2 - s2;
+          const float gal = G0[j] * d0 + G1[j] * d1 + G2[j] * d2;
+          const float sat = (ar < NTS_ALPHA_MAX) ? 1.0f : 0.0f;
+          const float gar = gal * sat;
+          const float gpow = gar * sigma * g;
+          acc[0] += gpow * (A.z * dx + A.w * dy);
+          acc[1] += gpow * (A.w * dx + B.x * dy);
+          acc[2] += gar * g;
+          acc[3] += gpow * -0.5f * (dx * dx);
+          acc[4] += gpow * -0.5f * (2.0f * dx * dy);
+          acc[5] += gpow * -0.5f * (dy * dy);
+          acc[6] += G0[j] * ta;
+          acc[7] += G1[j] * ta;
+          acc[8] += G2[j] * ta;
+          S0[j] = S0[j] + ta * c0;
+          S1[j] = S1[j] + ta * c1;
+          S2[j] = S2[j] + ta * c2;
+          contrib = true;
         }
-        const float ta = T[j] * alpha;
-        const float c0 = B.z, c1 = B.w, c2 = Cc.x;
-        const float d0 = T[j] * c0 - S0[j] / oma;
-        const float d1 = T[j] * c1 - S1[j] / oma;
-        const float d2 = T[j] * c2 - S2[j] / oma;
-        const float gal = G0[j] * d0 + G1[j] * d1 + G2[j] * d2;
-        const float sat = (ar < NTS_ALPHA_MAX) ? 1.0f : 0.0f;
-        const float gar = gal * sat;
-        const float gpow = gar * sigma * g;
-        acc[0] += gpow * (A.z * dx + A.w * dy);
-        acc[1] += gpow * (A.w * dx + B.x * dy);
-        acc[2] += gar * g;
-        acc[3] += gpow * -0.5f * (dx * dx);
-        acc[4] += gpow * -0.5f * (2.0f * dx * dy);
-        acc[5] += gpow * -0.5f * (dy * dy);
-        acc[6] += G0[j] * ta;
-        acc[7] += G1[j] * ta;
-        acc[8] += G2[j] * ta;
-        S0[j] = S0[j] + ta * c0;
-        S1[j] = S1[j] + ta * c1;
-        S2[j] = S2[j] + ta * c2;
-        contrib = true;
-      }
-      if (__ballot(contrib)) {
-        const float r = wave_reduce9(acc);
-        if (term_lane) s_part[wave][k][my_term] = r;
-      } else if (term_lane) {
-        s_part[wave][k][my_term] = 0.0f;
+        if (__ballot(contrib)) {
+          const float r = wave_reduce9(acc);
+          if (term_lane) s_part[wave][k][my_term] = r;
+          cm[i] |= 1ull << bt;
+        }
       }
     }
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < NM; ++i) s_cm[wave][i] = cm[i];
+    }
     __syncthreads();
-    // one set of float atomics per (tile, splat): sum the waves' partials
+    // one set of float atomics per (tile, splat): sum the partials of the
+    // waves that wrote one
     for (uint32_t k = threadIdx.x; k < take; k += NT) {
+      const uint64_t bit = 1ull << (k & 63u);
+      float sum[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) sum[q] = 0.0f;
+      bool any = false;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        if (!(s_cm[w][k >> 6] & bit)) continue;
+        any = true;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) sum[q] += s_part[w][k][q];
+      }
+      if (!any) continue;
       float* o = reinterpret_cast<float*>(pgrad + vd.proj_base + s_idx[k]);
 #pragma unroll
-      for (int q = 0; q < 9; ++q) {
-        float sum = 0.0f;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) sum += s_part[w][k][q];
-        if (sum != 0.0f) unsafeAtomicAdd(o + (q < 3 ? q : (q < 6 ? q + 1 : q + 2)), sum);
-      }
+      for (int q = 0; q < 9; ++q)
+        if (sum[q] != 0.0f) unsafeAtomicAdd(o + (q < 3 ? q : (q < 6 ? q + 1 : q + 2)), sum[q]);
     }
     rem -= take;
   }
@@ -447,8 +529,6 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
-#define NTS_BATCH 256
-
 static int fill_single_view(const nts_bins* bins, const nts_tiling* t, ViewTable& vt) {
   memset(&vt, 0, sizeof vt);
   vt.n = 1;
@@ -466,56 +546,91 @@ static int fill_single_view(const nts_bins* bins, const nts_tiling* t, ViewTable
   return NTS_OK;
 }
 
-int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, const Projected* proj, const uint32_t* payloads,
-                      const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx,
-                      const LossArgs* loss) {
+bool nts_blend_config_ok(uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_batch) {
+  const bool ts = tile_size == 8 || tile_size == 16 || tile_size == 32;
+  const bool fb = fwd_batch == 64 || fwd_batch == 128 || fwd_batch == 256 || fwd_batch == 512;
+  const bool bb = bwd_batch == 64 || bwd_batch == 128 || bwd_batch == 256;
+  return ts && fb && bb;
+}
+
+template <int TS, int B>
+static int launch_fwd(nts_ctx* ctx, dim3 grid, bool loss, const Projected* proj, const uint32_t* payloads,
+                       const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx,
+                       const LossArgs& la) {
+  if (loss)
+    NTS_LAUNCH(ctx, "blend_fwd_loss", (k_blend_fwd<TS, B, true>), grid, dim3(Blend<TS>::NT), 0, proj, payloads,
+               ranges, vt, out, final_t, last_idx, la);
+  else
+    NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<TS, B, false>), grid, dim3(Blend<TS>::NT), 0, proj, payloads, ranges,
+               vt, out, final_t, last_idx, la);
+  return NTS_OK;
+}
+
+template <int TS>
+static int fwd_batch_switch(nts_ctx* ctx, uint32_t lds_batch, dim3 grid, bool loss, const Projected* proj,
+                            const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, float4* out,
+                            float* final_t, uint32_t* last_idx, const LossArgs& la) {
+  switch (lds_batch) {
+    case 64: return launch_fwd<TS, 64>(ctx, grid, loss, proj, payloads, ranges, vt, out, final_t, last_idx, la);
+    case 128: return launch_fwd<TS, 128>(ctx, grid, loss, proj, payloads, ranges, vt, out, final_t, last_idx, la);
+    case 256: return launch_fwd<TS, 256>(ctx, grid, loss, proj, payloads, ranges, vt, out, final_t, last_idx, la);
+    case 512: return launch_fwd<TS, 512>(ctx, grid, loss, proj, payloads, ranges, vt, out, final_t, last_idx, la);
+    default: return nts_fail(NTS_E_INVALID, "forward LDS batch %u not in {64, 128, 256, 512}", lds_batch);
+  }
+}
+
+int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, const Projected* proj,
+                      const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, float4* out, float* final_t,
+                      uint32_t* last_idx, const LossArgs* loss) {
   if (vt.total_tiles == 0) return NTS_OK;
   const dim3 grid(vt.total_tiles);
   LossArgs la{};
   if (loss) la = *loss;
-#define NTS_FWD_CASE(TSZ)                                                                                        \
-  case TSZ:                                                                                                      \
-    if (loss)                                                                                                    \
-      NTS_LAUNCH(ctx, "blend_fwd_loss", (k_blend_fwd<TSZ, NTS_BATCH, true>), grid, dim3(Blend<TSZ>::NT), 0, proj, \
-                 payloads, ranges, vt, out, final_t, last_idx, la);                                              \
-    else                                                                                                         \
-      NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<TSZ, NTS_BATCH, false>), grid, dim3(Blend<TSZ>::NT), 0, proj,     \
-                 payloads, ranges, vt, out, final_t, last_idx, la);                                              \
-    break;
   switch (tile_size) {
-    NTS_FWD_CASE(8)
-    NTS_FWD_CASE(16)
-    NTS_FWD_CASE(32)
-    default:
-      return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
+    case 8: return fwd_batch_switch<8>(ctx, lds_batch, grid, loss, proj, payloads, ranges, vt, out, final_t, last_idx, la);
+    case 16: return fwd_batch_switch<16>(ctx, lds_batch, grid, loss, proj, payloads, ranges, vt, out, final_t, last_idx, la);
+    case 32: return fwd_batch_switch<32>(ctx, lds_batch, grid, loss, proj, payloads, ranges, vt, out, final_t, last_idx, la);
+    default: return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
   }
-#undef NTS_FWD_CASE
+}
+
+template <int TS, int B>
+static int launch_bwd(nts_ctx* ctx, dim3 grid, int mode, const Projected* proj, const uint32_t* payloads,
+                       const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
+                       const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
+  if (mode == NTS_BWD_RECOMPUTE)
+    NTS_LAUNCH(ctx, "blend_bwd_recompute", (k_blend_bwd<TS, B, true>), grid, dim3(Blend<TS>::NT), 0, proj, payloads,
+               ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+  else
+    NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd<TS, B, false>), grid, dim3(Blend<TS>::NT), 0, proj, payloads, ranges,
+               vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
   return NTS_OK;
 }
 
-int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, int mode, const Projected* proj, const uint32_t* payloads,
-                       const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
-                       const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
+template <int TS>
+static int bwd_batch_switch(nts_ctx* ctx, uint32_t lds_batch, dim3 grid, int mode, const Projected* proj,
+                            const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, const float4* fwd_out,
+                            const float* final_t, const uint32_t* last_idx, const float4* dl_dc,
+                            ProjectedGrad* pgrad) {
+  switch (lds_batch) {
+    case 64: return launch_bwd<TS, 64>(ctx, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+    case 128: return launch_bwd<TS, 128>(ctx, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+    case 256: return launch_bwd<TS, 256>(ctx, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+    default: return nts_fail(NTS_E_INVALID, "backward LDS batch %u not in {64, 128, 256}", lds_batch);
+  }
+}
+
+int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, int mode, const Projected* proj,
+                       const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, const float4* fwd_out,
+                       const float* final_t, const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
   if (vt.total_tiles == 0) return NTS_OK;
   const dim3 grid(vt.total_tiles);
-#define NTS_BWD_CASE(TSZ)                                                                                     \
-  case TSZ:                                                                                                   \
-    if (mode == NTS_BWD_RECOMPUTE)                                                                            \
-      NTS_LAUNCH(ctx, "blend_bwd_recompute", (k_blend_bwd<TSZ, NTS_BATCH, true>), grid, dim3(Blend<TSZ>::NT), \
-                 0, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);                    \
-    else                                                                                                      \
-      NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd<TSZ, NTS_BATCH, false>), grid, dim3(Blend<TSZ>::NT), 0, proj, \
-                 payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);                             \
-    break;
   switch (tile_size) {
-    NTS_BWD_CASE(8)
-    NTS_BWD_CASE(16)
-    NTS_BWD_CASE(32)
-    default:
-      return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
+    case 8: return bwd_batch_switch<8>(ctx, lds_batch, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+    case 16: return bwd_batch_switch<16>(ctx, lds_batch, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+    case 32: return bwd_batch_switch<32>(ctx, lds_batch, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+    default: return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
   }
-#undef NTS_BWD_CASE
-  return NTS_OK;
 }
 
 extern "C" int nts_composite_ex(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins, nts_buffer* image,
@@ -528,7 +643,8 @@ extern "C" int nts_composite_ex(nts_ctx* ctx, const nts_buffer* projected, const
   ViewTable vt;
   int rc = fill_single_view(bins, tiling, vt);
   if (rc) return rc;
-  return nts_blend_forward(ctx, tiling->tile_size, (const Projected*)projected->ptr, bins->payloads, bins->ranges, vt,
+  return nts_blend_forward(ctx, tiling->tile_size, NTS_DEFAULT_LDS_BATCH, (const Projected*)projected->ptr,
+                           bins->payloads, bins->ranges, vt,
                            (float4*)image->ptr, final_t ? (float*)final_t->ptr : nullptr,
                            last_idx ? (uint32_t*)last_idx->ptr : nullptr, nullptr);
 }
@@ -553,7 +669,8 @@ extern "C" int nts_composite_backward_ex(nts_ctx* ctx, const nts_buffer* project
   ViewTable vt;
   int rc = fill_single_view(bins, tiling, vt);
   if (rc) return rc;
-  return nts_blend_backward(ctx, tiling->tile_size, mode, (const Projected*)projected->ptr, bins->payloads,
+  return nts_blend_backward(ctx, tiling->tile_size, NTS_DEFAULT_LDS_BATCH, mode, (const Projected*)projected->ptr,
+                            bins->payloads,
                             bins->ranges, vt, forward_out ? (const float4*)forward_out->ptr : nullptr,
                             final_t ? (const float*)final_t->ptr : nullptr,
                             last_idx ? (const uint32_t*)last_idx->ptr : nullptr, (const float4*)dl_dc->ptr,

@@ -67,10 +67,16 @@ __device__ __forceinline__ float nts_expf_nonpos(float x) {
   return p * nts_pow2i(k);
 }
 
-// Host-side launchers of the blend kernels (k_raster.hip).
-int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, const Projected* proj, const uint32_t* payloads,
-                      const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx,
-                      const LossArgs* loss);
-int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, int mode, const Projected* proj, const uint32_t* payloads,
-                       const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
-                       const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad);
+// Splats staged through LDS per batch when the caller does not choose.
+#define NTS_DEFAULT_LDS_BATCH 256
+
+// Host-side launchers of the blend kernels (k_raster.hip).  tile_size in
+// {8, 16, 32}; lds_batch in {64, 128, 256, 512} forward, {64, 128, 256}
+// backward.
+bool nts_blend_config_ok(uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_batch);
+int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, const Projected* proj,
+                      const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, float4* out, float* final_t,
+                      uint32_t* last_idx, const LossArgs* loss);
+int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, int mode, const Projected* proj,
+                       const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, const float4* fwd_out,
+                       const float* final_t, const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad);
