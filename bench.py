@@ -9,7 +9,9 @@ batched (default): one scene of 1,000,000 splats (SH order 3, fp32); each
   resolutions drawn from {640x480, 1280x720, 1920x1080, 2560x1440}, sums the
   per-view L1 losses and runs backward + fused Adam: one launch per stage
   for all views, the whole iteration captured once in a hipGraph and
-  replayed.  `value` = training iterations per second.
+  replayed.  The blend tile size (8/16/32) and LDS batches (64-512) are
+  autotuned on the workload during warm-up (--tile / --lds-batch fix
+  them).  `value` = training iterations per second.
 plain: the single-view path (one 1920x1080 view per iteration, reference
   call sequence).
 
@@ -38,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 N_SPLATS = 1_000_000
 N_VIEWS = 16
 SEED = 0x5EED0002
-TILE = 16
-LDS_BATCH = (256, 256)
+TILE = "auto"
+LDS_BATCH = "auto"
 
 
 def make_views(n):
@@ -99,7 +101,7 @@ def roofline_of(kernels):
                 alg_bytes_per_launch=dk["alg_bytes"])
 
 
-def render_latency(ctx, g, n):
+def render_latency(ctx, g, n, tile):
     """Single-view 1920x1080 forward (project, bin, blend) latency."""
     from nanotracer_amd import synth
     from nanotracer_amd.api import CameraUniform, ViewBatch, ViewSpec
@@ -112,7 +114,7 @@ def render_latency(ctx, g, n):
     for _ in range(40):
         ctx.sync()
         a = time.perf_counter()
-        b.render(g, [spec], TILE)
+        b.render(g, [spec], tile)
         ctx.sync()
         lat.append((time.perf_counter() - a) * 1e3)
     b.free()
@@ -122,7 +124,7 @@ def render_latency(ctx, g, n):
                 render_p99_ms=round(float(np.percentile(lat, 99)), 3))
 
 
-def cpu_baseline_batched(s, views, targets):
+def cpu_baseline_batched(s, views, targets, tile):
     """The host restatement of the reference path on one full iteration of
     the same 16-view workload (recompute backward as on the GPU)."""
     from nanotracer_amd import synth
@@ -132,7 +134,7 @@ def cpu_baseline_batched(s, views, targets):
     total = None
     for (spec, eye), tgt in zip(views, targets):
         cam = O.camera_like(spec.camera)
-        t = O.tiling(spec.width, spec.height, TILE, spec.depth_max)
+        t = O.tiling(spec.width, spec.height, tile, spec.depth_max)
         op = O.project(s, cam)
         keys, pay, rg = O.bin(op, t)
         img, ft, li = O.composite(op, pay, rg, t, with_state=True)
@@ -159,11 +161,14 @@ def run_batched(args, ctx):
     views = make_views(s.n)
     specs = [v[0] for v in views]
     pixels = sum(v.width * v.height for v in specs)
-    # targets: the scene with perturbed means, rendered on the device
+    # targets: the scene with perturbed means, rendered on the device at the
+    # reference's tile size
     tgt_scene = GpuSplatBuffer.upload(ctx, synth.perturb_means(s, SEED + 1))
-    probe = ViewBatch(ctx, N_VIEWS, s.n, pixels, 400_000_000)
-    probe.render(tgt_scene, specs, TILE)
+    probe = ViewBatch(ctx, N_VIEWS, s.n, pixels, 600_000_000)
+    probe.render(tgt_scene, specs, 16)
     targets = [probe.image(k) for k in range(N_VIEWS)]
+    # pair capacity for the smallest tile size the run may use
+    probe.render(tgt_scene, specs, 8 if TILE == "auto" else TILE)
     pairs_t, _, _ = probe.stats()
     probe.free()
     tgt_scene.free()
@@ -171,17 +176,28 @@ def run_batched(args, ctx):
     g = GpuSplatBuffer.upload(ctx, s)
     cap = int(max(pairs_t, 1) * 1.3) + 1_000_000
     b = ViewBatch(ctx, N_VIEWS, s.n, pixels, cap)
-    b.set_lds_batch(*LDS_BATCH)
     mode = L.BWD_RECOMPUTE if args.backward == "recompute" else L.BWD_REF_PARITY
-    t = 0
+    t = 1
+    tune = None
+    if TILE == "auto" or LDS_BATCH == "auto":
+        # the tuning steps are training steps: they count as warm-up
+        from nanotracer_amd.autotune import TILES, autotune_batch
+
+        tune = autotune_batch(ctx, b, g, specs, tbuf, t, L.LOSS_L1, mode,
+                              tiles=TILES if TILE == "auto" else (TILE,))
+        t = tune.next_t
+        tile, lds = tune.tile, (tune.fwd_batch, tune.bwd_batch)
+    else:
+        tile, lds = TILE, LDS_BATCH
+        b.set_lds_batch(*lds)
     for _ in range(max(args.warmup, 1)):
+        b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, mode, graph=True)
         t += 1
-        b.train_step(g, specs, tbuf, t, TILE, L.LOSS_L1, mode, graph=True)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, mode, graph=True)
         t += 1
-        b.train_step(g, specs, tbuf, t, TILE, L.LOSS_L1, mode, graph=True)
     ctx.sync()
     dt = time.perf_counter() - t0
     pairs, visible, overflow = b.stats()
@@ -193,11 +209,11 @@ def run_batched(args, ctx):
     ctx.prof_reset()
     ctx.prof_enable(True)
     for _ in range(3):
+        b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, mode, graph=False)
         t += 1
-        b.train_step(g, specs, tbuf, t, TILE, L.LOSS_L1, mode, graph=False)
     ctx.prof_enable(False)
     kernels = kernel_table(ctx.prof_report(), s.n, visible, pairs, pixels,
-                           sum((-(-v.width // TILE)) * (-(-v.height // TILE)) for v in specs), N_VIEWS)
+                           sum((-(-v.width // tile)) * (-(-v.height // tile)) for v in specs), N_VIEWS)
     b.free()
     out = dict(
         metric="train iters/s + render Mpx/s at 1M splats 1080p; p50/p99 render latency",
@@ -205,18 +221,21 @@ def run_batched(args, ctx):
         ms_per_step=round(dt * 1e3 / args.steps, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32", data="synthetic",
         config=dict(workload="batched training iteration: 1M splats, 16 views of mixed resolution per launch, "
-                             "hipGraph replay", splats=s.n, views=N_VIEWS, pixels=pixels, tile=TILE,
-                    lds_batch=dict(forward=LDS_BATCH[0], backward=LDS_BATCH[1]), pairs=pairs,
+                             "hipGraph replay", splats=s.n, views=N_VIEWS, pixels=pixels, tile=tile,
+                    lds_batch=dict(forward=lds[0], backward=lds[1]),
+                    tuning="autotuned during warm-up" if tune else "fixed", pairs=pairs,
                     visible_records=visible, backward=args.backward, loss="L1 summed over views"),
         views_per_s=round(iters_s * N_VIEWS, 2), train_mpx_s=round(iters_s * pixels / 1e6, 2),
         final_loss=float(losses.sum()),
     )
-    out.update(render_latency(ctx, g, s.n))
+    out.update(render_latency(ctx, g, s.n, tile))
     out["roofline"] = roofline_of(kernels)
+    if tune:
+        out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
                               gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_batched(s, views, targets)
+        out["cpu_baseline"] = cpu_baseline_batched(s, views, targets, tile)
     return out
 
 
@@ -264,8 +283,10 @@ def run_plain(args, ctx):
         config=dict(workload="plain single-view training step (reference call sequence)", splats=N_SPLATS,
                     width=W, height=H, tile=16, pairs=pairs, visible=Nv, backward=args.backward),
     )
-    out.update(render_latency(ctx, g, s.n))
+    out.update(render_latency(ctx, g, s.n, tile))
     out["roofline"] = roofline_of(kernels)
+    if tune:
+        out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
                               gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
     return out
@@ -280,12 +301,13 @@ def main():
     ap.add_argument("--backward", choices=("ref", "recompute"), default="recompute",
                     help="reference-parity backward or the recompute (final-T, last-index) backward")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tile", type=int, choices=(8, 16, 32), default=16)
-    ap.add_argument("--lds-batch", default="256,256", help="blend forward,backward LDS batch")
+    ap.add_argument("--tile", choices=("auto", "8", "16", "32"), default="auto",
+                    help="blend tile size, or autotuned during warm-up")
+    ap.add_argument("--lds-batch", default="auto", help="blend forward,backward LDS batch, or auto")
     args = ap.parse_args()
     global TILE, LDS_BATCH
-    TILE = args.tile
-    LDS_BATCH = tuple(int(x) for x in args.lds_batch.split(","))
+    TILE = args.tile if args.tile == "auto" else int(args.tile)
+    LDS_BATCH = args.lds_batch if args.lds_batch == "auto" else tuple(int(x) for x in args.lds_batch.split(","))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is view-parallel with one gradient reduce-scatter); run with --gpus 1",

@@ -21,12 +21,6 @@
 #include "nts_project.h"
 #include "nts_raster.h"
 
-struct TileTable {
-  uint32_t n;
-  uint32_t pad[3];
-  ViewTiles t[NTS_MAX_VIEWS];
-};
-
 struct nts_batch {
   uint32_t max_views = 0, max_splats = 0;
   uint64_t max_pixels = 0, pair_cap = 0, max_tiles = 0;
@@ -128,33 +122,6 @@ __global__ void k_view_segments(const uint32_t* __restrict__ offsets, uint32_t n
   flags[0] = total > cap ? 1u : 0u;
   flags[2] = (uint32_t)total;
   flags[3] = (uint32_t)(total >> 32);
-}
-
-template <typename K>
-__global__ void __launch_bounds__(256) k_emit_views(const Projected* __restrict__ proj, uint32_t n, TileTable tt,
-                                                    const uint32_t* __restrict__ offsets, uint64_t capacity,
-                                                    K* __restrict__ keys, uint32_t* __restrict__ payloads) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t v = blockIdx.y;
-  if (i >= n) return;
-  const Projected p = proj[(size_t)v * n + i];
-  if (p.b.w == 0.0f) return;
-  const ViewTiles vt = tt.t[v];
-  int32_t x0, y0, x1, y1;
-  if (!tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1,
-                     y1))
-    return;
-  const K d16 = (K)depth16(p.a.z, vt.depth_max);
-  uint32_t w = offsets[(size_t)v * n + i];
-  for (int32_t ty = y0; ty <= y1; ++ty)
-    for (int32_t tx = x0; tx <= x1; ++tx) {
-      const K tile = (K)((uint32_t)ty * vt.tiles_x + (uint32_t)tx);
-      if ((uint64_t)w < capacity) {
-        keys[w] = (tile << 16) | d16;
-        payloads[w] = i;
-      }
-      ++w;
-    }
 }
 
 __global__ void k_set_step(uint64_t* d_step, uint64_t t) {
@@ -414,14 +381,15 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
              (const uint64_t*)b->d_total, b->pair_cap, b->seg_begin, b->flags);
   bool alt = false;
   if (n > 0) {
+    const uint32_t nrec = (uint32_t)vn;
     if (pl.key_bytes == 4)
-      NTS_LAUNCH(ctx, "emit_views", k_emit_views<uint32_t>, dim3(nts_div_up(n, 256), V), dim3(256), 0,
-                 (const Projected*)b->proj, n, pl.tt, (const uint32_t*)b->counts, b->pair_cap, (uint32_t*)b->keys_a,
-                 b->vals_a);
+      NTS_LAUNCH(ctx, "emit_views", k_emit_balanced<uint32_t>, dim3(nts_div_up(nrec, 256)), dim3(256), 0,
+                 (const Projected*)b->proj, n, nrec, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
+                 b->pair_cap, (uint32_t*)b->keys_a, b->vals_a);
     else
-      NTS_LAUNCH(ctx, "emit_views", k_emit_views<uint64_t>, dim3(nts_div_up(n, 256), V), dim3(256), 0,
-                 (const Projected*)b->proj, n, pl.tt, (const uint32_t*)b->counts, b->pair_cap, (uint64_t*)b->keys_a,
-                 b->vals_a);
+      NTS_LAUNCH(ctx, "emit_views", k_emit_balanced<uint64_t>, dim3(nts_div_up(nrec, 256)), dim3(256), 0,
+                 (const Projected*)b->proj, n, nrec, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
+                 b->pair_cap, (uint64_t*)b->keys_a, b->vals_a);
   }
   if (pl.key_bytes == 4)
     rc = nts_sort_ws_launch32(ctx, (uint32_t*)b->keys_a, b->vals_a, (uint32_t*)b->keys_b, b->vals_b, b->seg_begin, V,

@@ -146,31 +146,37 @@ __global__ void k_sort_parts(const uint64_t* __restrict__ seg_begin, uint32_t nu
   part_begin[num_segs] = p;
 }
 
+// Digit histograms of every pass for every segment.  A fixed grid walks the
+// partitions (the live count is only known on the device), so a launch
+// sized for the capacity costs nothing for partitions that are empty.
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                         const uint32_t* __restrict__ part_begin, uint32_t num_segs,
                                                         uint32_t passes, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_hist[8 * RS_RADIX];
-  const uint32_t p = blockIdx.x;
-  if (p >= part_begin[num_segs]) return;
-  for (uint32_t i = threadIdx.x; i < passes * RS_RADIX; i += RS_BLOCK) s_hist[i] = 0;
-  __syncthreads();
-  const uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p);
-  const uint64_t base = seg_begin[s] + (uint64_t)(p - part_begin[s]) * RS_TILE;
-  const uint64_t end = seg_begin[s + 1];
+  const uint32_t total_parts = part_begin[num_segs];
+  for (uint32_t p = blockIdx.x; p < total_parts; p += gridDim.x) {
+    for (uint32_t i = threadIdx.x; i < passes * RS_RADIX; i += RS_BLOCK) s_hist[i] = 0;
+    __syncthreads();
+    const uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p);
+    const uint64_t base = seg_begin[s] + (uint64_t)(p - part_begin[s]) * RS_TILE;
+    const uint64_t end = seg_begin[s + 1];
 #pragma unroll 4
-  for (int k = 0; k < RS_ITEMS; ++k) {
-    const uint64_t i = base + (uint64_t)k * RS_BLOCK + threadIdx.x;
-    if (i < end) {
-      const K key = keys[i];
-      for (uint32_t ps = 0; ps < passes; ++ps) atomicAdd(&s_hist[ps * RS_RADIX + (uint32_t)((key >> (8 * ps)) & 255u)], 1u);
+    for (int k = 0; k < RS_ITEMS; ++k) {
+      const uint64_t i = base + (uint64_t)k * RS_BLOCK + threadIdx.x;
+      if (i < end) {
+        const K key = keys[i];
+        for (uint32_t ps = 0; ps < passes; ++ps)
+          atomicAdd(&s_hist[ps * RS_RADIX + (uint32_t)((key >> (8 * ps)) & 255u)], 1u);
+      }
     }
-  }
-  __syncthreads();
-  uint32_t* h = hist + (size_t)s * passes * RS_RADIX;
-  for (uint32_t i = threadIdx.x; i < passes * RS_RADIX; i += RS_BLOCK) {
-    const uint32_t c = s_hist[i];
-    if (c) atomicAdd(&h[i], c);
+    __syncthreads();
+    uint32_t* h = hist + (size_t)s * passes * RS_RADIX;
+    for (uint32_t i = threadIdx.x; i < passes * RS_RADIX; i += RS_BLOCK) {
+      const uint32_t c = s_hist[i];
+      if (c) atomicAdd(&h[i], c);
+    }
+    __syncthreads();
   }
 }
 
@@ -203,117 +209,123 @@ __global__ void __launch_bounds__(RS_BLOCK) k_onesweep(const K* __restrict__ key
   __shared__ uint32_t s_part;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  if (tid == 0) s_part = atomicAdd(part_counter, 1u);
-#pragma unroll
-  for (int w = 0; w < 4; ++w) s_wave_hist[w][tid] = 0;
-  __syncthreads();
-  const uint32_t p = s_part;
   const uint32_t total_parts = part_begin[num_segs];
-  if (p >= total_parts) return;
-  const uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p);
-  const uint32_t first_part = part_begin[s];
-  const uint64_t base = seg_begin[s] + (uint64_t)(p - first_part) * RS_TILE;
-  const uint64_t end = seg_begin[s + 1];
+  // A fixed grid takes partitions in the order of an atomic counter, so the
+  // partition a block waits on in the look-back was taken earlier by a
+  // resident block, which publishes its aggregate before waiting itself.
+  for (;;) {
+    if (tid == 0) s_part = atomicAdd(part_counter, 1u);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s_wave_hist[w][tid] = 0;
+    __syncthreads();
+    const uint32_t p = s_part;
+    if (p >= total_parts) return;
+    const uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p);
+    const uint32_t first_part = part_begin[s];
+    const uint64_t base = seg_begin[s] + (uint64_t)(p - first_part) * RS_TILE;
+    const uint64_t end = seg_begin[s + 1];
 
-  // ---- load and rank (order within the block = (wave, round, lane)) ----
-  K key[RS_ITEMS];
-  uint32_t val[RS_ITEMS];
-  uint32_t rank[RS_ITEMS];
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
+    // ---- load and rank (order within the block = (wave, round, lane)) ----
+    K key[RS_ITEMS];
+    uint32_t val[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    const uint64_t i = base + (uint64_t)wave * (RS_TILE / 4) + (uint64_t)r * 64 + lane;
-    const bool valid = i < end;
-    key[r] = valid ? keys_in[i] : (K)~(K)0;
-    val[r] = valid ? vals_in[i] : 0u;
-  }
-#pragma unroll
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
-    uint64_t match = ~0ull;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      match &= bit ? bal : ~bal;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      const uint64_t i = base + (uint64_t)wave * (RS_TILE / 4) + (uint64_t)r * 64 + lane;
+      const bool valid = i < end;
+      key[r] = valid ? keys_in[i] : (K)~(K)0;
+      val[r] = valid ? vals_in[i] : 0u;
     }
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)match) - 1u;
-    uint32_t old = 0;
-    if (lane == leader) {
-      old = s_wave_hist[wave][d];
-      s_wave_hist[wave][d] = old + (uint32_t)__popcll(match);
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+      uint64_t match = ~0ull;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        match &= bit ? bal : ~bal;
+      }
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)match) - 1u;
+      uint32_t old = 0;
+      if (lane == leader) {
+        old = s_wave_hist[wave][d];
+        s_wave_hist[wave][d] = old + (uint32_t)__popcll(match);
+      }
+      old = __shfl(old, (int)leader, 64);
+      rank[r] = old + (uint32_t)__popcll(match & lt_mask);
     }
-    old = __shfl(old, (int)leader, 64);
-    rank[r] = old + (uint32_t)__popcll(match & lt_mask);
-  }
-  __syncthreads();
+    __syncthreads();
 
-  // ---- per-digit block totals, wave prefixes, block-local digit offsets ----
-  const uint32_t d = tid;
-  const uint32_t c0 = s_wave_hist[0][d], c1 = s_wave_hist[1][d], c2 = s_wave_hist[2][d], c3 = s_wave_hist[3][d];
-  const uint32_t tot = c0 + c1 + c2 + c3;
-  s_wave_hist[0][d] = 0;
-  s_wave_hist[1][d] = c0;
-  s_wave_hist[2][d] = c0 + c1;
-  s_wave_hist[3][d] = c0 + c1 + c2;
-  uint32_t block_total;
-  const uint32_t ex = block_excl_scan<RS_BLOCK>(tot, s_scan, block_total);
-  s_excl[d] = ex;
+    // ---- per-digit block totals, wave prefixes, block-local digit offsets ----
+    const uint32_t d = tid;
+    const uint32_t c0 = s_wave_hist[0][d], c1 = s_wave_hist[1][d], c2 = s_wave_hist[2][d], c3 = s_wave_hist[3][d];
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    s_wave_hist[0][d] = 0;
+    s_wave_hist[1][d] = c0;
+    s_wave_hist[2][d] = c0 + c1;
+    s_wave_hist[3][d] = c0 + c1 + c2;
+    uint32_t block_total;
+    const uint32_t ex = block_excl_scan<RS_BLOCK>(tot, s_scan, block_total);
+    s_excl[d] = ex;
 
-  // ---- decoupled look-back over the preceding partitions of this segment ----
-  uint32_t* st = status;  // this pass's status words (offset by the host)
-  uint32_t prefix = 0;
-  if (p == first_part) {
-    __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_INC | (tot & VALUE_MASK), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_AGG | (tot & VALUE_MASK), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t j = p - 1;
-    uint32_t spins = 0;
-    while (true) {
-      const uint32_t w = __hip_atomic_load(&st[(size_t)j * RS_RADIX + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t flag = w & FLAG_MASK;
-      if (flag == 0u) {
-        if (++spins > (1u << 22)) {
-          atomicOr(err, 1u);
-          break;
+    // ---- decoupled look-back over the preceding partitions of this segment ----
+    uint32_t* st = status;  // this pass's status words (offset by the host)
+    uint32_t prefix = 0;
+    if (p == first_part) {
+      __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_INC | (tot & VALUE_MASK), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_AGG | (tot & VALUE_MASK), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t j = p - 1;
+      uint32_t spins = 0;
+      while (true) {
+        const uint32_t w = __hip_atomic_load(&st[(size_t)j * RS_RADIX + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t flag = w & FLAG_MASK;
+        if (flag == 0u) {
+          if (++spins > (1u << 22)) {
+            atomicOr(err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
         }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
+        prefix += w & VALUE_MASK;
+        if (flag == FLAG_INC || j == first_part) break;
+        --j;
       }
-      prefix += w & VALUE_MASK;
-      if (flag == FLAG_INC || j == first_part) break;
-      --j;
+      __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_INC | ((prefix + tot) & VALUE_MASK), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
-    __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_INC | ((prefix + tot) & VALUE_MASK), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-  s_gbase[d] = offsets[((size_t)s * passes + pass) * RS_RADIX + d] + prefix - ex;
-  __syncthreads();
+    s_gbase[d] = offsets[((size_t)s * passes + pass) * RS_RADIX + d] + prefix - ex;
+    __syncthreads();
 
-  // ---- reorder in LDS by digit, then write digit runs contiguously ----
+    // ---- reorder in LDS by digit, then write digit runs contiguously ----
 #pragma unroll
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    const uint32_t dd = (uint32_t)(key[r] >> shift) & 255u;
-    const uint32_t lp = s_excl[dd] + s_wave_hist[wave][dd] + rank[r];
-    s_keys[lp] = key[r];
-    s_vals[lp] = val[r];
-  }
-  __syncthreads();
-  const uint64_t valid_n = end - base;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      const uint32_t dd = (uint32_t)(key[r] >> shift) & 255u;
+      const uint32_t lp = s_excl[dd] + s_wave_hist[wave][dd] + rank[r];
+      s_keys[lp] = key[r];
+      s_vals[lp] = val[r];
+    }
+    __syncthreads();
+    const uint64_t valid_n = end - base;
 #pragma unroll
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    const uint32_t j = (uint32_t)r * RS_BLOCK + tid;
-    if (j < valid_n) {
-      const K k = s_keys[j];
-      const uint32_t dd = (uint32_t)(k >> shift) & 255u;
-      const uint32_t pos = s_gbase[dd] + j;
-      if ((uint64_t)pos < end) {  // always true unless a look-back timed out
-        keys_out[pos] = k;
-        vals_out[pos] = s_vals[j];
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      const uint32_t j = (uint32_t)r * RS_BLOCK + tid;
+      if (j < valid_n) {
+        const K k = s_keys[j];
+        const uint32_t dd = (uint32_t)(k >> shift) & 255u;
+        const uint32_t pos = s_gbase[dd] + j;
+        if ((uint64_t)pos < end) {  // always true unless a look-back timed out
+          keys_out[pos] = k;
+          vals_out[pos] = s_vals[j];
+        }
       }
     }
+    __syncthreads();  // LDS and s_part are reused by the next partition
   }
 }
 
@@ -355,7 +367,8 @@ static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt,
   if (rc) return rc;
   if (passes == 0 || max_pairs == 0) return NTS_OK;
   NTS_LAUNCH(ctx, "sort_parts", k_sort_parts, dim3(1), dim3(64), 0, d_seg_begin, num_segs, part_begin);
-  NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(max_parts), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
+  const uint32_t grid = max_parts < RS_GRID ? max_parts : RS_GRID;
+  NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
              (const uint32_t*)part_begin, num_segs, passes, hist);
   NTS_LAUNCH(ctx, "sort_offsets", k_sort_offsets, dim3(num_segs * passes), dim3(RS_RADIX), 0, (const uint32_t*)hist,
              d_seg_begin, passes, offsets);
@@ -364,7 +377,7 @@ static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt,
   K* dst_k = keys_alt;
   uint32_t* dst_v = vals_alt;
   for (uint32_t ps = 0; ps < passes; ++ps) {
-    NTS_LAUNCH(ctx, "sort_onesweep", k_onesweep<K>, dim3(max_parts), dim3(RS_BLOCK), 0, (const K*)src_k,
+    NTS_LAUNCH(ctx, "sort_onesweep", k_onesweep<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
                (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs, 8 * ps,
                passes, ps, (const uint32_t*)offsets, status + (size_t)ps * max_parts * RS_RADIX, counters + ps, err);
     K* tk = src_k; src_k = dst_k; dst_k = tk;

@@ -3,6 +3,9 @@
 #include "nts_geom.h"
 #include "nts_sort.h"
 
+// Most views one launch bins (the batched step's limit).
+#define NTS_MAX_VIEW_TILES 32
+
 // Tile geometry of one view.
 struct ViewTiles {
   uint32_t tiles_x, tiles_y;
@@ -10,51 +13,96 @@ struct ViewTiles {
   float depth_max;
 };
 
-// tile_emit.wgsl:91-121: key = tile << 16 | depth16, payload = splat index,
-// written from offsets[i] in ty-major, tx-minor order.
+// Tile geometry of every view of a launch.
+struct TileTable {
+  uint32_t n;
+  uint32_t pad[3];
+  ViewTiles t[NTS_MAX_VIEW_TILES];
+};
+
+// tile_emit.wgsl:91-121 with the work spread over pairs instead of splats:
+// key = tile << 16 | depth16, payload = splat index, a record's pairs
+// written from offsets[record] in ty-major, tx-minor order.
+//
+// Records are (view, splat) with record f = v * n + i.  A block takes 256
+// consecutive records, stages each record's first pair, first tile, bbox
+// width and depth bucket in LDS, then its threads walk the block's pair
+// range [offsets[f0], offsets[f0 + 256]) together: pair j belongs to the
+// last record whose first pair is <= j (binary search in LDS), so every
+// store is coalesced whatever the footprints.  offsets is the exclusive
+// scan of the per-record tile counts; *total the pair total.
 template <typename K>
-__global__ void __launch_bounds__(256) k_tile_emit(const Projected* __restrict__ proj, uint32_t n, ViewTiles vt,
-                                                   const uint32_t* __restrict__ offsets, uint64_t capacity,
-                                                   K* __restrict__ keys, uint32_t* __restrict__ payloads) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const Projected p = proj[i];
-  if (p.b.w == 0.0f) return;
-  int32_t x0, y0, x1, y1;
-  if (!tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1,
-                     y1))
-    return;
-  const K d16 = (K)depth16(p.a.z, vt.depth_max);
-  uint32_t w = offsets[i];
-  for (int32_t ty = y0; ty <= y1; ++ty)
-    for (int32_t tx = x0; tx <= x1; ++tx) {
-      const K tile = (K)((uint32_t)ty * vt.tiles_x + (uint32_t)tx);
-      if ((uint64_t)w < capacity) {
-        keys[w] = (tile << 16) | d16;
-        payloads[w] = i;
-      }
-      ++w;
+__global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restrict__ proj, uint32_t n, uint32_t nrec,
+                                                       TileTable tt, const uint32_t* __restrict__ offsets,
+                                                       const uint64_t* __restrict__ total, uint64_t capacity,
+                                                       K* __restrict__ keys, uint32_t* __restrict__ payloads) {
+  __shared__ uint32_t s_off[257];
+  __shared__ uint32_t s_tile0[256], s_bw[256], s_tx[256], s_d[256], s_pay[256];
+  const uint32_t t = threadIdx.x;
+  const uint32_t f0 = blockIdx.x * 256u, f = f0 + t;
+  uint32_t off = 0xFFFFFFFFu;  // past every pair: records beyond nrec own none
+  if (f < nrec) {
+    off = offsets[f];
+    const uint32_t v = f / n, i = f - v * n;
+    const ViewTiles vt = tt.t[v];
+    const Projected p = proj[f];
+    int32_t x0 = 0, y0 = 0, x1 = -1, y1 = -1;
+    if (p.b.w != 0.0f)
+      tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1, y1);
+    s_tile0[t] = (uint32_t)y0 * vt.tiles_x + (uint32_t)x0;
+    s_bw[t] = x1 >= x0 ? (uint32_t)(x1 - x0 + 1) : 1u;
+    s_tx[t] = vt.tiles_x;
+    s_d[t] = depth16(p.a.z, vt.depth_max);
+    s_pay[t] = i;
+  }
+  s_off[t] = off;
+  if (t == 0) s_off[256] = f0 + 256u < nrec ? offsets[f0 + 256u] : (uint32_t)*total;
+  __syncthreads();
+  const uint32_t j0 = s_off[0], j1 = s_off[256];
+  for (uint32_t j = j0 + t; j < j1; j += 256u) {
+    uint32_t lo = 0, hi = 256;  // s_off[lo] <= j < s_off[hi]
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_off[mid] <= j) lo = mid; else hi = mid;
     }
+    const uint32_t r = j - s_off[lo], bw = s_bw[lo];
+    const uint32_t row = r / bw;
+    const uint32_t tile = s_tile0[lo] + row * s_tx[lo] + (r - row * bw);
+    if ((uint64_t)j < capacity) {
+      keys[j] = ((K)tile << 16) | (K)s_d[lo];
+      payloads[j] = s_pay[lo];
+    }
+  }
 }
 
 // tile_ranges.wgsl:22-41 over segment s = [seg_begin[s], seg_begin[s+1]):
 // ranges[tile_base[s] + tile] = [begin, end); tiles >= tiles_in[s] ignored.
+// The segment tables sit in LDS; a thread's indices only grow, so its
+// segment only moves forward.
 template <typename K>
 __global__ void __launch_bounds__(256) k_tile_ranges(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                      uint32_t num_segs, const uint32_t* __restrict__ tile_base,
                                                      uint2* __restrict__ ranges) {
-  const uint64_t total = seg_begin[num_segs];
+  __shared__ uint64_t s_sb[NTS_MAX_VIEW_TILES + 1];
+  __shared__ uint32_t s_tb[NTS_MAX_VIEW_TILES + 1];
+  for (uint32_t k = threadIdx.x; k <= num_segs; k += blockDim.x) {
+    s_sb[k] = seg_begin[k];
+    s_tb[k] = tile_base[k];
+  }
+  __syncthreads();
+  const uint64_t total = s_sb[num_segs];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t s = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const uint32_t s = seg_of<uint64_t>(seg_begin, num_segs, i);
-    const uint64_t sb = seg_begin[s], se = seg_begin[s + 1];
+    while (s + 1 < num_segs && i >= s_sb[s + 1]) ++s;
+    const uint64_t sb = s_sb[s], se = s_sb[s + 1];
     const K tile = keys[i] >> 16;
     const bool starts = (i == sb) || ((keys[i - 1] >> 16) != tile);
     const bool ends = (i + 1 == se) || ((keys[i + 1] >> 16) != tile);
-    const uint32_t tb = tile_base[s], tn = tile_base[s + 1] - tb;
+    const uint32_t tb = s_tb[s], tn = s_tb[s + 1] - tb;
     if (tile >= (K)tn) continue;
     if (starts) ranges[tb + (uint32_t)tile].x = (uint32_t)i;
     if (ends) ranges[tb + (uint32_t)tile].y = (uint32_t)(i + 1);
   }
 }
-

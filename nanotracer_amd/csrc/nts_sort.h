@@ -14,6 +14,7 @@
 #define RS_TILE (RS_BLOCK * RS_ITEMS)
 #define RS_RADIX 256
 #define RS_MAX_SEGS 64
+#define RS_GRID 2048  // blocks of the partition-walking sort kernels
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 

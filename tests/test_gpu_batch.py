@@ -166,3 +166,96 @@ def test_overflow_is_flagged_not_written_past(ctx):
     tp, vis, ov = b.stats()
     assert ov == 1 and tp > 10_000
     b.free()
+
+
+@pytest.mark.parametrize("tile,fwd_batch", [(8, 64), (16, 512), (32, 128), (32, 512)])
+def test_lds_batch_keeps_images_bit_exact(ctx, tile, fwd_batch):
+    s = synth.make_scene(20_000, 0x5EED0002, 3.0)
+    views = _views(s.n)
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 2_000_000)
+    b.set_lds_batch(fwd_batch, 256)
+    b.render(g, [v[0] for v in views], tile)
+    for k, (spec, ocam, _) in enumerate(views):
+        t = O.tiling(spec.width, spec.height, tile, spec.depth_max)
+        op = O.project(s, ocam)
+        ok, opay, orr = O.bin(op, t)
+        img, ft, li = O.composite(op, opay, orr, t, with_state=True)
+        assert np.array_equal(b.image(k).view(np.uint32), img.view(np.uint32)), k
+        gft, gli = b.state(k)
+        assert np.array_equal(gli, li)
+    b.free()
+
+
+@pytest.mark.parametrize("tile,bwd_batch", [(32, 64), (8, 128), (16, 64)])
+def test_lds_batch_backward_matches_oracle(ctx, tile, bwd_batch):
+    s = synth.make_scene(20_000, 0x5EED0002, 3.0)
+    views = _views(s.n)
+    targets = _targets(synth.perturb_means(s, 79), views, tile)
+    tbuf = ctx.storage_buffer(np.concatenate([x.reshape(-1) for x in targets]))
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 2_000_000)
+    b.set_lds_batch(128, bwd_batch)
+    cfg = AdamConfig()
+    m = [np.zeros(a.size, np.float32) for a in s.arrays()]
+    v = [np.zeros(a.size, np.float32) for a in s.arrays()]
+    acc = np.zeros(s.n, np.float32)
+    b.train_step(g, [x[0] for x in views], tbuf, 1, tile, L.LOSS_L1, L.BWD_RECOMPUTE, write_grads=True)
+    _, ref_g, ref_losses = _oracle_batch_step(s, views, targets, tile, L.BWD_RECOMPUTE, m, v, acc, 1, cfg)
+    assert np.allclose(b.losses(), ref_losses, rtol=1e-9, atol=0)
+    for name, x, y in zip(("pos", "rot", "scale", "logit", "dc", "rest"), g.grads().arrays(), ref_g.arrays()):
+        assert rel_norm_err(x, y) < 1e-4, (name, rel_norm_err(x, y))
+    b.free()
+
+
+def test_wave_culling_exact_on_elongated_and_faint_splats(ctx):
+    """Needle-like splats (axis ratios up to e^7), splats whose peak alpha
+    sits just above and below 1/255, and large flat ones: the wave and pixel
+    culls must not change a single bit of the image or the saved state."""
+    rng = np.random.default_rng(11)
+    n = 6000
+    s = synth.make_scene(n, 0x5EED0003, 1.5)
+    ls = np.full((n, 3), -4.0, np.float32)
+    ls[:, 0] = rng.uniform(-1.5, -0.5, n)
+    ls[:, 1:] = rng.uniform(-8.0, -5.0, (n, 2))
+    big = rng.random(n) < 0.1
+    ls[big] = rng.uniform(-2.0, -1.0, (int(big.sum()), 3))
+    s.log_scale[:] = ls
+    thr = math.log((1.0 / 255.0) / (1.0 - 1.0 / 255.0))
+    s.logit[:] = np.where(rng.random(n) < 0.3, thr + rng.normal(0, 0.01, n),
+                             rng.normal(0.5, 1.5, n)).astype(np.float32)
+    res = ((320, 240), (256, 256))
+    views = _views(s.n, 2, seed=9, res=res)
+    g = GpuSplatBuffer.upload(ctx, s)
+    for tile in (8, 16, 32):
+        b = ViewBatch(ctx, 2, s.n, 2 * max(w * h for w, h in res), 4_000_000)
+        b.render(g, [v[0] for v in views], tile)
+        for k, (spec, ocam, _) in enumerate(views):
+            t = O.tiling(spec.width, spec.height, tile, spec.depth_max)
+            op = O.project(s, ocam)
+            ok, opay, orr = O.bin(op, t)
+            img, ft, li = O.composite(op, opay, orr, t, with_state=True)
+            assert np.array_equal(b.image(k).view(np.uint32), img.view(np.uint32)), (tile, k)
+            gft, gli = b.state(k)
+            assert np.array_equal(gft.view(np.uint32), ft.view(np.uint32)) and np.array_equal(gli, li)
+        b.free()
+
+
+def test_autotuner_picks_a_candidate_and_trains(ctx):
+    from nanotracer_amd.autotune import autotune_batch
+
+    s = synth.make_scene(20_000, 0x5EED0002, 3.0)
+    views = _views(s.n)
+    specs = [x[0] for x in views]
+    targets = _targets(synth.perturb_means(s, 80), views, 16)
+    tbuf = ctx.storage_buffer(np.concatenate([x.reshape(-1) for x in targets]))
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 4_000_000)
+    r = autotune_batch(ctx, b, g, specs, tbuf, 1, tiles=(16, 32), fwd_batches=(128, 256), bwd_batches=(64, 128))
+    assert r.tile in (16, 32) and r.fwd_batch in (128, 256) and r.bwd_batch in (64, 128)
+    assert r.next_t == 1 + 2 * 2 * 2 and r.step_ms > 0
+    assert len([row for row in r.table if "fwd_ms" in row]) == 4
+    b.train_step(g, specs, tbuf, r.next_t, r.tile, graph=True)
+    ctx.sync()
+    assert np.isfinite(b.losses()).all()
+    b.free()
