@@ -9,24 +9,38 @@
 // the tile once every pixel is saturated.
 #include "nts_raster.h"
 
-template <int TS>
+// Thread layout of a (view, tile) workgroup with at most MAXT threads.
+// Every wave covers a square sub-tile: 8x8 pixels one per lane, or 16x16
+// as a 2x2 quad per lane when the tile has more pixels than threads; the
+// waves tile the tile row-major.  Square wave rectangles are what the
+// per-wave culling rejects best.  The forward runs one pixel per thread
+// (up to 1024 threads); the backward caps the workgroup at 256 threads,
+// which keeps its per-batch barriers and the LDS partials of its waves
+// small.
+template <int TS, int MAXT>
 struct Blend {
   static constexpr int NPIX = TS * TS;
-  static constexpr int NT = NPIX >= 256 ? 256 : NPIX;
-  static constexpr int PPT = NPIX / NT;  // 1, or 4 (a 2x2 quad per thread at TS = 32)
+  static constexpr int NT = NPIX < MAXT ? NPIX : MAXT;
+  static constexpr int PPT = NPIX / NT;  // 1 or 4
+  static constexpr int SUB = PPT == 1 ? 8 : 16;
+  static_assert(PPT == 1 || PPT == 4, "one pixel or a 2x2 quad per thread");
 };
+#define NTS_FWD_THREADS 1024
+#define NTS_BWD_THREADS 256
 
-// Tile-local pixel j of thread t: one pixel per thread in row-major order,
-// or a 2x2 quad per thread so that each wave covers a compact 32x8 strip.
-template <int TS>
+// Tile-local pixel j of thread t.
+template <int TS, int MAXT>
 __device__ __forceinline__ void tile_pixel(uint32_t t, int j, uint32_t& lx, uint32_t& ly) {
-  if (Blend<TS>::PPT == 1) {
-    lx = t % TS;
-    ly = t / TS;
+  using B = Blend<TS, MAXT>;
+  constexpr uint32_t WX = TS / B::SUB;  // waves per tile row
+  const uint32_t w = t >> 6, l = t & 63u;
+  const uint32_t bx = B::SUB * (w % WX), by = B::SUB * (w / WX);
+  if (B::PPT == 1) {
+    lx = bx + (l & 7u);
+    ly = by + (l >> 3);
   } else {
-    const uint32_t qx = t % (TS / 2), qy = t / (TS / 2);
-    lx = 2 * qx + (j & 1);
-    ly = 2 * qy + (j >> 1);
+    lx = bx + 2u * (l & 7u) + ((uint32_t)j & 1u);
+    ly = by + 2u * (l >> 3) + ((uint32_t)j >> 1);
   }
 }
 
@@ -47,13 +61,13 @@ struct WaveRect {
   float cx, cy, hw, hh;
 };
 
-template <int TS>
+template <int TS, int MAXT>
 __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
   float x0 = 3.4e38f, x1 = -3.4e38f, y0 = 3.4e38f, y1 = -3.4e38f;
 #pragma unroll
-  for (int j = 0; j < Blend<TS>::PPT; ++j) {
+  for (int j = 0; j < Blend<TS, MAXT>::PPT; ++j) {
     uint32_t lx, ly;
-    tile_pixel<TS>(threadIdx.x, j, lx, ly);
+    tile_pixel<TS, MAXT>(threadIdx.x, j, lx, ly);
     const float fx = (float)(tx * TS + lx) + 0.5f, fy = (float)(ty * TS + ly) + 0.5f;
     x0 = fminf(x0, fx);
     x1 = fmaxf(x1, fx);
@@ -67,7 +81,7 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
   return WaveRect{x0, x1, y0, y1, 0.5f * (x0 + x1), 0.5f * (y0 + y1), 0.5f * (x1 - x0), 0.5f * (y1 - y0)};
 }
 
-// Staged splat: the fields the blend reads plus a conservative bound of the
+// Staged splat (mean, conic a, 2b, c, opacity, colour) plus a conservative bound of the
 // region where alpha >= 1/255:  -0.5 x^T Q x >= -ln(255 o)  is the ellipse
 // x^T Q x <= 2 ln(255 o), whose half extents are sqrt(2 ln(255 o) Sigma_xx)
 // and sqrt(2 ln(255 o) Sigma_yy) with Sigma = Q^-1.  Extents get a 1% + 0.05 px
@@ -75,7 +89,7 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
 // one that would have failed the alpha >= 1/255 test: images and gradients
 // are unchanged.
 __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c) {
-  a = make_float4(p.a.x, p.a.y, p.b.x, p.b.y);
+  a = make_float4(p.a.x, p.a.y, p.b.x, 2.0f * p.b.y);  // 2b: exact, the blend's dx dy coefficient
   b = make_float4(p.b.z, p.c.w, p.c.x, p.c.y);
   const float o = p.c.w, qa = p.b.x, qb = p.b.y, qc = p.b.z;
   const float det = qa * qc - qb * qb;
@@ -110,7 +124,7 @@ __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float
 // unchanged.
 __device__ __forceinline__ bool wave_hits(const WaveRect& r, float4 A, float4 B, float4 Cc) {
   if (fabsf(r.cx - A.x) > r.hw + Cc.y || fabsf(r.cy - A.y) > r.hh + Cc.z) return false;
-  const float a = A.z, b = A.w, c = B.x;
+  const float a = A.z, b = 0.5f * A.w, c = B.x;
   const float det = a * c - b * b;
   if (!(Cc.w > -1e30f) || !(a > 0.0f) || !(c > 0.0f) || !(det > 0.0f)) return true;
   const float s = sqrtf(a * c) + fabsf(b);
@@ -175,12 +189,13 @@ __device__ __forceinline__ float4 pixel_loss_grad(float4 p, float4 t, float inv_
 }
 
 template <int TS, int BATCH, bool LOSS>
-__global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __restrict__ proj,
+__global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(const Projected* __restrict__ proj,
                                                              const uint32_t* __restrict__ payloads,
                                                              const uint2* __restrict__ ranges, ViewTable vt,
                                                              float4* __restrict__ out, float* __restrict__ final_t,
                                                              uint32_t* __restrict__ last_idx, LossArgs la) {
-  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT, NM = BATCH / 64;
+  using BL = Blend<TS, NTS_FWD_THREADS>;
+  constexpr int NT = BL::NT, PPT = BL::PPT, NM = BATCH / 64;
   __shared__ float4 s_a[BATCH];  // mean.x, mean.y, conic.a, conic.b
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
   __shared__ float4 s_c[BATCH];  // b, cull half-extents x / y, power threshold
@@ -191,7 +206,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
   const uint32_t tx = lt % vd.tiles_x, ty = lt / vd.tiles_x;
   const uint2 rg = ranges[gt];
   const uint32_t begin = rg.x, end = rg.y;
-  const WaveRect wr = wave_rect<TS>(tx, ty);
+  const WaveRect wr = wave_rect<TS, NTS_FWD_THREADS>(tx, ty);
 
   float pxf[PPT], pyf[PPT], T[PPT], C0[PPT], C1[PPT], C2[PPT];
   uint32_t last[PPT], pix_x[PPT], pix_y[PPT];
@@ -199,7 +214,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     uint32_t lx, ly;
-    tile_pixel<TS>(threadIdx.x, j, lx, ly);
+    tile_pixel<TS, NTS_FWD_THREADS>(threadIdx.x, j, lx, ly);
     pix_x[j] = tx * TS + lx;
     pix_y[j] = ty * TS + ly;
     inb[j] = pix_x[j] < vd.width && pix_y[j] < vd.height;
@@ -236,7 +251,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
         for (int j = 0; j < PPT; ++j) {
           if (done[j]) continue;
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
-          const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
+          const float power = -0.5f * (A.z * dx * dx + A.w * dx * dy + B.x * dy * dy);
           if (power > 0.0f || power < Cc.w) continue;
           const float g = nts_expf_nonpos(power);
           const float ar = B.y * g;
@@ -286,6 +301,9 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_fwd(const Projected* __
 //   recompute mode: T = saved final T, walk from the pixel's last
 //     contributing splat, no bail (the exact chain rule of the forward);
 //     the three divisions by 1 - a become one hardware reciprocal (1 ulp).
+// Both modes take every power / alpha decision exactly as the forward does;
+// the gradient arithmetic uses fused multiply-adds, and the recompute mode
+// the hardware exponential (within 1e-6 relative of the exact forms).
 // Each splat's nine per-pixel terms are reduce-scattered across the wave,
 // the waves' partial sums meet in LDS, and one set of float atomics per
 // (tile, splat) commits them.
@@ -303,9 +321,23 @@ __device__ __forceinline__ float dpp_f(float v) {
 #define NTS_DPP_QUAD_XOR2 0x4E  // quad_perm [2, 3, 0, 1]
 #define NTS_DPP_QUAD_XOR1 0xB1  // quad_perm [1, 0, 3, 2]
 
-// One halving stage of the reduce-scatter: lanes whose bit is clear keep
-// terms [0, H) of the N, the others terms [H, N), each adding the partner's
-// copy of the terms it keeps.
+// v_permlane32_swap(a, b) leaves (a[0:32], b[0:32]) and (a[32:64],
+// b[32:64]) in its two registers; their sum holds, in lanes 0-31, a summed
+// over each lane and lane + 32, and in lanes 32-63 the same for b: two
+// values reduce-scattered across bit 5 in one swap and one add.
+// v_permlane16_swap does the same across bit 4 (rows of 16 lanes).
+__device__ __forceinline__ float swap_sum32(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap_sum16(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// One DPP halving stage: lanes whose bit is clear keep terms [0, H) of the
+// N, the others terms [H, N), each adding the partner's copy of the terms it
+// keeps.
 template <int N, int H, int CTRL>
 __device__ __forceinline__ void rs_stage(const float (&v)[N], float (&w)[H], bool hi_half) {
 #pragma unroll
@@ -317,38 +349,46 @@ __device__ __forceinline__ void rs_stage(const float (&v)[N], float (&w)[H], boo
   }
 }
 
-// Reduce-scatter of the 9 per-lane gradient terms across the wave: four DPP
-// stages inside each row of 16 lanes, then two cross-row sums.  Afterwards
-// lane l < 16 holds the wave sum of term grad_term_of_lane(l) (none when
-// that is >= 9); rows 1-3 hold copies.
+// Reduce-scatter of the 9 per-lane gradient terms across the wave: swaps
+// across bits 5 and 4 (9 -> 5 -> 3 terms), DPP halvings across bits 3 and
+// 2 (3 -> 2 -> 1), then full sums across bits 1 and 0.  Afterwards every
+// lane of quad q = lane >> 2 holds the wave sum of term grad_term_of_quad(q)
+// (none when that is >= 9).
 __device__ __forceinline__ float wave_reduce9(const float (&v)[9]) {
   const uint32_t lane = threadIdx.x & 63u;
   float w[5], x[3], y[2], z[1];
-  rs_stage<9, 5, NTS_DPP_ROW_MIRROR>(v, w, lane & 8u);
-  rs_stage<5, 3, NTS_DPP_ROW_HALF_MIRROR>(w, x, lane & 4u);
-  rs_stage<3, 2, NTS_DPP_QUAD_XOR2>(x, y, lane & 2u);
-  rs_stage<2, 1, NTS_DPP_QUAD_XOR1>(y, z, lane & 1u);
-  float r = z[0] + __shfl_xor(z[0], 16, 64);
-  r += __shfl_xor(r, 32, 64);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) w[j] = swap_sum32(v[j], v[j + 5]);
+  w[4] = swap_sum32(v[4], 0.0f);
+  x[0] = swap_sum16(w[0], w[3]);
+  x[1] = swap_sum16(w[1], w[4]);
+  x[2] = swap_sum16(w[2], 0.0f);
+  rs_stage<3, 2, NTS_DPP_ROW_MIRROR>(x, y, lane & 8u);
+  rs_stage<2, 1, NTS_DPP_ROW_HALF_MIRROR>(y, z, lane & 4u);
+  float r = z[0] + dpp_f<NTS_DPP_QUAD_XOR2>(z[0]);
+  r += dpp_f<NTS_DPP_QUAD_XOR1>(r);
   return r;
 }
 
-// Term index held by lane l (< 16) after wave_reduce9 (9 = none).
-__device__ __forceinline__ uint32_t grad_term_of_lane(uint32_t l) {
-  const uint32_t xs = ((l & 2u) ? 2u : 0u) + (l & 1u);
+// Term index held by quad q after wave_reduce9 (9 = none): bit 5 of the
+// lane picks terms 5-8, bit 4 the upper of 5, bit 3 the upper of 3, bit 2
+// the upper of 2.
+__device__ __forceinline__ uint32_t grad_term_of_quad(uint32_t q) {
+  const uint32_t xs = ((q & 2u) ? 2u : 0u) + (q & 1u);
   if (xs >= 3u) return 9u;
-  const uint32_t ws = ((l & 4u) ? 3u : 0u) + xs;
+  const uint32_t ws = ((q & 4u) ? 3u : 0u) + xs;
   if (ws >= 5u) return 9u;
-  const uint32_t vi = ((l & 8u) ? 5u : 0u) + ws;
+  const uint32_t vi = ((q & 8u) ? 5u : 0u) + ws;
   return vi < 9u ? vi : 9u;
 }
 
 template <int TS, int BATCH, bool RECOMP>
-__global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
+__global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
     ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
     const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
-  constexpr int NT = Blend<TS>::NT, PPT = Blend<TS>::PPT, NW = NT / 64, NM = BATCH / 64;
+  using BL = Blend<TS, NTS_BWD_THREADS>;
+  constexpr int NT = BL::NT, PPT = BL::PPT, NW = NT / 64, NM = BATCH / 64;
   __shared__ float4 s_a[BATCH];
   __shared__ float4 s_b[BATCH];
   __shared__ float4 s_c[BATCH];
@@ -364,9 +404,9 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
   const uint2 rg = ranges[gt];
   const uint32_t begin = rg.x, end = rg.y;
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  const uint32_t my_term = lane < 16u ? grad_term_of_lane(lane) : 9u;
+  const uint32_t my_term = (lane & 3u) == 0u ? grad_term_of_quad(lane >> 2) : 9u;
   const bool term_lane = my_term < 9u;
-  const WaveRect wr = wave_rect<TS>(tx, ty);
+  const WaveRect wr = wave_rect<TS, NTS_BWD_THREADS>(tx, ty);
 
   float pxf[PPT], pyf[PPT], T[PPT], G0[PPT], G1[PPT], G2[PPT], S0[PPT], S1[PPT], S2[PPT];
   uint32_t lastj[PPT];
@@ -376,7 +416,7 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     uint32_t lx, ly;
-    tile_pixel<TS>(threadIdx.x, j, lx, ly);
+    tile_pixel<TS, NTS_BWD_THREADS>(threadIdx.x, j, lx, ly);
     const uint32_t px = tx * TS + lx, py = ty * TS + ly;
     const bool inb = px < vd.width && py < vd.height;
     pxf[j] = (float)px + 0.5f;
@@ -441,11 +481,27 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
         for (int j = 0; j < PPT; ++j) {
           if (done[j] || rel0 + k >= lastj[j]) continue;
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
-          const float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
+          // the forward's power and thresholds, bit for bit
+          const float power = -0.5f * (A.z * dx * dx + A.w * dx * dy + B.x * dy * dy);
           if (power > 0.0f || power < Cc.w) continue;
-          const float g = nts_expf_nonpos(power);
           const float sigma = B.y;
-          const float ar = sigma * g;
+          // recompute mode: hardware 2^x (error well under 1e-6 relative),
+          // and where alpha lies that close to 1/255 or 0.99 the forward's
+          // exact exponential, so every alpha test decides as in the
+          // forward.  Reference-parity mode keeps the exact exponential: its
+          // 1.0001 bail on the divided-back T is sensitive to the last bit.
+          float g, ar;
+          if (RECOMP) {
+            g = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
+            ar = sigma * g;
+            if (fabsf(ar - NTS_ALPHA_MIN) <= 4e-6f * NTS_ALPHA_MIN || fabsf(ar - NTS_ALPHA_MAX) <= 4e-6f) {
+              g = nts_expf_nonpos(power);
+              ar = sigma * g;
+            }
+          } else {
+            g = nts_expf_nonpos(power);
+            ar = sigma * g;
+          }
           const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
           if (alpha < NTS_ALPHA_MIN) continue;
           float s0, s1, s2;
@@ -466,27 +522,28 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
             s1 = S1[j] / oma;
             s2 = S2[j] / oma;
           }
+          // gradient terms with fused multiply-adds (no bit parity needed)
           const float ta = T[j] * alpha;
           const float c0 = B.z, c1 = B.w, c2 = Cc.x;
-          const float d0 = T[j] * c0 - s0;
-          const float d1 = T[j] * c1 - s1;
-          const float d2 = T[j] * c2 - s2;
-          const float gal = G0[j] * d0 + G1[j] * d1 + G2[j] * d2;
-          const float sat = (ar < NTS_ALPHA_MAX) ? 1.0f : 0.0f;
-          const float gar = gal * sat;
-          const float gpow = gar * sigma * g;
-          acc[0] += gpow * (A.z * dx + A.w * dy);
-          acc[1] += gpow * (A.w * dx + B.x * dy);
-          acc[2] += gar * g;
-          acc[3] += gpow * -0.5f * (dx * dx);
-          acc[4] += gpow * -0.5f * (2.0f * dx * dy);
-          acc[5] += gpow * -0.5f * (dy * dy);
-          acc[6] += G0[j] * ta;
-          acc[7] += G1[j] * ta;
-          acc[8] += G2[j] * ta;
-          S0[j] = S0[j] + ta * c0;
-          S1[j] = S1[j] + ta * c1;
-          S2[j] = S2[j] + ta * c2;
+          const float gal = __builtin_fmaf(G0[j], __builtin_fmaf(T[j], c0, -s0),
+                                           __builtin_fmaf(G1[j], __builtin_fmaf(T[j], c1, -s1),
+                                                          G2[j] * __builtin_fmaf(T[j], c2, -s2)));
+          const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;  // alpha clamp gate
+          const float gpow = gar * ar;                           // dL/dpower = dL/dalpha * sigma * G
+          const float hb = 0.5f * A.w;                           // conic b
+          const float hp = -0.5f * gpow;
+          acc[0] = __builtin_fmaf(gpow, __builtin_fmaf(A.z, dx, hb * dy), acc[0]);
+          acc[1] = __builtin_fmaf(gpow, __builtin_fmaf(hb, dx, B.x * dy), acc[1]);
+          acc[2] = __builtin_fmaf(gar, g, acc[2]);
+          acc[3] = __builtin_fmaf(hp, dx * dx, acc[3]);
+          acc[4] = __builtin_fmaf(-gpow, dx * dy, acc[4]);
+          acc[5] = __builtin_fmaf(hp, dy * dy, acc[5]);
+          acc[6] = __builtin_fmaf(G0[j], ta, acc[6]);
+          acc[7] = __builtin_fmaf(G1[j], ta, acc[7]);
+          acc[8] = __builtin_fmaf(G2[j], ta, acc[8]);
+          S0[j] = __builtin_fmaf(ta, c0, S0[j]);
+          S1[j] = __builtin_fmaf(ta, c1, S1[j]);
+          S2[j] = __builtin_fmaf(ta, c2, S2[j]);
           contrib = true;
         }
         if (__ballot(contrib)) {
@@ -501,26 +558,20 @@ __global__ void __launch_bounds__(Blend<TS>::NT) k_blend_bwd(
       for (int i = 0; i < NM; ++i) s_cm[wave][i] = cm[i];
     }
     __syncthreads();
-    // one set of float atomics per (tile, splat): sum the partials of the
-    // waves that wrote one
-    for (uint32_t k = threadIdx.x; k < take; k += NT) {
+    // one set of float atomics per (tile, splat): each thread sums one
+    // (slot, term) over the waves that wrote a partial for the slot, so the
+    // nine terms of a splat are consecutive lanes and consecutive words
+    for (uint32_t it = threadIdx.x; it < take * 9u; it += NT) {
+      const uint32_t k = it / 9u, q = it - k * 9u;
       const uint64_t bit = 1ull << (k & 63u);
-      float sum[9];
+      float sum = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) sum[q] = 0.0f;
-      bool any = false;
-#pragma unroll
-      for (int w = 0; w < NW; ++w) {
-        if (!(s_cm[w][k >> 6] & bit)) continue;
-        any = true;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) sum[q] += s_part[w][k][q];
+      for (int w = 0; w < NW; ++w)
+        if (s_cm[w][k >> 6] & bit) sum += s_part[w][k][q];
+      if (sum != 0.0f) {
+        float* o = reinterpret_cast<float*>(pgrad + vd.proj_base + s_idx[k]);
+        unsafeAtomicAdd(o + (q < 3u ? q : (q < 6u ? q + 1u : q + 2u)), sum);
       }
-      if (!any) continue;
-      float* o = reinterpret_cast<float*>(pgrad + vd.proj_base + s_idx[k]);
-#pragma unroll
-      for (int q = 0; q < 9; ++q)
-        if (sum[q] != 0.0f) unsafeAtomicAdd(o + (q < 3 ? q : (q < 6 ? q + 1 : q + 2)), sum[q]);
     }
     rem -= take;
   }
@@ -558,10 +609,10 @@ static int launch_fwd(nts_ctx* ctx, dim3 grid, bool loss, const Projected* proj,
                        const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx,
                        const LossArgs& la) {
   if (loss)
-    NTS_LAUNCH(ctx, "blend_fwd_loss", (k_blend_fwd<TS, B, true>), grid, dim3(Blend<TS>::NT), 0, proj, payloads,
+    NTS_LAUNCH(ctx, "blend_fwd_loss", (k_blend_fwd<TS, B, true>), grid, dim3(Blend<TS, NTS_FWD_THREADS>::NT), 0, proj, payloads,
                ranges, vt, out, final_t, last_idx, la);
   else
-    NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<TS, B, false>), grid, dim3(Blend<TS>::NT), 0, proj, payloads, ranges,
+    NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<TS, B, false>), grid, dim3(Blend<TS, NTS_FWD_THREADS>::NT), 0, proj, payloads, ranges,
                vt, out, final_t, last_idx, la);
   return NTS_OK;
 }
@@ -599,10 +650,10 @@ static int launch_bwd(nts_ctx* ctx, dim3 grid, int mode, const Projected* proj, 
                        const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
                        const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
   if (mode == NTS_BWD_RECOMPUTE)
-    NTS_LAUNCH(ctx, "blend_bwd_recompute", (k_blend_bwd<TS, B, true>), grid, dim3(Blend<TS>::NT), 0, proj, payloads,
+    NTS_LAUNCH(ctx, "blend_bwd_recompute", (k_blend_bwd<TS, B, true>), grid, dim3(Blend<TS, NTS_BWD_THREADS>::NT), 0, proj, payloads,
                ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
   else
-    NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd<TS, B, false>), grid, dim3(Blend<TS>::NT), 0, proj, payloads, ranges,
+    NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd<TS, B, false>), grid, dim3(Blend<TS, NTS_BWD_THREADS>::NT), 0, proj, payloads, ranges,
                vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
   return NTS_OK;
 }
