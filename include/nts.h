@@ -268,6 +268,27 @@ int nts_batch_download_projected_grad(nts_ctx* ctx, const nts_batch* batch, uint
 int nts_batch_download_bins(nts_ctx* ctx, const nts_batch* batch, uint32_t view, uint64_t* count, uint32_t* keys,
                             uint32_t* payloads, uint32_t* ranges);
 
+/* ---- prune / densify (train.rs:414-602, run on the device) -------------- */
+/* AdamGpu::zero_grad_acc (adam_gpu.rs:492-541): grad_acc = 0. */
+int nts_scene_zero_grad_acc(nts_ctx* ctx, nts_scene* scene);
+/* prune_low_opacity (train.rs:572-602): removes splats with logit < threshold
+ * (the reference uses -5.3) from parameters, both Adam moments and grad_acc,
+ * leaving exactly the order of the reference's descending swap_remove.
+ * *removed receives the count.  Synchronises with the host. */
+int nts_prune(nts_ctx* ctx, nts_scene* scene, float logit_threshold, uint32_t* removed);
+/* densify (train.rs:414-549): candidates grad_acc / max(grad_count, 1) >
+ * grad_threshold in index order up to max_splats - n; clone when the largest
+ * log-scale is <= split_log_scale (copy appended, both logits + ln 1/2),
+ * else split (two children at parent + q (N(0,1) sigma), log-scales - ln 1.6,
+ * the LCG stream seeded iter * 1664525 + 1013904223; parents removed by
+ * descending swap_remove).  New splats get zero moments and grad_acc.
+ * counts = {cloned, split children, parents removed}.  The scene needs room
+ * for n + clones + 2 splits (NTS_E_CAPACITY otherwise).  grad_acc is left as
+ * the reference's densify leaves it; the reference zeroes it next
+ * (train.rs:327-331): nts_scene_zero_grad_acc.  Synchronises with the host. */
+int nts_densify(nts_ctx* ctx, nts_scene* scene, uint32_t grad_count, float grad_threshold, float split_log_scale,
+                uint32_t max_splats, uint32_t iter, uint32_t counts[3]);
+
 /* ---- viewer (Rasterizer::tonemap, raster.rs:405-455) ------------------- */
 int nts_tonemap(nts_ctx* ctx, const nts_buffer* image, nts_buffer* rgba8, uint32_t width, uint32_t height);
 

@@ -147,6 +147,10 @@ _SIGS = {
                                        C.POINTER(AdamCfg), C.c_uint64, C.POINTER(Constraints), C.c_int]),
     "nts_render_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ViewDesc), C.c_uint32,
                                    C.c_uint32]),
+    "nts_scene_zero_grad_acc": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "nts_prune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.POINTER(C.c_uint32)]),
+    "nts_densify": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
+                              C.POINTER(C.c_uint32)]),
     "nts_batch_set_lds_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "nts_batch_losses": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nts_batch_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),

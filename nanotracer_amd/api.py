@@ -53,6 +53,12 @@ class Buffer:
             pass
 
 
+# prune / densify constants (train.rs:103-114)
+PRUNE_OPACITY_LOGIT = -5.3
+DENSIFY_GRAD_THRESHOLD = 2e-4
+SPLIT_SCALE_THRESHOLD = -2.3
+
+
 class GpuCtx:
     """WgpuCtx (gpu.rs:21-152) over a HIP device and stream."""
 
@@ -175,6 +181,28 @@ class GpuSplatBuffer:
     def set_grad_acc(self, a: np.ndarray) -> None:
         a = np.ascontiguousarray(a, dtype=np.float32)
         L.check(L.lib().nts_scene_upload_grad_acc(self.ctx.h, self.h, L.vptr(a)))
+
+    def zero_grad_acc(self) -> None:
+        """AdamGpu::zero_grad_acc (adam_gpu.rs:492-541)."""
+        L.check(L.lib().nts_scene_zero_grad_acc(self.ctx.h, self.h))
+
+    def prune(self, logit_threshold: float = PRUNE_OPACITY_LOGIT) -> int:
+        """prune_low_opacity (train.rs:572-602) on the device; returns the
+        number removed."""
+        removed = C.c_uint32()
+        L.check(L.lib().nts_prune(self.ctx.h, self.h, C.c_float(logit_threshold), C.byref(removed)))
+        return int(removed.value)
+
+    def densify(self, grad_count: int, max_splats: int, iteration: int,
+                grad_threshold: float = DENSIFY_GRAD_THRESHOLD,
+                split_log_scale: float = SPLIT_SCALE_THRESHOLD):
+        """densify (train.rs:414-549) on the device; returns (cloned, split
+        children, parents removed).  The scene needs room for the clones and
+        both children of every split."""
+        counts = (C.c_uint32 * 3)()
+        L.check(L.lib().nts_densify(self.ctx.h, self.h, int(grad_count), C.c_float(grad_threshold),
+                                    C.c_float(split_log_scale), int(max_splats), int(iteration), counts))
+        return tuple(int(c) for c in counts)
 
     def free(self) -> None:
         if self.h:

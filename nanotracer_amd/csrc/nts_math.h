@@ -102,3 +102,101 @@ NTS_HD int32_t nts_maxi(int32_t a, int32_t b) { return (a < b) ? b : a; }
 #define NTS_ONE_MINUS_ALPHA_EPS 1e-4f
 #define NTS_T_BAIL 1.0001f
 #define NTS_COV_DILATION 0.3f
+
+// ---- densify random numbers (train.rs:552-564) ------------------------------
+// nts_logf: x = 2^e m with m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s),
+// s = (m - 1)/(m + 1), |s| <= 0.1716, series to s^11 (truncation below
+// 1e-10), e ln2 with the two-word ln2 of nts_expf.  Positive normal x only
+// (the densify draws are in [1e-8, 1)).
+NTS_HD float nts_logf(float x) {
+  uint32_t b = nts_f32_to_bits(x);
+  int e = (int)((b >> 23) & 0xffu) - 127;
+  float m = nts_bits_to_f32((b & 0x007fffffu) | 0x3f800000u);  // [1, 2)
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  float p = 1.0f / 11.0f;
+  p = nts_fmaf(p, s2, 1.0f / 9.0f);
+  p = nts_fmaf(p, s2, 1.0f / 7.0f);
+  p = nts_fmaf(p, s2, 1.0f / 5.0f);
+  p = nts_fmaf(p, s2, 1.0f / 3.0f);
+  const float lm = nts_fmaf(2.0f * s * s2, p, 2.0f * s);
+  const float fe = (float)e;
+  return nts_fmaf(fe, 0.693359375f, nts_fmaf(fe, -2.12194440e-4f, lm));
+}
+
+// cos(2 pi u): v = u - rint(u) in [-1/2, 1/2] (exact), a = |v|, then the
+// octant reduction a -> 1/4 - a or 1/2 - a (exact for the octants that use
+// them) to an argument x = 2 pi w with |x| <= pi/4 and the Taylor series of
+// cos (to x^10) or sin (to x^11).
+NTS_HD float nts_cos2pi(float u) {
+  const float a = __builtin_fabsf(u - __builtin_rintf(u));
+  float w, sign = 1.0f;
+  bool use_sin = false;
+  if (a <= 0.125f) {
+    w = a;
+  } else if (a <= 0.375f) {
+    w = 0.25f - a;
+    use_sin = true;
+  } else {
+    w = 0.5f - a;
+    sign = -1.0f;
+  }
+  const float x = w * 6.28318548f;
+  const float x2 = x * x;
+  float r;
+  if (use_sin) {
+    float p = 2.50521084e-8f;            //  1/11!
+    p = nts_fmaf(p, -x2, 2.75573192e-6f);  //  1/9!
+    p = nts_fmaf(p, -x2, 1.98412698e-4f);  //  1/7!
+    p = nts_fmaf(p, -x2, 8.33333333e-3f);  //  1/5!
+    p = nts_fmaf(p, -x2, 1.66666667e-1f);  //  1/3!
+    r = nts_fmaf(-x * x2, p, x);
+  } else {
+    float p = 2.75573192e-7f;            //  1/10!
+    p = nts_fmaf(p, -x2, 2.48015873e-5f);  //  1/8!
+    p = nts_fmaf(p, -x2, 1.38888889e-3f);  //  1/6!
+    p = nts_fmaf(p, -x2, 4.16666667e-2f);  //  1/4!
+    p = nts_fmaf(p, -x2, 0.5f);            //  1/2!
+    r = nts_fmaf(-x2, p, 1.0f);
+  }
+  return sign * r;
+}
+
+// The densify LCG (train.rs:552-556): state' = 1664525 state + 1013904223
+// (mod 2^32), uniform = state' / 2^32.
+NTS_HD float nts_lcg01(uint32_t* st) {
+  *st = *st * 1664525u + 1013904223u;
+  return (float)(*st) / 4294967296.0f;
+}
+
+// State after k steps, by squaring the affine map s -> a s + c.
+NTS_HD uint32_t nts_lcg_jump(uint32_t st, uint64_t k) {
+  uint32_t a = 1664525u, c = 1013904223u;  // the map applied 2^bit times
+  uint32_t A = 1u, C = 0u;                 // accumulated map
+  while (k) {
+    if (k & 1u) {
+      A = a * A;
+      C = a * C + c;
+    }
+    c = a * c + c;
+    a = a * a;
+    k >>= 1;
+  }
+  return A * st + C;
+}
+
+// train.rs:558-564: sqrt(-2 ln max(u1, 1e-8)) cos(2 pi u2).
+NTS_HD float nts_box_muller(uint32_t* st) {
+  const float u1 = nts_maxf(nts_lcg01(st), 1e-8f);
+  const float u2 = nts_lcg01(st);
+  return __builtin_sqrtf(-2.0f * nts_logf(u1)) * nts_cos2pi(u2);
+}
+
+// ln(1/2) and -ln(1.6) rounded to float: the clone opacity step and the
+// split scale step (train.rs:455, 489).
+#define NTS_LN_HALF (-0.693147182f)
+#define NTS_NEG_LN_1_6 (-0.470003635f)

@@ -329,6 +329,27 @@ void orc_expf_array(const float* x, float* y, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) y[i] = nts_expf(x[i]);
 }
 
+void orc_logf_array(const float* x, float* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = nts_logf(x[i]);
+}
+
+void orc_cos2pi_array(const float* x, float* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = nts_cos2pi(x[i]);
+}
+
+uint32_t orc_lcg_jump(uint32_t state, uint64_t k) { return nts_lcg_jump(state, k); }
+
+// n draws of the densify Box-Muller stream from `state`, plus the uniforms
+// each consumed (2 per draw) for checking the transform.
+void orc_box_muller_stream(uint32_t state, uint64_t n, float* out, float* uniforms) {
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t probe = state;
+    uniforms[2 * i] = nts_lcg01(&probe);
+    uniforms[2 * i + 1] = nts_lcg01(&probe);
+    out[i] = nts_box_muller(&state);
+  }
+}
+
 // raster.rs:79-99 CameraUniform::from_pose with glam's right-handed look-at
 // (view = [s | u | -f] rows, translation -eye.s, -eye.u, eye.f).
 void orc_camera_from_pose(const float eye[3], const float target[3], const float up[3], float fov_y,
@@ -974,16 +995,10 @@ void push_splat(orc_splats_mut* s, uint32_t n, const float* pos, const float* ro
   if (grad_acc) grad_acc[n] = 0.0f;
 }
 
-// train.rs:552-564
-float lcg01(uint32_t* st) {
-  *st = *st * 1664525u + 1013904223u;
-  return (float)(*st) / (4294967295.0f + 1.0f);
-}
-float box_muller(uint32_t* st) {
-  const float u1 = nts_maxf(lcg01(st), 1e-8f);
-  const float u2 = lcg01(st);
-  return sqrtf(-2.0f * logf(u1)) * cosf(2.0f * 3.14159265358979323846f * u2);
-}
+// train.rs:552-564 with the shared scalar functions (nts_box_muller): the
+// logarithm and cosine bits of the reference's libm are not reproducible,
+// so both sides use the same restated functions.
+float box_muller(uint32_t* st) { return nts_box_muller(st); }
 }  // namespace
 
 uint32_t orc_prune(orc_splats_mut* s, uint32_t n, float threshold, float** moments, float* grad_acc) {
@@ -1018,8 +1033,8 @@ uint32_t orc_densify(orc_splats_mut* s, uint32_t n, uint32_t grad_count, float g
     }
   }
   uint32_t rng = iter * 1664525u + 1013904223u;
-  const float half = -0.6931471805599453f;
-  const float log_split = -logf(1.6f);
+  const float half = NTS_LN_HALF;          // -LN_2 as f32 (train.rs:462)
+  const float log_split = NTS_NEG_LN_1_6;  // -(1.6f32).ln() (train.rs:463)
   for (uint32_t i : to_clone) {
     const float op_new = s->logit[i] + half;
     float pos[3], rot[4], sc[3], dc[3], rest[45];
@@ -1042,7 +1057,7 @@ uint32_t orc_densify(orc_splats_mut* s, uint32_t n, uint32_t grad_count, float g
     memcpy(rest, s->sh_rest + 45 * (size_t)i, 180);
     const float op = s->logit[i];
     const float nsc[3] = {sc[0] + log_split, sc[1] + log_split, sc[2] + log_split};
-    const float sg[3] = {expf(sc[0]), expf(sc[1]), expf(sc[2])};
+    const float sg[3] = {nts_expf(sc[0]), nts_expf(sc[1]), nts_expf(sc[2])};
     // glam Quat (x, y, z, w) * Vec3: v + 2w(q x v) + 2 q x (q x v)
     const float qw = rot[0], qx = rot[1], qy = rot[2], qz = rot[3];
     for (int child = 0; child < 2; ++child) {

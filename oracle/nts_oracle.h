@@ -58,6 +58,12 @@ int orc_num_threads(void);
 
 // exp / sigmoid from the shared scalar header (pinned against float64 exp).
 void orc_expf_array(const float* x, float* y, uint64_t n);
+// logarithm / cos(2 pi u) / LCG / Box-Muller of the densify step, shared
+// with the device (nts_math.h).
+void orc_logf_array(const float* x, float* y, uint64_t n);
+void orc_cos2pi_array(const float* x, float* y, uint64_t n);
+uint32_t orc_lcg_jump(uint32_t state, uint64_t k);
+void orc_box_muller_stream(uint32_t state, uint64_t n, float* out, float* uniforms);
 
 void orc_camera_from_pose(const float eye[3], const float target[3], const float up[3],
                           float fov_y, uint32_t width, uint32_t height, uint32_t n_splats,

@@ -90,6 +90,32 @@ def expf(x: np.ndarray) -> np.ndarray:
     return y
 
 
+def logf(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_logf_array(_p(x), _p(y), C.c_uint64(x.size))
+    return y
+
+
+def cos2pi(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_cos2pi_array(_p(x), _p(y), C.c_uint64(x.size))
+    return y
+
+
+def lcg_jump(state: int, k: int) -> int:
+    lib().orc_lcg_jump.restype = C.c_uint32
+    return int(lib().orc_lcg_jump(C.c_uint32(state), C.c_uint64(k)))
+
+
+def box_muller_stream(state: int, n: int):
+    out = np.empty(n, np.float32)
+    uni = np.empty(2 * n, np.float32)
+    lib().orc_box_muller_stream(C.c_uint32(state), C.c_uint64(n), _p(out), _p(uni))
+    return out, uni.reshape(n, 2)
+
+
 def camera_from_pose(eye, target, up, fov_y, width, height, n_splats) -> Camera:
     f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
     cam = Camera()

@@ -94,3 +94,54 @@ def test_sh_basis_is_the_real_spherical_harmonics():
         pos = sgn * refs > 1e-3
         assert pos.sum() > 3
         assert np.allclose(vals[pos], sgn * refs[pos], rtol=2e-5, atol=2e-6), (k, l, m)
+
+
+def test_logf_within_two_ulp():
+    x = np.concatenate([np.geomspace(1e-8, 1.0, 2_000_001), np.linspace(0.5, 2.0, 400_001),
+                        np.geomspace(1e-30, 1e30, 200_001)]).astype(np.float32)
+    y = O.logf(x).astype(np.float64)
+    ref = np.log(x.astype(np.float64))
+    # near x = 1 the result is tiny: measure in ulps of the result, with an
+    # absolute floor of one ulp of 1e-7 there
+    ulp = np.maximum(np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64), 2.0 ** -47)
+    assert (np.abs(y - ref) / ulp).max() <= 2.0
+    assert O.logf(np.array([1.0], np.float32))[0] == 0.0
+
+
+def test_cos2pi_accuracy_and_symmetry():
+    u = np.concatenate([np.linspace(0.0, 1.0, 2_000_001, endpoint=False),
+                        np.array([0.0, 0.125, 0.25, 0.375, 0.5, 0.625, 0.75, 0.875])]).astype(np.float32)
+    y = O.cos2pi(u).astype(np.float64)
+    ref = np.cos(2.0 * np.pi * u.astype(np.float64))
+    assert np.abs(y - ref).max() <= 1.5e-7
+    # closed forms at the octant boundaries
+    c = O.cos2pi(np.array([0.0, 0.25, 0.5, 0.75], np.float32))
+    assert c[0] == 1.0 and abs(c[1]) < 1e-7 and c[2] == -1.0 and abs(c[3]) < 1e-7
+    # cos(2 pi u) = cos(2 pi (1 - u)), on multiples of 2^-20 where 1 - u is exact
+    v = (np.arange(1, 2 ** 19, 37) / 2.0 ** 20).astype(np.float32)
+    assert np.array_equal(O.cos2pi(v), O.cos2pi((1.0 - v.astype(np.float64)).astype(np.float32)))
+
+
+def test_lcg_jump_equals_stepping():
+    # train.rs:552-556: s' = 1664525 s + 1013904223 mod 2^32
+    s0 = 7 * 1664525 + 1013904223 & 0xFFFFFFFF
+    s = s0
+    for k in range(1, 600):
+        s = (s * 1664525 + 1013904223) & 0xFFFFFFFF
+        if k in (1, 2, 3, 12, 13, 24, 255, 256, 599):
+            assert O.lcg_jump(s0, k) == s
+    assert O.lcg_jump(s0, 0) == s0
+
+
+def test_box_muller_is_the_transform_of_its_uniforms():
+    out, uni = O.box_muller_stream(123456789, 200_000)
+    u1 = np.maximum(uni[:, 0].astype(np.float64), 1e-8)
+    ref = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * uni[:, 1].astype(np.float64))
+    assert np.abs(out - ref).max() <= 2e-6 * np.maximum(1.0, np.abs(ref)).max()
+    # the uniforms are the LCG's state / 2^32
+    s = 123456789
+    for i in range(4):
+        s = (s * 1664525 + 1013904223) & 0xFFFFFFFF
+        assert uni[i // 2, i % 2] == np.float32(s / 2.0 ** 32)
+        s2 = s
+    assert abs(out.mean()) < 0.01 and abs(out.std() - 1.0) < 0.01
