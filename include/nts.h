@@ -268,6 +268,53 @@ int nts_batch_download_projected_grad(nts_ctx* ctx, const nts_batch* batch, uint
 int nts_batch_download_bins(nts_ctx* ctx, const nts_batch* batch, uint32_t view, uint64_t* count, uint32_t* keys,
                             uint32_t* payloads, uint32_t* ranges);
 
+/* ---- render server (no reference counterpart: the viewer renders one view
+ *      per frame, nano-view/src/lib.rs:761-796) --------------------------- */
+/* Many scenes resident in HBM; each request is one camera of one scene and
+ * is admitted into the batch being formed while the batch's views, pixels,
+ * projected records, tiles and estimated (splat, tile) pairs fit a slot's
+ * workspace.  `slots` batches are in flight at once, each on its own
+ * stream; a dispatcher thread retires finished batches and refills free
+ * slots from the FIFO of pending requests (continuous batching).  Every
+ * result is bit-identical to rendering its request alone. */
+typedef struct nts_server nts_server;
+typedef struct nts_server_cfg {
+  uint64_t hbm_budget;     /* bytes for resident scenes + slot workspaces (0 = no limit) */
+  uint32_t slots;          /* batches in flight (1..8) */
+  uint32_t max_views;      /* requests per batch (<= 32) */
+  uint64_t max_pixels;     /* per batch */
+  uint64_t max_records;    /* per batch: sum of the requested scenes' splat counts */
+  uint64_t pair_capacity;  /* per batch (splat, tile) pairs */
+  uint32_t tile_size;      /* 8, 16 or 32 */
+  uint32_t lds_batch;      /* blend forward LDS batch: 64, 128, 256 or 512 */
+} nts_server_cfg;
+typedef struct nts_serve_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t batches, views, overflow_reruns;
+  uint32_t max_in_flight, scenes, pending, pad;
+  uint64_t hbm_in_use;
+} nts_serve_stats;
+typedef struct nts_request_info {
+  double latency_ms;     /* submit to result in the caller's buffer */
+  uint64_t pairs;        /* (splat, tile) pairs of the view */
+  uint32_t batch_views;  /* views in the batch that rendered it */
+  int status;            /* NTS_OK or the failure code */
+} nts_request_info;
+#define NTS_OUT_RGBA8 0  /* tonemapped (tonemap.wgsl), 4 B per pixel */
+#define NTS_OUT_F32 1    /* (r, g, b, 1 - T) float, 16 B per pixel */
+int nts_server_create(int device, const nts_server_cfg* cfg, nts_server** out);
+int nts_server_destroy(nts_server* server);
+/* Uploads a resident scene (parameters only); NTS_E_NOMEM over the budget. */
+int nts_server_add_scene(nts_server* server, const nts_splats_host* splats, uint32_t* scene_id);
+/* Queues a render; `out` (host, width * height * 4 or 16 bytes) must stay
+ * valid until the request is polled done or waited for. */
+int nts_server_submit(nts_server* server, uint32_t scene_id, const nts_view_desc* view, int format, void* out,
+                      uint64_t* request_id);
+/* Non-blocking: *done = 1 once finished (the request is then released). */
+int nts_server_poll(nts_server* server, uint64_t request_id, int* done, nts_request_info* info);
+int nts_server_wait(nts_server* server, uint64_t request_id, nts_request_info* info);
+int nts_server_stats(nts_server* server, nts_serve_stats* out);
+
 /* ---- prune / densify (train.rs:414-602, run on the device) -------------- */
 /* AdamGpu::zero_grad_acc (adam_gpu.rs:492-541): grad_acc = 0. */
 int nts_scene_zero_grad_acc(nts_ctx* ctx, nts_scene* scene);

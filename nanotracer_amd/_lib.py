@@ -14,6 +14,7 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libnts.so"
 
 NTS_OK = 0
+NTS_E_INVALID, NTS_E_HIP, NTS_E_NOMEM, NTS_E_CAPACITY, NTS_E_STATE = -1, -2, -3, -4, -5
 ERRORS = {-1: "NTS_E_INVALID", -2: "NTS_E_HIP", -3: "NTS_E_NOMEM", -4: "NTS_E_CAPACITY", -5: "NTS_E_STATE"}
 LOSS_L1, LOSS_MSE = 0, 1
 BWD_REF_PARITY, BWD_RECOMPUTE = 0, 1
@@ -75,6 +76,31 @@ class ViewDesc(C.Structure):
 
 
 STEP_GRAPH, STEP_WRITE_GRADS = 1, 2
+OUT_RGBA8, OUT_F32 = 0, 1
+
+
+class ServerCfg(C.Structure):
+    """nts_server_cfg."""
+
+    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("max_views", C.c_uint32),
+                ("max_pixels", C.c_uint64), ("max_records", C.c_uint64), ("pair_capacity", C.c_uint64),
+                ("tile_size", C.c_uint32), ("lds_batch", C.c_uint32)]
+
+
+class ServeStats(C.Structure):
+    """nts_serve_stats."""
+
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("batches", C.c_uint64), ("views", C.c_uint64), ("overflow_reruns", C.c_uint64),
+                ("max_in_flight", C.c_uint32), ("scenes", C.c_uint32), ("pending", C.c_uint32),
+                ("pad", C.c_uint32), ("hbm_in_use", C.c_uint64)]
+
+
+class RequestInfo(C.Structure):
+    """nts_request_info."""
+
+    _fields_ = [("latency_ms", C.c_double), ("pairs", C.c_uint64), ("batch_views", C.c_uint32),
+                ("status", C.c_int)]
 
 assert C.sizeof(Camera) == 96
 assert C.sizeof(ViewDesc) == 112
@@ -147,6 +173,14 @@ _SIGS = {
                                        C.POINTER(AdamCfg), C.c_uint64, C.POINTER(Constraints), C.c_int]),
     "nts_render_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ViewDesc), C.c_uint32,
                                    C.c_uint32]),
+    "nts_server_create": (C.c_int, [C.c_int, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
+    "nts_server_destroy": (C.c_int, [C.c_void_p]),
+    "nts_server_add_scene": (C.c_int, [C.c_void_p, C.POINTER(SplatsHost), C.POINTER(C.c_uint32)]),
+    "nts_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(ViewDesc), C.c_int, C.c_void_p,
+                                    C.POINTER(C.c_uint64)]),
+    "nts_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(RequestInfo)]),
+    "nts_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestInfo)]),
+    "nts_server_stats": (C.c_int, [C.c_void_p, C.POINTER(ServeStats)]),
     "nts_scene_zero_grad_acc": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nts_prune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.POINTER(C.c_uint32)]),
     "nts_densify": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float, C.c_uint32, C.c_uint32,

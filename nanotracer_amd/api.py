@@ -519,3 +519,68 @@ class ViewBatch:
         L.check(L.lib().nts_batch_download_bins(self.ctx.h, self.h, v, C.byref(cnt), L.vptr(keys), L.vptr(pay),
                                                 L.vptr(rng)))
         return keys[:k], pay[:k], rng
+
+
+class RenderServer:
+    """The render server (nts_server): scenes resident in HBM, requests
+    batched as they arrive, `slots` batches in flight on their own streams.
+
+    submit() returns a request id at once; the image lands in the array it
+    returns once poll() reports the request done (or wait() returns)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 2, max_views: int = 32,
+                 max_pixels: int = 16 << 20, max_records: int = 16 << 20, pair_capacity: int = 120_000_000,
+                 tile_size: int = 16, lds_batch: int = 128):
+        cfg = L.ServerCfg(int(hbm_budget), int(slots), int(max_views), int(max_pixels), int(max_records),
+                          int(pair_capacity), int(tile_size), int(lds_batch))
+        h = C.c_void_p()
+        L.check(L.lib().nts_server_create(int(device), C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._out = {}  # request id -> output array (kept alive until done)
+
+    def close(self) -> None:
+        if self.h:
+            L.lib().nts_server_destroy(self.h)
+            self.h = None
+
+    def add_scene(self, splats: HostSplats) -> int:
+        sid = C.c_uint32()
+        L.check(L.lib().nts_server_add_scene(self.h, C.byref(_host_struct(splats)), C.byref(sid)))
+        return int(sid.value)
+
+    def submit(self, scene: int, view: ViewSpec, fmt: int = L.OUT_RGBA8):
+        if fmt == L.OUT_RGBA8:
+            out = np.zeros((view.height, view.width, 4), np.uint8)
+        else:
+            out = np.zeros((view.height, view.width, 4), np.float32)
+        rid = C.c_uint64()
+        vd = view.c()
+        L.check(L.lib().nts_server_submit(self.h, int(scene), C.byref(vd), int(fmt), L.vptr(out), C.byref(rid)))
+        self._out[rid.value] = out
+        return int(rid.value), out
+
+    def poll(self, rid: int):
+        """None while pending, else (image, info dict)."""
+        done = C.c_int()
+        info = L.RequestInfo()
+        try:
+            L.check(L.lib().nts_server_poll(self.h, int(rid), C.byref(done), C.byref(info)))
+        except L.NtsError:
+            self._out.pop(rid, None)
+            raise
+        if not done.value:
+            return None
+        return self._out.pop(rid), dict(latency_ms=info.latency_ms, pairs=info.pairs, batch_views=info.batch_views)
+
+    def wait(self, rid: int):
+        info = L.RequestInfo()
+        try:
+            L.check(L.lib().nts_server_wait(self.h, int(rid), C.byref(info)))
+        finally:
+            out = self._out.pop(rid, None)
+        return out, dict(latency_ms=info.latency_ms, pairs=info.pairs, batch_views=info.batch_views)
+
+    def stats(self) -> dict:
+        st = L.ServeStats()
+        L.check(L.lib().nts_server_stats(self.h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in L.ServeStats._fields_ if name != "pad"}

@@ -109,16 +109,16 @@ __global__ void __launch_bounds__(256) k_project_views(const float* __restrict__
 
 // Segment bounds per view from the scanned counts, clamped to the pair
 // capacity (overflow is flagged, never written past).
-__global__ void k_view_segments(const uint32_t* __restrict__ offsets, uint32_t n, uint32_t n_views,
-                                const uint64_t* __restrict__ d_total, uint64_t cap, uint64_t* __restrict__ seg_begin,
-                                uint32_t* __restrict__ flags) {
+__global__ void k_view_segments(const uint32_t* __restrict__ offsets, RecTable rt, const uint64_t* __restrict__ d_total,
+                                uint64_t cap, uint64_t* __restrict__ seg_begin, uint32_t* __restrict__ flags) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint64_t total = *d_total;
-  for (uint32_t v = 0; v < n_views; ++v) {
-    const uint64_t b = (n == 0) ? 0 : (uint64_t)offsets[(size_t)v * n];
+  const uint32_t nrec = rt.base[rt.n];
+  for (uint32_t v = 0; v < rt.n; ++v) {
+    const uint64_t b = rt.base[v] < nrec ? (uint64_t)offsets[rt.base[v]] : total;
     seg_begin[v] = b < cap ? b : cap;
   }
-  seg_begin[n_views] = total < cap ? total : cap;
+  seg_begin[rt.n] = total < cap ? total : cap;
   flags[0] = total > cap ? 1u : 0u;
   flags[2] = (uint32_t)total;
   flags[3] = (uint32_t)(total >> 32);
@@ -377,18 +377,22 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
                scene->cap, n, (const DevCamera*)b->d_cams, pl.tt, b->proj, b->counts, b->flags + 1);
   rc = nts_scan_u32_ws(ctx, b->counts, b->counts, vn, b->d_total, b->scan_ws);
   if (rc) return rc;
-  NTS_LAUNCH(ctx, "view_segments", k_view_segments, dim3(1), dim3(64), 0, (const uint32_t*)b->counts, n, V,
+  RecTable rt;
+  memset(&rt, 0, sizeof rt);
+  rt.n = V;
+  for (uint32_t v = 0; v <= V; ++v) rt.base[v] = v * n;
+  NTS_LAUNCH(ctx, "view_segments", k_view_segments, dim3(1), dim3(64), 0, (const uint32_t*)b->counts, rt,
              (const uint64_t*)b->d_total, b->pair_cap, b->seg_begin, b->flags);
   bool alt = false;
   if (n > 0) {
     const uint32_t nrec = (uint32_t)vn;
     if (pl.key_bytes == 4)
       NTS_LAUNCH(ctx, "emit_views", k_emit_balanced<uint32_t>, dim3(nts_div_up(nrec, 256)), dim3(256), 0,
-                 (const Projected*)b->proj, n, nrec, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
+                 (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
                  b->pair_cap, (uint32_t*)b->keys_a, b->vals_a);
     else
       NTS_LAUNCH(ctx, "emit_views", k_emit_balanced<uint64_t>, dim3(nts_div_up(nrec, 256)), dim3(256), 0,
-                 (const Projected*)b->proj, n, nrec, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
+                 (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
                  b->pair_cap, (uint64_t*)b->keys_a, b->vals_a);
   }
   if (pl.key_bytes == 4)

@@ -128,15 +128,19 @@ extern "C" int nts_bin(nts_ctx* ctx, const nts_buffer* projected, uint32_t n, co
   memset(&tt, 0, sizeof tt);
   tt.n = 1;
   tt.t[0] = vt;
+  RecTable rt;
+  memset(&rt, 0, sizeof rt);
+  rt.n = 1;
+  rt.base[1] = n;
   if (kb == 4) {
     NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint32_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
-               (const Projected*)projected->ptr, n, n, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
+               (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
                total, (uint32_t*)keys_a, vals_a);
     rc = nts_sort_pairs32_dev(ctx, (uint32_t*)keys_a, vals_a, (uint32_t*)keys_b, vals_b, seg_begin, 1, total,
                               key_bits, &alt);
   } else {
     NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint64_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
-               (const Projected*)projected->ptr, n, n, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
+               (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
                total, (uint64_t*)keys_a, vals_a);
     rc = nts_sort_pairs_dev(ctx, (uint64_t*)keys_a, vals_a, (uint64_t*)keys_b, vals_b, seg_begin, 1, total,
                             key_bits, &alt);

@@ -816,6 +816,12 @@ __global__ void __launch_bounds__(256) k_tonemap(const float4* __restrict__ img,
   out[i] = o;
 }
 
+int nts_tonemap_dev(nts_ctx* ctx, const float4* image, uint32_t* rgba8, uint64_t np) {
+  if (np == 0) return NTS_OK;
+  NTS_LAUNCH(ctx, "tonemap", k_tonemap, dim3(nts_div_up(np, 256)), dim3(256), 0, image, rgba8, np);
+  return NTS_OK;
+}
+
 extern "C" int nts_tonemap(nts_ctx* ctx, const nts_buffer* image, nts_buffer* rgba8, uint32_t width, uint32_t height) {
   NTS_CHECK_ARG(ctx && image && rgba8);
   const uint64_t np = (uint64_t)width * height;

@@ -296,6 +296,24 @@ extern "C" int nts_scene_create(nts_ctx* ctx, uint32_t capacity, nts_scene** out
   return NTS_OK;
 }
 
+// A scene of parameter planes only (no gradients, moments or grad_acc), for
+// render-only residents.
+int nts_scene_alloc_params(nts_ctx* ctx, uint32_t capacity, nts_scene** out) {
+  *out = nullptr;
+  nts_scene* s = new nts_scene();
+  const uint64_t cap = ((uint64_t)(capacity ? capacity : 1) + 63) & ~63ull;
+  s->bytes = cap * 4 * P_COUNT;
+  int rc = nts_dev_alloc(ctx, &s->base, s->bytes);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  s->cap = (uint32_t)cap;
+  s->params = (float*)s->base;
+  *out = s;
+  return NTS_OK;
+}
+
 extern "C" int nts_scene_destroy(nts_ctx* ctx, nts_scene* scene) {
   if (!ctx || !scene) return NTS_OK;
   (void)hipStreamSynchronize(ctx->stream);
@@ -307,7 +325,7 @@ extern "C" int nts_scene_destroy(nts_ctx* ctx, nts_scene* scene) {
 extern "C" uint32_t nts_scene_count(const nts_scene* scene) { return scene ? scene->n : 0; }
 
 // Host SoA (per-splat interleaved attributes) <-> device planes.
-static int upload_planes(nts_ctx* ctx, const nts_scene* s, float* dst_planes, const nts_splats_host* h) {
+int nts_upload_planes(nts_ctx* ctx, const nts_scene* s, float* dst_planes, const nts_splats_host* h) {
   const uint32_t n = h->n;
   const float* src[6] = {h->pos, h->rot, h->log_scale, h->logit, h->sh_dc, h->sh_rest};
   std::vector<float> staging((size_t)P_COUNT * n);
@@ -355,7 +373,7 @@ extern "C" int nts_scene_upload(nts_ctx* ctx, nts_scene* scene, const nts_splats
   int rc = nts_fill_u32(ctx, scene->grads, 0, (uint64_t)scene->cap * (3 * P_COUNT + 1));
   if (rc) return rc;
   scene->n = host->n;
-  return upload_planes(ctx, scene, scene->params, host);
+  return nts_upload_planes(ctx, scene, scene->params, host);
 }
 
 extern "C" int nts_scene_download(nts_ctx* ctx, const nts_scene* scene, nts_splats_host* host) {
@@ -370,7 +388,7 @@ extern "C" int nts_scene_download_grads(nts_ctx* ctx, const nts_scene* scene, nt
 
 extern "C" int nts_scene_upload_grads(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* grads) {
   NTS_CHECK_ARG(ctx && scene && grads && grads->n == scene->n);
-  return upload_planes(ctx, scene, scene->grads, grads);
+  return nts_upload_planes(ctx, scene, scene->grads, grads);
 }
 
 extern "C" int nts_scene_zero_grads(nts_ctx* ctx, nts_scene* scene) {
@@ -389,9 +407,9 @@ extern "C" int nts_scene_download_moments(nts_ctx* ctx, const nts_scene* scene, 
 extern "C" int nts_scene_upload_moments(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* m,
                                         const nts_splats_host* v) {
   NTS_CHECK_ARG(ctx && scene && m && v && m->n == scene->n && v->n == scene->n);
-  int rc = upload_planes(ctx, scene, scene->m, m);
+  int rc = nts_upload_planes(ctx, scene, scene->m, m);
   if (rc) return rc;
-  return upload_planes(ctx, scene, scene->v, v);
+  return nts_upload_planes(ctx, scene, scene->v, v);
 }
 
 extern "C" int nts_scene_download_grad_acc(nts_ctx* ctx, const nts_scene* scene, float* host) {

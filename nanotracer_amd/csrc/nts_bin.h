@@ -20,11 +20,28 @@ struct TileTable {
   ViewTiles t[NTS_MAX_VIEW_TILES];
 };
 
+// Records of a launch: view v owns records [base[v], base[v+1]) (one per
+// splat of the view's scene, in splat order).
+struct RecTable {
+  uint32_t n;
+  uint32_t pad[3];
+  uint32_t base[NTS_MAX_VIEW_TILES + 1];
+};
+
+__device__ __forceinline__ uint32_t rec_view(const RecTable& rt, uint32_t f) {
+  uint32_t lo = 0, hi = rt.n;  // base[lo] <= f < base[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rt.base[mid] <= f) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // tile_emit.wgsl:91-121 with the work spread over pairs instead of splats:
 // key = tile << 16 | depth16, payload = splat index, a record's pairs
 // written from offsets[record] in ty-major, tx-minor order.
 //
-// Records are (view, splat) with record f = v * n + i.  A block takes 256
+// Records are (view, splat): record f = base[v] + i.  A block takes 256
 // consecutive records, stages each record's first pair, first tile, bbox
 // width and depth bucket in LDS, then its threads walk the block's pair
 // range [offsets[f0], offsets[f0 + 256]) together: pair j belongs to the
@@ -32,18 +49,19 @@ struct TileTable {
 // store is coalesced whatever the footprints.  offsets is the exclusive
 // scan of the per-record tile counts; *total the pair total.
 template <typename K>
-__global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restrict__ proj, uint32_t n, uint32_t nrec,
+__global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restrict__ proj, RecTable rt,
                                                        TileTable tt, const uint32_t* __restrict__ offsets,
                                                        const uint64_t* __restrict__ total, uint64_t capacity,
                                                        K* __restrict__ keys, uint32_t* __restrict__ payloads) {
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_tile0[256], s_bw[256], s_tx[256], s_d[256], s_pay[256];
   const uint32_t t = threadIdx.x;
+  const uint32_t nrec = rt.base[rt.n];
   const uint32_t f0 = blockIdx.x * 256u, f = f0 + t;
   uint32_t off = 0xFFFFFFFFu;  // past every pair: records beyond nrec own none
   if (f < nrec) {
     off = offsets[f];
-    const uint32_t v = f / n, i = f - v * n;
+    const uint32_t v = rec_view(rt, f), i = f - rt.base[v];
     const ViewTiles vt = tt.t[v];
     const Projected p = proj[f];
     int32_t x0 = 0, y0 = 0, x1 = -1, y1 = -1;

@@ -125,6 +125,10 @@ int nts_fail(int code, const char* fmt, ...);
 // Device allocation charged against the context's HBM budget.
 int nts_dev_alloc(nts_ctx* ctx, void** p, uint64_t bytes);
 void nts_dev_free(nts_ctx* ctx, void* p, uint64_t bytes);
+// Host SoA -> device planes (transposed on the host), synchronous.
+int nts_upload_planes(nts_ctx* ctx, const nts_scene* s, float* dst_planes, const nts_splats_host* h);
+// Scene with parameter planes only (render-only residents).
+int nts_scene_alloc_params(nts_ctx* ctx, uint32_t capacity, nts_scene** out);
 // Context scratch of at least `bytes` (contents undefined).
 int nts_scratch(nts_ctx* ctx, uint64_t bytes, void** out);
 

@@ -74,6 +74,8 @@ __device__ __forceinline__ float nts_expf_nonpos(float x) {
 // {8, 16, 32}; lds_batch in {64, 128, 256, 512} forward, {64, 128, 256}
 // backward.
 bool nts_blend_config_ok(uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_batch);
+// tonemap.wgsl:27-41 over np pixels (rgba8 out).
+int nts_tonemap_dev(nts_ctx* ctx, const float4* image, uint32_t* rgba8, uint64_t np);
 int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, const Projected* proj,
                       const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, float4* out, float* final_t,
                       uint32_t* last_idx, const LossArgs* loss);
