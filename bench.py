@@ -16,8 +16,11 @@ plain: the single-view path (one 1920x1080 view per iteration, reference
   call sequence).
 
 Also reported: single-view 1080p render throughput and p50/p99 latency at
-1M splats, per-kernel HIP-event timings (library stream) with the dominant
-kernel against the HBM roofline, and the CPU baseline (the host
+1M splats; the render server (64 resident scenes of 50k-3M splats, Poisson
+arrivals at --serve-rate req/s, resolutions 512^2 to 3840x2160, up to 32
+views per batch, 3 batches in flight) with p50/p99 arrival-to-host latency
+and Mpx/s; per-kernel HIP-event timings (library stream) with the dominant
+kernel against the HBM roofline; and the CPU baseline (the host
 restatement of the reference path, run on this host's cores on the same
 workload).
 """
@@ -152,6 +155,94 @@ def cpu_baseline_batched(s, views, targets, tile):
                        f"(project, bin, blend, L1, recompute backward, project_backward, Adam): {dt:.2f} s")
 
 
+SERVE_SCENES = 64
+SERVE_RES = ((512, 512), (1280, 720), (1920, 1080), (3840, 2160))
+
+
+def serving_scenes(seed):
+    """64 resident scenes, splat counts log-uniform in [50k, 3M]: windows of
+    one 3M-splat pool (per-scene offset and scale), so that setting up 46M
+    resident splats costs seconds."""
+    from nanotracer_amd import synth
+
+    pool = synth.fast_scene(3_000_000, seed, 3.0)
+    rng = np.random.default_rng(seed + 1)
+    counts = np.exp(rng.uniform(math.log(50_000), math.log(3_000_000), SERVE_SCENES)).astype(np.int64)
+    for k, n in enumerate(counts):
+        start = int(rng.integers(0, pool.n))
+        idx = (start + np.arange(n)) % pool.n
+        sc = float(rng.uniform(0.8, 1.2))
+        arrs = [np.ascontiguousarray(a[idx]) for a in pool.arrays()]
+        arrs[0] *= np.float32(sc)
+        yield synth.HostSplats(*arrs)
+
+
+def run_serving(rate, duration, seed=SEED + 5):
+    """Poisson arrivals at `rate` req/s for `duration` s, each one camera of a
+    random resident scene at a random resolution; latency = arrival to the
+    tonemapped image in host memory."""
+    from nanotracer_amd import synth
+    from nanotracer_amd.api import CameraUniform, RenderServer, ViewSpec
+
+    srv = RenderServer(slots=3, max_views=32, max_pixels=48 << 20, max_records=64 << 20,
+                       pair_capacity=600_000_000, tile_size=16, lds_batch=128)
+    sizes = []
+    t_setup = time.perf_counter()
+    for sc in serving_scenes(seed):
+        sizes.append(sc.n)
+        srv.add_scene(sc)
+    t_setup = time.perf_counter() - t_setup
+    rng = np.random.default_rng(seed + 2)
+    out = dict(rate_req_s=rate, scenes=len(sizes), resident_splats=int(sum(sizes)))
+    n_req = max(1, int(rate * duration))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for k in range(n_req):
+        sid = int(rng.integers(len(sizes)))
+        w, h = SERVE_RES[int(rng.integers(len(SERVE_RES)))]
+        eye = synth.orbit_eye(float(rng.uniform(6.0, 9.0)), float(rng.uniform(0, 2 * math.pi)),
+                              float(rng.uniform(-0.5, 0.5)))
+        cam = CameraUniform.from_pose(eye, (0, 0, 0), (0, 1, 0), math.pi / 3, w, h, sizes[sid])
+        reqs.append((sid, ViewSpec(cam, w, h, 24.0)))
+    # warm-up: one request per resolution
+    for w, h in SERVE_RES:
+        cam = CameraUniform.from_pose((0, 0, 8), (0, 0, 0), (0, 1, 0), math.pi / 3, w, h, sizes[0])
+        srv.wait(srv.submit(0, ViewSpec(cam, w, h, 24.0))[0])
+    lat, pix, views = [], 0, []
+    inflight = {}
+    t0 = time.perf_counter()
+    i = 0
+    while i < n_req or inflight:
+        now = time.perf_counter() - t0
+        while i < n_req and arrivals[i] <= now:
+            rid, _ = srv.submit(*reqs[i])
+            inflight[rid] = (now - arrivals[i], reqs[i][1].width * reqs[i][1].height)
+            i += 1
+        done = []
+        for rid, (lag, px) in inflight.items():
+            r = srv.poll(rid)
+            if r is not None:
+                lat.append(lag * 1e3 + r[1]["latency_ms"])
+                views.append(r[1]["batch_views"])
+                pix += px
+                done.append(rid)
+        for rid in done:
+            del inflight[rid]
+        if not done and (i >= n_req or arrivals[i] > time.perf_counter() - t0):
+            time.sleep(0.0002)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    out.update(requests=n_req, p50_ms=round(float(np.percentile(lat, 50)), 3),
+               p99_ms=round(float(np.percentile(lat, 99)), 3), mean_ms=round(float(lat.mean()), 3),
+               mpx_s=round(pix / wall / 1e6, 1), mean_views_per_batch=round(float(np.mean(views)), 2),
+               batches=int(st["batches"]), overflow_reruns=int(st["overflow_reruns"]),
+               max_in_flight=int(st["max_in_flight"]), failed=int(st["failed"]),
+               hbm_gb=round(st["hbm_in_use"] / 1e9, 1), setup_s=round(t_setup, 1))
+    return out
+
+
 def run_batched(args, ctx):
     from nanotracer_amd import _lib as L
     from nanotracer_amd import synth
@@ -234,6 +325,8 @@ def run_batched(args, ctx):
         out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
                               gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
+    if args.serve_rate > 0:
+        out["serving"] = run_serving(args.serve_rate, args.serve_seconds)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_batched(s, views, targets, tile)
     return out
@@ -304,6 +397,9 @@ def main():
     ap.add_argument("--tile", choices=("auto", "8", "16", "32"), default="auto",
                     help="blend tile size, or autotuned during warm-up")
     ap.add_argument("--lds-batch", default="auto", help="blend forward,backward LDS batch, or auto")
+    ap.add_argument("--serve-rate", type=float, default=200.0,
+                    help="render-server Poisson arrival rate (req/s); 0 skips the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     args = ap.parse_args()
     global TILE, LDS_BATCH
     TILE = args.tile if args.tile == "auto" else int(args.tile)

@@ -180,6 +180,21 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
   }
 }
 
+// Clears the look-back status words of the live partitions of every pass
+// (pass ps owns words [ps * max_parts * 256, ...)); the live count is read
+// on the device, so a launch sized for the capacity clears only what is used.
+__global__ void __launch_bounds__(256) k_clear_status(uint32_t* __restrict__ status,
+                                                      const uint32_t* __restrict__ part_begin, uint32_t num_segs,
+                                                      uint32_t passes, uint32_t max_parts) {
+  const uint64_t words = (uint64_t)part_begin[num_segs] * RS_RADIX;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint32_t ps = 0; ps < passes; ++ps) {
+    uint4* st = reinterpret_cast<uint4*>(status + (uint64_t)ps * max_parts * RS_RADIX);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words / 4; i += stride)
+      st[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
 // offsets[s][pass][d] = seg_begin[s] + sum of hist[s][pass][d' < d].
 __global__ void __launch_bounds__(RS_RADIX) k_sort_offsets(const uint32_t* __restrict__ hist,
                                                            const uint64_t* __restrict__ seg_begin, uint32_t passes,
@@ -363,10 +378,12 @@ static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt,
   if (err_out) *err_out = err;
   int rc = nts_fill_u32(ctx, hist, 0, n_hist);
   if (rc) return rc;
-  rc = nts_fill_u32(ctx, counters, 0, 64 + (uint64_t)passes * max_parts * RS_RADIX);
+  rc = nts_fill_u32(ctx, counters, 0, 64);
   if (rc) return rc;
   if (passes == 0 || max_pairs == 0) return NTS_OK;
   NTS_LAUNCH(ctx, "sort_parts", k_sort_parts, dim3(1), dim3(64), 0, d_seg_begin, num_segs, part_begin);
+  NTS_LAUNCH(ctx, "sort_clear", k_clear_status, dim3(1024), dim3(256), 0, status, (const uint32_t*)part_begin,
+             num_segs, passes, max_parts);
   const uint32_t grid = max_parts < RS_GRID ? max_parts : RS_GRID;
   NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
              (const uint32_t*)part_begin, num_segs, passes, hist);

@@ -92,6 +92,26 @@ def make_scene(n: int, seed: int, mean_half_extent: float = 1.0, log_scale_mu: f
     return HostSplats(f(pos, n, 3), f(q, n, 4), f(ls, n, 3), f(lg, n), f(dc, n, 3), f(rest, n, 45))
 
 
+def fast_scene(n: int, seed: int, mean_half_extent: float = 3.0, log_scale_mu: float = -4.0,
+               log_scale_sd: float = 0.5, logit_mu: float = 0.0, logit_sd: float = 1.0,
+               rest_sd: float = 0.05) -> HostSplats:
+    """The make_scene distributions drawn with numpy's PCG64 in float32 (the
+    higher SH bands as rows of a random pool): for large resident serving
+    scenes, where only the distribution matters."""
+    rng = np.random.default_rng(seed)
+    f32 = np.float32
+    pos = (rng.random((n, 3), dtype=f32) * f32(2.0) - f32(1.0)) * f32(mean_half_extent)
+    q = rng.standard_normal((n, 4), dtype=f32)
+    q /= np.maximum(np.linalg.norm(q, axis=1, keepdims=True), f32(1e-12))
+    ls = rng.standard_normal((n, 3), dtype=f32) * f32(log_scale_sd) + f32(log_scale_mu)
+    lg = rng.standard_normal(n, dtype=f32) * f32(logit_sd) + f32(logit_mu)
+    dc = rng.random((n, 3), dtype=f32)
+    # higher SH bands from a pool of 65536 N(0, rest_sd) rows
+    pool = rng.standard_normal((65536, 45), dtype=f32) * f32(rest_sd)
+    rest = pool[rng.integers(0, 65536, n)]
+    return HostSplats(pos, q, ls, lg, dc, rest)
+
+
 def perturb_means(s: HostSplats, seed: int, sd: float = 0.02) -> HostSplats:
     out = s.copy()
     out.pos = np.ascontiguousarray((s.pos.astype(np.float64) + normal((seed << 8) + 9, 3 * s.n).reshape(-1, 3) * sd)
