@@ -95,13 +95,28 @@ def kernel_table(rep, N, Nv, K, P, T, V):
     return kernels
 
 
-def roofline_of(kernels):
+# HBM bytes per launch measured with rocprofv3 (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors
+# read on a 1 GiB copy kernel) at the (kernel, tile, forward batch, backward batch) it was profiled
+# at, with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
+PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128, 128): (2.9635e9, 55_869_555),
+                    ("blend_fwd_loss", 32, 128, 128): (3.4496e9, 55_869_555)}
+
+
+def roofline_of(kernels, cfg=None, pairs=None):
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     dk = kernels[dom]
     achieved = dk["gbs"] or 0.0
-    return dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, avg_ms=round(dk["avg_ms"], 4),
-                alg_bytes_per_launch=dk["alg_bytes"])
+    traffic, note = None, None
+    if cfg is not None and (dom,) + tuple(cfg) in PROFILED_TRAFFIC and pairs:
+        t_bytes, t_pairs = PROFILED_TRAFFIC[(dom,) + tuple(cfg)]
+        traffic = round(t_bytes * pairs / t_pairs)
+        note = "rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel at this configuration, scaled by pairs"
+    out = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, avg_ms=round(dk["avg_ms"], 4),
+               alg_bytes_per_launch=dk["alg_bytes"])
+    if note:
+        out["traffic_note"] = note
+    return out
 
 
 def render_latency(ctx, g, n, tile):
@@ -320,7 +335,7 @@ def run_batched(args, ctx):
         final_loss=float(losses.sum()),
     )
     out.update(render_latency(ctx, g, s.n, tile))
-    out["roofline"] = roofline_of(kernels)
+    out["roofline"] = roofline_of(kernels, (tile, lds[0], lds[1]), pairs)
     if tune:
         out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
@@ -376,10 +391,8 @@ def run_plain(args, ctx):
         config=dict(workload="plain single-view training step (reference call sequence)", splats=N_SPLATS,
                     width=W, height=H, tile=16, pairs=pairs, visible=Nv, backward=args.backward),
     )
-    out.update(render_latency(ctx, g, s.n, tile))
+    out.update(render_latency(ctx, g, s.n, 16))
     out["roofline"] = roofline_of(kernels)
-    if tune:
-        out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
                               gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
     return out
