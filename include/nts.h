@@ -336,6 +336,17 @@ int nts_prune(nts_ctx* ctx, nts_scene* scene, float logit_threshold, uint32_t* r
 int nts_densify(nts_ctx* ctx, nts_scene* scene, uint32_t grad_count, float grad_threshold, float split_log_scale,
                 uint32_t max_splats, uint32_t iter, uint32_t counts[3]);
 
+/* ---- 3DGS PLY files (crates/nano-splat/src/ply.rs:17-256; host only) ---- */
+/* Vertex count of a binary little-endian 3DGS PLY (read_ply's header). */
+int nts_ply_count(const char* path, uint32_t* n);
+/* read_ply into caller arrays holding out->n >= the count splats; out->n
+ * is set to the count.  Properties are mapped by name; missing normals are
+ * allowed, missing f_rest_i read as 0, other missing properties, non-float
+ * properties and non-binary files are NTS_E_INVALID. */
+int nts_ply_read(const char* path, nts_splats_host* out);
+/* write_ply: the reference's header and record order, normals (0, 1, 0). */
+int nts_ply_write(const char* path, const nts_splats_host* splats);
+
 /* ---- viewer (Rasterizer::tonemap, raster.rs:405-455) ------------------- */
 int nts_tonemap(nts_ctx* ctx, const nts_buffer* image, nts_buffer* rgba8, uint32_t width, uint32_t height);
 

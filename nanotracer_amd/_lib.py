@@ -181,6 +181,9 @@ _SIGS = {
     "nts_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(RequestInfo)]),
     "nts_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestInfo)]),
     "nts_server_stats": (C.c_int, [C.c_void_p, C.POINTER(ServeStats)]),
+    "nts_ply_count": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32)]),
+    "nts_ply_read": (C.c_int, [C.c_char_p, C.POINTER(SplatsHost)]),
+    "nts_ply_write": (C.c_int, [C.c_char_p, C.POINTER(SplatsHost)]),
     "nts_scene_zero_grad_acc": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nts_prune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.POINTER(C.c_uint32)]),
     "nts_densify": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float, C.c_uint32, C.c_uint32,

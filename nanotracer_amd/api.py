@@ -123,6 +123,21 @@ def _host_struct(s: HostSplats) -> L.SplatsHost:
     return hs
 
 
+def read_ply(path) -> HostSplats:
+    """read_ply (nano-splat/src/ply.rs:116-134) through the library."""
+    n = C.c_uint32()
+    L.check(L.lib().nts_ply_count(str(path).encode(), C.byref(n)))
+    out = HostSplats.zeros(n.value)
+    hs = _host_struct(out)
+    L.check(L.lib().nts_ply_read(str(path).encode(), C.byref(hs)))
+    return out
+
+
+def write_ply(path, splats: HostSplats) -> None:
+    """write_ply (nano-splat/src/ply.rs:17-28) through the library."""
+    L.check(L.lib().nts_ply_write(str(path).encode(), C.byref(_host_struct(splats))))
+
+
 class GpuSplatBuffer:
     """GpuSplatBuffer + GradSplatBuffers + AdamMomentBuffers + grad_acc
     (splat_gpu.rs:36-384) as one device scene of attribute planes."""
