@@ -70,6 +70,16 @@ class HostSplats:
         """All 59 parameters of every splat, attribute-major."""
         return np.concatenate([a.reshape(-1) for a in self.arrays()])
 
+    @staticmethod
+    def from_flat(flat: np.ndarray, n: int) -> "HostSplats":
+        """Inverse of flat()."""
+        out, o = [], 0
+        for w in (3, 4, 3, 1, 3, 45):
+            a = np.ascontiguousarray(flat[o:o + w * n], np.float32)
+            out.append(a if w == 1 else a.reshape(n, w))
+            o += w * n
+        return HostSplats(*out)
+
     def subset(self, idx) -> "HostSplats":
         return HostSplats(*(np.ascontiguousarray(a[idx]) for a in self.arrays()))
 

@@ -103,3 +103,118 @@ class Trainer:
         total = bins.total_pairs
         bins.free()
         return StepResult(loss, total)
+
+
+# ---------------------------------------------------------------------------
+# train() driver (train.rs:65-401)
+# ---------------------------------------------------------------------------
+@dataclass
+class TrainConfig:
+    """TrainConfig (src/main.rs:418-437) and the cadence constants of
+    train.rs:103-114."""
+
+    iterations: int = 1000
+    max_splats: int = 1_000_000
+    tile_size: int = 16
+    loss_kind: int = L.LOSS_L1
+    backward_mode: int = L.BWD_REF_PARITY
+    adam_pos: AdamConfig = field(default_factory=AdamConfig)
+    adam_attr: AdamConfig = field(default_factory=AdamConfig)
+    prune_every: int = 100
+    prune_warmup: int = 50
+    prune_opacity_logit: float = -5.3
+    densify_every: int = 200
+    densify_warmup: int = 100
+    densify_grad_threshold: float = 2e-4
+    split_scale_threshold: float = -2.3
+    diagnostics_every: int = 100
+    checkpoint_every: int = 0          # 0: no periodic checkpoints
+    checkpoint_path: str | None = None  # base path: <base>.ply + <base>.npz
+
+
+@dataclass
+class TrainState:
+    iteration: int = 0   # next iteration to run
+    t: int = 0           # Adam step (adam_pos.t)
+    grad_count: int = 0  # steps accumulated into grad_acc since the last densify
+
+
+def save_checkpoint(base, splats: GpuSplatBuffer, state: TrainState) -> None:
+    """Parameters as a 3DGS PLY (base.ply) and the optimiser state the
+    reference does not persist (base.npz): both Adam moments, grad_acc,
+    the Adam step, grad_count and the next iteration."""
+    from .api import write_ply
+
+    write_ply(f"{base}.ply", splats.readback())
+    m, v = splats.moments()
+    np.savez(f"{base}.npz", m=m.flat(), v=v.flat(), grad_acc=splats.grad_acc(),
+             state=np.array([state.iteration, state.t, state.grad_count], np.int64))
+
+
+def load_checkpoint(ctx: GpuCtx, base, capacity: int):
+    """Inverse of save_checkpoint: (GpuSplatBuffer, TrainState)."""
+    from .api import read_ply
+
+    host = read_ply(f"{base}.ply")
+    d = np.load(f"{base}.npz")
+    g = GpuSplatBuffer.upload(ctx, host, capacity=max(capacity, host.n))
+    g.set_moments(HostSplats.from_flat(d["m"], host.n), HostSplats.from_flat(d["v"], host.n))
+    g.set_grad_acc(d["grad_acc"])
+    it, t, gc = (int(x) for x in d["state"])
+    return g, TrainState(iteration=it, t=t, grad_count=gc)
+
+
+def _norm(a) -> float:
+    return float(np.sqrt(np.sum(np.asarray(a, np.float64) ** 2)))
+
+
+def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None = None, on_iter=None,
+          resume: str | None = None, log=print):
+    """The training loop of train.rs:155-401 on the device: view iter %
+    len(views) each iteration, one fused Adam step, prune every
+    `prune_every` iterations from `prune_warmup`, densify every
+    `densify_every` from `densify_warmup` (then grad_acc = 0 and
+    grad_count = 0), gradient-norm diagnostics every `diagnostics_every`
+    and at the end, optional checkpoints.  `splats` is a HostSplats (fresh
+    start) or ignored when `resume` names a checkpoint.  on_iter(iter, loss,
+    n) is called after every iteration.  Returns the final HostSplats."""
+    cfg = cfg or TrainConfig()
+    cap = max(cfg.max_splats, splats.n if splats is not None else 0) + cfg.max_splats
+    if resume:
+        g, st = load_checkpoint(ctx, resume, cap)
+    else:
+        g, st = GpuSplatBuffer.upload(ctx, splats, capacity=cap), TrainState()
+    tr = Trainer(ctx, g, list(views), depth_max, cfg.tile_size, cfg.loss_kind, cfg.backward_mode, cfg.adam_pos,
+                 cfg.adam_attr, st.t)
+    for it in range(st.iteration, cfg.iterations):
+        res = tr.step(it)
+        st.t = tr.t
+        st.grad_count += 1
+        do_prune = it >= cfg.prune_warmup and it % cfg.prune_every == 0
+        do_densify = it >= cfg.densify_warmup and it % cfg.densify_every == 0
+        if do_prune:
+            before = g.n
+            removed = g.prune(cfg.prune_opacity_logit)
+            if removed:
+                log(f"[train] iter {it}: pruned {removed} low-opacity splats ({before}->{g.n})")
+        if do_densify and st.grad_count > 0:
+            cloned, split, parents = g.densify(st.grad_count, cfg.max_splats, it, cfg.densify_grad_threshold,
+                                               cfg.split_scale_threshold)
+            if cloned + split + parents:
+                log(f"[train] iter {it}: densify cloned {cloned} split {split} (removed {parents} parents) "
+                    f"-> {g.n} splats")
+            g.zero_grad_acc()
+            st.grad_count = 0
+        st.iteration = it + 1
+        if on_iter is not None:
+            on_iter(it, res.loss, g.n)
+        if cfg.diagnostics_every and (it % cfg.diagnostics_every == 0 or it + 1 == cfg.iterations):
+            gr = g.grads()
+            log(f"[train] iter {it:>5} | view {it % len(tr.views)}/{len(tr.views)} | loss {res.loss:.4f} | "
+                f"grad |pos|={_norm(gr.pos):.2e} |op|={_norm(gr.logit):.2e} |dc|={_norm(gr.sh_dc):.2e} "
+                f"|scale|={_norm(gr.log_scale):.2e} | {g.n} splats")
+        if cfg.checkpoint_every and cfg.checkpoint_path and (it + 1) % cfg.checkpoint_every == 0:
+            save_checkpoint(cfg.checkpoint_path, g, st)
+    out = g.readback()
+    g.free()
+    return out

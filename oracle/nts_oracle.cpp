@@ -1088,6 +1088,126 @@ uint32_t orc_densify(orc_splats_mut* s, uint32_t n, uint32_t grad_count, float g
   return n;
 }
 
+// ---- local central differences for the large-scene gradient check ----------
+// The mean-square loss L = sum (C - target)^2 / (3WH) depends on one splat's
+// parameters only through the tiles its record is binned to, so a central
+// difference needs only those tiles re-blended (in float64, with the splat's
+// record projected at theta +- h and every other splat at its float64
+// record), over the fixed float32 binning that the device gradient uses.
+// A parameter whose +-h evaluations change the splat's tile set, depth
+// bucket or visibility, or any pixel's sequence of contributing splats,
+// blend termination or alpha clamp, is reported as discontinuous (1).
+namespace {
+
+uint32_t depth16_of(double depth, float depth_max) {
+  float dn = (float)depth / depth_max;
+  dn = dn < 0.0f ? 0.0f : (dn > 1.0f ? 1.0f : dn);
+  return (uint32_t)(dn * 65535.0f);
+}
+
+// blend of one pixel over [begin, end) with splat `idx` drawn from `rec`;
+// returns sum_c (C_c - target_c)^2 and a signature of the contributions
+double pixel_sq_err(const double* proj64, const uint32_t* payloads, uint32_t begin, uint32_t end, uint32_t idx,
+                    const double* rec, double pxf, double pyf, const float* tgt, uint64_t* sig) {
+  double C[3] = {0.0, 0.0, 0.0}, Tr = 1.0;
+  uint64_t h = 1469598103934665603ull;
+  for (uint32_t i = begin; i < end; ++i) {
+    const uint32_t sidx = payloads[i];
+    const double* p = sidx == idx ? rec : proj64 + 12 * (size_t)sidx;
+    const double dx = pxf - p[0], dy = pyf - p[1];
+    const double power = -0.5 * (p[4] * dx * dx + 2.0 * p[5] * dx * dy + p[6] * dy * dy);
+    if (power > 0.0) continue;
+    const double ar = p[11] * exp(power);
+    const bool clamped = ar >= (double)NTS_ALPHA_MAX;
+    const double alpha = clamped ? (double)NTS_ALPHA_MAX : ar;
+    if (alpha < (double)NTS_ALPHA_MIN) continue;
+    h = (h ^ ((uint64_t)i * 2 + (clamped ? 1 : 0))) * 1099511628211ull;
+    const double ta = Tr * alpha;
+    C[0] += ta * p[8];
+    C[1] += ta * p[9];
+    C[2] += ta * p[10];
+    Tr = Tr * (1.0 - alpha);
+    if (Tr < (double)NTS_T_MIN) {
+      h = (h ^ 0xABCDull) * 1099511628211ull;
+      break;
+    }
+  }
+  *sig = h;
+  double e = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double d = C[c] - (double)tgt[c];
+    e += d * d;
+  }
+  return e;
+}
+
+const int kWidthOf[6] = {3, 4, 3, 1, 3, 45};
+
+}  // namespace
+
+int orc_fd_local_mse(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
+                     const double* proj64, const float* proj32, const uint32_t* payloads, const uint32_t* ranges,
+                     const float* target, uint32_t idx, int attr, int comp, double h, double* fd) {
+  *fd = 0.0;
+  if (idx >= n || attr < 0 || attr > 5 || comp < 0 || comp >= kWidthOf[attr]) return -1;
+  // this splat's parameters, perturbed
+  double pos[3], rot[4], ls[3], lg, dc[3], rest[45];
+  double* dst[6] = {pos, rot, ls, &lg, dc, rest};
+  const double* src[6] = {s->pos, s->rot, s->log_scale, s->logit, s->sh_dc, s->sh_rest};
+  double rec[2][12];
+  for (int sign = 0; sign < 2; ++sign) {
+    for (int a = 0; a < 6; ++a)
+      for (int k = 0; k < kWidthOf[a]; ++k) dst[a][k] = src[a][(size_t)kWidthOf[a] * idx + k];
+    dst[attr][comp] += sign == 0 ? h : -h;
+    const Splats<double> one{pos, rot, ls, &lg, dc, rest};
+    project_one<double>(one, 0, cam, rec[sign]);
+  }
+  // the fixed binning of this splat (float32 record) and its perturbed bins
+  const float* b32 = proj32 + 12 * (size_t)idx;
+  int32_t x0, y0, x1, y1;
+  const bool vis = tile_bbox<float>(b32, t, &x0, &y0, &x1, &y1);
+  int disc = 0;
+  for (int sign = 0; sign < 2; ++sign) {
+    float r32[12];
+    for (int k = 0; k < 12; ++k) r32[k] = (float)rec[sign][k];
+    int32_t a0, b0, a1, b1;
+    const bool v = tile_bbox<float>(r32, t, &a0, &b0, &a1, &b1);
+    if (v != vis || (v && (a0 != x0 || b0 != y0 || a1 != x1 || b1 != y1))) disc = 1;
+    if (v && depth16_of(rec[sign][2], t->depth_max) != depth16_of((double)b32[2], t->depth_max)) disc = 1;
+  }
+  if (!vis) return disc;  // contributes nowhere at theta
+  const uint32_t ts = t->tile_size, tiles_x = (t->width + ts - 1) / ts;
+  double sum[2] = {0.0, 0.0};
+  for (int32_t ty = y0; ty <= y1; ++ty)
+    for (int32_t tx = x0; tx <= x1; ++tx) {
+      const uint32_t tile = (uint32_t)ty * tiles_x + (uint32_t)tx;
+      const uint32_t begin = ranges[2 * tile], end = ranges[2 * tile + 1];
+      for (uint32_t ly = 0; ly < ts; ++ly)
+        for (uint32_t lx = 0; lx < ts; ++lx) {
+          const uint32_t px = tx * ts + lx, py = ty * ts + ly;
+          if (px >= t->width || py >= t->height) continue;
+          const double pxf = (double)((float)px + 0.5f), pyf = (double)((float)py + 0.5f);
+          const float* tg = target + 4 * ((size_t)py * t->width + px);
+          uint64_t sg[2];
+          for (int sign = 0; sign < 2; ++sign)
+            sum[sign] += pixel_sq_err(proj64, payloads, begin, end, idx, rec[sign], pxf, pyf, tg, &sg[sign]);
+          if (sg[0] != sg[1]) disc = 1;
+        }
+    }
+  *fd = (sum[0] - sum[1]) / (2.0 * h) / (3.0 * (double)t->width * (double)t->height);
+  return disc;
+}
+
+void orc_fd_local_mse_batch(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
+                            const double* proj64, const float* proj32, const uint32_t* payloads,
+                            const uint32_t* ranges, const float* target, uint32_t count, const uint32_t* idx,
+                            const int32_t* attr, const int32_t* comp, const double* h, double* fd, int32_t* disc) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t k = 0; k < (int64_t)count; ++k)
+    disc[k] = orc_fd_local_mse(s, n, cam, t, proj64, proj32, payloads, ranges, target, idx[k], attr[k], comp[k],
+                               h[k], &fd[k]);
+}
+
 // tonemap.wgsl:27-41.
 void orc_tonemap(const float* image, uint32_t w, uint32_t h, uint8_t* out) {
   for (uint64_t i = 0; i < (uint64_t)w * h; ++i) {

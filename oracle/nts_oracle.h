@@ -130,6 +130,23 @@ void orc_accumulate_grad(const float* dpos, float* acc, uint32_t n);
 double orc_loss_grad(const float* pred, const float* target, uint32_t w, uint32_t h,
                      int kind, float* dl_dc);
 
+// Central difference of the mean-square loss over 3WH with respect to one
+// parameter (splat idx, attribute attr in pos/rot/log_scale/logit/sh_dc/
+// sh_rest order, component comp), re-blending in float64 only the tiles of
+// the splat's float32 binning.  proj64 / proj32: float64 / float32
+// projections of every splat; payloads / ranges: the float32 binning;
+// target: float4 per pixel.  Returns 0, or 1 when the +-h evaluations are
+// discontinuous (tile set, depth bucket, visibility or any pixel's
+// contribution pattern changes), -1 on bad arguments.
+int orc_fd_local_mse(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
+                     const double* proj64, const float* proj32, const uint32_t* payloads, const uint32_t* ranges,
+                     const float* target, uint32_t idx, int attr, int comp, double h, double* fd);
+// The same for `count` parameters at once (threads over parameters).
+void orc_fd_local_mse_batch(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
+                            const double* proj64, const float* proj32, const uint32_t* payloads,
+                            const uint32_t* ranges, const float* target, uint32_t count, const uint32_t* idx,
+                            const int32_t* attr, const int32_t* comp, const double* h, double* fd, int32_t* disc);
+
 // Prune / densify on the host SoA (train.rs:414-602).  Arrays must have
 // capacity for the result.  moments: 12 slabs (m,v per attribute, in the
 // attribute order pos, rot, scale, logit, dc, rest), may be NULL.

@@ -356,6 +356,52 @@ def densify(s, grad_count, grad_threshold, split_thr, max_splats, it, moments=No
     return _unmut(arrs, n2), mo, ga[:n2], tuple(counts)
 
 
+ATTRS = ("pos", "rot", "log_scale", "logit", "sh_dc", "sh_rest")
+
+
+class FdContext:
+    """Float64 local central differences of the mean-square loss against a
+    fixed float32 binning (orc_fd_local_mse): build once per scene and
+    camera, then fd(idx, attr, comp, h) per parameter."""
+
+    def __init__(self, s, cam: Camera, t: Tiling, target: np.ndarray):
+        self.s64 = type(s)(*[np.ascontiguousarray(a, np.float64) for a in s.arrays()])
+        self.st, self._keep = _splats(self.s64, np.float64)
+        self.n = s.n
+        self.cam, self.t = cam, t
+        self.proj32 = project(s, cam)
+        self.proj64 = np.ascontiguousarray(project(self.s64, cam, np.float64), np.float64)
+        self.keys, self.pay, self.rg = bin(self.proj32, t)
+        if self.pay.size == 0:
+            self.pay = np.zeros(1, np.uint32)
+        self.target = np.ascontiguousarray(target, np.float32)
+
+    def fd(self, idx: int, attr: int, comp: int, h: float):
+        out = C.c_double()
+        rc = lib().orc_fd_local_mse(C.byref(self.st), C.c_uint32(self.n), C.byref(self.cam), C.byref(self.t),
+                                    _p(self.proj64), _p(self.proj32), _p(self.pay), _p(self.rg), _p(self.target),
+                                    C.c_uint32(idx), C.c_int(attr), C.c_int(comp), C.c_double(h), C.byref(out))
+        if rc < 0:
+            raise ValueError("bad parameter index")
+        return out.value, bool(rc)
+
+    def fd_many(self, idx, attr, comp, h):
+        """Arrays in, (fd, discontinuous) arrays out; threads over parameters."""
+        idx = np.ascontiguousarray(idx, np.uint32)
+        k = idx.size
+        attr = np.ascontiguousarray(np.broadcast_to(attr, (k,)), np.int32)
+        comp = np.ascontiguousarray(np.broadcast_to(comp, (k,)), np.int32)
+        h = np.ascontiguousarray(np.broadcast_to(h, (k,)), np.float64)
+        fd = np.zeros(k, np.float64)
+        disc = np.zeros(k, np.int32)
+        lib().orc_fd_local_mse_batch(C.byref(self.st), C.c_uint32(self.n), C.byref(self.cam), C.byref(self.t),
+                                     _p(self.proj64), _p(self.proj32), _p(self.pay), _p(self.rg), _p(self.target),
+                                     C.c_uint32(k), _p(idx), _p(attr), _p(comp), _p(h), _p(fd), _p(disc))
+        if (disc < 0).any():
+            raise ValueError("bad parameter index")
+        return fd, disc.astype(bool)
+
+
 def tonemap(image: np.ndarray) -> np.ndarray:
     image = np.ascontiguousarray(image, np.float32)
     h, w = image.shape[:2]

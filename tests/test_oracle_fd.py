@@ -92,3 +92,45 @@ def test_reference_chain_rule_is_caught_by_the_same_check():
     finite-difference check must flag its position gradients."""
     ok, bad, disc, worst = _fd_check(full=False)
     assert bad.get("pos", 0) >= 10, (ok, bad)
+
+
+def test_local_fd_equals_the_whole_image_difference():
+    """orc_fd_local_mse re-blends only the tiles of the perturbed splat; it
+    must equal the central difference of the whole float64 image over the
+    same fixed binning (the means of two full renders)."""
+    from nanotracer_amd import synth
+
+    s = synth.make_scene(400, 0xFD10CA1, 0.8, log_scale_mu=-3.0, log_scale_sd=0.6, logit_mu=0.5)
+    cam = O.camera_from_pose((0.0, 0.3, 2.5), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.pi / 3, 80, 64, s.n)
+    t = O.tiling(80, 64, 16, 5.0)
+    tp = O.project(synth.perturb_means(s, 77, 0.03), cam)
+    tk, tpay, trg = O.bin(tp, t)
+    target = O.composite(tp, tpay, trg, t)
+    ctx = O.FdContext(s, cam, t, target)
+    rng = np.random.default_rng(5)
+    vis = np.nonzero(ctx.proj32[:, 7] != 0)[0]
+    checked = 0
+    for k in range(24):
+        idx = int(rng.choice(vis))
+        attr = k % 6
+        comp = int(rng.integers((3, 4, 3, 1, 3, 45)[attr]))
+        h = 1e-4
+        fd, disc = ctx.fd(idx, attr, comp, h)
+        ims = []
+        for sign in (1, -1):
+            s64 = synth.HostSplats(*[np.array(a, np.float64) for a in s.arrays()])
+            a = s64.arrays()[attr]
+            if a.ndim == 1:
+                a[idx] += sign * h
+            else:
+                a[idx, comp] += sign * h
+            p64 = O.project(s64, cam, np.float64)
+            ims.append(O.composite_f64(p64, ctx.pay, ctx.rg, t))
+        sq = [np.sum((im[..., :3] - target[..., :3].astype(np.float64)) ** 2) for im in ims]
+        ref = (sq[0] - sq[1]) / (2 * h) / (3.0 * 80 * 64)
+        if disc:
+            continue
+        # the whole-image sums cancel to ~1e-16 of the image loss: 1e-7 relative
+        assert abs(fd - ref) <= 1e-7 * abs(ref) + 1e-13, (idx, attr, comp, fd, ref)
+        checked += 1
+    assert checked >= 12
