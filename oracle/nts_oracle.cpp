@@ -1096,7 +1096,9 @@ uint32_t orc_densify(orc_splats_mut* s, uint32_t n, uint32_t grad_count, float g
 // record), over the fixed float32 binning that the device gradient uses.
 // A parameter whose +-h evaluations change the splat's tile set, depth
 // bucket or visibility, or any pixel's sequence of contributing splats,
-// blend termination or alpha clamp, is reported as discontinuous (1).
+// blend termination or alpha clamp, is reported as discontinuous (1); one
+// whose float64 pattern differs from the float32 forward's (the device's) on
+// any of those pixels is reported as 2.
 namespace {
 
 uint32_t depth16_of(double depth, float depth_max) {
@@ -1108,7 +1110,9 @@ uint32_t depth16_of(double depth, float depth_max) {
 // blend of one pixel over [begin, end) with splat `idx` drawn from `rec`;
 // returns sum_c (C_c - target_c)^2 and a signature of the contributions
 double pixel_sq_err(const double* proj64, const uint32_t* payloads, uint32_t begin, uint32_t end, uint32_t idx,
-                    const double* rec, double pxf, double pyf, const float* tgt, uint64_t* sig) {
+                    const double* rec, double pxf, double pyf, const float* tgt, const float* w, uint64_t* sig,
+                    bool* touched) {
+  *touched = false;
   double C[3] = {0.0, 0.0, 0.0}, Tr = 1.0;
   uint64_t h = 1469598103934665603ull;
   for (uint32_t i = begin; i < end; ++i) {
@@ -1122,6 +1126,7 @@ double pixel_sq_err(const double* proj64, const uint32_t* payloads, uint32_t beg
     const double alpha = clamped ? (double)NTS_ALPHA_MAX : ar;
     if (alpha < (double)NTS_ALPHA_MIN) continue;
     h = (h ^ ((uint64_t)i * 2 + (clamped ? 1 : 0))) * 1099511628211ull;
+    if (sidx == idx) *touched = true;
     const double ta = Tr * alpha;
     C[0] += ta * p[8];
     C[1] += ta * p[9];
@@ -1135,10 +1140,41 @@ double pixel_sq_err(const double* proj64, const uint32_t* payloads, uint32_t beg
   *sig = h;
   double e = 0.0;
   for (int c = 0; c < 3; ++c) {
-    const double d = C[c] - (double)tgt[c];
-    e += d * d;
+    if (w) {
+      e += (double)w[c] * C[c];  // the linear functional <w, C>
+    } else {
+      const double d = C[c] - (double)tgt[c];
+      e += d * d;
+    }
   }
   return e;
+}
+
+// Contribution signature of one pixel under the float32 blend (the
+// device's forward, bit for bit).
+uint64_t pixel_sig_f32(const float* proj32, const uint32_t* payloads, uint32_t begin, uint32_t end, float pxf,
+                       float pyf, uint32_t idx, bool* touched) {
+  *touched = false;
+  float Tr = 1.0f;
+  uint64_t h = 1469598103934665603ull;
+  for (uint32_t i = begin; i < end; ++i) {
+    const float* p = proj32 + 12 * (size_t)payloads[i];
+    const float dx = pxf - p[0], dy = pyf - p[1];
+    const float power = -0.5f * (p[4] * dx * dx + 2.0f * p[5] * dx * dy + p[6] * dy * dy);
+    if (power > 0.0f) continue;
+    const float ar = p[11] * nts_expf(power);
+    const bool clamped = !(ar < NTS_ALPHA_MAX);
+    const float alpha = clamped ? NTS_ALPHA_MAX : ar;
+    if (alpha < NTS_ALPHA_MIN) continue;
+    h = (h ^ ((uint64_t)i * 2 + (clamped ? 1 : 0))) * 1099511628211ull;
+    if (payloads[i] == idx) *touched = true;
+    Tr = Tr * (1.0f - alpha);
+    if (Tr < NTS_T_MIN) {
+      h = (h ^ 0xABCDull) * 1099511628211ull;
+      break;
+    }
+  }
+  return h;
 }
 
 const int kWidthOf[6] = {3, 4, 3, 1, 3, 45};
@@ -1147,7 +1183,8 @@ const int kWidthOf[6] = {3, 4, 3, 1, 3, 45};
 
 int orc_fd_local_mse(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
                      const double* proj64, const float* proj32, const uint32_t* payloads, const uint32_t* ranges,
-                     const float* target, uint32_t idx, int attr, int comp, double h, double* fd) {
+                     const float* target, const float* weights, uint32_t idx, int attr, int comp, double h,
+                     double* fd) {
   *fd = 0.0;
   if (idx >= n || attr < 0 || attr > 5 || comp < 0 || comp >= kWidthOf[attr]) return -1;
   // this splat's parameters, perturbed
@@ -1188,24 +1225,36 @@ int orc_fd_local_mse(const orc_splats_f64* s, uint32_t n, const orc_camera* cam,
           if (px >= t->width || py >= t->height) continue;
           const double pxf = (double)((float)px + 0.5f), pyf = (double)((float)py + 0.5f);
           const float* tg = target + 4 * ((size_t)py * t->width + px);
+          const float* wp = weights ? weights + 4 * ((size_t)py * t->width + px) : nullptr;
           uint64_t sg[2];
+          bool hit[3];
           for (int sign = 0; sign < 2; ++sign)
-            sum[sign] += pixel_sq_err(proj64, payloads, begin, end, idx, rec[sign], pxf, pyf, tg, &sg[sign]);
+            sum[sign] +=
+                pixel_sq_err(proj64, payloads, begin, end, idx, rec[sign], pxf, pyf, tg, wp, &sg[sign], &hit[sign]);
+          // the float32 forward (the device's) must take the same pattern, or
+          // the difference measures another active set than the gradient;
+          // only pixels the splat reaches carry its derivative
+          const uint64_t s32 = pixel_sig_f32(proj32, payloads, begin, end, (float)px + 0.5f, (float)py + 0.5f, idx,
+                                             &hit[2]);
+          if (!(hit[0] || hit[1] || hit[2])) continue;
           if (sg[0] != sg[1]) disc = 1;
+          if (!disc && s32 != sg[0]) disc = 2;
         }
     }
-  *fd = (sum[0] - sum[1]) / (2.0 * h) / (3.0 * (double)t->width * (double)t->height);
+  *fd = (sum[0] - sum[1]) / (2.0 * h);
+  if (!weights) *fd /= 3.0 * (double)t->width * (double)t->height;
   return disc;
 }
 
 void orc_fd_local_mse_batch(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
                             const double* proj64, const float* proj32, const uint32_t* payloads,
-                            const uint32_t* ranges, const float* target, uint32_t count, const uint32_t* idx,
-                            const int32_t* attr, const int32_t* comp, const double* h, double* fd, int32_t* disc) {
+                            const uint32_t* ranges, const float* target, const float* weights, uint32_t count,
+                            const uint32_t* idx, const int32_t* attr, const int32_t* comp, const double* h,
+                            double* fd, int32_t* disc) {
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t k = 0; k < (int64_t)count; ++k)
-    disc[k] = orc_fd_local_mse(s, n, cam, t, proj64, proj32, payloads, ranges, target, idx[k], attr[k], comp[k],
-                               h[k], &fd[k]);
+    disc[k] = orc_fd_local_mse(s, n, cam, t, proj64, proj32, payloads, ranges, target, weights, idx[k], attr[k],
+                               comp[k], h[k], &fd[k]);
 }
 
 // tonemap.wgsl:27-41.

@@ -137,15 +137,22 @@ double orc_loss_grad(const float* pred, const float* target, uint32_t w, uint32_
 // projections of every splat; payloads / ranges: the float32 binning;
 // target: float4 per pixel.  Returns 0, or 1 when the +-h evaluations are
 // discontinuous (tile set, depth bucket, visibility or any pixel's
-// contribution pattern changes), -1 on bad arguments.
+// contribution pattern changes), 2 when the float64 contribution pattern
+// differs from the float32 forward's on one of the splat's pixels, -1 on
+// bad arguments.
+// With weights (float4 per pixel) the differentiated function is instead
+// the linear functional sum_pixels <w, C> (a vector-Jacobian product with
+// fixed dL/dC), not divided by 3WH.
 int orc_fd_local_mse(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
                      const double* proj64, const float* proj32, const uint32_t* payloads, const uint32_t* ranges,
-                     const float* target, uint32_t idx, int attr, int comp, double h, double* fd);
+                     const float* target, const float* weights, uint32_t idx, int attr, int comp, double h,
+                     double* fd);
 // The same for `count` parameters at once (threads over parameters).
 void orc_fd_local_mse_batch(const orc_splats_f64* s, uint32_t n, const orc_camera* cam, const orc_tiling* t,
                             const double* proj64, const float* proj32, const uint32_t* payloads,
-                            const uint32_t* ranges, const float* target, uint32_t count, const uint32_t* idx,
-                            const int32_t* attr, const int32_t* comp, const double* h, double* fd, int32_t* disc);
+                            const uint32_t* ranges, const float* target, const float* weights, uint32_t count,
+                            const uint32_t* idx, const int32_t* attr, const int32_t* comp, const double* h,
+                            double* fd, int32_t* disc);
 
 // Prune / densify on the host SoA (train.rs:414-602).  Arrays must have
 // capacity for the result.  moments: 12 slabs (m,v per attribute, in the

@@ -364,7 +364,7 @@ class FdContext:
     fixed float32 binning (orc_fd_local_mse): build once per scene and
     camera, then fd(idx, attr, comp, h) per parameter."""
 
-    def __init__(self, s, cam: Camera, t: Tiling, target: np.ndarray):
+    def __init__(self, s, cam: Camera, t: Tiling, target: np.ndarray, weights: np.ndarray | None = None):
         self.s64 = type(s)(*[np.ascontiguousarray(a, np.float64) for a in s.arrays()])
         self.st, self._keep = _splats(self.s64, np.float64)
         self.n = s.n
@@ -375,18 +375,26 @@ class FdContext:
         if self.pay.size == 0:
             self.pay = np.zeros(1, np.uint32)
         self.target = np.ascontiguousarray(target, np.float32)
+        # weights: differentiate sum <w, C> (fixed dL/dC) instead of the MSE
+        self.weights = None if weights is None else np.ascontiguousarray(weights, np.float32)
+
+    def _w(self):
+        return None if self.weights is None else _p(self.weights)
 
     def fd(self, idx: int, attr: int, comp: int, h: float):
         out = C.c_double()
         rc = lib().orc_fd_local_mse(C.byref(self.st), C.c_uint32(self.n), C.byref(self.cam), C.byref(self.t),
                                     _p(self.proj64), _p(self.proj32), _p(self.pay), _p(self.rg), _p(self.target),
-                                    C.c_uint32(idx), C.c_int(attr), C.c_int(comp), C.c_double(h), C.byref(out))
+                                    self._w(), C.c_uint32(idx), C.c_int(attr), C.c_int(comp), C.c_double(h),
+                                    C.byref(out))
         if rc < 0:
             raise ValueError("bad parameter index")
         return out.value, bool(rc)
 
     def fd_many(self, idx, attr, comp, h):
-        """Arrays in, (fd, discontinuous) arrays out; threads over parameters."""
+        """Arrays in, (fd, flag) arrays out (flag 0 continuous, 1 discontinuous
+        under +-h, 2 float32 / float64 patterns differ); threads over
+        parameters."""
         idx = np.ascontiguousarray(idx, np.uint32)
         k = idx.size
         attr = np.ascontiguousarray(np.broadcast_to(attr, (k,)), np.int32)
@@ -396,10 +404,10 @@ class FdContext:
         disc = np.zeros(k, np.int32)
         lib().orc_fd_local_mse_batch(C.byref(self.st), C.c_uint32(self.n), C.byref(self.cam), C.byref(self.t),
                                      _p(self.proj64), _p(self.proj32), _p(self.pay), _p(self.rg), _p(self.target),
-                                     C.c_uint32(k), _p(idx), _p(attr), _p(comp), _p(h), _p(fd), _p(disc))
+                                     self._w(), C.c_uint32(k), _p(idx), _p(attr), _p(comp), _p(h), _p(fd), _p(disc))
         if (disc < 0).any():
             raise ValueError("bad parameter index")
-        return fd, disc.astype(bool)
+        return fd, disc
 
 
 def tonemap(image: np.ndarray) -> np.ndarray:
