@@ -1257,6 +1257,147 @@ void orc_fd_local_mse_batch(const orc_splats_f64* s, uint32_t n, const orc_camer
                                comp[k], h[k], &fd[k]);
 }
 
+// ---- SSIM loss (loss.rs:23-415) ---------------------------------------------
+// Per-channel SSIM over an 11-tap separable Gaussian (sigma 1.5, normalised,
+// reflect-101 borders), C1 = 1e-4, C2 = 9e-4; loss = 1 - mean over pixels and
+// channels; the gradient through the blur adjoint (a scatter with the same
+// reflection map).  float32 arithmetic in the reference's loop orders; the
+// kernel taps use the shared exponential.
+namespace {
+
+const int kSsimWin = 11, kSsimR = 5;
+
+void ssim_kernel(float k[kSsimWin]) {
+  const float centre = (float)(kSsimWin - 1) * 0.5f;
+  const float two_sigma_sq = 2.0f * 1.5f * 1.5f;
+  float sum = 0.0f;
+  for (int i = 0; i < kSsimWin; ++i) {
+    const float d = (float)i - centre;
+    k[i] = nts_expf(-d * d / two_sigma_sq);
+    sum += k[i];
+  }
+  for (int i = 0; i < kSsimWin; ++i) k[i] = k[i] / sum;
+}
+
+int reflect101(int i, int n) {
+  if (n == 1) return 0;
+  const int period = 2 * (n - 1);
+  int x = i % period;
+  if (x < 0) x += period;
+  if (x >= n) x = period - x;
+  return x;
+}
+
+void blur(const float* src, int w, int h, const float* k, float* dst) {
+  std::vector<float> tmp((size_t)w * h);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      float acc = 0.0f;
+      for (int t = 0; t < kSsimWin; ++t) acc += src[(size_t)y * w + reflect101(x + t - kSsimR, w)] * k[t];
+      tmp[(size_t)y * w + x] = acc;
+    }
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      float acc = 0.0f;
+      for (int t = 0; t < kSsimWin; ++t) acc += tmp[(size_t)reflect101(y + t - kSsimR, h) * w + x] * k[t];
+      dst[(size_t)y * w + x] = acc;
+    }
+}
+
+void blur_adjoint(const float* src, int w, int h, const float* k, float* dst) {
+  std::vector<float> tmp((size_t)w * h, 0.0f);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const float sv = src[(size_t)y * w + x];
+      for (int t = 0; t < kSsimWin; ++t) tmp[(size_t)reflect101(y + t - kSsimR, h) * w + x] += sv * k[t];
+    }
+  for (size_t i = 0; i < (size_t)w * h; ++i) dst[i] = 0.0f;
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const float sv = tmp[(size_t)y * w + x];
+      for (int t = 0; t < kSsimWin; ++t) dst[(size_t)y * w + reflect101(x + t - kSsimR, w)] += sv * k[t];
+    }
+}
+
+// loss.rs:212-338 for one channel: returns 1 - mean(SSIM), grad = dL/dx
+float ssim_channel(const float* x, const float* y, int w, int h, const float* k, float* grad) {
+  const size_t n = (size_t)w * h;
+  std::vector<float> xx(n), yy(n), xy(n), mx(n), my(n), gxx(n), gyy(n), gxy(n);
+  for (size_t i = 0; i < n; ++i) {
+    xx[i] = x[i] * x[i];
+    yy[i] = y[i] * y[i];
+    xy[i] = x[i] * y[i];
+  }
+  blur(x, w, h, k, mx.data());
+  blur(y, w, h, k, my.data());
+  blur(xx.data(), w, h, k, gxx.data());
+  blur(yy.data(), w, h, k, gyy.data());
+  blur(xy.data(), w, h, k, gxy.data());
+  std::vector<float> P(n), Q(n), R(n), mxq(n), myr(n);
+  double sum = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const float mux = mx[i], muy = my[i];
+    const float mux2 = mux * mux, muy2 = muy * muy;
+    const float sxx = gxx[i] - mux2, syy = gyy[i] - muy2, sxy = gxy[i] - mux * muy;
+    const float av = 2.0f * mux * muy + 1e-4f;
+    const float bv = 2.0f * sxy + 9e-4f;
+    const float cv = mux2 + muy2 + 1e-4f;
+    const float dv = sxx + syy + 9e-4f;
+    sum += (double)((av * bv) / (cv * dv));
+    const float cd = cv * dv;
+    P[i] = 2.0f * bv * (muy * cv - mux * av) / (cv * cd);
+    Q[i] = -av * bv / (cv * dv * dv);
+    R[i] = 2.0f * av / cd;
+    mxq[i] = mux * Q[i];
+    myr[i] = muy * R[i];
+  }
+  std::vector<float> sp(n), sq(n), sr(n), smq(n), smr(n);
+  blur_adjoint(P.data(), w, h, k, sp.data());
+  blur_adjoint(Q.data(), w, h, k, sq.data());
+  blur_adjoint(R.data(), w, h, k, sr.data());
+  blur_adjoint(mxq.data(), w, h, k, smq.data());
+  blur_adjoint(myr.data(), w, h, k, smr.data());
+  const float inv_n = 1.0f / (float)n;
+  for (size_t i = 0; i < n; ++i) {
+    const float d = sp[i] + 2.0f * (x[i] * sq[i] - smq[i]) + (y[i] * sr[i] - smr[i]);
+    grad[i] = -inv_n * d;
+  }
+  return 1.0f - (float)(sum / (double)n);
+}
+
+}  // namespace
+
+void orc_ssim_kernel(float* k) { ssim_kernel(k); }
+
+void orc_ssim_blur(const float* src, int w, int h, float* dst, int adjoint) {
+  float k[kSsimWin];
+  ssim_kernel(k);
+  if (adjoint) blur_adjoint(src, w, h, k, dst);
+  else blur(src, w, h, k, dst);
+}
+
+// ssim_loss_grad (loss.rs:346-383): channels of float4 images, loss averaged
+// over the three channels, grad = float4 (w = 0).
+float orc_ssim_loss_grad(const float* pred, const float* target, int w, int h, float* grad) {
+  float k[kSsimWin];
+  ssim_kernel(k);
+  const size_t n = (size_t)w * h;
+  std::vector<float> px(n), tx(n), g(n);
+  float loss = 0.0f;
+  float l[3];
+  for (int c = 0; c < 3; ++c) {
+    for (size_t i = 0; i < n; ++i) {
+      px[i] = pred[4 * i + c];
+      tx[i] = target[4 * i + c];
+    }
+    l[c] = ssim_channel(px.data(), tx.data(), w, h, k, g.data());
+    for (size_t i = 0; i < n; ++i) grad[4 * i + c] = g[i] / 3.0f;
+  }
+  for (size_t i = 0; i < n; ++i) grad[4 * i + 3] = 0.0f;
+  loss = (l[0] + l[1] + l[2]) / 3.0f;
+  return loss;
+}
+
 // tonemap.wgsl:27-41.
 void orc_tonemap(const float* image, uint32_t w, uint32_t h, uint8_t* out) {
   for (uint64_t i = 0; i < (uint64_t)w * h; ++i) {

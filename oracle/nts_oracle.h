@@ -154,6 +154,13 @@ void orc_fd_local_mse_batch(const orc_splats_f64* s, uint32_t n, const orc_camer
                             const uint32_t* idx, const int32_t* attr, const int32_t* comp, const double* h,
                             double* fd, int32_t* disc);
 
+// SSIM (loss.rs:23-415): the normalised 11-tap kernel, one separable blur
+// (or its adjoint) of a w x h plane, and the 3-channel SSIM loss
+// 1 - mean(SSIM) with dL/dC (float4 images, w ignored / written 0).
+void orc_ssim_kernel(float* k);
+void orc_ssim_blur(const float* src, int w, int h, float* dst, int adjoint);
+float orc_ssim_loss_grad(const float* pred, const float* target, int w, int h, float* grad);
+
 // Prune / densify on the host SoA (train.rs:414-602).  Arrays must have
 // capacity for the result.  moments: 12 slabs (m,v per attribute, in the
 // attribute order pos, rot, scale, logit, dc, rest), may be NULL.

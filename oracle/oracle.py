@@ -356,6 +356,44 @@ def densify(s, grad_count, grad_threshold, split_thr, max_splats, it, moments=No
     return _unmut(arrs, n2), mo, ga[:n2], tuple(counts)
 
 
+def ssim_kernel() -> np.ndarray:
+    k = np.zeros(11, np.float32)
+    lib().orc_ssim_kernel(_p(k))
+    return k
+
+
+def ssim_blur(plane: np.ndarray, adjoint: bool = False) -> np.ndarray:
+    plane = np.ascontiguousarray(plane, np.float32)
+    out = np.zeros_like(plane)
+    h, w = plane.shape
+    lib().orc_ssim_blur(_p(plane), C.c_int(w), C.c_int(h), _p(out), C.c_int(int(adjoint)))
+    return out
+
+
+def ssim_loss_grad(pred: np.ndarray, target: np.ndarray):
+    """loss.rs:346-383: (1 - mean SSIM over channels, dL/dC float4)."""
+    pred = np.ascontiguousarray(pred, np.float32)
+    target = np.ascontiguousarray(target, np.float32)
+    h, w = pred.shape[:2]
+    grad = np.zeros((h, w, 4), np.float32)
+    lib().orc_ssim_loss_grad.restype = C.c_float
+    loss = lib().orc_ssim_loss_grad(_p(pred), _p(target), C.c_int(w), C.c_int(h), _p(grad))
+    return float(loss), grad
+
+
+def combined_loss_grad(pred: np.ndarray, target: np.ndarray, lam: float):
+    """loss.rs:390-415: (total, mse, ssim, dL/dC) of (1 - lam) MSE + lam (1 - SSIM)."""
+    lam = min(max(lam, 0.0), 1.0)
+    mse, mg = loss_grad(pred, target, 1)
+    if lam == 0.0:
+        return mse, mse, 0.0, mg
+    ssim, sg = ssim_loss_grad(pred, target)
+    if lam == 1.0:
+        return ssim, 0.0, ssim, sg
+    a, b = np.float32(1.0 - lam), np.float32(lam)
+    return float(np.float32(a * np.float32(mse) + b * np.float32(ssim))), mse, ssim, mg * a + sg * b
+
+
 ATTRS = ("pos", "rot", "log_scale", "logit", "sh_dc", "sh_rest")
 
 
