@@ -177,6 +177,14 @@ int nts_composite_ex(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* 
 /* dl_dc: 16 B per pixel (w = 0).  *loss receives the scalar loss. */
 int nts_loss_grad(nts_ctx* ctx, const nts_buffer* image, const nts_buffer* target, nts_buffer* dl_dc,
                   uint32_t width, uint32_t height, int kind, double* loss);
+/* combined_loss_grad (loss.rs:390-415): (1 - lambda) MSE + lambda (1 - SSIM)
+ * with lambda clamped to [0, 1]; SSIM per channel over an 11-tap Gaussian
+ * window (sigma 1.5, reflect-101 borders, C1 = 1e-4, C2 = 9e-4, loss.rs:23-338)
+ * averaged over the three channels (ssim_loss_grad, loss.rs:346-383).
+ * lambda = 0 is nts_loss_grad with NTS_LOSS_MSE.  *total, *mse and *ssim
+ * (any may be NULL) receive the three components the reference returns. */
+int nts_loss_grad_ssim(nts_ctx* ctx, const nts_buffer* image, const nts_buffer* target, nts_buffer* dl_dc,
+                       uint32_t width, uint32_t height, float lambda, double* total, double* mse, double* ssim);
 
 /* ---- backward (Rasterizer::composite_backward, raster.rs:653-724;
  *      project_backward, raster.rs:559-645) ---------------------------- */

@@ -396,6 +396,16 @@ def loss_grad(ctx: GpuCtx, image: Buffer, target: Buffer, dl_dc: Buffer, width: 
     return out.value
 
 
+def combined_loss_grad(ctx: GpuCtx, image: Buffer, target: Buffer, dl_dc: Buffer, width: int, height: int,
+                       lam: float) -> tuple:
+    """(1 - lam) MSE + lam (1 - SSIM) and its dL/dC on the device
+    (combined_loss_grad, loss.rs:390-415): returns (total, mse, ssim)."""
+    t, m, s = C.c_double(), C.c_double(), C.c_double()
+    L.check(L.lib().nts_loss_grad_ssim(ctx.h, image.h, target.h, dl_dc.h, width, height, float(lam), C.byref(t),
+                                       C.byref(m), C.byref(s)))
+    return t.value, m.value, s.value
+
+
 @dataclass
 class AdamConfig:
     """AdamConfig (adam.rs:22-39); defaults as the reference."""

@@ -383,15 +383,15 @@ def ssim_loss_grad(pred: np.ndarray, target: np.ndarray):
 
 def combined_loss_grad(pred: np.ndarray, target: np.ndarray, lam: float):
     """loss.rs:390-415: (total, mse, ssim, dL/dC) of (1 - lam) MSE + lam (1 - SSIM)."""
-    lam = min(max(lam, 0.0), 1.0)
+    lam = np.float32(min(max(np.float32(lam), np.float32(0.0)), np.float32(1.0)))
     mse, mg = loss_grad(pred, target, 1)
     if lam == 0.0:
         return mse, mse, 0.0, mg
     ssim, sg = ssim_loss_grad(pred, target)
     if lam == 1.0:
         return ssim, 0.0, ssim, sg
-    a, b = np.float32(1.0 - lam), np.float32(lam)
-    return float(np.float32(a * np.float32(mse) + b * np.float32(ssim))), mse, ssim, mg * a + sg * b
+    a, b = np.float32(1.0) - lam, lam
+    return float(a * np.float32(mse) + b * np.float32(ssim)), mse, ssim, mg * a + sg * b
 
 
 ATTRS = ("pos", "rot", "log_scale", "logit", "sh_dc", "sh_rest")

@@ -19,7 +19,7 @@ def test_kernel_is_the_normalised_gaussian():
     assert np.array_equal(O.ssim_kernel(), O.ssim_kernel()[::-1])
 
 
-@pytest.mark.parametrize("h,w", [(8, 8), (13, 6), (40, 57)])
+@pytest.mark.parametrize("h,w", [(8, 8), (13, 6), (40, 57), (3, 5), (1, 7), (2, 2), (1, 1)])
 def test_blur_is_scipy_mirror_gaussian(h, w):
     x = np.random.default_rng(h * w).random((h, w)).astype(np.float32)
     ref = ndimage.gaussian_filter(x.astype(np.float64), 1.5, mode="mirror", truncate=5 / 1.5)
