@@ -82,6 +82,7 @@ typedef struct nts_constraints {
 
 #define NTS_LOSS_L1 0
 #define NTS_LOSS_MSE 1
+#define NTS_LOSS_SSIM 2  /* (1 - lambda) MSE + lambda (1 - SSIM); batched steps */
 
 #define NTS_BWD_REF_PARITY 0  /* rasterize_backward.wgsl semantics */
 #define NTS_BWD_RECOMPUTE 1   /* final-T + last-index recompute, full chain rule */
@@ -243,10 +244,11 @@ int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_splats, uint
 int nts_batch_destroy(nts_ctx* ctx, nts_batch* batch);
 /* One training iteration over n_views views in single launches per stage:
  * project + count, scan, emit, view-segmented sort, ranges, blend forward
- * with the loss fused (targets: float4 per pixel, views concatenated),
- * blend backward, then per-splat project_backward summed over views fused
- * with accumulate_grad + Adam (step t) + constraints.  Per-view losses
- * (mean over 3WH) are summed into the loss reported by nts_batch_losses.
+ * with an L1 or MSE loss fused (targets: float4 per pixel, views
+ * concatenated; NTS_LOSS_SSIM adds two tiled SSIM passes over all views
+ * after the blend), blend backward, then per-splat project_backward summed
+ * over views fused with accumulate_grad + Adam (step t) + constraints.
+ * The per-view losses are reported by nts_batch_losses.
  * Nothing synchronises with the host; with NTS_STEP_GRAPH the step is
  * captured in a hipGraph on first use and replayed while the views and
  * settings stay the same and t advances by one.  Pairs beyond
@@ -263,6 +265,14 @@ int nts_render_views(nts_ctx* ctx, const nts_scene* scene, nts_batch* batch, con
  * default.  No reference counterpart: the reference stages 256
  * (rasterize.wgsl:40). */
 int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* batch, uint32_t fwd_batch, uint32_t bwd_batch);
+/* Blend factor lambda of NTS_LOSS_SSIM steps, (1 - lambda) MSE + lambda
+ * (1 - SSIM) (TrainConfig::ssim_lambda, train.rs:48-52; 0.2 by default,
+ * main.rs:189-190), clamped to [0, 1].  The first lambda > 0 allocates the
+ * SSIM planes (60 B per pixel of max_pixels), so the steps allocate
+ * nothing; NTS_LOSS_SSIM steps fail until then.  lambda = 0 makes such
+ * steps plain MSE (loss.rs:398-402).  nts_batch_losses then reports the
+ * combined per-view loss. */
+int nts_batch_set_ssim_lambda(nts_ctx* ctx, nts_batch* batch, float lambda);
 int nts_batch_losses(nts_ctx* ctx, const nts_batch* batch, double* losses);
 int nts_batch_stats(nts_ctx* ctx, const nts_batch* batch, uint64_t* total_pairs, uint32_t* visible,
                     uint32_t* overflow);

@@ -499,6 +499,11 @@ class ViewBatch:
         """Splats staged per LDS batch by the blend forward / backward."""
         L.check(L.lib().nts_batch_set_lds_batch(self.ctx.h, self.h, int(fwd_batch), int(bwd_batch)))
 
+    def set_ssim_lambda(self, lam: float = 0.2) -> None:
+        """Blend factor of LOSS_SSIM steps: (1 - lam) MSE + lam (1 - SSIM)
+        (train.rs:48-52); allocates the SSIM planes on first use."""
+        L.check(L.lib().nts_batch_set_ssim_lambda(self.ctx.h, self.h, float(lam)))
+
     def render(self, splats: GpuSplatBuffer, views, tile_size: int = 16) -> None:
         arr = views_array(views)
         self.sizes = [(v.width, v.height) for v in views]

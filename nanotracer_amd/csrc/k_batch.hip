@@ -63,6 +63,11 @@ struct nts_batch {
   uint32_t last_tile_base[NTS_MAX_VIEWS + 1] = {};
   // splats staged per LDS batch by the blend kernels
   uint32_t fwd_batch = NTS_DEFAULT_LDS_BATCH, bwd_batch = NTS_DEFAULT_LDS_BATCH;
+  // combined MSE + SSIM loss of NTS_LOSS_SSIM steps (planes allocated by the setter)
+  float ssim_lambda = 0.2f;
+  void* ssim_base = nullptr;
+  uint64_t ssim_bytes = 0;
+  bool last_loss_final = false;  // losses[] already hold the per-view loss
 };
 
 struct U32Table {
@@ -302,6 +307,7 @@ extern "C" int nts_batch_destroy(nts_ctx* ctx, nts_batch* b) {
   (void)hipStreamSynchronize(ctx->stream);
   drop_graph(b);
   if (b->h_cams) (void)hipHostFree(b->h_cams);
+  if (b->ssim_base) nts_dev_free(ctx, b->ssim_base, b->ssim_bytes);
   nts_dev_free(ctx, b->base, b->bytes);
   delete b;
   return NTS_OK;
@@ -345,7 +351,7 @@ static int plan_batch(nts_batch* b, const nts_scene* scene, const nts_view_desc*
     d.proj_base = v * scene->n;
     d.pix_base = pix;
     const float n_scalars = (float)w * (float)h * 3.0f;
-    d.inv_n = loss_kind == NTS_LOSS_MSE ? 2.0f / n_scalars : 1.0f / n_scalars;
+    d.inv_n = loss_kind == NTS_LOSS_L1 ? 1.0f / n_scalars : 2.0f / n_scalars;
     pl.tt.t[v] = ViewTiles{tx, ty, tile_size, views[v].depth_max};
     pix += (uint64_t)w * h;
     tiles += (uint64_t)tx * ty;
@@ -456,11 +462,30 @@ static int record_step(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const Batch
   if (rc) return rc;
   rc = nts_fill_u32(ctx, b->pgrad, 0, (uint64_t)V * n * (sizeof(ProjectedGrad) / 4));
   if (rc) return rc;
-  LossArgs la{(const float4*)targets->ptr, b->dl_dc, b->losses, loss_kind};
+  // NTS_LOSS_SSIM at lambda 0 is the fused MSE (loss.rs:398-402)
+  const bool ssim = loss_kind == NTS_LOSS_SSIM && b->ssim_lambda > 0.0f;
+  LossArgs la{(const float4*)targets->ptr, b->dl_dc, b->losses, loss_kind == NTS_LOSS_L1 ? NTS_LOSS_L1 : NTS_LOSS_MSE};
   bool alt = false;
   bool* altp = &alt;
-  rc = launch_forward(ctx, scene, b, pl, &la, mode == NTS_BWD_RECOMPUTE, &altp);
+  rc = launch_forward(ctx, scene, b, pl, ssim ? nullptr : &la, mode == NTS_BWD_RECOMPUTE, &altp);
   if (rc) return rc;
+  if (ssim) {
+    uint32_t ws[NTS_MAX_VIEWS], hs[NTS_MAX_VIEWS];
+    uint64_t pb[NTS_MAX_VIEWS];
+    for (uint32_t v = 0; v < V; ++v) {
+      ws[v] = pl.vt.v[v].width;
+      hs[v] = pl.vt.v[v].height;
+      pb[v] = pl.vt.v[v].pix_base;
+    }
+    SsimViews sv;
+    rc = nts_ssim_views(V, ws, hs, pb, &sv);
+    if (rc) return rc;
+    double* sums = (double*)b->ssim_base;
+    float* planes = (float*)((char*)b->ssim_base + 8 * 4 * NTS_MAX_VIEWS);
+    rc = nts_ssim_launch(ctx, sv, b->image, (const float4*)targets->ptr, planes, b->max_pixels, sums, b->dl_dc,
+                         b->ssim_lambda, b->losses);
+    if (rc) return rc;
+  }
   const uint32_t* pay = alt ? b->vals_b : b->vals_a;
   rc = nts_blend_backward(ctx, pl.tile_size, b->bwd_batch, mode, b->proj, pay, b->ranges, pl.vt, b->image,
                           b->final_t, b->last_idx, b->dl_dc, b->pgrad);
@@ -489,13 +514,16 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
                                     int bwd_mode, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr,
                                     uint64_t t, const nts_constraints* cons, int flags) {
   NTS_CHECK_ARG(ctx && scene && b && views && targets && cfg_pos && cfg_attr && cons && t > 0);
-  NTS_CHECK_ARG(loss_kind == NTS_LOSS_L1 || loss_kind == NTS_LOSS_MSE);
+  NTS_CHECK_ARG(loss_kind == NTS_LOSS_L1 || loss_kind == NTS_LOSS_MSE || loss_kind == NTS_LOSS_SSIM);
   NTS_CHECK_ARG(bwd_mode == NTS_BWD_REF_PARITY || bwd_mode == NTS_BWD_RECOMPUTE);
+  if (loss_kind == NTS_LOSS_SSIM && b->ssim_lambda > 0.0f && !b->ssim_base)
+    return nts_fail(NTS_E_INVALID, "NTS_LOSS_SSIM needs nts_batch_set_ssim_lambda first (it allocates the planes)");
   BatchPlan pl;
   int rc = plan_batch(b, scene, views, n_views, tile_size, loss_kind, pl);
   if (rc) return rc;
   NTS_CHECK_ARG(targets->bytes >= pl.pixels * 16);
   remember_shape(b, pl);
+  b->last_loss_final = loss_kind == NTS_LOSS_SSIM && b->ssim_lambda > 0.0f;
   const int write_grads = (flags & NTS_STEP_WRITE_GRADS) ? 1 : 0;
   const bool use_graph = (flags & NTS_STEP_GRAPH) != 0 && !ctx->prof;
   if (!use_graph) {
@@ -515,7 +543,10 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
   put(ids, sizeof ids);
   put(&scene->n, 4);
   put(views, sizeof(nts_view_desc) * n_views);
-  const int meta[6] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch};
+  int lam_bits;
+  memcpy(&lam_bits, &b->ssim_lambda, 4);
+  const int meta[7] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch,
+                       loss_kind == NTS_LOSS_SSIM ? lam_bits : 0};
   put(meta, sizeof meta);
   put(cfg_pos, sizeof *cfg_pos);
   put(cfg_attr, sizeof *cfg_attr);
@@ -578,11 +609,24 @@ extern "C" int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* b, uint32_t fwd_
   return NTS_OK;
 }
 
+extern "C" int nts_batch_set_ssim_lambda(nts_ctx* ctx, nts_batch* b, float lambda) {
+  NTS_CHECK_ARG(ctx && b && lambda == lambda);
+  const float lam = lambda < 0.0f ? 0.0f : (lambda > 1.0f ? 1.0f : lambda);
+  if (lam > 0.0f && !b->ssim_base) {
+    const uint64_t bytes = 8 * 4 * NTS_MAX_VIEWS + 15 * b->max_pixels * 4;
+    const int rc = nts_dev_alloc(ctx, &b->ssim_base, bytes);
+    if (rc) return rc;
+    b->ssim_bytes = bytes;
+  }
+  b->ssim_lambda = lam;
+  return NTS_OK;
+}
+
 extern "C" int nts_batch_losses(nts_ctx* ctx, const nts_batch* b, double* losses) {
   NTS_CHECK_ARG(ctx && b && losses);
   NTS_HIP_TRY(hipMemcpyAsync(losses, b->losses, 8 * (size_t)b->n_views, hipMemcpyDeviceToHost, ctx->stream));
   NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (uint32_t v = 0; v < b->n_views; ++v) {
+  for (uint32_t v = 0; v < b->n_views && !b->last_loss_final; ++v) {
     const uint64_t np = b->pix_base[v + 1] - b->pix_base[v];
     losses[v] = losses[v] / ((double)np * 3.0);
   }

@@ -1,27 +1,31 @@
 // Combined MSE + SSIM loss on the device (loss.rs:23-415): per channel,
-// SSIM over an 11-tap separable Gaussian (sigma 1.5, normalised, reflect-101
-// borders), C1 = 1e-4, C2 = 9e-4, loss 1 - mean(SSIM) averaged over the
-// three channels, the gradient through the transpose of the blur.
+// SSIM over an 11-tap separable Gaussian (sigma 1.5, normalised,
+// reflect-101 borders), C1 = 1e-4, C2 = 9e-4, loss 1 - mean(SSIM) averaged
+// over the three channels, the gradient through the transpose of the blur.
 //
-// Four passes over planar intermediates (3 channels x 5 planes):
-//   h      horizontal blur of x, y, x^2, y^2, xy;
-//   v      vertical blur, the SSIM map (summed in float64 per channel) and
-//          its partials P = dS/dmu_x, Q = dS/dsigma_x^2, R = dS/dsigma_xy with
-//          the centring products mu_x Q and mu_y R;
-//   vadj   the vertical transpose;
-//   hadj   the horizontal transpose, then dL/dx and the weighted sum with
-//          the MSE gradient into dL/dC.
-// The transpose is written as a gather in the scatter's summation order, so
-// every value matches the scatter form bit for bit; the per-channel SSIM sums
-// are float64 atomics (order-dependent only in the last bits of a double).
+// Two LDS-tiled passes over 64 x 16 pixel tiles of every view of a batch:
+//   moments  loads the tile's window (5-pixel reflected halo) of prediction
+//            and target once, then per channel blurs x, y, x^2, y^2, xy
+//            horizontally into LDS and vertically into registers, forms the
+//            SSIM map (summed in float64 per view and channel, with the MSE
+//            sum) and its partials P = dS/dmu_x, Q = dS/dsigma_x^2,
+//            R = dS/dsigma_xy and the centring products mu_x Q, mu_y R
+//            (15 planes, 60 B per pixel);
+//   grad     per channel loads those 5 planes over the halo, applies the
+//            vertical then horizontal transpose of the blur in LDS, and
+//            combines dL/dx with the MSE gradient into dL/dC.
+// The transpose is gathered in the scatter form's summation order, and
+// every float32 expression keeps the host restatement's operation order, so
+// dL/dC matches it bit for bit; the float64 sums differ only in order.
 #include "nts_raster.h"
 
 #define SSIM_TAPS 11
 #define SSIM_R 5
-
-struct SsimKernel {
-  float k[SSIM_TAPS];
-};
+#define SSIM_TW 64
+#define SSIM_TH 16
+#define SSIM_WW (SSIM_TW + 2 * SSIM_R)
+#define SSIM_WH (SSIM_TH + 2 * SSIM_R)
+#define SSIM_NT 256
 
 // reflect-101 over a period of 2(n - 1) (loss.rs:57-67); n >= 6 needs at
 // most one reflection for offsets up to 5
@@ -39,89 +43,6 @@ __device__ __forceinline__ int refl101(int i, int n) {
   return x;
 }
 
-__global__ void __launch_bounds__(256) k_ssim_h(const float4* __restrict__ pred, const float4* __restrict__ target,
-                                                uint32_t w, uint32_t h, SsimKernel K, float* __restrict__ H5) {
-  const uint64_t n = (uint64_t)w * h;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % w);
-  const uint64_t row = i - (uint64_t)x;
-  float s[3][5];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int q = 0; q < 5; ++q) s[c][q] = 0.0f;
-#pragma unroll
-  for (int t = 0; t < SSIM_TAPS; ++t) {
-    const uint64_t j = row + (uint64_t)refl101(x + t - SSIM_R, (int)w);
-    const float4 p = pred[j], g = target[j];
-    const float pc[3] = {p.x, p.y, p.z}, gc[3] = {g.x, g.y, g.z};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float a = pc[c], b = gc[c];
-      s[c][0] = s[c][0] + a * K.k[t];
-      s[c][1] = s[c][1] + b * K.k[t];
-      s[c][2] = s[c][2] + (a * a) * K.k[t];
-      s[c][3] = s[c][3] + (b * b) * K.k[t];
-      s[c][4] = s[c][4] + (a * b) * K.k[t];
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int q = 0; q < 5; ++q) H5[(uint64_t)(c * 5 + q) * n + i] = s[c][q];
-}
-
-__global__ void __launch_bounds__(256) k_ssim_v(const float* __restrict__ H5, uint32_t w, uint32_t h, SsimKernel K,
-                                                float* __restrict__ T5, double* __restrict__ sums) {
-  const uint64_t n = (uint64_t)w * h;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double local[3] = {0.0, 0.0, 0.0};
-  if (i < n) {
-    const int y = (int)(i / w), x = (int)(i - (uint64_t)y * w);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float m[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int t = 0; t < SSIM_TAPS; ++t) {
-        const uint64_t j = (uint64_t)refl101(y + t - SSIM_R, (int)h) * w + x;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) m[q] = m[q] + H5[(uint64_t)(c * 5 + q) * n + j] * K.k[t];
-      }
-      const float mux = m[0], muy = m[1];
-      const float mux2 = mux * mux, muy2 = muy * muy;
-      const float sxx = m[2] - mux2, syy = m[3] - muy2, sxy = m[4] - mux * muy;
-      const float av = 2.0f * mux * muy + 1e-4f;
-      const float bv = 2.0f * sxy + 9e-4f;
-      const float cv = mux2 + muy2 + 1e-4f;
-      const float dv = sxx + syy + 9e-4f;
-      local[c] = (double)((av * bv) / (cv * dv));
-      const float cd = cv * dv;
-      const float P = 2.0f * bv * (muy * cv - mux * av) / (cv * cd);
-      const float Q = -av * bv / (cv * dv * dv);
-      const float R = 2.0f * av / cd;
-      T5[(uint64_t)(c * 5 + 0) * n + i] = P;
-      T5[(uint64_t)(c * 5 + 1) * n + i] = Q;
-      T5[(uint64_t)(c * 5 + 2) * n + i] = R;
-      T5[(uint64_t)(c * 5 + 3) * n + i] = mux * Q;
-      T5[(uint64_t)(c * 5 + 4) * n + i] = muy * R;
-    }
-  }
-  __shared__ double s_sum[3][4];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double v = local[c];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63u) == 0) s_sum[c][threadIdx.x >> 6] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    const double v = s_sum[threadIdx.x][0] + s_sum[threadIdx.x][1] + s_sum[threadIdx.x][2] + s_sum[threadIdx.x][3];
-    unsafeAtomicAdd(sums + threadIdx.x, v);
-  }
-}
-
 // Transpose of one blur pass along an axis of n samples, for source index
 // i: value(j) is the pass's output-side value at index j.  Contributions are
 // summed in the order of the scatter form (output index j ascending, then
@@ -129,7 +50,8 @@ __global__ void __launch_bounds__(256) k_ssim_v(const float* __restrict__ H5, ui
 // n >= 6 the taps of j that land on i are t = 5 - i - j (through the low
 // reflection, i != 0), t = i - j + 5 (direct) and t = 2(n - 1) - i - j + 5
 // (through the high reflection, i != n - 1), in that order, and only
-// j in [i - 5, i + 5] has any.
+// j in [i - 5, i + 5] has any.  Below 6 samples every j in [0, n) lies
+// within 4 of i.
 template <typename F>
 __device__ __forceinline__ float adj_gather(int i, int n, const SsimKernel& K, F value) {
   float acc = 0.0f;
@@ -154,62 +76,225 @@ __device__ __forceinline__ float adj_gather(int i, int n, const SsimKernel& K, F
   return acc;
 }
 
-__global__ void __launch_bounds__(256) k_ssim_vadj(const float* __restrict__ T5, uint32_t w, uint32_t h, SsimKernel K,
-                                                   float* __restrict__ A5) {
-  const uint64_t n = (uint64_t)w * h;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int y = (int)(i / w), x = (int)(i - (uint64_t)y * w);
-  for (int p = 0; p < 15; ++p) {
-    const float* src = T5 + (uint64_t)p * n;
-    A5[(uint64_t)p * n + i] = adj_gather(y, (int)h, K, [&](int j) { return src[(uint64_t)j * w + x]; });
-  }
+struct TileLoc {
+  uint32_t view, w, h;
+  int x0, y0;
+  uint64_t pix_base;
+};
+
+__device__ __forceinline__ TileLoc locate(const SsimViews& sv, uint32_t b) {
+  uint32_t v = 0;
+  while (v + 1 < sv.n && b >= sv.tile_base[v + 1]) ++v;
+  const uint32_t t = b - sv.tile_base[v];
+  TileLoc L;
+  L.view = v;
+  L.w = sv.w[v];
+  L.h = sv.h[v];
+  L.x0 = (int)(t % sv.tiles_x[v]) * SSIM_TW;
+  L.y0 = (int)(t / sv.tiles_x[v]) * SSIM_TH;
+  L.pix_base = sv.pix_base[v];
+  return L;
 }
 
-__global__ void __launch_bounds__(256) k_ssim_hadj(const float* __restrict__ A5, const float4* __restrict__ pred,
-                                                   const float4* __restrict__ target, uint32_t w, uint32_t h,
-                                                   SsimKernel K, float lam, float inv_mse, float4* __restrict__ dl_dc) {
-  const uint64_t n = (uint64_t)w * h;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % w);
-  const uint64_t row = i - (uint64_t)x;
-  float adj[15];
-#pragma unroll
-  for (int p = 0; p < 15; ++p) {
-    const float* src = A5 + (uint64_t)p * n + row;
-    adj[p] = adj_gather(x, (int)w, K, [&](int j) { return src[j]; });
+__global__ void __launch_bounds__(SSIM_NT) k_ssim_moments(const float4* __restrict__ pred,
+                                                          const float4* __restrict__ target, SsimViews sv,
+                                                          SsimKernel K, float* __restrict__ T5, uint64_t plane,
+                                                          double* __restrict__ sums) {
+  __shared__ float s_in[6][SSIM_WH][SSIM_WW];   // x r g b, y r g b
+  __shared__ float s_h[5][SSIM_WH][SSIM_TW];    // horizontal blurs of one channel
+  __shared__ double s_red[4][SSIM_NT / 64];
+  const TileLoc L = locate(sv, blockIdx.x);
+  const int w = (int)L.w, h = (int)L.h;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < SSIM_WH * SSIM_WW; k += SSIM_NT) {
+    const int ly = k / SSIM_WW, lx = k - ly * SSIM_WW;
+    const int ry = L.y0 - SSIM_R + ly, rx = L.x0 - SSIM_R + lx;
+    // cells beyond 5 past the last row or column feed no output pixel
+    const int gy = ry < h + SSIM_R ? refl101(ry, h) : 0;
+    const int gx = rx < w + SSIM_R ? refl101(rx, w) : 0;
+    const uint64_t idx = L.pix_base + (uint64_t)gy * w + gx;
+    const float4 p = pred[idx], g = target[idx];
+    s_in[0][ly][lx] = p.x;
+    s_in[1][ly][lx] = p.y;
+    s_in[2][ly][lx] = p.z;
+    s_in[3][ly][lx] = g.x;
+    s_in[4][ly][lx] = g.y;
+    s_in[5][ly][lx] = g.z;
   }
-  const float4 pv = pred[i], tv = target[i];
-  const float xs[3] = {pv.x, pv.y, pv.z}, ys[3] = {tv.x, tv.y, tv.z};
-  const float inv_n = 1.0f / (float)n;
-  float out[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const float* a = adj + 5 * c;
-    const float d = a[0] + 2.0f * (xs[c] * a[1] - a[3]) + (ys[c] * a[2] - a[4]);
-    const float gs = (-inv_n * d) / 3.0f;          // ssim_loss_grad's channel average
-    const float gm = (xs[c] - ys[c]) * inv_mse;    // 2 (x - y) / (3WH)
-    out[c] = gm * (1.0f - lam) + gs * lam;
-  }
-  dl_dc[i] = make_float4(out[0], out[1], out[2], 0.0f);
-}
-
-__global__ void __launch_bounds__(256) k_mse_sum(const float4* __restrict__ pred, const float4* __restrict__ target,
-                                                 uint64_t n, double* __restrict__ acc) {
-  double local = 0.0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const float4 p = pred[i], t = target[i];
-    const float d0 = p.x - t.x, d1 = p.y - t.y, d2 = p.z - t.z;
-    local += (double)(d0 * d0 + d1 * d1 + d2 * d2);
-  }
-  __shared__ double s_part[4];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off, 64);
-  if ((threadIdx.x & 63u) == 0) s_part[threadIdx.x >> 6] = local;
   __syncthreads();
-  if (threadIdx.x == 0) unsafeAtomicAdd(acc, s_part[0] + s_part[1] + s_part[2] + s_part[3]);
+  const int lx = tid & (SSIM_TW - 1), ly0 = tid / SSIM_TW;
+  constexpr int RS = SSIM_NT / SSIM_TW;  // rows per sweep of the block
+  double acc_s[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    for (int k = tid; k < SSIM_WH * SSIM_TW; k += SSIM_NT) {
+      const int ry = k / SSIM_TW, rx = k - ry * SSIM_TW;
+      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
+#pragma unroll
+      for (int t = 0; t < SSIM_TAPS; ++t) {
+        const float a = s_in[c][ry][rx + t], b = s_in[3 + c][ry][rx + t];
+        s0 = s0 + a * K.k[t];
+        s1 = s1 + b * K.k[t];
+        s2 = s2 + (a * a) * K.k[t];
+        s3 = s3 + (b * b) * K.k[t];
+        s4 = s4 + (a * b) * K.k[t];
+      }
+      s_h[0][ry][rx] = s0;
+      s_h[1][ry][rx] = s1;
+      s_h[2][ry][rx] = s2;
+      s_h[3][ry][rx] = s3;
+      s_h[4][ry][rx] = s4;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SSIM_TH / RS; ++r) {
+      const int ly = ly0 + r * RS;
+      const int ox = L.x0 + lx, oy = L.y0 + ly;
+      if (ox >= w || oy >= h) continue;
+      float m[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int t = 0; t < SSIM_TAPS; ++t)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) m[q] = m[q] + s_h[q][ly + t][lx] * K.k[t];
+      const float mux = m[0], muy = m[1];
+      const float mux2 = mux * mux, muy2 = muy * muy;
+      const float sxx = m[2] - mux2, syy = m[3] - muy2, sxy = m[4] - mux * muy;
+      const float av = 2.0f * mux * muy + 1e-4f;
+      const float bv = 2.0f * sxy + 9e-4f;
+      const float cv = mux2 + muy2 + 1e-4f;
+      const float dv = sxx + syy + 9e-4f;
+      acc_s[c] += (double)((av * bv) / (cv * dv));
+      const float cd = cv * dv;
+      const float P = 2.0f * bv * (muy * cv - mux * av) / (cv * cd);
+      const float Q = -av * bv / (cv * dv * dv);
+      const float R = 2.0f * av / cd;
+      float* base = T5 + (uint64_t)(c * 5) * plane + L.pix_base + (uint64_t)oy * w + ox;
+      base[0] = P;
+      base[plane] = Q;
+      base[2 * plane] = R;
+      base[3 * plane] = mux * Q;
+      base[4 * plane] = muy * R;
+    }
+    __syncthreads();
+  }
+  double acc_m = 0.0;
+#pragma unroll
+  for (int r = 0; r < SSIM_TH / RS; ++r) {
+    const int ly = ly0 + r * RS;
+    if (L.x0 + lx >= w || L.y0 + ly >= h) continue;
+    const float d0 = s_in[0][ly + SSIM_R][lx + SSIM_R] - s_in[3][ly + SSIM_R][lx + SSIM_R];
+    const float d1 = s_in[1][ly + SSIM_R][lx + SSIM_R] - s_in[4][ly + SSIM_R][lx + SSIM_R];
+    const float d2 = s_in[2][ly + SSIM_R][lx + SSIM_R] - s_in[5][ly + SSIM_R][lx + SSIM_R];
+    acc_m += (double)(d0 * d0 + d1 * d1 + d2 * d2);
+  }
+  double part[4] = {acc_s[0], acc_s[1], acc_s[2], acc_m};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double v = part[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((tid & 63) == 0) s_red[q][tid >> 6] = v;
+  }
+  __syncthreads();
+  if (tid < 4) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < SSIM_NT / 64; ++k) v += s_red[tid][k];
+    unsafeAtomicAdd(sums + 4 * L.view + tid, v);
+  }
+}
+
+__global__ void __launch_bounds__(SSIM_NT) k_ssim_grad(const float* __restrict__ T5, uint64_t plane,
+                                                       const float4* __restrict__ pred,
+                                                       const float4* __restrict__ target, SsimViews sv,
+                                                       SsimKernel K, float lam, float4* __restrict__ dl_dc) {
+  __shared__ float s_t[5][SSIM_WH][SSIM_WW];   // the channel's planes over the halo
+  __shared__ float s_v[5][SSIM_TH][SSIM_WW];   // after the vertical transpose
+  const TileLoc L = locate(sv, blockIdx.x);
+  const int w = (int)L.w, h = (int)L.h;
+  const int tid = threadIdx.x;
+  const int lx = tid & (SSIM_TW - 1), ly0 = tid / SSIM_TW;
+  constexpr int RS = SSIM_NT / SSIM_TW;
+  constexpr int PER = SSIM_TH / RS;
+  float4 xv[PER], yv[PER];
+  float out[PER][3];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int ox = L.x0 + lx, oy = L.y0 + ly0 + r * RS;
+    const bool in = ox < w && oy < h;
+    const uint64_t o = L.pix_base + (uint64_t)(in ? oy : 0) * w + (in ? ox : 0);
+    xv[r] = pred[o];
+    yv[r] = target[o];
+    out[r][0] = out[r][1] = out[r][2] = 0.0f;
+  }
+  const float inv_n = 1.0f / (float)((uint64_t)w * h);
+  const float inv_mse = sv.inv_mse[L.view];
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    for (int k = tid; k < SSIM_WH * SSIM_WW; k += SSIM_NT) {
+      const int ly = k / SSIM_WW, lxx = k - ly * SSIM_WW;
+      const int gy = L.y0 - SSIM_R + ly, gx = L.x0 - SSIM_R + lxx;
+      if (gy < 0 || gy >= h || gx < 0 || gx >= w) continue;  // outside the view: never read
+      const float* src = T5 + (uint64_t)(c * 5) * plane + L.pix_base + (uint64_t)gy * w + gx;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) s_t[q][ly][lxx] = src[(uint64_t)q * plane];
+    }
+    __syncthreads();
+    for (int k = tid; k < SSIM_TH * SSIM_WW; k += SSIM_NT) {
+      const int ly = k / SSIM_WW, lxx = k - ly * SSIM_WW;
+      const int gy = L.y0 + ly, gx = L.x0 - SSIM_R + lxx;
+      if (gy >= h || gx < 0 || gx >= w) continue;
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        s_v[q][ly][lxx] = adj_gather(gy, h, K, [&](int j) { return s_t[q][j - (L.y0 - SSIM_R)][lxx]; });
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int ly = ly0 + r * RS;
+      const int ox = L.x0 + lx, oy = L.y0 + ly;
+      if (ox >= w || oy >= h) continue;
+      float a[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        a[q] = adj_gather(ox, w, K, [&](int j) { return s_v[q][ly][j - (L.x0 - SSIM_R)]; });
+      const float xs = c == 0 ? xv[r].x : (c == 1 ? xv[r].y : xv[r].z);
+      const float ys = c == 0 ? yv[r].x : (c == 1 ? yv[r].y : yv[r].z);
+      const float d = a[0] + 2.0f * (xs * a[1] - a[3]) + (ys * a[2] - a[4]);
+      const float gs = (-inv_n * d) / 3.0f;    // ssim_loss_grad's channel average
+      const float gm = (xs - ys) * inv_mse;    // 2 (x - y) / (3WH)
+      out[r][c] = gm * (1.0f - lam) + gs * lam;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int ox = L.x0 + lx, oy = L.y0 + ly0 + r * RS;
+    if (ox >= w || oy >= h) continue;
+    dl_dc[L.pix_base + (uint64_t)oy * w + ox] = make_float4(out[r][0], out[r][1], out[r][2], 0.0f);
+  }
+}
+
+// Per-view loss (loss.rs:404-409): (1 - lam) MSE + lam (1 - SSIM), or the
+// SSIM term alone at lam = 1.
+__device__ __host__ __forceinline__ void ssim_totals(const double* s, uint64_t np, float lam, float* total,
+                                                     float* mse, float* ssim) {
+  const float m = (float)(s[3] / ((double)np * 3.0));
+  float l[3];
+  for (int c = 0; c < 3; ++c) l[c] = 1.0f - (float)(s[c] / (double)np);
+  const float sl = (l[0] + l[1] + l[2]) / 3.0f;
+  *mse = lam == 1.0f ? 0.0f : m;
+  *ssim = sl;
+  *total = lam == 1.0f ? sl : (1.0f - lam) * m + lam * sl;
+}
+
+__global__ void k_ssim_finalize(const double* __restrict__ sums, SsimViews sv, float lam,
+                                double* __restrict__ losses) {
+  const uint32_t v = threadIdx.x;
+  if (v >= sv.n) return;
+  float t, m, s;
+  ssim_totals(sums + 4 * v, (uint64_t)sv.w[v] * sv.h[v], lam, &t, &m, &s);
+  losses[v] = (double)t;
 }
 
 static SsimKernel ssim_kernel() {
@@ -226,13 +311,48 @@ static SsimKernel ssim_kernel() {
   return K;
 }
 
+int nts_ssim_views(uint32_t n_views, const uint32_t* widths, const uint32_t* heights, const uint64_t* pix_base,
+                   SsimViews* out) {
+  if (n_views == 0 || n_views > NTS_MAX_VIEWS) return nts_fail(NTS_E_INVALID, "SSIM over %u views", n_views);
+  memset(out, 0, sizeof *out);
+  out->n = n_views;
+  uint64_t tiles = 0;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    const uint32_t w = widths[v], h = heights[v];
+    if (w == 0 || h == 0) return nts_fail(NTS_E_INVALID, "SSIM of an empty image (view %u)", v);
+    out->w[v] = w;
+    out->h[v] = h;
+    out->tiles_x[v] = (w + SSIM_TW - 1) / SSIM_TW;
+    out->tile_base[v] = (uint32_t)tiles;
+    out->pix_base[v] = pix_base[v];
+    out->inv_mse[v] = 2.0f / ((float)w * (float)h * 3.0f);
+    tiles += (uint64_t)out->tiles_x[v] * ((h + SSIM_TH - 1) / SSIM_TH);
+  }
+  if (tiles >= (1ull << 31)) return nts_fail(NTS_E_CAPACITY, "SSIM over %llu tiles", (unsigned long long)tiles);
+  out->tile_base[n_views] = (uint32_t)tiles;
+  return NTS_OK;
+}
+
+int nts_ssim_launch(nts_ctx* ctx, const SsimViews& sv, const float4* pred, const float4* target, float* planes,
+                    uint64_t plane, double* sums, float4* dl_dc, float lam, double* losses) {
+  int rc = nts_fill_u32(ctx, sums, 0, 8 * (uint64_t)sv.n);
+  if (rc) return rc;
+  const SsimKernel K = ssim_kernel();
+  const dim3 grid(sv.tile_base[sv.n]);
+  NTS_LAUNCH(ctx, "ssim_moments", k_ssim_moments, grid, dim3(SSIM_NT), 0, pred, target, sv, K, planes, plane, sums);
+  NTS_LAUNCH(ctx, "ssim_grad", k_ssim_grad, grid, dim3(SSIM_NT), 0, (const float*)planes, plane, pred, target, sv, K,
+             lam, dl_dc);
+  if (losses)
+    NTS_LAUNCH(ctx, "ssim_finalize", k_ssim_finalize, dim3(1), dim3(64), 0, (const double*)sums, sv, lam, losses);
+  return NTS_OK;
+}
+
 extern "C" int nts_loss_grad_ssim(nts_ctx* ctx, const nts_buffer* image, const nts_buffer* target,
                                   nts_buffer* dl_dc, uint32_t width, uint32_t height, float lambda, double* total,
                                   double* mse, double* ssim) {
   NTS_CHECK_ARG(ctx && image && target && dl_dc);
   const uint64_t n = (uint64_t)width * height;
   NTS_CHECK_ARG(image->bytes >= n * 16 && target->bytes >= n * 16 && dl_dc->bytes >= n * 16);
-  if (n == 0) return nts_fail(NTS_E_INVALID, "SSIM of an empty image");
   const float lam = lambda < 0.0f ? 0.0f : (lambda > 1.0f ? 1.0f : lambda);
   if (lam == 0.0f) {
     // loss.rs:398-402: the plain MSE gradient, SSIM skipped
@@ -244,40 +364,24 @@ extern "C" int nts_loss_grad_ssim(nts_ctx* ctx, const nts_buffer* image, const n
     if (ssim) *ssim = 0.0;
     return NTS_OK;
   }
+  SsimViews sv;
+  const uint64_t zero = 0;
+  int rc = nts_ssim_views(1, &width, &height, &zero, &sv);
+  if (rc) return rc;
   void* scr;
-  const uint64_t plane_bytes = ((n * 4 + 255) & ~255ull);
-  int rc = nts_scratch(ctx, 256 + 30 * plane_bytes, &scr);
+  rc = nts_scratch(ctx, 256 + 15 * n * 4, &scr);
   if (rc) return rc;
-  double* sums = (double*)scr;  // [0..2] SSIM per channel, [3] MSE
-  float* H5 = (float*)((char*)scr + 256);
-  float* T5 = H5 + 15 * (plane_bytes / 4);
-  rc = nts_fill_u32(ctx, sums, 0, 8);
+  double* sums = (double*)scr;
+  rc = nts_ssim_launch(ctx, sv, (const float4*)image->ptr, (const float4*)target->ptr, (float*)((char*)scr + 256),
+                       n, sums, (float4*)dl_dc->ptr, lam, nullptr);
   if (rc) return rc;
-  const float4* P = (const float4*)image->ptr;
-  const float4* G = (const float4*)target->ptr;
-  const dim3 grid(nts_div_up(n, 256));
-  uint32_t mgrid = nts_div_up(n, 256);
-  if (mgrid > 2048) mgrid = 2048;
-  NTS_LAUNCH(ctx, "mse_sum", k_mse_sum, dim3(mgrid), dim3(256), 0, P, G, n, sums + 3);
-  const SsimKernel K = ssim_kernel();
-  const float inv_mse = 2.0f / ((float)width * (float)height * 3.0f);
-  {
-    NTS_LAUNCH(ctx, "ssim_h", k_ssim_h, grid, dim3(256), 0, P, G, width, height, K, H5);
-    NTS_LAUNCH(ctx, "ssim_v", k_ssim_v, grid, dim3(256), 0, (const float*)H5, width, height, K, T5, sums);
-    NTS_LAUNCH(ctx, "ssim_vadj", k_ssim_vadj, grid, dim3(256), 0, (const float*)T5, width, height, K, H5);
-    NTS_LAUNCH(ctx, "ssim_hadj", k_ssim_hadj, grid, dim3(256), 0, (const float*)H5, P, G, width, height, K, lam,
-               inv_mse, (float4*)dl_dc->ptr);
-  }
   double h_sums[4];
   NTS_HIP_TRY(hipMemcpyAsync(h_sums, sums, 32, hipMemcpyDeviceToHost, ctx->stream));
   NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  const float m = (float)(h_sums[3] / ((double)n * 3.0));
-  float l[3];
-  for (int c = 0; c < 3; ++c) l[c] = 1.0f - (float)(h_sums[c] / (double)n);
-  const float sl = (l[0] + l[1] + l[2]) / 3.0f;
-  // loss.rs:404-409
-  if (mse) *mse = lam == 1.0f ? 0.0 : (double)m;
-  if (ssim) *ssim = (double)sl;
-  if (total) *total = lam == 1.0f ? (double)sl : (double)((1.0f - lam) * m + lam * sl);
+  float t, m, s;
+  ssim_totals(h_sums, n, lam, &t, &m, &s);
+  if (total) *total = t;
+  if (mse) *mse = m;
+  if (ssim) *ssim = s;
   return NTS_OK;
 }

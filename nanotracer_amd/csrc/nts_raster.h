@@ -82,3 +82,28 @@ int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, cons
 int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, int mode, const Projected* proj,
                        const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, const float4* fwd_out,
                        const float* final_t, const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad);
+
+// ---- combined MSE + SSIM loss (k_ssim.hip) ---------------------------------
+// Normalised 11-tap Gaussian window (sigma 1.5), taps computed on the host.
+struct SsimKernel {
+  float k[11];
+};
+// Views of one SSIM launch (kernel argument): 64 x 16 pixel tiles per view,
+// tile_base prefix over the views, pixel offsets into the concatenated
+// images, and the MSE gradient scale 2 / (3WH) per view.
+struct SsimViews {
+  uint32_t n;
+  uint32_t tile_base[NTS_MAX_VIEWS + 1];
+  uint32_t tiles_x[NTS_MAX_VIEWS];
+  uint32_t w[NTS_MAX_VIEWS], h[NTS_MAX_VIEWS];
+  float inv_mse[NTS_MAX_VIEWS];
+  uint64_t pix_base[NTS_MAX_VIEWS];
+};
+int nts_ssim_views(uint32_t n_views, const uint32_t* widths, const uint32_t* heights, const uint64_t* pix_base,
+                   SsimViews* out);
+// dL/dC of (1 - lam) MSE + lam (1 - SSIM) for every view (0 < lam <= 1).
+// planes: 15 float planes of `plane` floats (plane >= the pixels of all
+// views); sums: 4 doubles per view (SSIM r, g, b, squared error), cleared
+// here.  losses (optional): the per-view total, written on the device.
+int nts_ssim_launch(nts_ctx* ctx, const SsimViews& sv, const float4* pred, const float4* target, float* planes,
+                    uint64_t plane, double* sums, float4* dl_dc, float lam, double* losses);

@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _lib as L
 from .api import (AdamConfig, AdamGpu, CameraUniform, GpuCtx, GpuSplatBuffer, Rasterizer, TileBinner,
-                  TilingParams, loss_grad)
+                  TilingParams, combined_loss_grad, loss_grad)
 from .synth import HostSplats
 
 
@@ -44,6 +44,7 @@ class Trainer:
     adam_pos: AdamConfig = field(default_factory=AdamConfig)
     adam_attr: AdamConfig = field(default_factory=AdamConfig)
     t: int = 0
+    ssim_lambda: float = 0.2   # LOSS_SSIM only: (1 - lambda) MSE + lambda (1 - SSIM)
 
     def __post_init__(self):
         self.r = Rasterizer()
@@ -89,7 +90,10 @@ class Trainer:
             tgt = self.ctx.storage_buffer(np.ascontiguousarray(v.target_image, np.float32))
             self._targets[key] = tgt
         dl = self._buf(("dl", v.width, v.height), v.width * v.height * 16)
-        loss = loss_grad(self.ctx, img, tgt, dl, v.width, v.height, self.loss_kind)
+        if self.loss_kind == L.LOSS_SSIM:
+            loss = combined_loss_grad(self.ctx, img, tgt, dl, v.width, v.height, self.ssim_lambda)[0]
+        else:
+            loss = loss_grad(self.ctx, img, tgt, dl, v.width, v.height, self.loss_kind)
         pg = self._buf("pgrad", max(n, 1) * 48)
         pg.zero()
         if self.backward_mode == L.BWD_RECOMPUTE:
@@ -117,6 +121,7 @@ class TrainConfig:
     max_splats: int = 1_000_000
     tile_size: int = 16
     loss_kind: int = L.LOSS_L1
+    ssim_lambda: float = 0.2           # LOSS_SSIM: TrainConfig::ssim_lambda (train.rs:48-52)
     backward_mode: int = L.BWD_REF_PARITY
     adam_pos: AdamConfig = field(default_factory=AdamConfig)
     adam_attr: AdamConfig = field(default_factory=AdamConfig)
@@ -185,7 +190,7 @@ def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None 
     else:
         g, st = GpuSplatBuffer.upload(ctx, splats, capacity=cap), TrainState()
     tr = Trainer(ctx, g, list(views), depth_max, cfg.tile_size, cfg.loss_kind, cfg.backward_mode, cfg.adam_pos,
-                 cfg.adam_attr, st.t)
+                 cfg.adam_attr, st.t, cfg.ssim_lambda)
     for it in range(st.iteration, cfg.iterations):
         res = tr.step(it)
         st.t = tr.t

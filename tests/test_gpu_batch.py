@@ -60,9 +60,10 @@ def test_render_views_bit_exact_per_view(ctx, tile):
     b.free()
 
 
-def _oracle_batch_step(s, views, targets, tile, mode, m, v, acc, t, cfg):
+def _oracle_batch_step(s, views, targets, tile, mode, m, v, acc, t, cfg, ssim_lambda=None):
     """The reference sequence view by view; gradients summed over views in
-    view order (float32), then accumulate_grad + Adam + constraints."""
+    view order (float32), then accumulate_grad + Adam + constraints.  The
+    loss is L1, or the combined MSE + SSIM loss with ssim_lambda."""
     from .test_gpu_optim import _oracle_fused_step
 
     total = None
@@ -72,7 +73,10 @@ def _oracle_batch_step(s, views, targets, tile, mode, m, v, acc, t, cfg):
         op = O.project(s, ocam)
         k, pay, rg = O.bin(op, tt)
         img, ft, li = O.composite(op, pay, rg, tt, with_state=True)
-        loss, dl = O.loss_grad(img, tgt, 0)
+        if ssim_lambda is None:
+            loss, dl = O.loss_grad(img, tgt, 0)
+        else:
+            loss, _, _, dl = O.combined_loss_grad(img, tgt, ssim_lambda)
         losses.append(loss)
         if mode == L.BWD_RECOMPUTE:
             pg = O.composite_backward_recompute(op, pay, rg, tt, ft, li, dl)
@@ -110,6 +114,50 @@ def test_train_step_views_matches_oracle(ctx, mode):
             d = np.abs(x.astype(np.float64) - y)
             assert d.max() <= 2.0 * cfg.lr + 1e-6
             assert (d > 1e-6).mean() < 2e-3
+    b.free()
+
+
+@pytest.mark.parametrize("lam,mode", [(0.2, L.BWD_RECOMPUTE), (1.0, L.BWD_REF_PARITY), (0.0, L.BWD_RECOMPUTE)])
+def test_train_step_views_ssim_matches_oracle(ctx, lam, mode):
+    s = synth.make_scene(20_000, 0x5EED0003, 3.0)
+    views = _views(s.n)
+    tile = 16
+    targets = _targets(synth.perturb_means(s, 79), views, tile)
+    tbuf = ctx.storage_buffer(np.concatenate([x.reshape(-1) for x in targets]))
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 2_000_000)
+    b.set_ssim_lambda(lam)
+    cfg = AdamConfig()
+    for step in (1, 2):
+        before = g.readback()
+        mg, vg = g.moments()
+        m = [a.reshape(-1).copy() for a in mg.arrays()]
+        v = [a.reshape(-1).copy() for a in vg.arrays()]
+        acc = g.grad_acc()
+        b.train_step(g, [x[0] for x in views], tbuf, step, tile, L.LOSS_SSIM, mode, write_grads=True,
+                     graph=(step == 2))
+        ref_s, ref_g, ref_losses = _oracle_batch_step(before, views, targets, tile, mode, m, v, acc, step, cfg,
+                                                      ssim_lambda=lam)
+        assert np.allclose(b.losses(), ref_losses, rtol=1e-6, atol=0), (b.losses(), ref_losses)
+        gg = g.grads()
+        for name, x, y in zip(("pos", "rot", "scale", "logit", "dc", "rest"), gg.arrays(), ref_g.arrays()):
+            assert rel_norm_err(x, y) < 1e-4, (step, name, rel_norm_err(x, y))
+        after = g.readback()
+        for x, y in zip(after.arrays(), ref_s.arrays()):
+            d = np.abs(x.astype(np.float64) - y)
+            assert d.max() <= 2.0 * cfg.lr + 1e-6
+            assert (d > 1e-6).mean() < 2e-3
+    b.free()
+
+
+def test_ssim_step_needs_the_setter(ctx):
+    s = synth.make_scene(1000, 0x5EED0004, 3.0)
+    views = _views(s.n, 1)
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 2, s.n, 320 * 240, 200_000)
+    tbuf = ctx.storage_buffer(np.zeros((240, 320, 4), np.float32))
+    with pytest.raises(L.NtsError, match="nts_batch_set_ssim_lambda"):
+        b.train_step(g, [views[0][0]], tbuf, 1, 16, L.LOSS_SSIM, L.BWD_RECOMPUTE)
     b.free()
 
 
