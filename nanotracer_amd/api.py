@@ -66,6 +66,7 @@ class GpuCtx:
         h = C.c_void_p()
         L.check(L.lib().nts_ctx_create(device, hbm_budget, C.byref(h)))
         self.h = h
+        self.device = device
 
     def close(self) -> None:
         if self.h:

@@ -174,7 +174,7 @@ def _norm(a) -> float:
 
 
 def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None = None, on_iter=None,
-          resume: str | None = None, log=print):
+          resume: str | None = None, log=print, on_snapshot=None, snapshot_every: int = 1):
     """The training loop of train.rs:155-401 on the device: view iter %
     len(views) each iteration, one fused Adam step, prune every
     `prune_every` iterations from `prune_warmup`, densify every
@@ -182,7 +182,10 @@ def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None 
     grad_count = 0), gradient-norm diagnostics every `diagnostics_every`
     and at the end, optional checkpoints.  `splats` is a HostSplats (fresh
     start) or ignored when `resume` names a checkpoint.  on_iter(iter, loss,
-    n) is called after every iteration.  Returns the final HostSplats."""
+    n) is called after every iteration; on_snapshot(iter, HostSplats, loss),
+    the reference's on_iter with the splats (train.rs:55-64), every
+    `snapshot_every` iterations and after the last.  Returns the final
+    HostSplats."""
     cfg = cfg or TrainConfig()
     cap = max(cfg.max_splats, splats.n if splats is not None else 0) + cfg.max_splats
     if resume:
@@ -213,6 +216,8 @@ def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None 
         st.iteration = it + 1
         if on_iter is not None:
             on_iter(it, res.loss, g.n)
+        if on_snapshot is not None and ((it + 1) % max(1, snapshot_every) == 0 or it + 1 == cfg.iterations):
+            on_snapshot(it, g.readback(), res.loss)
         if cfg.diagnostics_every and (it % cfg.diagnostics_every == 0 or it + 1 == cfg.iterations):
             gr = g.grads()
             log(f"[train] iter {it:>5} | view {it % len(tr.views)}/{len(tr.views)} | loss {res.loss:.4f} | "
