@@ -82,6 +82,8 @@ def alg_bytes(name: str, N: int, Nv: int, K: int, P: int, T: int, V: int, kb: in
         "project_backward_full": (236 + 48 + 236) * N,
         "adam_step": 7 * 236 * N + 8 * N,
         "backward_adam_views": 236 * N + 48 * N * V + 6 * 236 * N + 8 * N,
+        "ssim_moments": 32 * P + 60 * P,
+        "ssim_grad": 60 * P + 32 * P + 16 * P,
     }
     return float(table.get(name, 0))
 
@@ -318,8 +320,10 @@ def run_batched(args, ctx):
         b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, mode, graph=False)
         t += 1
     ctx.prof_enable(False)
-    kernels = kernel_table(ctx.prof_report(), s.n, visible, pairs, pixels,
-                           sum((-(-v.width // tile)) * (-(-v.height // tile)) for v in specs), N_VIEWS)
+    n_tiles = sum((-(-v.width // tile)) * (-(-v.height // tile)) for v in specs)
+    kernels = kernel_table(ctx.prof_report(), s.n, visible, pairs, pixels, n_tiles, N_VIEWS)
+    ssim = ssim_step(ctx, b, g, specs, tbuf, t, tile, mode, min(args.steps, 10), s.n, visible, pairs, pixels,
+                     n_tiles, args.ssim_lambda) if args.ssim_lambda > 0 else None
     b.free()
     out = dict(
         metric="train iters/s + render Mpx/s at 1M splats 1080p; p50/p99 render latency",
@@ -336,6 +340,8 @@ def run_batched(args, ctx):
     )
     out.update(render_latency(ctx, g, s.n, tile))
     out["roofline"] = roofline_of(kernels, (tile, lds[0], lds[1]), pairs)
+    if ssim:
+        out["ssim_step"] = ssim
     if tune:
         out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
@@ -345,6 +351,35 @@ def run_batched(args, ctx):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_batched(s, views, targets, tile)
     return out
+
+
+def ssim_step(ctx, b, g, specs, tbuf, t, tile, mode, steps, N, Nv, K, P, T, lam=0.2):
+    """The same batched iteration with the reference's default training loss,
+    (1 - lam) MSE + lam (1 - SSIM) (train.rs:48-52, main.rs:189-190): graph
+    replays timed after the headline run, then per-kernel HIP-event times."""
+    from nanotracer_amd import _lib as L
+
+    b.set_ssim_lambda(lam)
+    b.train_step(g, specs, tbuf, t, tile, L.LOSS_SSIM, mode, graph=True)
+    t += 1
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.train_step(g, specs, tbuf, t, tile, L.LOSS_SSIM, mode, graph=True)
+        t += 1
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for _ in range(2):
+        b.train_step(g, specs, tbuf, t, tile, L.LOSS_SSIM, mode, graph=False)
+        t += 1
+    ctx.prof_enable(False)
+    ks = kernel_table(ctx.prof_report(), N, Nv, K, P, T, len(specs))
+    return dict(ssim_lambda=lam, steps=steps, ms_per_step=round(dt * 1e3 / steps, 3),
+                iters_s=round(steps / dt, 3), final_loss=float(b.losses().sum()),
+                kernels={k: dict(avg_ms=round(v["avg_ms"], 4), gbs=None if v["gbs"] is None else round(v["gbs"], 1))
+                         for k, v in ks.items() if k.startswith("ssim") or k.startswith("blend")})
 
 
 def run_plain(args, ctx):
@@ -413,6 +448,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0,
                     help="render-server Poisson arrival rate (req/s); 0 skips the serving leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--ssim-lambda", type=float, default=0.2,
+                    help="also time the iteration with the combined MSE + SSIM loss; 0 skips it")
     args = ap.parse_args()
     global TILE, LDS_BATCH
     TILE = args.tile if args.tile == "auto" else int(args.tile)

@@ -324,6 +324,9 @@ int nts_server_create(int device, const nts_server_cfg* cfg, nts_server** out);
 int nts_server_destroy(nts_server* server);
 /* Uploads a resident scene (parameters only); NTS_E_NOMEM over the budget. */
 int nts_server_add_scene(nts_server* server, const nts_splats_host* splats, uint32_t* scene_id);
+/* The same from a binary little-endian 3DGS PLY file (nts_ply_read): the
+ * serving loader for scenes on disk. */
+int nts_server_add_scene_ply(nts_server* server, const char* path, uint32_t* scene_id);
 /* Queues a render; `out` (host, width * height * 4 or 16 bytes) must stay
  * valid until the request is polled done or waited for. */
 int nts_server_submit(nts_server* server, uint32_t scene_id, const nts_view_desc* view, int format, void* out,

@@ -179,6 +179,7 @@ _SIGS = {
     "nts_server_create": (C.c_int, [C.c_int, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "nts_server_destroy": (C.c_int, [C.c_void_p]),
     "nts_server_add_scene": (C.c_int, [C.c_void_p, C.POINTER(SplatsHost), C.POINTER(C.c_uint32)]),
+    "nts_server_add_scene_ply": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint32)]),
     "nts_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(ViewDesc), C.c_int, C.c_void_p,
                                     C.POINTER(C.c_uint64)]),
     "nts_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.POINTER(RequestInfo)]),

@@ -579,6 +579,12 @@ class RenderServer:
         L.check(L.lib().nts_server_add_scene(self.h, C.byref(_host_struct(splats)), C.byref(sid)))
         return int(sid.value)
 
+    def add_scene_ply(self, path) -> int:
+        """A resident scene read from a 3DGS PLY file."""
+        sid = C.c_uint32()
+        L.check(L.lib().nts_server_add_scene_ply(self.h, str(path).encode(), C.byref(sid)))
+        return int(sid.value)
+
     def submit(self, scene: int, view: ViewSpec, fmt: int = L.OUT_RGBA8):
         if fmt == L.OUT_RGBA8:
             out = np.zeros((view.height, view.width, 4), np.uint8)

@@ -26,6 +26,8 @@
 #define SSIM_WW (SSIM_TW + 2 * SSIM_R)
 #define SSIM_WH (SSIM_TH + 2 * SSIM_R)
 #define SSIM_NT 256
+// window cells per thread, rounded up to even
+#define WIN_ITEMS ((((SSIM_WH * SSIM_WW + SSIM_NT - 1) / SSIM_NT) + 1) & ~1)
 
 // reflect-101 over a period of 2(n - 1) (loss.rs:57-67); n >= 6 needs at
 // most one reflection for offsets up to 5
@@ -50,28 +52,35 @@ __device__ __forceinline__ int refl101(int i, int n) {
 // n >= 6 the taps of j that land on i are t = 5 - i - j (through the low
 // reflection, i != 0), t = i - j + 5 (direct) and t = 2(n - 1) - i - j + 5
 // (through the high reflection, i != n - 1), in that order, and only
-// j in [i - 5, i + 5] has any.  Below 6 samples every j in [0, n) lies
-// within 4 of i.
+// j in [i - 5, i + 5] has any; for 6 <= i <= n - 7 only the direct tap
+// exists, with static tap indices.  Below 6 samples every j in [0, n) lies
+// within 4 of i.  kl: the taps in LDS, for the data-dependent tap indices
+// of the border.
 template <typename F>
-__device__ __forceinline__ float adj_gather(int i, int n, const SsimKernel& K, F value) {
+__device__ __forceinline__ float adj_gather(int i, int n, const SsimKernel& K, const float* kl, F value) {
   float acc = 0.0f;
+  if (i >= SSIM_R + 1 && i <= n - 2 - SSIM_R) {
+#pragma unroll
+    for (int d = -SSIM_R; d <= SSIM_R; ++d) acc = acc + value(i + d) * K.k[SSIM_R - d];
+    return acc;
+  }
   if (n >= 6) {
     const int lo = i - SSIM_R < 0 ? 0 : i - SSIM_R;
     const int hi = i + SSIM_R > n - 1 ? n - 1 : i + SSIM_R;
     for (int j = lo; j <= hi; ++j) {
       const float v = value(j);
       const int t2 = SSIM_R - i - j;
-      if (i != 0 && t2 >= 0) acc = acc + v * K.k[t2];
-      acc = acc + v * K.k[i - j + SSIM_R];
+      if (i != 0 && t2 >= 0) acc = acc + v * kl[t2];
+      acc = acc + v * kl[i - j + SSIM_R];
       const int t3 = 2 * (n - 1) - i - j + SSIM_R;
-      if (i != n - 1 && t3 < SSIM_TAPS) acc = acc + v * K.k[t3];
+      if (i != n - 1 && t3 < SSIM_TAPS) acc = acc + v * kl[t3];
     }
     return acc;
   }
   for (int j = 0; j < n; ++j) {
     const float v = value(j);
     for (int t = 0; t < SSIM_TAPS; ++t)
-      if (refl101(j + t - SSIM_R, n) == i) acc = acc + v * K.k[t];
+      if (refl101(j + t - SSIM_R, n) == i) acc = acc + v * kl[t];
   }
   return acc;
 }
@@ -101,64 +110,83 @@ __global__ void __launch_bounds__(SSIM_NT) k_ssim_moments(const float4* __restri
                                                           SsimKernel K, float* __restrict__ T5, uint64_t plane,
                                                           double* __restrict__ sums) {
   __shared__ float s_in[6][SSIM_WH][SSIM_WW];   // x r g b, y r g b
-  __shared__ float s_h[5][SSIM_WH][SSIM_TW];    // horizontal blurs of one channel
+  __shared__ float s_h[SSIM_WH][SSIM_TW];       // horizontal blur of one moment plane
   __shared__ double s_red[4][SSIM_NT / 64];
   const TileLoc L = locate(sv, blockIdx.x);
   const int w = (int)L.w, h = (int)L.h;
   const int tid = threadIdx.x;
-  for (int k = tid; k < SSIM_WH * SSIM_WW; k += SSIM_NT) {
-    const int ly = k / SSIM_WW, lx = k - ly * SSIM_WW;
-    const int ry = L.y0 - SSIM_R + ly, rx = L.x0 - SSIM_R + lx;
-    // cells beyond 5 past the last row or column feed no output pixel
-    const int gy = ry < h + SSIM_R ? refl101(ry, h) : 0;
-    const int gx = rx < w + SSIM_R ? refl101(rx, w) : 0;
-    const uint64_t idx = L.pix_base + (uint64_t)gy * w + gx;
-    const float4 p = pred[idx], g = target[idx];
-    s_in[0][ly][lx] = p.x;
-    s_in[1][ly][lx] = p.y;
-    s_in[2][ly][lx] = p.z;
-    s_in[3][ly][lx] = g.x;
-    s_in[4][ly][lx] = g.y;
-    s_in[5][ly][lx] = g.z;
+  // the window in two halves of statically unrolled loads, so each thread
+  // has up to 8 loads in flight
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    float4 pv[WIN_ITEMS / 2], gv[WIN_ITEMS / 2];
+#pragma unroll
+    for (int u = 0; u < WIN_ITEMS / 2; ++u) {
+      const int k = tid + (half * (WIN_ITEMS / 2) + u) * SSIM_NT;
+      const int ly = k / SSIM_WW, lx = k - ly * SSIM_WW;
+      const int ry = L.y0 - SSIM_R + ly, rx = L.x0 - SSIM_R + lx;
+      // cells beyond 5 past the last row or column feed no output pixel
+      const int gy = ry < h + SSIM_R ? refl101(ry, h) : 0;
+      const int gx = rx < w + SSIM_R ? refl101(rx, w) : 0;
+      const uint64_t idx = L.pix_base + (uint64_t)(k < SSIM_WH * SSIM_WW ? gy : 0) * w + gx;
+      pv[u] = pred[idx];
+      gv[u] = target[idx];
+    }
+#pragma unroll
+    for (int u = 0; u < WIN_ITEMS / 2; ++u) {
+      const int k = tid + (half * (WIN_ITEMS / 2) + u) * SSIM_NT;
+      if (k >= SSIM_WH * SSIM_WW) continue;
+      const int ly = k / SSIM_WW, lx = k - ly * SSIM_WW;
+      s_in[0][ly][lx] = pv[u].x;
+      s_in[1][ly][lx] = pv[u].y;
+      s_in[2][ly][lx] = pv[u].z;
+      s_in[3][ly][lx] = gv[u].x;
+      s_in[4][ly][lx] = gv[u].y;
+      s_in[5][ly][lx] = gv[u].z;
+    }
   }
   __syncthreads();
   const int lx = tid & (SSIM_TW - 1), ly0 = tid / SSIM_TW;
   constexpr int RS = SSIM_NT / SSIM_TW;  // rows per sweep of the block
+  constexpr int PER = SSIM_TH / RS;
   double acc_s[3] = {0.0, 0.0, 0.0};
 #pragma unroll 1
   for (int c = 0; c < 3; ++c) {
-    for (int k = tid; k < SSIM_WH * SSIM_TW; k += SSIM_NT) {
-      const int ry = k / SSIM_TW, rx = k - ry * SSIM_TW;
-      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f;
+    float m[PER][5];
+    // one moment plane at a time: x, y, x^2, y^2, xy blurred horizontally
+    // into LDS over the halo rows, then vertically into registers
 #pragma unroll
-      for (int t = 0; t < SSIM_TAPS; ++t) {
-        const float a = s_in[c][ry][rx + t], b = s_in[3 + c][ry][rx + t];
-        s0 = s0 + a * K.k[t];
-        s1 = s1 + b * K.k[t];
-        s2 = s2 + (a * a) * K.k[t];
-        s3 = s3 + (b * b) * K.k[t];
-        s4 = s4 + (a * b) * K.k[t];
+    for (int q = 0; q < 5; ++q) {
+      for (int k = tid; k < SSIM_WH * SSIM_TW; k += SSIM_NT) {
+        const int ry = k / SSIM_TW, rx = k - ry * SSIM_TW;
+        float acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < SSIM_TAPS; ++t) {
+          const float a = s_in[c][ry][rx + t], b = s_in[3 + c][ry][rx + t];
+          const float f = q == 0 ? a : (q == 1 ? b : (q == 2 ? a * a : (q == 3 ? b * b : a * b)));
+          acc = acc + f * K.k[t];
+        }
+        s_h[ry][rx] = acc;
       }
-      s_h[0][ry][rx] = s0;
-      s_h[1][ry][rx] = s1;
-      s_h[2][ry][rx] = s2;
-      s_h[3][ry][rx] = s3;
-      s_h[4][ry][rx] = s4;
-    }
-    __syncthreads();
+      __syncthreads();
 #pragma unroll
-    for (int r = 0; r < SSIM_TH / RS; ++r) {
+      for (int r = 0; r < PER; ++r) {
+        const int ly = ly0 + r * RS;
+        float acc = 0.0f;
+#pragma unroll
+        for (int t = 0; t < SSIM_TAPS; ++t) acc = acc + s_h[ly + t][lx] * K.k[t];
+        m[r][q] = acc;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
       const int ly = ly0 + r * RS;
       const int ox = L.x0 + lx, oy = L.y0 + ly;
       if (ox >= w || oy >= h) continue;
-      float m[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int t = 0; t < SSIM_TAPS; ++t)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) m[q] = m[q] + s_h[q][ly + t][lx] * K.k[t];
-      const float mux = m[0], muy = m[1];
+      const float mux = m[r][0], muy = m[r][1];
       const float mux2 = mux * mux, muy2 = muy * muy;
-      const float sxx = m[2] - mux2, syy = m[3] - muy2, sxy = m[4] - mux * muy;
+      const float sxx = m[r][2] - mux2, syy = m[r][3] - muy2, sxy = m[r][4] - mux * muy;
       const float av = 2.0f * mux * muy + 1e-4f;
       const float bv = 2.0f * sxy + 9e-4f;
       const float cv = mux2 + muy2 + 1e-4f;
@@ -175,11 +203,10 @@ __global__ void __launch_bounds__(SSIM_NT) k_ssim_moments(const float4* __restri
       base[3 * plane] = mux * Q;
       base[4 * plane] = muy * R;
     }
-    __syncthreads();
   }
   double acc_m = 0.0;
 #pragma unroll
-  for (int r = 0; r < SSIM_TH / RS; ++r) {
+  for (int r = 0; r < PER; ++r) {
     const int ly = ly0 + r * RS;
     if (L.x0 + lx >= w || L.y0 + ly >= h) continue;
     const float d0 = s_in[0][ly + SSIM_R][lx + SSIM_R] - s_in[3][ly + SSIM_R][lx + SSIM_R];
@@ -208,64 +235,84 @@ __global__ void __launch_bounds__(SSIM_NT) k_ssim_grad(const float* __restrict__
                                                        const float4* __restrict__ pred,
                                                        const float4* __restrict__ target, SsimViews sv,
                                                        SsimKernel K, float lam, float4* __restrict__ dl_dc) {
-  __shared__ float s_t[5][SSIM_WH][SSIM_WW];   // the channel's planes over the halo
-  __shared__ float s_v[5][SSIM_TH][SSIM_WW];   // after the vertical transpose
+  __shared__ float s_t[SSIM_WH][SSIM_WW];   // one plane over the halo
+  __shared__ float s_v[SSIM_TH][SSIM_WW];   // after the vertical transpose
+  __shared__ float s_k[SSIM_TAPS];
   const TileLoc L = locate(sv, blockIdx.x);
   const int w = (int)L.w, h = (int)L.h;
   const int tid = threadIdx.x;
+  if (tid < SSIM_TAPS) s_k[tid] = K.k[tid];  // published by the first barrier below
   const int lx = tid & (SSIM_TW - 1), ly0 = tid / SSIM_TW;
   constexpr int RS = SSIM_NT / SSIM_TW;
   constexpr int PER = SSIM_TH / RS;
-  float4 xv[PER], yv[PER];
   float out[PER][3];
-#pragma unroll
-  for (int r = 0; r < PER; ++r) {
-    const int ox = L.x0 + lx, oy = L.y0 + ly0 + r * RS;
-    const bool in = ox < w && oy < h;
-    const uint64_t o = L.pix_base + (uint64_t)(in ? oy : 0) * w + (in ? ox : 0);
-    xv[r] = pred[o];
-    yv[r] = target[o];
-    out[r][0] = out[r][1] = out[r][2] = 0.0f;
-  }
   const float inv_n = 1.0f / (float)((uint64_t)w * h);
   const float inv_mse = sv.inv_mse[L.view];
+  // Window cell u of this thread: k = tid + u * 256.  Cells outside the
+  // view are never read by the transposes; they load a valid dummy.
+  uint32_t cell_off[WIN_ITEMS];
+#pragma unroll
+  for (int u = 0; u < WIN_ITEMS; ++u) {
+    const int k = tid + u * SSIM_NT;
+    const int ly = k / SSIM_WW, lxx = k - ly * SSIM_WW;
+    const int gy = L.y0 - SSIM_R + ly, gx = L.x0 - SSIM_R + lxx;
+    const bool in = k < SSIM_WH * SSIM_WW && gy >= 0 && gy < h && gx >= 0 && gx < w;
+    cell_off[u] = in ? (uint32_t)(gy * w + gx) : 0u;
+  }
+  float pre[WIN_ITEMS];
+  auto load_plane = [&](int p) {
+    const float* src = T5 + (uint64_t)p * plane + L.pix_base;
+#pragma unroll
+    for (int u = 0; u < WIN_ITEMS; ++u) pre[u] = src[cell_off[u]];
+  };
+  load_plane(0);
 #pragma unroll 1
   for (int c = 0; c < 3; ++c) {
-    for (int k = tid; k < SSIM_WH * SSIM_WW; k += SSIM_NT) {
-      const int ly = k / SSIM_WW, lxx = k - ly * SSIM_WW;
-      const int gy = L.y0 - SSIM_R + ly, gx = L.x0 - SSIM_R + lxx;
-      if (gy < 0 || gy >= h || gx < 0 || gx >= w) continue;  // outside the view: never read
-      const float* src = T5 + (uint64_t)(c * 5) * plane + L.pix_base + (uint64_t)gy * w + gx;
+    float a[PER][5];
+    // one plane at a time: the vertical transpose over the halo columns into
+    // LDS, then the horizontal transpose at the block's own pixels; the next
+    // plane's loads are in flight meanwhile
 #pragma unroll
-      for (int q = 0; q < 5; ++q) s_t[q][ly][lxx] = src[(uint64_t)q * plane];
-    }
-    __syncthreads();
-    for (int k = tid; k < SSIM_TH * SSIM_WW; k += SSIM_NT) {
-      const int ly = k / SSIM_WW, lxx = k - ly * SSIM_WW;
-      const int gy = L.y0 + ly, gx = L.x0 - SSIM_R + lxx;
-      if (gy >= h || gx < 0 || gx >= w) continue;
+    for (int q = 0; q < 5; ++q) {
 #pragma unroll
-      for (int q = 0; q < 5; ++q)
-        s_v[q][ly][lxx] = adj_gather(gy, h, K, [&](int j) { return s_t[q][j - (L.y0 - SSIM_R)][lxx]; });
+      for (int u = 0; u < WIN_ITEMS; ++u) {
+        const int k = tid + u * SSIM_NT;
+        if (k < SSIM_WH * SSIM_WW) (&s_t[0][0])[k] = pre[u];
+      }
+      __syncthreads();
+      if (c * 5 + q + 1 < 15) load_plane(c * 5 + q + 1);
+      for (int k = tid; k < SSIM_TH * SSIM_WW; k += SSIM_NT) {
+        const int ly = k / SSIM_WW, lxx = k - ly * SSIM_WW;
+        const int gy = L.y0 + ly, gx = L.x0 - SSIM_R + lxx;
+        if (gy >= h || gx < 0 || gx >= w) continue;
+        s_v[ly][lxx] = adj_gather(gy, h, K, s_k, [&](int j) { return s_t[j - (L.y0 - SSIM_R)][lxx]; });
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const int ly = ly0 + r * RS;
+        const int ox = L.x0 + lx;
+        a[r][q] = ox < w && L.y0 + ly < h
+                      ? adj_gather(ox, w, K, s_k, [&](int j) { return s_v[ly][j - (L.x0 - SSIM_R)]; })
+                      : 0.0f;
+      }
+      // the next plane's stores to s_t follow this plane's last read of it
+      // (before the barrier above); its writes to s_v follow the next
+      // barrier, which every thread reaches after these reads of s_v
     }
-    __syncthreads();
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-      const int ly = ly0 + r * RS;
-      const int ox = L.x0 + lx, oy = L.y0 + ly;
+      const int ox = L.x0 + lx, oy = L.y0 + ly0 + r * RS;
       if (ox >= w || oy >= h) continue;
-      float a[5];
-#pragma unroll
-      for (int q = 0; q < 5; ++q)
-        a[q] = adj_gather(ox, w, K, [&](int j) { return s_v[q][ly][j - (L.x0 - SSIM_R)]; });
-      const float xs = c == 0 ? xv[r].x : (c == 1 ? xv[r].y : xv[r].z);
-      const float ys = c == 0 ? yv[r].x : (c == 1 ? yv[r].y : yv[r].z);
-      const float d = a[0] + 2.0f * (xs * a[1] - a[3]) + (ys * a[2] - a[4]);
+      const uint64_t o = L.pix_base + (uint64_t)oy * w + ox;
+      const float4 xv = pred[o], yv = target[o];
+      const float xs = c == 0 ? xv.x : (c == 1 ? xv.y : xv.z);
+      const float ys = c == 0 ? yv.x : (c == 1 ? yv.y : yv.z);
+      const float d = a[r][0] + 2.0f * (xs * a[r][1] - a[r][3]) + (ys * a[r][2] - a[r][4]);
       const float gs = (-inv_n * d) / 3.0f;    // ssim_loss_grad's channel average
       const float gm = (xs - ys) * inv_mse;    // 2 (x - y) / (3WH)
       out[r][c] = gm * (1.0f - lam) + gs * lam;
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int r = 0; r < PER; ++r) {

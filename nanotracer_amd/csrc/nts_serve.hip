@@ -26,6 +26,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <vector>
 
 #include "nts_bin.h"
 #include "nts_project.h"
@@ -550,6 +551,28 @@ extern "C" int nts_server_add_scene(nts_server* sv, const nts_splats_host* host,
   sv->scenes.push_back(r);
   *scene_id = (uint32_t)(sv->scenes.size() - 1);
   return NTS_OK;
+}
+
+// A resident scene straight from a 3DGS PLY file (nano-splat ply.rs:116-134).
+extern "C" int nts_server_add_scene_ply(nts_server* sv, const char* path, uint32_t* scene_id) {
+  NTS_CHECK_ARG(sv && path && scene_id);
+  uint32_t n = 0;
+  int rc = nts_ply_count(path, &n);
+  if (rc) return rc;
+  if (n == 0) return nts_fail(NTS_E_INVALID, "%s holds no splats", path);
+  std::vector<float> pos(3 * (size_t)n), rot(4 * (size_t)n), ls(3 * (size_t)n), lg(n), dc(3 * (size_t)n),
+      rest(45 * (size_t)n);
+  nts_splats_host h;
+  h.n = n;
+  h.pos = pos.data();
+  h.rot = rot.data();
+  h.log_scale = ls.data();
+  h.logit = lg.data();
+  h.sh_dc = dc.data();
+  h.sh_rest = rest.data();
+  rc = nts_ply_read(path, &h);
+  if (rc) return rc;
+  return nts_server_add_scene(sv, &h, scene_id);
 }
 
 extern "C" int nts_server_submit(nts_server* sv, uint32_t scene_id, const nts_view_desc* view, int format, void* out,

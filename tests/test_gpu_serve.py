@@ -125,3 +125,25 @@ def test_admission_limits(ctx):
             tight.add_scene(synth.make_scene(30_000, 9, 2.0))
     assert tight.stats()["hbm_in_use"] <= 200 << 20
     tight.close()
+
+
+def test_scenes_load_from_ply(ctx, tmp_path):
+    from nanotracer_amd.api import write_ply
+
+    scenes = _scenes()[:2]
+    paths = []
+    for k, s in enumerate(scenes):
+        p = tmp_path / f"scene{k}.ply"
+        write_ply(p, s)
+        paths.append(p)
+    srv = RenderServer(slots=1, max_views=4, max_pixels=1 << 20, max_records=200_000, pair_capacity=4_000_000)
+    a = [srv.add_scene(s) for s in scenes]
+    b = [srv.add_scene_ply(p) for p in paths]
+    assert a == [0, 1] and b == [2, 3]
+    for sid, v in _requests(scenes, 6, 3):
+        x, _ = srv.wait(srv.submit(sid, v, L.OUT_F32)[0])
+        y, _ = srv.wait(srv.submit(sid + 2, v, L.OUT_F32)[0])
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    with pytest.raises(L.NtsError):
+        srv.add_scene_ply(tmp_path / "missing.ply")
+    srv.close()
