@@ -208,9 +208,12 @@ __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(
   const uint32_t begin = rg.x, end = rg.y;
   const WaveRect wr = wave_rect<TS, NTS_FWD_THREADS>(tx, ty);
 
+  // A pixel that is finished (saturated, or outside the image) has its
+  // centre set to NaN: every later power test then fails, which costs the
+  // evaluation loop no separate check.
   float pxf[PPT], pyf[PPT], T[PPT], C0[PPT], C1[PPT], C2[PPT];
   uint32_t last[PPT], pix_x[PPT], pix_y[PPT];
-  bool done[PPT], inb[PPT];
+  bool inb[PPT];
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     uint32_t lx, ly;
@@ -218,17 +221,16 @@ __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(
     pix_x[j] = tx * TS + lx;
     pix_y[j] = ty * TS + ly;
     inb[j] = pix_x[j] < vd.width && pix_y[j] < vd.height;
-    pxf[j] = (float)pix_x[j] + 0.5f;
+    pxf[j] = inb[j] ? (float)pix_x[j] + 0.5f : __builtin_nanf("");
     pyf[j] = (float)pix_y[j] + 0.5f;
     T[j] = 1.0f;
     C0[j] = C1[j] = C2[j] = 0.0f;
     last[j] = 0;
-    done[j] = !inb[j];
   }
   for (uint32_t base = begin; base < end; base += BATCH) {
     int live = 0;
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) live |= !done[j];
+    for (int j = 0; j < PPT; ++j) live |= pxf[j] == pxf[j];
     if (!__syncthreads_or(live)) break;
     const uint32_t nb = (end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH;
     for (uint32_t k = threadIdx.x; k < nb; k += NT)
@@ -242,20 +244,20 @@ __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(
       while (m) {
         int lv = 0;
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) lv |= !done[j];
-        if (!__ballot(lv)) break;  // every pixel of the wave is saturated
+        for (int j = 0; j < PPT; ++j) lv |= pxf[j] == pxf[j];
+        if (!__ballot(lv)) break;  // every pixel of the wave is finished
         const uint32_t k = (uint32_t)i * 64u + (uint32_t)__builtin_ctzll(m);
         m &= m - 1;
         const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-          if (done[j]) continue;
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
           const float power = -0.5f * (A.z * dx * dx + A.w * dx * dy + B.x * dy * dy);
-          if (power > 0.0f || power < Cc.w) continue;
-          const float g = nts_expf_nonpos(power);
+          // power > 0, a finished pixel (NaN), or below the alpha >= 1/255 bound
+          if (!(power <= 0.0f) || power < Cc.w) continue;
+          const float g = nts_exp_blend(power);
           const float ar = B.y * g;
-          const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
+          const float alpha = __builtin_amdgcn_fmed3f(ar, 0.0f, NTS_ALPHA_MAX);  // min(ar, 0.99), ar >= 0
           if (alpha < NTS_ALPHA_MIN) continue;
           const float ta = T[j] * alpha;
           C0[j] = C0[j] + ta * B.z;
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(
           C2[j] = C2[j] + ta * Cc.x;
           T[j] = T[j] * (1.0f - alpha);
           last[j] = base + k - begin + 1;
-          if (T[j] < NTS_T_MIN) done[j] = true;
+          if (T[j] < NTS_T_MIN) pxf[j] = __builtin_nanf("");
         }
       }
     }
@@ -408,9 +410,13 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
   const bool term_lane = my_term < 9u;
   const WaveRect wr = wave_rect<TS, NTS_BWD_THREADS>(tx, ty);
 
-  float pxf[PPT], pyf[PPT], T[PPT], G0[PPT], G1[PPT], G2[PPT], S0[PPT], S1[PPT], S2[PPT];
+  // Per pixel: T walked back, dL/dC, U = dL/dC . S with S the sum of
+  // T alpha c over the splats behind the current one (the gradient needs S
+  // only through dL/dC . S / (1 - alpha)), and the slice-relative end of its
+  // splats (0 once the pixel needs no more: no contributor, or the
+  // reference-parity bail).
+  float pxf[PPT], pyf[PPT], T[PPT], G0[PPT], G1[PPT], G2[PPT], U[PPT];
   uint32_t lastj[PPT];
-  bool done[PPT];
   uint32_t my_max = 0;
   if (threadIdx.x == 0) s_max = 0;
 #pragma unroll
@@ -438,8 +444,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
       G1[j] = g.y;
       G2[j] = g.z;
     }
-    S0[j] = S1[j] = S2[j] = 0.0f;
-    done[j] = lastj[j] == 0;
+    U[j] = 0.0f;
     my_max = lastj[j] > my_max ? lastj[j] : my_max;
   }
   const uint32_t wave_last = wave_max_u32(my_max);  // slice entries this wave can need
@@ -450,7 +455,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
   while (rem > 0) {
     int live = 0;
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) live |= !done[j];
+    for (int j = 0; j < PPT; ++j) live |= lastj[j] != 0u;
     if (!__syncthreads_or(live)) break;
     const uint32_t take = rem < (uint32_t)BATCH ? rem : (uint32_t)BATCH;
     const uint32_t rel0 = rem - take;  // slice-relative index of LDS slot 0
@@ -473,18 +478,23 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
         m &= ~(1ull << bt);
         const uint32_t k = (uint32_t)i * 64u + (uint32_t)bt;
         const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+        // Per-pixel moments of the splat's dL/dpower, mapped through the
+        // conic once per splat at the commit:
+        //   0, 1   sum gpow dx, sum gpow dy        -> dL/dmean = Q (X, Y)
+        //   2      sum dL/dalpha G                  -> dL/dopacity
+        //   3-5    sum gpow dx^2, dx dy, dy^2       -> dL/dconic (-1/2, -1, -1/2)
+        //   6-8    sum dL/dC_i T alpha              -> dL/dcolour
         float acc[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
         bool contrib = false;
+        const float sigma = B.y, c0 = B.z, c1 = B.w, c2 = Cc.x;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-          if (done[j] || rel0 + k >= lastj[j]) continue;
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
           // the forward's power and thresholds, bit for bit
           const float power = -0.5f * (A.z * dx * dx + A.w * dx * dy + B.x * dy * dy);
-          if (power > 0.0f || power < Cc.w) continue;
-          const float sigma = B.y;
+          // alpha for every lane, then one branch on all the forward's tests
           // recompute mode: hardware 2^x (error well under 1e-6 relative),
           // and where alpha lies that close to 1/255 or 0.99 the forward's
           // exact exponential, so every alpha test decides as in the
@@ -495,55 +505,49 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
             g = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
             ar = sigma * g;
             if (fabsf(ar - NTS_ALPHA_MIN) <= 4e-6f * NTS_ALPHA_MIN || fabsf(ar - NTS_ALPHA_MAX) <= 4e-6f) {
-              g = nts_expf_nonpos(power);
+              g = nts_exp_blend(power);
               ar = sigma * g;
             }
           } else {
-            g = nts_expf_nonpos(power);
+            g = nts_exp_blend(power);
             ar = sigma * g;
           }
-          const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
-          if (alpha < NTS_ALPHA_MIN) continue;
-          float s0, s1, s2;
+          const float alpha = __builtin_amdgcn_fmed3f(ar, 0.0f, NTS_ALPHA_MAX);  // min(ar, 0.99), ar >= 0
+          if (!(rel0 + k < lastj[j] && power <= 0.0f && power >= Cc.w && alpha >= NTS_ALPHA_MIN)) continue;
+          float Tn, Un;
           if (RECOMP) {
             const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
-            T[j] = T[j] * r;
-            s0 = S0[j] * r;
-            s1 = S1[j] * r;
-            s2 = S2[j] * r;
+            Tn = T[j] * r;
+            Un = U[j] * r;
           } else {
             const float oma = nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
-            T[j] = T[j] / oma;
-            if (T[j] > NTS_T_BAIL) {
-              done[j] = true;
+            Tn = T[j] / oma;
+            T[j] = Tn;
+            if (Tn > NTS_T_BAIL) {
+              lastj[j] = 0u;
               continue;
             }
-            s0 = S0[j] / oma;
-            s1 = S1[j] / oma;
-            s2 = S2[j] / oma;
+            Un = U[j] / oma;
           }
-          // gradient terms with fused multiply-adds (no bit parity needed)
-          const float ta = T[j] * alpha;
-          const float c0 = B.z, c1 = B.w, c2 = Cc.x;
-          const float gal = __builtin_fmaf(G0[j], __builtin_fmaf(T[j], c0, -s0),
-                                           __builtin_fmaf(G1[j], __builtin_fmaf(T[j], c1, -s1),
-                                                          G2[j] * __builtin_fmaf(T[j], c2, -s2)));
+          T[j] = Tn;
+          // dL/dalpha = T (dL/dC . c) - dL/dC . S_behind (fused multiply-adds;
+          // no bit parity is needed in the gradient)
+          const float gc = __builtin_fmaf(G2[j], c2, __builtin_fmaf(G1[j], c1, G0[j] * c0));
+          const float gal = __builtin_fmaf(Tn, gc, -Un);
           const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;  // alpha clamp gate
-          const float gpow = gar * ar;                           // dL/dpower = dL/dalpha * sigma * G
-          const float hb = 0.5f * A.w;                           // conic b
-          const float hp = -0.5f * gpow;
-          acc[0] = __builtin_fmaf(gpow, __builtin_fmaf(A.z, dx, hb * dy), acc[0]);
-          acc[1] = __builtin_fmaf(gpow, __builtin_fmaf(hb, dx, B.x * dy), acc[1]);
+          const float gpow = gar * ar;                           // dL/dpower = dL/dalpha sigma G
+          const float ta = Tn * alpha;
+          const float gx = gpow * dx, gy = gpow * dy;
+          acc[0] += gx;
+          acc[1] += gy;
           acc[2] = __builtin_fmaf(gar, g, acc[2]);
-          acc[3] = __builtin_fmaf(hp, dx * dx, acc[3]);
-          acc[4] = __builtin_fmaf(-gpow, dx * dy, acc[4]);
-          acc[5] = __builtin_fmaf(hp, dy * dy, acc[5]);
+          acc[3] = __builtin_fmaf(gx, dx, acc[3]);
+          acc[4] = __builtin_fmaf(gx, dy, acc[4]);
+          acc[5] = __builtin_fmaf(gy, dy, acc[5]);
           acc[6] = __builtin_fmaf(G0[j], ta, acc[6]);
           acc[7] = __builtin_fmaf(G1[j], ta, acc[7]);
           acc[8] = __builtin_fmaf(G2[j], ta, acc[8]);
-          S0[j] = __builtin_fmaf(ta, c0, S0[j]);
-          S1[j] = __builtin_fmaf(ta, c1, S1[j]);
-          S2[j] = __builtin_fmaf(ta, c2, S2[j]);
+          U[j] = __builtin_fmaf(ta, gc, U[j]);  // S += T alpha c (S itself is not rescaled)
           contrib = true;
         }
         if (__ballot(contrib)) {
@@ -564,10 +568,23 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
     for (uint32_t it = threadIdx.x; it < take * 9u; it += NT) {
       const uint32_t k = it / 9u, q = it - k * 9u;
       const uint64_t bit = 1ull << (k & 63u);
-      float sum = 0.0f;
+      const uint32_t qs = q < 2u ? 0u : q;  // dL/dmean needs both first moments
+      float sum = 0.0f, sum2 = 0.0f;
 #pragma unroll
       for (int w = 0; w < NW; ++w)
-        if (s_cm[w][k >> 6] & bit) sum += s_part[w][k][q];
+        if (s_cm[w][k >> 6] & bit) {
+          sum += s_part[w][k][qs];
+          if (q < 2u) sum2 += s_part[w][k][1];
+        }
+      if (q < 2u) {
+        const float4 A = s_a[k];
+        const float cb = 0.5f * A.w;
+        sum = q == 0u ? __builtin_fmaf(A.z, sum, cb * sum2) : __builtin_fmaf(cb, sum, s_b[k].x * sum2);
+      } else if (q == 3u || q == 5u) {
+        sum = -0.5f * sum;
+      } else if (q == 4u) {
+        sum = -sum;
+      }
       if (sum != 0.0f) {
         float* o = reinterpret_cast<float*>(pgrad + vd.proj_base + s_idx[k]);
         unsafeAtomicAdd(o + (q < 3u ? q : (q < 6u ? q + 1u : q + 2u)), sum);

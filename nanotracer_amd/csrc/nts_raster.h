@@ -67,6 +67,28 @@ __device__ __forceinline__ float nts_expf_nonpos(float x) {
   return p * nts_pow2i(k);
 }
 
+// e^x for the blend's power test, x <= 0: equal to nts_expf for
+// x >= -80 (the same reduction and polynomial; the result there is a normal
+// number, so the scaling is one ldexp).  Below -80 it returns e^-80, which
+// like e^x gives alpha = opacity e^x < 1/255 for any opacity <= 1: every
+// blend decision, and so every image, is the one nts_expf gives.  No range,
+// NaN or subnormal branches.
+__device__ __forceinline__ float nts_exp_blend(float x) {
+  x = __builtin_amdgcn_fmed3f(x, -80.0f, 0.0f);  // one clamp instruction; x <= 0 on entry
+  const float n = __builtin_rintf(x * 1.44269504088896341f);
+  float r = nts_fmaf(n, -0.693359375f, x);
+  r = nts_fmaf(n, 2.12194440e-4f, r);
+  float p = 1.98412698e-4f;
+  p = nts_fmaf(p, r, 1.38888889e-3f);
+  p = nts_fmaf(p, r, 8.33333333e-3f);
+  p = nts_fmaf(p, r, 4.16666667e-2f);
+  p = nts_fmaf(p, r, 1.66666667e-1f);
+  p = nts_fmaf(p, r, 0.5f);
+  p = nts_fmaf(p, r, 1.0f);
+  p = nts_fmaf(p, r, 1.0f);
+  return __builtin_ldexpf(p, (int)n);
+}
+
 // Splats staged through LDS per batch when the caller does not choose.
 #define NTS_DEFAULT_LDS_BATCH 256
 

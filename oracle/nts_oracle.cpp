@@ -272,7 +272,7 @@ void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* r
           const T* p = projected + 12 * (size_t)payloads[i];
           const T dx = pxf - p[0], dy = pyf - p[1];
           const T power = T(-0.5) * (p[4] * dx * dx + T(2) * p[5] * dx * dy + p[6] * dy * dy);
-          if (power > T(0)) continue;
+          if (!(power <= T(0))) continue;  // also a NaN power
           const T g = M<T>::exp(power);
           const T ar = p[11] * g;
           const T alpha = ar < T(NTS_ALPHA_MAX) ? ar : T(NTS_ALPHA_MAX);
@@ -550,7 +550,7 @@ void orc_composite_backward_ref(const float* projected, const uint32_t* payloads
             const float dx = pxf - p[0], dy = pyf - p[1];
             const float ca = p[4], cb = p[5], cc = p[6];
             const float power = -0.5f * (ca * dx * dx + 2.0f * cb * dx * dy + cc * dy * dy);
-            if (power > 0.0f) continue;
+            if (!(power <= 0.0f)) continue;  // also a NaN power
             const float g = nts_expf(power);
             const float sigma = p[11];
             const float ar = sigma * g;
@@ -641,7 +641,7 @@ void orc_composite_backward_recompute(const float* projected, const uint32_t* pa
             const float dx = pxf - p[0], dy = pyf - p[1];
             const float ca = p[4], cb = p[5], cc = p[6];
             const float power = -0.5f * (ca * dx * dx + 2.0f * cb * dx * dy + cc * dy * dy);
-            if (power > 0.0f) continue;
+            if (!(power <= 0.0f)) continue;  // also a NaN power
             const float g = nts_expf(power);
             const float sigma = p[11];
             const float ar = sigma * g;
@@ -1120,7 +1120,7 @@ double pixel_sq_err(const double* proj64, const uint32_t* payloads, uint32_t beg
     const double* p = sidx == idx ? rec : proj64 + 12 * (size_t)sidx;
     const double dx = pxf - p[0], dy = pyf - p[1];
     const double power = -0.5 * (p[4] * dx * dx + 2.0 * p[5] * dx * dy + p[6] * dy * dy);
-    if (power > 0.0) continue;
+    if (!(power <= 0.0)) continue;  // also a NaN power
     const double ar = p[11] * exp(power);
     const bool clamped = ar >= (double)NTS_ALPHA_MAX;
     const double alpha = clamped ? (double)NTS_ALPHA_MAX : ar;
@@ -1161,7 +1161,7 @@ uint64_t pixel_sig_f32(const float* proj32, const uint32_t* payloads, uint32_t b
     const float* p = proj32 + 12 * (size_t)payloads[i];
     const float dx = pxf - p[0], dy = pyf - p[1];
     const float power = -0.5f * (p[4] * dx * dx + 2.0f * p[5] * dx * dy + p[6] * dy * dy);
-    if (power > 0.0f) continue;
+    if (!(power <= 0.0f)) continue;  // also a NaN power
     const float ar = p[11] * nts_expf(power);
     const bool clamped = !(ar < NTS_ALPHA_MAX);
     const float alpha = clamped ? NTS_ALPHA_MAX : ar;
