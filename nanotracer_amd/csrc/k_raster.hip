@@ -87,78 +87,83 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
 // and sqrt(2 ln(255 o) Sigma_yy) with Sigma = Q^-1.  Extents get a 1% + 0.05 px
 // margin and the power threshold a 1e-3 margin, so a culled evaluation is
 // one that would have failed the alpha >= 1/255 test: images and gradients
-// are unchanged.
-__device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c) {
+// are unchanged.  The fourth record holds what the per-wave test needs that
+// depends on the splat alone (computed once here, not once per wave): the
+// threshold R of that test with its float-error margin (+inf where the test
+// cannot decide: degenerate conics, kappa >= 6e4), the edge minimisers'
+// slopes -b/c and -b/a, and c.
+__device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c, float4& d) {
   a = make_float4(p.a.x, p.a.y, p.b.x, 2.0f * p.b.y);  // 2b: exact, the blend's dx dy coefficient
   b = make_float4(p.b.z, p.c.w, p.c.x, p.c.y);
   const float o = p.c.w, qa = p.b.x, qb = p.b.y, qc = p.b.z;
   const float det = qa * qc - qb * qb;
-  float ex, ey, pmin;
+  float ex, ey, pmin, R = 3.4e38f, rbc = 0.0f, rba = 0.0f;
   if (o * 255.0f < 0.999f) {  // alpha = o G < 1/255 everywhere
     ex = ey = -3.4e38f;
     pmin = 3.4e38f;
   } else if (!(det > 0.0f)) {  // degenerate conic: no culling
     ex = ey = 3.4e38f;
     pmin = -3.4e38f;
+    R = __builtin_inff();
   } else {
     const float l = logf(o * 255.0f);
     const float r2 = fmaxf(2.0f * l, 0.0f);
     ex = sqrtf(r2 * (qc / det)) * 1.01f + 0.05f;
     ey = sqrtf(r2 * (qa / det)) * 1.01f + 0.05f;
     pmin = -l - 1e-3f;
+    const float s = sqrtf(qa * qc) + fabsf(qb);
+    const float kap = s * s / det;
+    R = (qa > 0.0f && qc > 0.0f && kap < 6.0e4f) ? -2.0f * pmin * (1.0f + 1e-5f + 4e-6f * kap)
+                                                  : __builtin_inff();
+    if (qc > 0.0f) rbc = -qb / qc;
+    if (qa > 0.0f) rba = -qb / qa;
   }
   c = make_float4(p.c.z, ex, ey, pmin);
+  d = make_float4(R, rbc, rba, qc);
 }
 
 // Whether any pixel centre of the wave's rectangle can pass the blend's
-// power test  q(d) = a dx^2 + 2b dx dy + c dy^2 <= R  (R = -2 pmin) for this
+// power test  q(d) = a dx^2 + 2b dx dy + c dy^2 <= -2 pmin  for this
 // splat.  First the bounding-box test of the alpha >= 1/255 ellipse, then
 // the exact minimum of the convex q over the rectangle: it is 0 when the
 // centre lies inside, otherwise it lies on an edge that faces the centre,
-// where it is a clamped 1-D minimum.  Every pixel's float offset lies in the
+// where it is a clamped 1-D minimum (at dy = -b U / c on the edge dx = U,
+// dx = -b V / a on the edge dy = V).  Every pixel's float offset lies in the
 // float rectangle [x0 - mx, x1 - mx] x [y0 - my, y1 - my], and the float
 // evaluation of q at any point is within 6u * kappa * q of the exact value,
 // kappa = (sqrt(ac) + |b|)^2 / det (u = 2^-24); the threshold carries four
-// times that margin, and splats with kappa >= 6e4 skip the exact test.  A
-// rejected splat therefore fails every pixel's power test: results are
-// unchanged.
-__device__ __forceinline__ bool wave_hits(const WaveRect& r, float4 A, float4 B, float4 Cc) {
+// times that margin, and splats with kappa >= 6e4 skip the exact test.  The
+// rounding of the minimiser moves q by a second-order amount.  A rejected
+// splat therefore fails every pixel's power test: results are unchanged.
+// Both edges are evaluated without branches (a wave's lanes hold different
+// splats).
+__device__ __forceinline__ bool wave_hits(const WaveRect& r, float4 A, float4 Cc, float4 D) {
   if (fabsf(r.cx - A.x) > r.hw + Cc.y || fabsf(r.cy - A.y) > r.hh + Cc.z) return false;
-  const float a = A.z, b = 0.5f * A.w, c = B.x;
-  const float det = a * c - b * b;
-  if (!(Cc.w > -1e30f) || !(a > 0.0f) || !(c > 0.0f) || !(det > 0.0f)) return true;
-  const float s = sqrtf(a * c) + fabsf(b);
-  const float kap = s * s / det;
-  if (!(kap < 6.0e4f)) return true;
+  const float a = A.z, b = 0.5f * A.w, c = D.w;
   const float u0 = r.x0 - A.x, u1 = r.x1 - A.x, v0 = r.y0 - A.y, v1 = r.y1 - A.y;
   const bool out_x = u0 > 0.0f || u1 < 0.0f, out_y = v0 > 0.0f || v1 < 0.0f;
-  if (!out_x && !out_y) return true;
-  float q = 3.4e38f;
-  if (out_x) {
-    const float U = u0 > 0.0f ? u0 : u1;
-    const float dy = fminf(fmaxf(-b * U / c, v0), v1);
-    q = fminf(q, a * U * U + 2.0f * b * U * dy + c * dy * dy);
-  }
-  if (out_y) {
-    const float V = v0 > 0.0f ? v0 : v1;
-    const float dx = fminf(fmaxf(-b * V / a, u0), u1);
-    q = fminf(q, a * dx * dx + 2.0f * b * dx * V + c * V * V);
-  }
-  return q <= -2.0f * Cc.w * (1.0f + 1e-5f + 4e-6f * kap);
+  const float U = u0 > 0.0f ? u0 : u1;
+  const float dy = __builtin_amdgcn_fmed3f(U * D.y, v0, v1);
+  const float qx = a * U * U + 2.0f * b * U * dy + c * dy * dy;
+  const float V = v0 > 0.0f ? v0 : v1;
+  const float dx = __builtin_amdgcn_fmed3f(V * D.z, u0, u1);
+  const float qy = a * dx * dx + 2.0f * b * dx * V + c * V * V;
+  const float q = fminf(out_x ? qx : 3.4e38f, out_y ? qy : 3.4e38f);
+  return (!out_x && !out_y) || q <= D.x;
 }
 
 // Per-wave hit masks of the staged batch: bit b of mk[i] is set when slot
 // 64 i + b (< klim) can reach one of the wave's pixels.  Each lane tests
 // NM slots; the masks are wave-uniform.
 template <int NM>
-__device__ __forceinline__ void wave_masks(const WaveRect& wr, const float4* s_a, const float4* s_b,
-                                           const float4* s_c, uint32_t klim, uint64_t (&mk)[NM]) {
+__device__ __forceinline__ void wave_masks(const WaveRect& wr, const float4* s_a, const float4* s_c,
+                                           const float4* s_d, uint32_t klim, uint64_t (&mk)[NM]) {
   const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
     const uint32_t k = (uint32_t)i * 64u + lane;
     bool h = false;
-    if (k < klim) h = wave_hits(wr, s_a[k], s_b[k], s_c[k]);
+    if (k < klim) h = wave_hits(wr, s_a[k], s_c[k], s_d[k]);
     mk[i] = __ballot(h);
   }
 }
@@ -199,6 +204,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(
   __shared__ float4 s_a[BATCH];  // mean.x, mean.y, conic.a, conic.b
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
   __shared__ float4 s_c[BATCH];  // b, cull half-extents x / y, power threshold
+  __shared__ float4 s_d[BATCH];  // per-wave test: threshold, -b/c, -b/a, c
   const uint32_t gt = blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
   const ViewDesc vd = vt.v[vi];
@@ -234,10 +240,10 @@ __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(
     if (!__syncthreads_or(live)) break;
     const uint32_t nb = (end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH;
     for (uint32_t k = threadIdx.x; k < nb; k += NT)
-      stage_splat(proj[vd.proj_base + payloads[base + k]], s_a[k], s_b[k], s_c[k]);
+      stage_splat(proj[vd.proj_base + payloads[base + k]], s_a[k], s_b[k], s_c[k], s_d[k]);
     __syncthreads();
     uint64_t mk[NM];
-    wave_masks<NM>(wr, s_a, s_b, s_c, __ballot(live) ? nb : 0u, mk);
+    wave_masks<NM>(wr, s_a, s_c, s_d, __ballot(live) ? nb : 0u, mk);
 #pragma unroll
     for (int i = 0; i < NM; ++i) {
       uint64_t m = mk[i];
@@ -394,6 +400,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
   __shared__ float4 s_a[BATCH];
   __shared__ float4 s_b[BATCH];
   __shared__ float4 s_c[BATCH];
+  __shared__ float4 s_d[BATCH];
   __shared__ uint32_t s_idx[BATCH];
   __shared__ float s_part[NW][BATCH][9];  // per-wave sums of each splat's terms
   __shared__ uint64_t s_cm[NW][NM];      // which waves wrote s_part for a slot
@@ -461,14 +468,14 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
     const uint32_t rel0 = rem - take;  // slice-relative index of LDS slot 0
     for (uint32_t k = threadIdx.x; k < take; k += NT) {
       const uint32_t sidx = payloads[begin + rel0 + k];
-      stage_splat(proj[vd.proj_base + sidx], s_a[k], s_b[k], s_c[k]);
+      stage_splat(proj[vd.proj_base + sidx], s_a[k], s_b[k], s_c[k], s_d[k]);
       s_idx[k] = sidx;
     }
     __syncthreads();
     uint32_t klim = 0;
     if (__ballot(live) && wave_last > rel0) klim = wave_last - rel0 < take ? wave_last - rel0 : take;
     uint64_t mk[NM], cm[NM];
-    wave_masks<NM>(wr, s_a, s_b, s_c, klim, mk);
+    wave_masks<NM>(wr, s_a, s_c, s_d, klim, mk);
 #pragma unroll
     for (int i = NM - 1; i >= 0; --i) {
       uint64_t m = mk[i];
