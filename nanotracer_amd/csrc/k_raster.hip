@@ -520,7 +520,35 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
             ar = sigma * g;
           }
           const float alpha = __builtin_amdgcn_fmed3f(ar, 0.0f, NTS_ALPHA_MAX);  // min(ar, 0.99), ar >= 0
-          if (!(rel0 + k < lastj[j] && power <= 0.0f && power >= Cc.w && alpha >= NTS_ALPHA_MIN)) continue;
+          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && power >= Cc.w && alpha >= NTS_ALPHA_MIN;
+          if (RECOMP && j == 0) {
+            // The first pixel of the quad without a branch: a lane that fails
+            // the tests adds exact zeros and keeps T (acc needs no zeroing
+            // on a path around it, which costs nine moves per join).
+            const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
+            const float Tn = T[j] * r, Un = U[j] * r;
+            const float gc = __builtin_fmaf(G2[j], c2, __builtin_fmaf(G1[j], c1, G0[j] * c0));
+            const float gal = __builtin_fmaf(Tn, gc, -Un);
+            const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;
+            const float gpow = ok ? gar * ar : 0.0f;
+            const float og = ok ? gar * g : 0.0f;
+            const float ta = ok ? Tn * alpha : 0.0f;
+            const float gx = gpow * dx, gy = gpow * dy;
+            acc[0] += gx;
+            acc[1] += gy;
+            acc[2] += og;
+            acc[3] = __builtin_fmaf(gx, dx, acc[3]);
+            acc[4] = __builtin_fmaf(gx, dy, acc[4]);
+            acc[5] = __builtin_fmaf(gy, dy, acc[5]);
+            acc[6] = __builtin_fmaf(G0[j], ta, acc[6]);
+            acc[7] = __builtin_fmaf(G1[j], ta, acc[7]);
+            acc[8] = __builtin_fmaf(G2[j], ta, acc[8]);
+            U[j] = __builtin_fmaf(ta, gc, U[j]);
+            T[j] = ok ? Tn : T[j];
+            contrib = contrib || ok;
+            continue;
+          }
+          if (!ok) continue;
           float Tn, Un;
           if (RECOMP) {
             const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
