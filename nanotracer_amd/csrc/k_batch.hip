@@ -170,15 +170,16 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
   float gs[P_COUNT];
 #pragma unroll
   for (int k = 0; k < P_COUNT; ++k) gs[k] = 0.0f;
+  // the next view's record is loaded while this one is chained back (one
+  // wave per SIMD fits this kernel's registers: loads must overlap compute)
+  ProjectedGrad next = pgrad[i];
   for (uint32_t v = 0; v < n_views; ++v) {
-    const ProjectedGrad pg = pgrad[(size_t)v * n + i];
+    const ProjectedGrad pg = next;
+    if (v + 1 < n_views) next = pgrad[(size_t)(v + 1) * n + i];
     if (pg.a.x == 0.0f && pg.a.y == 0.0f && pg.a.z == 0.0f && pg.b.x == 0.0f && pg.b.y == 0.0f && pg.b.z == 0.0f &&
         pg.c.x == 0.0f && pg.c.y == 0.0f && pg.c.z == 0.0f)
       continue;  // the view gave this splat no gradient: its terms are all zero
-    float g[P_COUNT];
-    project_backward_p<FULL>(s, cams[v], pg, g);
-#pragma unroll
-    for (int k = 0; k < P_COUNT; ++k) gs[k] = gs[k] + g[k];
+    project_backward_p<FULL, true>(s, cams[v], pg, gs);
   }
   const size_t c = cap;
   if (write_grads) {
@@ -192,25 +193,28 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
   kp.bc1 = dev_bias(cfg.b1_pos, t); kp.bc2 = dev_bias(cfg.b2_pos, t);
   ka.lr = cfg.lr_attr; ka.b1 = cfg.b1_attr; ka.b2 = cfg.b2_attr; ka.eps = cfg.eps_attr;
   ka.bc1 = dev_bias(cfg.b1_attr, t); ka.bc2 = dev_bias(cfg.b2_attr, t);
-  float np[P_COUNT];
+  // Adam plane by plane, storing as it goes; only the quaternion waits for
+  // its renormalisation (apply_constraints.wgsl:24-41).
+  float q[4];
 #pragma unroll
   for (int k = 0; k < P_COUNT; ++k) {
     const size_t j = (size_t)k * c + i;
     float mm = M[j], vv = V[j];
-    np[k] = adam_lane(s.v[k], gs[k], mm, vv, k < P_ROT ? kp : ka);
+    float np = adam_lane(s.v[k], gs[k], mm, vv, k < P_ROT ? kp : ka);
     M[j] = mm;
     V[j] = vv;
+    if (k >= P_ROT && k < P_ROT + 4) {
+      q[k - P_ROT] = np;
+      continue;
+    }
+    if (k >= P_SCALE && k < P_SCALE + 3) np = nts_minf(nts_maxf(np, cfg.smin), cfg.smax);
+    if (k == P_LOGIT) np = nts_minf(nts_maxf(np, -cfg.omax), cfg.omax);
+    P[j] = np;
   }
-  const float n2 = np[P_ROT] * np[P_ROT] + np[P_ROT + 1] * np[P_ROT + 1] + np[P_ROT + 2] * np[P_ROT + 2] +
-                   np[P_ROT + 3] * np[P_ROT + 3];
+  const float n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   const float nrm = nts_maxf(sqrtf(n2), 1e-8f);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) np[P_ROT + k] = np[P_ROT + k] / nrm;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) np[P_SCALE + k] = nts_minf(nts_maxf(np[P_SCALE + k], cfg.smin), cfg.smax);
-  np[P_LOGIT] = nts_minf(nts_maxf(np[P_LOGIT], -cfg.omax), cfg.omax);
-#pragma unroll
-  for (int k = 0; k < P_COUNT; ++k) P[(size_t)k * c + i] = np[k];
+  for (int k = 0; k < 4; ++k) P[(size_t)(P_ROT + k) * c + i] = q[k] / nrm;
 }
 
 // ---------------------------------------------------------------------------

@@ -35,19 +35,29 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ sums, 
                                                    const unsigned long long* __restrict__ tot64,
                                                    uint64_t* __restrict__ d_total) {
   __shared__ uint32_t s_wave[16];
+  __shared__ unsigned long long s_tot[16];
   uint32_t carry = 0;
+  unsigned long long t64 = 0;  // the 64-bit grand total, summed across the block
   for (uint32_t c = 0; c < nblocks; c += 1024) {
     const uint32_t i = c + threadIdx.x;
     const uint32_t v = i < nblocks ? sums[i] : 0u;
+    if (i < nblocks) t64 += tot64[i];
     uint32_t tot;
     const uint32_t ex = block_excl_scan<1024>(v, s_wave, tot);
     if (i < nblocks) sums[i] = carry + ex;
     carry += tot;
   }
-  if (threadIdx.x == 0 && d_total) {
-    uint64_t t = 0;
-    for (uint32_t i = 0; i < nblocks; ++i) t += tot64[i];
-    *d_total = t;
+  if (d_total) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t64 += __shfl_xor(t64, off, 64);
+    if ((threadIdx.x & 63u) == 0) s_tot[threadIdx.x >> 6] = t64;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+#pragma unroll
+      for (int w = 0; w < 16; ++w) t += s_tot[w];
+      *d_total = t;
+    }
   }
 }
 

@@ -77,11 +77,17 @@ __device__ __forceinline__ Projected project_p(const SplatP& s, const DevCamera&
 // the reference drops (project_backward.wgsl:17-24): the colour clamp gate,
 // the view direction's dependence on the position and the Jacobian's
 // dependence on the camera-space mean.
-template <bool FULL>
+// ACC adds each term to g[k] (g[k] = g[k] + term) instead of storing it:
+// a sum over views then needs no per-view temporary of 59 registers.
+template <bool FULL, bool ACC = false>
 __device__ __forceinline__ void project_backward_p(const SplatP& s, const DevCamera& cam, const ProjectedGrad& pg,
                                                    float (&g)[P_COUNT]) {
+  if (!ACC) {
 #pragma unroll
-  for (int k = 0; k < P_COUNT; ++k) g[k] = 0.0f;
+    for (int k = 0; k < P_COUNT; ++k) g[k] = 0.0f;
+  }
+  // store or accumulate one gradient term
+  auto put = [&](int k, float v) { g[k] = ACC ? g[k] + v : v; };
   const float px = s.v[P_POS + 0];
   const float py = s.v[P_POS + 1];
   const float pz = s.v[P_POS + 2];
@@ -106,7 +112,7 @@ __device__ __forceinline__ void project_backward_p(const SplatP& s, const DevCam
     float dcol[3] = {pg.c.x, pg.c.y, pg.c.z};
 
     const float os = nts_sigmoidf(s.v[P_LOGIT]);
-    g[P_LOGIT] = dop * os * (1.0f - os);
+    put(P_LOGIT, dop * os * (1.0f - os));
 
     float dx = cam.cam_pos[0] - px, dy = cam.cam_pos[1] - py, dz = cam.cam_pos[2] - pz;
     const float inv_len = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
@@ -126,12 +132,12 @@ __device__ __forceinline__ void project_backward_p(const SplatP& s, const DevCam
       }
     }
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) g[P_DC + ch] = dcol[ch] * NTS_SH_C0;
+    for (int ch = 0; ch < 3; ++ch) put(P_DC + ch, dcol[ch] * NTS_SH_C0);
 #pragma unroll
     for (int k = 0; k < 15; ++k) {
-      g[P_REST + k] = dcol[0] * b[k];
-      g[P_REST + 15 + k] = dcol[1] * b[k];
-      g[P_REST + 30 + k] = dcol[2] * b[k];
+      put(P_REST + k, dcol[0] * b[k]);
+      put(P_REST + 15 + k, dcol[1] * b[k]);
+      put(P_REST + 30 + k, dcol[2] * b[k]);
     }
 
     // dSigma2d = -(C dconic C) with the off-diagonal halved / summed.
@@ -187,12 +193,12 @@ __device__ __forceinline__ void project_backward_p(const SplatP& s, const DevCam
     for (int k = 0; k < 3; ++k) {
       const float3 rk = make_float3(cv.R.c[k][0], cv.R.c[k][1], cv.R.c[k][2]);
       const float ds2 = dot3f(rk, m3_mulv(dcw, rk));
-      g[P_SCALE + k] = ds2 * 2.0f * cv.s2[k];
+      put(P_SCALE + k, ds2 * 2.0f * cv.s2[k]);
     }
-    g[P_ROT + 0] = 2.0f * (qz * (G10 - G01) + qy * (G02 - G20) + qx * (G21 - G12));
-    g[P_ROT + 1] = 2.0f * (qy * (G01 + G10) + qz * (G02 + G20) + qw * (G21 - G12)) - 4.0f * qx * (G11 + G22);
-    g[P_ROT + 2] = -4.0f * qy * (G00 + G22) + 2.0f * (qx * (G01 + G10) + qw * (G02 - G20) + qz * (G12 + G21));
-    g[P_ROT + 3] = -4.0f * qz * (G00 + G11) + 2.0f * (qw * (G10 - G01) + qx * (G02 + G20) + qy * (G12 + G21));
+    put(P_ROT + 0, 2.0f * (qz * (G10 - G01) + qy * (G02 - G20) + qx * (G21 - G12)));
+    put(P_ROT + 1, 2.0f * (qy * (G01 + G10) + qz * (G02 + G20) + qw * (G21 - G12)) - 4.0f * qx * (G11 + G22));
+    put(P_ROT + 2, -4.0f * qy * (G00 + G22) + 2.0f * (qx * (G01 + G10) + qw * (G02 - G20) + qz * (G12 + G21)));
+    put(P_ROT + 3, -4.0f * qz * (G00 + G11) + 2.0f * (qw * (G10 - G01) + qx * (G02 + G20) + qy * (G12 + G21)));
 
     const float3 dtx = make_float3(f * inv_z, 0.0f, -f * t.x * inv_z2);
     const float3 dty = make_float3(0.0f, -f * inv_z, f * t.y * inv_z2);
@@ -232,9 +238,9 @@ __device__ __forceinline__ void project_backward_p(const SplatP& s, const DevCam
       dp.y = dp.y - (vy - dy * pr) * inv_len;
       dp.z = dp.z - (vz - dz * pr) * inv_len;
     }
-    g[P_POS + 0] = dp.x;
-    g[P_POS + 1] = dp.y;
-    g[P_POS + 2] = dp.z;
+    put(P_POS + 0, dp.x);
+    put(P_POS + 1, dp.y);
+    put(P_POS + 2, dp.z);
   }
 }
 
