@@ -98,21 +98,25 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 
 
 # HBM bytes per launch measured with rocprofv3 (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors
-# read on a 1 GiB copy kernel) at the (kernel, tile, forward batch, backward batch) it was profiled
-# at, with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128, 128): (2.9635e9, 55_869_555),
-                    ("blend_fwd_loss", 32, 128, 128): (3.4496e9, 55_869_555)}
+# read on a 1 GiB copy kernel) at the (kernel, tile size, that kernel's LDS batch) it was profiled at,
+# with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
+PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.9753e9, 55_869_555),
+                    ("blend_fwd_loss", 32, 512): (3.9109e9, 55_869_555)}
 
 
 def roofline_of(kernels, cfg=None, pairs=None):
+    """cfg = (tile, forward LDS batch, backward LDS batch) of the run."""
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     dk = kernels[dom]
     achieved = dk["gbs"] or 0.0
     traffic, note = None, None
-    if cfg is not None and (dom,) + tuple(cfg) in PROFILED_TRAFFIC and pairs:
-        t_bytes, t_pairs = PROFILED_TRAFFIC[(dom,) + tuple(cfg)]
-        traffic = round(t_bytes * pairs / t_pairs)
-        note = "rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel at this configuration, scaled by pairs"
+    if cfg is not None and pairs:
+        tile, fb, bb = cfg
+        key = (dom, tile, bb if dom.startswith("blend_bwd") else fb)
+        if key in PROFILED_TRAFFIC:
+            t_bytes, t_pairs = PROFILED_TRAFFIC[key]
+            traffic = round(t_bytes * pairs / t_pairs)
+            note = "rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel at this tile and LDS batch, scaled by pairs"
     out = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, avg_ms=round(dk["avg_ms"], 4),
                alg_bytes_per_launch=dk["alg_bytes"])

@@ -10,7 +10,9 @@
 #include "nts_internal.h"
 
 #define RS_BLOCK 256
+#ifndef RS_ITEMS
 #define RS_ITEMS 16
+#endif
 #define RS_TILE (RS_BLOCK * RS_ITEMS)
 #define RS_RADIX 256
 #define RS_MAX_SEGS 64
