@@ -102,6 +102,12 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 # with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
 PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.9753e9, 55_869_555),
                     ("blend_fwd_loss", 32, 512): (3.9109e9, 55_869_555)}
+# Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
+PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (4.1652e9, 55_869_555),
+                 ("blend_fwd_loss", 32, 512): (2.5284e9, 55_869_555)}
+N_SIMDS, CLOCK_HZ = 1024, 2.4e9
+# A wave64 FP32 VALU instruction occupies a SIMD for 4 cycles: 0.25 issues per SIMD per cycle.
+VALU_ISSUE_CEILING = 0.25
 
 
 def roofline_of(kernels, cfg=None, pairs=None):
@@ -122,6 +128,22 @@ def roofline_of(kernels, cfg=None, pairs=None):
                alg_bytes_per_launch=dk["alg_bytes"])
     if note:
         out["traffic_note"] = note
+    return out
+
+
+def valu_of(kernels, cfg, pairs):
+    """VALU issue rate of the blend kernels against the 4-cycle wave64 issue ceiling: the profiled
+    instruction count, scaled by this run's pairs, over this run's HIP-event kernel time."""
+    out = {}
+    tile, fb, bb = cfg
+    for name in ("blend_bwd_recompute", "blend_fwd_loss"):
+        key = (name, tile, bb if name.startswith("blend_bwd") else fb)
+        if name not in kernels or key not in PROFILED_VALU or not pairs:
+            continue
+        instr = PROFILED_VALU[key][0] * pairs / PROFILED_VALU[key][1]
+        rate = instr / N_SIMDS / (kernels[name]["avg_ms"] * 1e-3 * CLOCK_HZ)
+        out[name] = dict(instructions=round(instr), issue_per_simd_cycle=round(rate, 4),
+                         ceiling=VALU_ISSUE_CEILING, frac=round(rate / VALU_ISSUE_CEILING, 3))
     return out
 
 
@@ -344,6 +366,7 @@ def run_batched(args, ctx):
     )
     out.update(render_latency(ctx, g, s.n, tile))
     out["roofline"] = roofline_of(kernels, (tile, lds[0], lds[1]), pairs)
+    out["valu"] = valu_of(kernels, (tile, lds[0], lds[1]), pairs)
     if ssim:
         out["ssim_step"] = ssim
     if tune:
