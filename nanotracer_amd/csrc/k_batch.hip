@@ -99,8 +99,9 @@ __global__ void __launch_bounds__(256) k_project_views(const float* __restrict__
       uint32_t c = 0;
       int32_t x0, y0, x1, y1;
       const ViewTiles vt = tt.t[v];
-      if (p.b.w != 0.0f && tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x,
-                                         (int32_t)vt.tiles_y, x0, y0, x1, y1)) {
+      // only the tiles the splat's alpha >= 1/255 ellipse can reach (the
+      // emit uses the same rule)
+      if (record_tiles<true>(p, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1, y1)) {
         c = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1));
         ++vis;
       }
@@ -397,11 +398,11 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
   if (n > 0) {
     const uint32_t nrec = (uint32_t)vn;
     if (pl.key_bytes == 4)
-      NTS_LAUNCH(ctx, "emit_views", k_emit_balanced<uint32_t>, dim3(nts_div_up(nrec, 256)), dim3(256), 0,
+      NTS_LAUNCH(ctx, "emit_views", (k_emit_balanced<uint32_t, true>), dim3(nts_div_up(nrec, 256)), dim3(256), 0,
                  (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
                  b->pair_cap, (uint32_t*)b->keys_a, b->vals_a);
     else
-      NTS_LAUNCH(ctx, "emit_views", k_emit_balanced<uint64_t>, dim3(nts_div_up(nrec, 256)), dim3(256), 0,
+      NTS_LAUNCH(ctx, "emit_views", (k_emit_balanced<uint64_t, true>), dim3(nts_div_up(nrec, 256)), dim3(256), 0,
                  (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
                  b->pair_cap, (uint64_t*)b->keys_a, b->vals_a);
   }

@@ -48,7 +48,9 @@ __device__ __forceinline__ uint32_t rec_view(const RecTable& rt, uint32_t f) {
 // last record whose first pair is <= j (binary search in LDS), so every
 // store is coalesced whatever the footprints.  offsets is the exclusive
 // scan of the per-record tile counts; *total the pair total.
-template <typename K>
+// CULL: only the tiles the record's alpha >= 1/255 ellipse can reach
+// (record_tiles), in the same order; the counts must come from the same rule.
+template <typename K, bool CULL = false>
 __global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restrict__ proj, RecTable rt,
                                                        TileTable tt, const uint32_t* __restrict__ offsets,
                                                        const uint64_t* __restrict__ total, uint64_t capacity,
@@ -65,8 +67,7 @@ __global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restri
     const ViewTiles vt = tt.t[v];
     const Projected p = proj[f];
     int32_t x0 = 0, y0 = 0, x1 = -1, y1 = -1;
-    if (p.b.w != 0.0f)
-      tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1, y1);
+    record_tiles<CULL>(p, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1, y1);
     s_tile0[t] = (uint32_t)y0 * vt.tiles_x + (uint32_t)x0;
     s_bw[t] = x1 >= x0 ? (uint32_t)(x1 - x0 + 1) : 1u;
     s_tx[t] = vt.tiles_x;

@@ -170,6 +170,16 @@ __device__ __forceinline__ bool tile_bbox_dev(float mx, float my, float r, float
   return !(x1 < x0 || y1 < y0);
 }
 
+// The tile rectangle of a projected record: the reference's (CULL false) or
+// that one cut to the tiles its alpha >= 1/255 ellipse can reach
+// (nts_alpha_tile_rect).  false when empty.
+template <bool CULL>
+__device__ __forceinline__ bool record_tiles(const Projected& p, float ts, int32_t tiles_x, int32_t tiles_y,
+                                             int32_t& x0, int32_t& y0, int32_t& x1, int32_t& y1) {
+  if (p.b.w == 0.0f || !tile_bbox_dev(p.a.x, p.a.y, p.a.w, ts, tiles_x, tiles_y, x0, y0, x1, y1)) return false;
+  return !CULL || nts_alpha_tile_rect(p.a.x, p.a.y, p.b.x, p.b.y, p.b.z, p.c.w, ts, &x0, &y0, &x1, &y1);
+}
+
 // 16-bit depth bucket (tile_emit.wgsl:109-110).
 __device__ __forceinline__ uint32_t depth16(float depth, float depth_max) {
   float dn = depth / depth_max;

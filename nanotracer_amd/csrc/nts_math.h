@@ -200,3 +200,29 @@ NTS_HD float nts_box_muller(uint32_t* st) {
 // split scale step (train.rs:455, 489).
 #define NTS_LN_HALF (-0.693147182f)
 #define NTS_NEG_LN_1_6 (-0.470003635f)
+
+// The tiles of a splat that can hold a pixel with alpha >= 1/255: the
+// reference's tile rectangle [x0, x1] x [y0, y1] (in: non-empty) cut to the
+// tiles overlapping the axis-aligned box of the ellipse x^T Q x <= 2 ln(255 o)
+// (half extents sqrt(2 ln(255 o) Sigma_xx) and sqrt(2 ln(255 o) Sigma_yy),
+// Sigma = Q^-1, with a 1% + 0.05 px margin).  A pixel outside that box has
+// o exp(-x^T Q x / 2) < 1/255, so it fails the blend's alpha test: dropping
+// the other tiles' (splat, tile) pairs leaves every image unchanged.  Conic
+// Q = (qa, qb; qb, qc), o = opacity.  False when no tile remains (opacity
+// below 1/255 included); a conic that is not positive definite keeps the
+// reference rectangle.
+NTS_HD bool nts_alpha_tile_rect(float mx, float my, float qa, float qb, float qc, float o, float ts, int32_t* x0,
+                                int32_t* y0, int32_t* x1, int32_t* y1) {
+  if (!(o * 255.0f >= 0.999f)) return false;
+  const float det = qa * qc - qb * qb;
+  if (!(det > 0.0f) || !(qa > 0.0f) || !(qc > 0.0f)) return true;
+  const float l = nts_logf(o * 255.0f);
+  const float r2 = nts_maxf(2.0f * l, 0.0f);
+  const float ex = __builtin_sqrtf(r2 * (qc / det)) * 1.01f + 0.05f;
+  const float ey = __builtin_sqrtf(r2 * (qa / det)) * 1.01f + 0.05f;
+  *x0 = nts_maxi(*x0, nts_f2i_sat(__builtin_floorf((mx - ex) / ts)));
+  *y0 = nts_maxi(*y0, nts_f2i_sat(__builtin_floorf((my - ey) / ts)));
+  *x1 = nts_mini(*x1, nts_f2i_sat(__builtin_floorf((mx + ex) / ts)));
+  *y1 = nts_mini(*y1, nts_f2i_sat(__builtin_floorf((my + ey) / ts)));
+  return !(*x1 < *x0 || *y1 < *y0);
+}

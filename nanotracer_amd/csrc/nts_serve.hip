@@ -57,8 +57,7 @@ __global__ void __launch_bounds__(256) k_project_requests(SceneTab st, RecTable 
   uint32_t c = 0;
   int32_t x0, y0, x1, y1;
   const ViewTiles vt = tt.t[v];
-  if (p.b.w != 0.0f && tile_bbox_dev(p.a.x, p.a.y, p.a.w, (float)vt.tile_size, (int32_t)vt.tiles_x,
-                                     (int32_t)vt.tiles_y, x0, y0, x1, y1))
+  if (record_tiles<true>(p, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1, y1))
     c = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1));
   counts[f] = c;
 }
@@ -297,11 +296,11 @@ int launch_batch(nts_server* sv, Slot& sl) {
   bool alt = false;
   if (rec > 0) {
     if (kb == 4)
-      NTS_LAUNCH(ctx, "serve_emit", k_emit_balanced<uint32_t>, dim3(nts_div_up(rec, 256)), dim3(256), 0,
+      NTS_LAUNCH(ctx, "serve_emit", (k_emit_balanced<uint32_t, true>), dim3(nts_div_up(rec, 256)), dim3(256), 0,
                  (const Projected*)sl.proj, rt, tt, (const uint32_t*)sl.counts, (const uint64_t*)sl.d_total,
                  sv->cfg.pair_capacity, (uint32_t*)sl.keys_a, sl.vals_a);
     else
-      NTS_LAUNCH(ctx, "serve_emit", k_emit_balanced<uint64_t>, dim3(nts_div_up(rec, 256)), dim3(256), 0,
+      NTS_LAUNCH(ctx, "serve_emit", (k_emit_balanced<uint64_t, true>), dim3(nts_div_up(rec, 256)), dim3(256), 0,
                  (const Projected*)sl.proj, rt, tt, (const uint32_t*)sl.counts, (const uint64_t*)sl.d_total,
                  sv->cfg.pair_capacity, (uint64_t*)sl.keys_a, sl.vals_a);
   }

@@ -392,18 +392,31 @@ void orc_project_f64(const orc_splats_f64* s, uint32_t n, const orc_camera* cam,
   project_all<double>(sp, n, cam, projected);
 }
 
-uint64_t orc_tile_count(const float* projected, uint32_t n, const orc_tiling* t, uint32_t* counts) {
+// The tile rectangle of record p: the reference's, or with cull the tiles
+// its alpha >= 1/255 ellipse can reach (nts_alpha_tile_rect; the device's
+// batched and serving paths bin this way).
+static bool record_tiles(const float* p, const orc_tiling* t, int cull, int32_t* x0, int32_t* y0, int32_t* x1,
+                         int32_t* y1) {
+  if (!tile_bbox<float>(p, t, x0, y0, x1, y1)) return false;
+  return !cull || nts_alpha_tile_rect(p[0], p[1], p[4], p[5], p[6], p[11], (float)t->tile_size, x0, y0, x1, y1);
+}
+
+static uint64_t tile_count_ex(const float* projected, uint32_t n, const orc_tiling* t, int cull, uint32_t* counts) {
   uint64_t total = 0;
 #pragma omp parallel for reduction(+ : total) schedule(static)
   for (int64_t i = 0; i < (int64_t)n; ++i) {
     int32_t x0, y0, x1, y1;
     uint32_t c = 0;
-    if (tile_bbox<float>(projected + 12 * i, t, &x0, &y0, &x1, &y1))
+    if (record_tiles(projected + 12 * i, t, cull, &x0, &y0, &x1, &y1))
       c = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1));
     counts[i] = c;
     total += c;
   }
   return total;
+}
+
+uint64_t orc_tile_count(const float* projected, uint32_t n, const orc_tiling* t, uint32_t* counts) {
+  return tile_count_ex(projected, n, t, 0, counts);
 }
 
 // scan_block.wgsl + scan_add_offsets.wgsl: exclusive, wrapping u32.
@@ -417,14 +430,14 @@ void orc_exclusive_scan_u32(const uint32_t* in, uint64_t n, uint32_t* out) {
 }
 
 // tile_emit.wgsl:91-121.
-void orc_tile_emit(const float* projected, uint32_t n, const uint32_t* offsets, const orc_tiling* t,
-                   uint32_t* keys, uint32_t* payloads) {
+static void tile_emit_ex(const float* projected, uint32_t n, const uint32_t* offsets, const orc_tiling* t, int cull,
+                         uint32_t* keys, uint32_t* payloads) {
   const uint32_t tiles_x = (t->width + t->tile_size - 1) / t->tile_size;
 #pragma omp parallel for schedule(dynamic, 256)
   for (int64_t i = 0; i < (int64_t)n; ++i) {
     const float* p = projected + 12 * i;
     int32_t x0, y0, x1, y1;
-    if (!tile_bbox<float>(p, t, &x0, &y0, &x1, &y1)) continue;
+    if (!record_tiles(p, t, cull, &x0, &y0, &x1, &y1)) continue;
     float dn = p[2] / t->depth_max;
     dn = dn < 0.0f ? 0.0f : (dn > 1.0f ? 1.0f : dn);
     const uint32_t d16 = (uint32_t)(dn * 65535.0f);
@@ -437,6 +450,11 @@ void orc_tile_emit(const float* projected, uint32_t n, const uint32_t* offsets, 
         ++w;
       }
   }
+}
+
+void orc_tile_emit(const float* projected, uint32_t n, const uint32_t* offsets, const orc_tiling* t,
+                   uint32_t* keys, uint32_t* payloads) {
+  tile_emit_ex(projected, n, offsets, t, 0, keys, payloads);
 }
 
 // radix_sort.rs:1-26: 32 stable 1-bit LSD passes == a stable sort by key.
@@ -480,10 +498,10 @@ void orc_tile_ranges(const uint32_t* sorted_keys, uint64_t k, uint32_t num_tiles
   }
 }
 
-uint64_t orc_bin(const float* projected, uint32_t n, const orc_tiling* t, uint32_t* keys,
-                 uint32_t* payloads, uint32_t* ranges) {
+uint64_t orc_bin_ex(const float* projected, uint32_t n, const orc_tiling* t, int cull, uint32_t* keys,
+                    uint32_t* payloads, uint32_t* ranges) {
   std::vector<uint32_t> counts(n > 0 ? n : 1), offsets(n > 0 ? n : 1);
-  const uint64_t total = orc_tile_count(projected, n, t, counts.data());
+  const uint64_t total = tile_count_ex(projected, n, t, cull, counts.data());
   if (!keys) return total;
   const uint32_t tiles_x = (t->width + t->tile_size - 1) / t->tile_size;
   const uint32_t tiles_y = (t->height + t->tile_size - 1) / t->tile_size;
@@ -493,10 +511,15 @@ uint64_t orc_bin(const float* projected, uint32_t n, const orc_tiling* t, uint32
     return 0;
   }
   orc_exclusive_scan_u32(counts.data(), n, offsets.data());
-  orc_tile_emit(projected, n, offsets.data(), t, keys, payloads);
+  tile_emit_ex(projected, n, offsets.data(), t, cull, keys, payloads);
   orc_sort_pairs_u32(keys, payloads, total);
   orc_tile_ranges(keys, total, num_tiles, ranges);
   return total;
+}
+
+uint64_t orc_bin(const float* projected, uint32_t n, const orc_tiling* t, uint32_t* keys,
+                 uint32_t* payloads, uint32_t* ranges) {
+  return orc_bin_ex(projected, n, t, 0, keys, payloads, ranges);
 }
 
 void orc_composite(const float* projected, const uint32_t* payloads, const uint32_t* ranges,

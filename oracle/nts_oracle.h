@@ -85,6 +85,10 @@ void orc_tile_ranges(const uint32_t* sorted_keys, uint64_t k, uint32_t num_tiles
 // keys == NULL to get the pair count, then again with buffers of that size.
 uint64_t orc_bin(const float* projected, uint32_t n, const orc_tiling* t,
                  uint32_t* keys, uint32_t* payloads, uint32_t* ranges);
+// orc_bin with cull: only the tiles each record's alpha >= 1/255 ellipse can
+// reach (nts_alpha_tile_rect), as the device's batched and serving paths bin.
+uint64_t orc_bin_ex(const float* projected, uint32_t n, const orc_tiling* t, int cull, uint32_t* keys,
+                    uint32_t* payloads, uint32_t* ranges);
 
 // Forward alpha blend (rasterize.wgsl:48-118).  out: 4 floats/pixel
 // (r, g, b, 1 - T).  final_t / last_idx may be NULL; last_idx is the number

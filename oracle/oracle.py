@@ -64,7 +64,7 @@ def lib():
         if not LIB.exists():
             build()
         _h = C.CDLL(str(LIB))
-        for name in ("orc_tile_count", "orc_bin"):
+        for name in ("orc_tile_count", "orc_bin", "orc_bin_ex"):
             getattr(_h, name).restype = C.c_uint64
         _h.orc_loss_grad.restype = C.c_double
         _h.orc_powi_f32.restype = C.c_float
@@ -176,16 +176,19 @@ def sort_pairs(keys: np.ndarray, vals: np.ndarray):
     return k, v
 
 
-def bin(projected: np.ndarray, t: Tiling):
-    """Full TileBinner::bin: (sorted u32 keys, payloads, ranges[2*tiles])."""
+def bin(projected: np.ndarray, t: Tiling, cull: bool = False):
+    """Full TileBinner::bin: (sorted u32 keys, payloads, ranges[2*tiles]).
+    cull: only the tiles each record's alpha >= 1/255 ellipse can reach, as
+    the device's batched and serving paths bin."""
     projected = np.ascontiguousarray(projected, np.float32)
     n = projected.shape[0]
-    total = lib().orc_bin(_p(projected), C.c_uint32(n), C.byref(t), None, None, None)
+    f = lib().orc_bin_ex
+    total = f(_p(projected), C.c_uint32(n), C.byref(t), C.c_int(int(cull)), None, None, None)
     ntiles = (-(-t.width // t.tile_size)) * (-(-t.height // t.tile_size))
     keys = np.zeros(max(total, 1), np.uint32)
     pay = np.zeros(max(total, 1), np.uint32)
     ranges = np.zeros(2 * ntiles, np.uint32)
-    lib().orc_bin(_p(projected), C.c_uint32(n), C.byref(t), _p(keys), _p(pay), _p(ranges))
+    f(_p(projected), C.c_uint32(n), C.byref(t), C.c_int(int(cull)), _p(keys), _p(pay), _p(ranges))
     return keys[:total], pay[:total], ranges
 
 
