@@ -156,6 +156,13 @@ struct AdamCfgs {
   float smin, smax, omax;
 };
 
+// Per splat: the V views' projected gradients chained back to the
+// parameters and summed, then accumulate_grad + Adam + constraints.  The
+// geometry (through the 2-D mean, conic and opacity) is summed over the
+// views first, then the colour one channel at a time (its SH planes take
+// their Adam step as soon as their sums are complete), then the position,
+// rotation, scale and opacity planes: no more than 23 parameters and sums
+// are live across a view loop, so several waves share each SIMD.
 template <bool FULL>
 __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__ P, float* __restrict__ G,
                                                              float* __restrict__ M, float* __restrict__ V,
@@ -166,56 +173,95 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
                                                              int write_grads) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  SplatP s;
-  load_splat(P, cap, i, s);
-  float gs[P_COUNT];
-#pragma unroll
-  for (int k = 0; k < P_COUNT; ++k) gs[k] = 0.0f;
-  // the next view's record is loaded while this one is chained back (one
-  // wave per SIMD fits this kernel's registers: loads must overlap compute)
-  ProjectedGrad next = pgrad[i];
-  for (uint32_t v = 0; v < n_views; ++v) {
-    const ProjectedGrad pg = next;
-    if (v + 1 < n_views) next = pgrad[(size_t)(v + 1) * n + i];
-    if (pg.a.x == 0.0f && pg.a.y == 0.0f && pg.a.z == 0.0f && pg.b.x == 0.0f && pg.b.y == 0.0f && pg.b.z == 0.0f &&
-        pg.c.x == 0.0f && pg.c.y == 0.0f && pg.c.z == 0.0f)
-      continue;  // the view gave this splat no gradient: its terms are all zero
-    project_backward_p<FULL, true>(s, cams[v], pg, gs);
-  }
   const size_t c = cap;
-  if (write_grads) {
-#pragma unroll
-    for (int k = 0; k < P_COUNT; ++k) G[(size_t)k * c + i] = gs[k];
-  }
-  grad_acc[i] = grad_acc[i] + fabsf(gs[P_POS + 0]) + fabsf(gs[P_POS + 1]) + fabsf(gs[P_POS + 2]);
   const uint64_t t = *d_step;
   AdamK kp, ka;
   kp.lr = cfg.lr_pos; kp.b1 = cfg.b1_pos; kp.b2 = cfg.b2_pos; kp.eps = cfg.eps_pos;
   kp.bc1 = dev_bias(cfg.b1_pos, t); kp.bc2 = dev_bias(cfg.b2_pos, t);
   ka.lr = cfg.lr_attr; ka.b1 = cfg.b1_attr; ka.b2 = cfg.b2_attr; ka.eps = cfg.eps_attr;
   ka.bc1 = dev_bias(cfg.b1_attr, t); ka.bc2 = dev_bias(cfg.b2_attr, t);
-  // Adam plane by plane, storing as it goes; only the quaternion waits for
-  // its renormalisation (apply_constraints.wgsl:24-41).
-  float q[4];
-#pragma unroll
-  for (int k = 0; k < P_COUNT; ++k) {
+  // one plane's gradient into G and its Adam step; returns the new value
+  auto step = [&](int k, float p, float gk) {
     const size_t j = (size_t)k * c + i;
+    if (write_grads) G[j] = gk;
     float mm = M[j], vv = V[j];
-    float np = adam_lane(s.v[k], gs[k], mm, vv, k < P_ROT ? kp : ka);
+    const float np = adam_lane(p, gk, mm, vv, k < P_ROT ? kp : ka);
     M[j] = mm;
     V[j] = vv;
-    if (k >= P_ROT && k < P_ROT + 4) {
-      q[k - P_ROT] = np;
-      continue;
+    return np;
+  };
+
+  // ---- geometry over the views ----
+  float q[11];  // pos, rot, log-scale, logit
+#pragma unroll
+  for (int k = 0; k < 3; ++k) q[k] = P[(size_t)(P_POS + k) * c + i];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[3 + k] = P[(size_t)(P_ROT + k) * c + i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) q[7 + k] = P[(size_t)(P_SCALE + k) * c + i];
+  q[10] = P[(size_t)P_LOGIT * c + i];
+  float gq[11];
+#pragma unroll
+  for (int k = 0; k < 11; ++k) gq[k] = 0.0f;
+  uint32_t kept = 0;  // views that keep the splat and passed it a gradient
+  {
+    ProjectedGrad next = pgrad[i];  // the next view's record loads while this one is chained back
+    for (uint32_t v = 0; v < n_views; ++v) {
+      const ProjectedGrad pg = next;
+      if (v + 1 < n_views) next = pgrad[(size_t)(v + 1) * n + i];
+      if (pg.a.x == 0.0f && pg.a.y == 0.0f && pg.a.z == 0.0f && pg.b.x == 0.0f && pg.b.y == 0.0f &&
+          pg.b.z == 0.0f && pg.c.x == 0.0f && pg.c.y == 0.0f && pg.c.z == 0.0f)
+        continue;  // the view gave this splat no gradient: its terms are all zero
+      if (project_backward_geo<FULL>(q, cams[v], pg.a, pg.b, gq)) kept |= 1u << v;
     }
-    if (k >= P_SCALE && k < P_SCALE + 3) np = nts_minf(nts_maxf(np, cfg.smin), cfg.smax);
-    if (k == P_LOGIT) np = nts_minf(nts_maxf(np, -cfg.omax), cfg.omax);
-    P[j] = np;
   }
-  const float n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+
+  // ---- colour, one channel at a time ----
+  float3 psh = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+  for (int ch = 0; ch < 3; ++ch) {
+    const float dc = P[(size_t)(P_DC + ch) * c + i];
+    float rest[15], grest[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) {
+      rest[k] = P[(size_t)(P_REST + 15 * ch + k) * c + i];
+      grest[k] = 0.0f;
+    }
+    float gdc = 0.0f;
+    for (uint32_t m = kept; m; m &= m - 1) {
+      const uint32_t v = (uint32_t)__builtin_ctz(m);
+      const float4 pc = pgrad[(size_t)v * n + i].c;
+      const float dcol = ch == 0 ? pc.x : (ch == 1 ? pc.y : pc.z);
+      project_backward_colour<FULL>(q[0], q[1], q[2], dc, rest, cams[v], dcol, gdc, grest, psh);
+    }
+    P[(size_t)(P_DC + ch) * c + i] = step(P_DC + ch, dc, gdc);
+#pragma unroll
+    for (int k = 0; k < 15; ++k) {
+      const int pk = P_REST + 15 * ch + k;
+      P[(size_t)pk * c + i] = step(pk, rest[k], grest[k]);
+    }
+  }
+
+  // ---- position, rotation, scale, opacity ----
+  if (FULL) {
+    gq[0] = gq[0] - psh.x;
+    gq[1] = gq[1] - psh.y;
+    gq[2] = gq[2] - psh.z;
+  }
+  grad_acc[i] = grad_acc[i] + fabsf(gq[0]) + fabsf(gq[1]) + fabsf(gq[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) P[(size_t)(P_POS + k) * c + i] = step(P_POS + k, q[k], gq[k]);
+  float nq[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) nq[k] = step(P_ROT + k, q[3 + k], gq[3 + k]);
+  const float n2 = nq[0] * nq[0] + nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3];
   const float nrm = nts_maxf(sqrtf(n2), 1e-8f);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) P[(size_t)(P_ROT + k) * c + i] = q[k] / nrm;
+  for (int k = 0; k < 4; ++k) P[(size_t)(P_ROT + k) * c + i] = nq[k] / nrm;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    P[(size_t)(P_SCALE + k) * c + i] = nts_minf(nts_maxf(step(P_SCALE + k, q[7 + k], gq[7 + k]), cfg.smin), cfg.smax);
+  P[(size_t)P_LOGIT * c + i] = nts_minf(nts_maxf(step(P_LOGIT, q[10], gq[10]), -cfg.omax), cfg.omax);
 }
 
 // ---------------------------------------------------------------------------
