@@ -163,10 +163,10 @@ __device__ __forceinline__ bool screen_cov(const DevCamera& cam, float3 t, float
 // Tile bbox (tile_count.wgsl:39-50).  false when the bbox is empty.
 __device__ __forceinline__ bool tile_bbox_dev(float mx, float my, float r, float ts, int32_t tiles_x, int32_t tiles_y,
                                               int32_t& x0, int32_t& y0, int32_t& x1, int32_t& y1) {
-  x0 = nts_maxi(nts_f2i_sat(floorf((mx - r) / ts)), 0);
-  y0 = nts_maxi(nts_f2i_sat(floorf((my - r) / ts)), 0);
-  x1 = nts_mini(nts_f2i_sat(floorf((mx + r) / ts)), tiles_x - 1);
-  y1 = nts_mini(nts_f2i_sat(floorf((my + r) / ts)), tiles_y - 1);
+  x0 = nts_maxi(nts_f2i_sat(floorf(nts_div_tile(mx - r, ts))), 0);
+  y0 = nts_maxi(nts_f2i_sat(floorf(nts_div_tile(my - r, ts))), 0);
+  x1 = nts_mini(nts_f2i_sat(floorf(nts_div_tile(mx + r, ts))), tiles_x - 1);
+  y1 = nts_mini(nts_f2i_sat(floorf(nts_div_tile(my + r, ts))), tiles_y - 1);
   return !(x1 < x0 || y1 < y0);
 }
 

@@ -72,6 +72,13 @@ NTS_HD int32_t nts_f2i_sat(float x) {
   return (int32_t)x;
 }
 
+// x / ts for a (positive, normal) tile size: a power of two divides exactly
+// as a multiplication by its reciprocal, which is what it costs then.
+NTS_HD float nts_div_tile(float x, float ts) {
+  const uint32_t b = nts_f32_to_bits(ts);
+  return (b & 0x007FFFFFu) == 0u ? x * nts_bits_to_f32(0x7F000000u - (b & 0x7F800000u)) : x / ts;
+}
+
 NTS_HD float nts_minf(float a, float b) { return (b < a) ? b : a; }
 NTS_HD float nts_maxf(float a, float b) { return (a < b) ? b : a; }
 NTS_HD int32_t nts_mini(int32_t a, int32_t b) { return (b < a) ? b : a; }
@@ -220,9 +227,9 @@ NTS_HD bool nts_alpha_tile_rect(float mx, float my, float qa, float qb, float qc
   const float r2 = nts_maxf(2.0f * l, 0.0f);
   const float ex = __builtin_sqrtf(r2 * (qc / det)) * 1.01f + 0.05f;
   const float ey = __builtin_sqrtf(r2 * (qa / det)) * 1.01f + 0.05f;
-  *x0 = nts_maxi(*x0, nts_f2i_sat(__builtin_floorf((mx - ex) / ts)));
-  *y0 = nts_maxi(*y0, nts_f2i_sat(__builtin_floorf((my - ey) / ts)));
-  *x1 = nts_mini(*x1, nts_f2i_sat(__builtin_floorf((mx + ex) / ts)));
-  *y1 = nts_mini(*y1, nts_f2i_sat(__builtin_floorf((my + ey) / ts)));
+  *x0 = nts_maxi(*x0, nts_f2i_sat(__builtin_floorf(nts_div_tile(mx - ex, ts))));
+  *y0 = nts_maxi(*y0, nts_f2i_sat(__builtin_floorf(nts_div_tile(my - ey, ts))));
+  *x1 = nts_mini(*x1, nts_f2i_sat(__builtin_floorf(nts_div_tile(mx + ex, ts))));
+  *y1 = nts_mini(*y1, nts_f2i_sat(__builtin_floorf(nts_div_tile(my + ey, ts))));
   return !(*x1 < *x0 || *y1 < *y0);
 }

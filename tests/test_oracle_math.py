@@ -145,3 +145,14 @@ def test_box_muller_is_the_transform_of_its_uniforms():
         assert uni[i // 2, i % 2] == np.float32(s / 2.0 ** 32)
         s2 = s
     assert abs(out.mean()) < 0.01 and abs(out.std() - 1.0) < 0.01
+
+
+def test_tile_division_by_a_power_of_two_is_exact():
+    # nts_div_tile multiplies by the reciprocal for power-of-two tile sizes;
+    # the quotient must equal the correctly rounded division bit for bit
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-5000, 5000, 200000), rng.normal(0, 3, 1000)]).astype(np.float32)
+    for ts in (1.0, 8.0, 16.0, 32.0, 64.0):
+        q = x / np.float32(ts)
+        m = x * np.float32(1.0 / ts)
+        assert np.array_equal(q.view(np.uint32), m.view(np.uint32)), ts
