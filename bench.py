@@ -100,11 +100,11 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 # HBM bytes per launch measured with rocprofv3 (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors
 # read on a 1 GiB copy kernel) at the (kernel, tile size, that kernel's LDS batch) it was profiled at,
 # with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.9753e9, 55_869_555),
-                    ("blend_fwd_loss", 32, 512): (3.9109e9, 55_869_555)}
+PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.7017e+09, 45_319_956),
+                    ("blend_fwd_loss", 32, 512): (3.5572e+09, 45_319_956)}
 # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
-PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (4.1652e9, 55_869_555),
-                 ("blend_fwd_loss", 32, 512): (2.5284e9, 55_869_555)}
+PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (4.1179e9, 45_319_956),
+                 ("blend_fwd_loss", 32, 512): (2.4825e9, 45_319_956)}
 N_SIMDS, CLOCK_HZ = 1024, 2.4e9
 # A wave64 FP32 VALU instruction occupies a SIMD for 4 cycles: 0.25 issues per SIMD per cycle.
 VALU_ISSUE_CEILING = 0.25
