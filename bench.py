@@ -105,9 +105,16 @@ PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.7017e+09, 45_319_956),
 # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
 PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (4.1179e9, 45_319_956),
                  ("blend_fwd_loss", 32, 512): (2.4825e9, 45_319_956)}
+# LDS bytes per launch moved by the lanes (rocprofv3 SQ_INSTS_LDS_LOAD_BANDWIDTH + _STORE_BANDWIDTH,
+# 64-byte units) and the share of the kernel's cycles the LDS spends on indexed operations
+# (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
+PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((846562491.4 + 11953168.2) * 64, 45_298_241, 0.097),
+                ("blend_fwd_loss", 32, 512): ((1602792019.3 + 15178916.0) * 64, 45_298_241, 0.427)}
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 128 B per clock per CU
 N_SIMDS, CLOCK_HZ = 1024, 2.4e9
-# A wave64 FP32 VALU instruction occupies a SIMD for 4 cycles: 0.25 issues per SIMD per cycle.
-VALU_ISSUE_CEILING = 0.25
+# Independent FP32 FMA chains at full occupancy issue 0.244 wave instructions per SIMD per cycle
+# (a wave64 instruction occupies a SIMD for 4 cycles); mixed code co-issues slightly above it.
+FMA_PROBE_RATE = 0.244
 
 
 def roofline_of(kernels, cfg=None, pairs=None):
@@ -132,18 +139,25 @@ def roofline_of(kernels, cfg=None, pairs=None):
 
 
 def valu_of(kernels, cfg, pairs):
-    """VALU issue rate of the blend kernels against the 4-cycle wave64 issue ceiling: the profiled
-    instruction count, scaled by this run's pairs, over this run's HIP-event kernel time."""
+    """VALU issue rate and LDS bandwidth of the blend kernels: the profiled instruction count and LDS
+    bytes, scaled by this run's pairs, over this run's HIP-event kernel time; the issue rate beside
+    the FMA-chain probe's, the LDS rate beside the LDS peak."""
     out = {}
     tile, fb, bb = cfg
     for name in ("blend_bwd_recompute", "blend_fwd_loss"):
         key = (name, tile, bb if name.startswith("blend_bwd") else fb)
         if name not in kernels or key not in PROFILED_VALU or not pairs:
             continue
+        sec = kernels[name]["avg_ms"] * 1e-3
         instr = PROFILED_VALU[key][0] * pairs / PROFILED_VALU[key][1]
-        rate = instr / N_SIMDS / (kernels[name]["avg_ms"] * 1e-3 * CLOCK_HZ)
-        out[name] = dict(instructions=round(instr), issue_per_simd_cycle=round(rate, 4),
-                         ceiling=VALU_ISSUE_CEILING, frac=round(rate / VALU_ISSUE_CEILING, 3))
+        rate = instr / N_SIMDS / (sec * CLOCK_HZ)
+        out[name] = dict(valu_instructions=round(instr), valu_issue_per_simd_cycle=round(rate, 4),
+                         fma_probe_rate=FMA_PROBE_RATE, vs_fma_probe=round(rate / FMA_PROBE_RATE, 3))
+        if key in PROFILED_LDS:
+            lb, lp, busy = PROFILED_LDS[key]
+            tbs = lb * pairs / lp / sec / 1e12
+            out[name].update(lds_tb_s=round(tbs, 2), lds_peak_tb_s=round(LDS_PEAK_TBS, 2),
+                             lds_frac=round(tbs / LDS_PEAK_TBS, 3), lds_busy_profiled=busy)
     return out
 
 
