@@ -109,7 +109,7 @@ PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (4.1179e9, 45_319_956),
 # 64-byte units) and the share of the kernel's cycles the LDS spends on indexed operations
 # (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
 PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((846562491.4 + 11953168.2) * 64, 45_298_241, 0.097),
-                ("blend_fwd_loss", 32, 512): ((1602792019.3 + 15178916.0) * 64, 45_298_241, 0.427)}
+                ("blend_fwd_loss", 32, 512): ((1602792019.3 + 15178916.0) * 64, 45_298_241, 0.417)}
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 128 B per clock per CU
 N_SIMDS, CLOCK_HZ = 1024, 2.4e9
 # Independent FP32 FMA chains at full occupancy issue 0.244 wave instructions per SIMD per cycle
