@@ -81,46 +81,52 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
   return WaveRect{x0, x1, y0, y1, 0.5f * (x0 + x1), 0.5f * (y0 + y1), 0.5f * (x1 - x0), 0.5f * (y1 - y0)};
 }
 
-// Staged splat (mean, conic a, 2b, c, opacity, colour) plus a conservative bound of the
-// region where alpha >= 1/255:  -0.5 x^T Q x >= -ln(255 o)  is the ellipse
-// x^T Q x <= 2 ln(255 o), whose half extents are sqrt(2 ln(255 o) Sigma_xx)
-// and sqrt(2 ln(255 o) Sigma_yy) with Sigma = Q^-1.  Extents get a 1% + 0.05 px
-// margin and the power threshold a 1e-3 margin, so a culled evaluation is
-// one that would have failed the alpha >= 1/255 test: images and gradients
-// are unchanged.  The fourth record holds what the per-wave test needs that
-// depends on the splat alone (computed once here, not once per wave): the
-// threshold R of that test with its float-error margin (+inf where the test
-// cannot decide: degenerate conics, kappa >= 6e4), the edge minimisers'
-// slopes -b/c and -b/a, and c.
-__device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c, float4& d) {
-  a = make_float4(p.a.x, p.a.y, p.b.x, 2.0f * p.b.y);  // 2b: exact, the blend's dx dy coefficient
-  b = make_float4(p.b.z, p.c.w, p.c.x, p.c.y);
-  const float o = p.c.w, qa = p.b.x, qb = p.b.y, qc = p.b.z;
+// Staged splat (mean, conic a, 2b, c, opacity, colour) plus a conservative
+// bound of the region where alpha >= 1/255:  -0.5 x^T Q x >= -ln(255 o) is
+// the ellipse x^T Q x <= R = 2 ln(255 o), whose half extents are
+// sqrt(R Sigma_xx) and sqrt(R Sigma_yy) with Sigma = Q^-1.  Extents get a
+// 1% + 0.05 px margin and the power threshold a 1e-3 margin, so a culled
+// evaluation is one that would have failed the alpha >= 1/255 test: images
+// and gradients are unchanged.  The margin covers the float evaluation of
+// the power for kappa < NTS_CULL_KAPPA_MAX (nts_cull_conic_ok); other
+// conics, and a NaN or infinite opacity, are not culled.  The fourth record
+// holds what the per-wave test needs that depends on the splat alone
+// (computed once here, not once per wave): the threshold of that test with
+// its float-error margin (+inf where nothing is culled), the edge
+// minimisers' slopes -b/c and -b/a, and c.
+__device__ __forceinline__ void stage_splat(float4 pa, float4 pb, float4 pc, float4& a, float4& b, float4& c,
+                                            float4& d) {
+  a = make_float4(pa.x, pa.y, pb.x, 2.0f * pb.y);  // 2b: exact, the blend's dx dy coefficient
+  b = make_float4(pb.z, pc.w, pc.x, pc.y);
+  const float o = pc.w, qa = pb.x, qb = pb.y, qc = pb.z;
   const float det = qa * qc - qb * qb;
-  float ex, ey, pmin, R = 3.4e38f, rbc = 0.0f, rba = 0.0f;
+  float ex = 3.4e38f, ey = 3.4e38f, pmin, R = __builtin_inff(), rbc = 0.0f, rba = 0.0f;
   if (o * 255.0f < 0.999f) {  // alpha = o G < 1/255 everywhere
     ex = ey = -3.4e38f;
     pmin = 3.4e38f;
   } else if (!(det > 0.0f)) {  // degenerate conic: no culling
-    ex = ey = 3.4e38f;
     pmin = -3.4e38f;
-    R = __builtin_inff();
   } else {
     const float l = logf(o * 255.0f);
-    const float r2 = fmaxf(2.0f * l, 0.0f);
-    ex = sqrtf(r2 * (qc / det)) * 1.01f + 0.05f;
-    ey = sqrtf(r2 * (qa / det)) * 1.01f + 0.05f;
     pmin = -l - 1e-3f;
-    const float s = sqrtf(qa * qc) + fabsf(qb);
-    const float kap = s * s / det;
-    R = (qa > 0.0f && qc > 0.0f && kap < 6.0e4f) ? -2.0f * pmin * (1.0f + 1e-5f + 4e-6f * kap)
-                                                  : __builtin_inff();
-    if (qc > 0.0f) rbc = -qb / qc;
-    if (qa > 0.0f) rba = -qb / qa;
+    if (l <= 3.4e38f && nts_cull_conic_ok(qa, qb, qc, det)) {  // not for a NaN or infinite opacity
+      const float r2 = fmaxf(2.0f * l, 0.0f);
+      ex = sqrtf(r2 * (qc / det)) * 1.01f + 0.05f;
+      ey = sqrtf(r2 * (qa / det)) * 1.01f + 0.05f;
+      const float s = sqrtf(qa * qc) + fabsf(qb);
+      const float kap = s * s / det;
+      R = -2.0f * pmin * (1.0f + 1e-5f + 4e-6f * kap);
+      rbc = -qb / qc;
+      rba = -qb / qa;
+    }
   }
-  c = make_float4(p.c.z, ex, ey, pmin);
+  c = make_float4(pc.z, ex, ey, pmin);
   d = make_float4(R, rbc, rba, qc);
 }
+__device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c, float4& d) {
+  stage_splat(p.a, p.b, p.c, a, b, c, d);
+}
+
 
 // Whether any pixel centre of the wave's rectangle can pass the blend's
 // power test  q(d) = a dx^2 + 2b dx dy + c dy^2 <= -2 pmin  for this
@@ -131,8 +137,9 @@ __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float
 // dx = -b V / a on the edge dy = V).  Every pixel's float offset lies in the
 // float rectangle [x0 - mx, x1 - mx] x [y0 - my, y1 - my], and the float
 // evaluation of q at any point is within 6u * kappa * q of the exact value,
-// kappa = (sqrt(ac) + |b|)^2 / det (u = 2^-24); the threshold carries four
-// times that margin, and splats with kappa >= 6e4 skip the exact test.  The
+// kappa = (sqrt(ac) + |b|)^2 / det (u = 2^-24); the threshold carries a
+// larger margin, 4e-6 kappa + 1e-5, and only splats with kappa below
+// NTS_CULL_KAPPA_MAX are culled at all.  The
 // rounding of the minimiser moves q by a second-order amount.  A rejected
 // splat therefore fails every pixel's power test: results are unchanged.
 // Both edges are evaluated without branches (a wave's lanes hold different
@@ -193,8 +200,12 @@ __device__ __forceinline__ float4 pixel_loss_grad(float4 p, float4 t, float inv_
                      d2 > 0.f ? inv_n : (d2 < 0.f ? -inv_n : 0.f), 0.0f);
 }
 
+// A 1024-thread workgroup is 4 waves per SIMD: two of them share a CU only
+// within 64 VGPRs (8 waves per SIMD), which the attribute holds the compiler
+// to; one per CU leaves half the SIMD slots idle at every barrier.
 template <int TS, int BATCH, bool LOSS>
-__global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(const Projected* __restrict__ proj,
+__global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT))
+__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 1))) k_blend_fwd(const Projected* __restrict__ proj,
                                                              const uint32_t* __restrict__ payloads,
                                                              const uint2* __restrict__ ranges, ViewTable vt,
                                                              float4* __restrict__ out, float* __restrict__ final_t,
@@ -263,7 +274,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT)) k_blend_fwd(
           if (!(power <= 0.0f) || power < Cc.w) continue;
           const float g = nts_exp_blend(power);
           const float ar = B.y * g;
-          const float alpha = __builtin_amdgcn_fmed3f(ar, 0.0f, NTS_ALPHA_MAX);  // min(ar, 0.99), ar >= 0
+          const float alpha = fminf(ar, NTS_ALPHA_MAX);  // min(alpha_raw, 0.99): a NaN gives 0.99
           if (alpha < NTS_ALPHA_MIN) continue;
           const float ta = T[j] * alpha;
           C0[j] = C0[j] + ta * B.z;
@@ -507,20 +518,26 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
           // exact exponential, so every alpha test decides as in the
           // forward.  Reference-parity mode keeps the exact exponential: its
           // 1.0001 bail on the divided-back T is sensitive to the last bit.
-          float g, ar;
+          // alpha = min(alpha_raw, 0.99), a NaN giving 0.99, taken next to each
+          // product (a product is canonical: the min needs no quieting first)
+          float g, ar, alpha;
           if (RECOMP) {
             g = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
             ar = sigma * g;
+            alpha = fminf(ar, NTS_ALPHA_MAX);
             if (fabsf(ar - NTS_ALPHA_MIN) <= 4e-6f * NTS_ALPHA_MIN || fabsf(ar - NTS_ALPHA_MAX) <= 4e-6f) {
               g = nts_exp_blend(power);
               ar = sigma * g;
+              alpha = fminf(ar, NTS_ALPHA_MAX);
             }
           } else {
             g = nts_exp_blend(power);
             ar = sigma * g;
+            alpha = fminf(ar, NTS_ALPHA_MAX);
           }
-          const float alpha = __builtin_amdgcn_fmed3f(ar, 0.0f, NTS_ALPHA_MAX);  // min(ar, 0.99), ar >= 0
-          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && power >= Cc.w && alpha >= NTS_ALPHA_MIN;
+          // the exact complement of the forward's skip tests (a NaN threshold
+          // or alpha must be kept here as it is there)
+          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && !(power < Cc.w) && !(alpha < NTS_ALPHA_MIN);
           if (RECOMP && j == 0) {
             // The first pixel of the quad without a branch: a lane that fails
             // the tests adds exact zeros and keeps T (acc needs no zeroing

@@ -216,13 +216,29 @@ NTS_HD float nts_box_muller(uint32_t* st) {
 // o exp(-x^T Q x / 2) < 1/255, so it fails the blend's alpha test: dropping
 // the other tiles' (splat, tile) pairs leaves every image unchanged.  Conic
 // Q = (qa, qb; qb, qc), o = opacity.  False when no tile remains (opacity
-// below 1/255 included); a conic that is not positive definite keeps the
-// reference rectangle.
+// below 1/255 included); a conic that is not positive definite, or too
+// ill-conditioned for the margin (nts_cull_conic_ok), and a NaN or infinite
+// opacity (whose alpha is not below 1/255 anywhere) keep the reference
+// rectangle.
+//
+// The margin: the float evaluation of q(d) = qa dx^2 + 2 qb dx dy + qc dy^2
+// (the blend's power) is within 6 u kappa q of the exact value, with
+// kappa = (sqrt(qa qc) + |qb|)^2 / det the largest ratio of
+// qa dx^2 + 2 |qb dx dy| + qc dy^2 to q (u = 2^-24).  A point outside a
+// half-width grown by 1% has q >= 1.0201 R; with kappa < 2e4 the evaluation
+// error (7.2e-3 R) and the rounding of det and of the half-widths (below
+// 8e-3 R together) cannot bring its float power back over the threshold.
+#define NTS_CULL_KAPPA_MAX 2.0e4f
+NTS_HD bool nts_cull_conic_ok(float qa, float qb, float qc, float det) {
+  if (!(det > 0.0f) || !(qa > 0.0f) || !(qc > 0.0f)) return false;
+  const float s = __builtin_sqrtf(qa * qc) + __builtin_fabsf(qb);
+  return s * s / det < NTS_CULL_KAPPA_MAX;
+}
 NTS_HD bool nts_alpha_tile_rect(float mx, float my, float qa, float qb, float qc, float o, float ts, int32_t* x0,
                                 int32_t* y0, int32_t* x1, int32_t* y1) {
-  if (!(o * 255.0f >= 0.999f)) return false;
+  if (o * 255.0f < 0.999f) return false;
   const float det = qa * qc - qb * qb;
-  if (!(det > 0.0f) || !(qa > 0.0f) || !(qc > 0.0f)) return true;
+  if (!nts_cull_conic_ok(qa, qb, qc, det) || !(o * 255.0f <= 3.4e38f)) return true;  // NaN opacity: kept
   const float l = nts_logf(o * 255.0f);
   const float r2 = nts_maxf(2.0f * l, 0.0f);
   const float ex = __builtin_sqrtf(r2 * (qc / det)) * 1.01f + 0.05f;

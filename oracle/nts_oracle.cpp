@@ -577,7 +577,7 @@ void orc_composite_backward_ref(const float* projected, const uint32_t* payloads
             const float g = nts_expf(power);
             const float sigma = p[11];
             const float ar = sigma * g;
-            const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
+            const float alpha = ar < NTS_ALPHA_MAX ? ar : NTS_ALPHA_MAX;  // a NaN clamps, as in the forward
             if (alpha < NTS_ALPHA_MIN) continue;
             const float oma = nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
             T = T / oma;
@@ -668,7 +668,7 @@ void orc_composite_backward_recompute(const float* projected, const uint32_t* pa
             const float g = nts_expf(power);
             const float sigma = p[11];
             const float ar = sigma * g;
-            const float alpha = nts_minf(ar, NTS_ALPHA_MAX);
+            const float alpha = ar < NTS_ALPHA_MAX ? ar : NTS_ALPHA_MAX;  // a NaN clamps, as in the forward
             if (alpha < NTS_ALPHA_MIN) continue;
             const float oma = 1.0f - alpha;
             T = T / oma;
@@ -1145,7 +1145,7 @@ double pixel_sq_err(const double* proj64, const uint32_t* payloads, uint32_t beg
     const double power = -0.5 * (p[4] * dx * dx + 2.0 * p[5] * dx * dy + p[6] * dy * dy);
     if (!(power <= 0.0)) continue;  // also a NaN power
     const double ar = p[11] * exp(power);
-    const bool clamped = ar >= (double)NTS_ALPHA_MAX;
+    const bool clamped = !(ar < (double)NTS_ALPHA_MAX);  // a NaN clamps, as in the forward
     const double alpha = clamped ? (double)NTS_ALPHA_MAX : ar;
     if (alpha < (double)NTS_ALPHA_MIN) continue;
     h = (h ^ ((uint64_t)i * 2 + (clamped ? 1 : 0))) * 1099511628211ull;

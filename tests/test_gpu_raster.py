@@ -282,3 +282,43 @@ def test_recompute_and_parity_agree_without_saturation(ctx):
     ga = a.download(np.float32, s.n * 12).reshape(s.n, 12)
     gb = b.download(np.float32, s.n * 12).reshape(s.n, 12)
     assert rel_norm_err(ga, gb) < 1e-4
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_edge_records_blend_bit_exact(ctx, tile):
+    """The blend's per-wave culling on records that stress its bounds
+    (needles across the kappa limit, opacities at 1/255, NaN and infinite
+    opacities, degenerate conics, off-screen means): image, final
+    transmittance and last contributor equal the oracle's, which culls
+    nothing (a NaN alpha clamps to 0.99 on both sides), and the recompute
+    backward matches it to the usual bar."""
+    from .helpers import edge_records
+    W, H = 192, 128
+    p = edge_records(3000, 0xED6E + tile, W, H)
+    buf = ctx.storage_buffer(p)
+    params = TilingParams(W, H, tile, 60.0)
+    bins = TileBinner().bin(ctx, buf, len(p), params)
+    r = Rasterizer()
+    img = r.alloc_image(ctx, W, H)
+    ft = ctx.storage_buffer_zeroed(W * H * 4)
+    li = ctx.storage_buffer_zeroed(W * H * 4)
+    r.composite_ex(ctx, buf, bins, img, params, ft, li)
+    t = O.tiling(W, H, tile, 60.0)
+    ok, opay, orr = O.bin(p, t)
+    oimg, oft, oli = O.composite(p, opay, orr, t, with_state=True)
+    gimg = img.download(np.float32, W * H * 4).reshape(H, W, 4)
+    assert np.array_equal(gimg.view(np.uint32), oimg.view(np.uint32))
+    assert np.array_equal(ft.download(np.float32, W * H).reshape(H, W).view(np.uint32), oft.view(np.uint32))
+    assert np.array_equal(li.download(np.uint32, W * H).reshape(H, W), oli)
+    rng = np.random.default_rng(29)
+    dl = rng.normal(0, 1e-3, (H, W, 4)).astype(np.float32)
+    dl[..., 3] = 0
+    pg = r.alloc_projected_grad(ctx, len(p))
+    r.composite_backward_recompute(ctx, buf, bins, ft, li, ctx.storage_buffer(dl), pg, params)
+    g = pg.download(np.float32, len(p) * 12).reshape(len(p), 12)
+    og = O.composite_backward_recompute(p, opay, orr, t, oft, oli, dl)
+    rows = np.isfinite(p[:, 11])  # a NaN or infinite opacity has NaN gradients of its own
+    assert np.isfinite(og[rows]).all()
+    for name, cols in (("dmean", [0, 1]), ("dopacity", [2]), ("dconic", [4, 5, 6]), ("dcolor", [8, 9, 10])):
+        err = rel_norm_err(g[rows][:, cols], og[rows][:, cols])
+        assert err < GRAD_TOL, f"{name}: relative error {err:.2e}"

@@ -56,3 +56,33 @@ def test_culled_binning_keeps_every_pixel(tile):
     assert np.array_equal(img0.view(np.uint32), img1.view(np.uint32))
     assert np.array_equal(ft0.view(np.uint32), ft1.view(np.uint32))
     assert np.array_equal(_last_splat(p0, r0, li0, tile, w, h), _last_splat(p1, r1, li1, tile, w, h))
+
+
+@pytest.mark.parametrize("tile", [8, 32])
+def test_culled_binning_edge_records(tile):
+    """Needles across the kappa limit, opacities at 1/255, NaN and infinite
+    opacities (a NaN alpha clamps to 0.99 in the blend, so such a splat is
+    drawn), degenerate conics: the culled binning still renders every pixel
+    exactly as the reference binning, keeps the reference tiles of every
+    record it may not cull, and culls some of the others."""
+    from .helpers import conic_kappa, edge_records
+    w, h = 192, 128
+    p = edge_records(3000, 0xED6E + tile, w, h)
+    t = O.tiling(w, h, tile, 60.0)
+    k0, p0, r0 = O.bin(p, t)
+    k1, p1, r1 = O.bin(p, t, cull=True)
+    assert 0 < len(k1) < len(k0)
+    img0, ft0, li0 = O.composite(p, p0, r0, t, with_state=True)
+    img1, ft1, li1 = O.composite(p, p1, r1, t, with_state=True)
+    assert np.array_equal(img0.view(np.uint32), img1.view(np.uint32))
+    assert np.array_equal(ft0.view(np.uint32), ft1.view(np.uint32))
+    assert np.array_equal(_last_splat(p0, r0, li0, tile, w, h), _last_splat(p1, r1, li1, tile, w, h))
+    c0 = np.bincount(p0, minlength=len(p))
+    c1 = np.bincount(p1, minlength=len(p))
+    kap = conic_kappa(p)
+    with np.errstate(invalid="ignore"):
+        keep = ~(kap < 2e4) | ~(p[:, 11] * np.float32(255.0) <= np.float32(3.4e38))
+    keep &= ~(p[:, 11] * np.float32(255.0) < np.float32(0.999))
+    assert keep.sum() >= 10
+    assert np.array_equal(c0[keep], c1[keep])
+    assert (c1[~keep] <= c0[~keep]).all() and (c1[~keep] < c0[~keep]).any()
