@@ -89,7 +89,9 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
 // evaluation is one that would have failed the alpha >= 1/255 test: images
 // and gradients are unchanged.  The margin covers the float evaluation of
 // the power for kappa < NTS_CULL_KAPPA_MAX (nts_cull_conic_ok); other
-// conics, and a NaN or infinite opacity, are not culled.  The fourth record
+// conics, and a NaN or infinite opacity, are not culled.  Records:
+// a = (mean, conic a, 2b), b = (conic c, opacity, red, green),
+// c = (blue, power threshold, box half extents x / y); the fourth record
 // holds what the per-wave test needs that depends on the splat alone
 // (computed once here, not once per wave): the threshold of that test with
 // its float-error margin (+inf where nothing is culled), the edge
@@ -120,7 +122,7 @@ __device__ __forceinline__ void stage_splat(float4 pa, float4 pb, float4 pc, flo
       rba = -qb / qa;
     }
   }
-  c = make_float4(pc.z, ex, ey, pmin);
+  c = make_float4(pc.z, pmin, ex, ey);
   d = make_float4(R, rbc, rba, qc);
 }
 __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c, float4& d) {
@@ -145,7 +147,7 @@ __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float
 // Both edges are evaluated without branches (a wave's lanes hold different
 // splats).
 __device__ __forceinline__ bool wave_hits(const WaveRect& r, float4 A, float4 Cc, float4 D) {
-  if (fabsf(r.cx - A.x) > r.hw + Cc.y || fabsf(r.cy - A.y) > r.hh + Cc.z) return false;
+  if (fabsf(r.cx - A.x) > r.hw + Cc.z || fabsf(r.cy - A.y) > r.hh + Cc.w) return false;
   const float a = A.z, b = 0.5f * A.w, c = D.w;
   const float u0 = r.x0 - A.x, u1 = r.x1 - A.x, v0 = r.y0 - A.y, v1 = r.y1 - A.y;
   const bool out_x = u0 > 0.0f || u1 < 0.0f, out_y = v0 > 0.0f || v1 < 0.0f;
@@ -214,7 +216,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
   constexpr int NT = BL::NT, PPT = BL::PPT, NM = BATCH / 64;
   __shared__ float4 s_a[BATCH];  // mean.x, mean.y, conic.a, conic.b
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
-  __shared__ float4 s_c[BATCH];  // b, cull half-extents x / y, power threshold
+  __shared__ float4 s_c[BATCH];  // b, power threshold, cull half-extents x / y
   __shared__ float4 s_d[BATCH];  // per-wave test: threshold, -b/c, -b/a, c
   const uint32_t gt = blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
@@ -266,12 +268,16 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
         const uint32_t k = (uint32_t)i * 64u + (uint32_t)__builtin_ctzll(m);
         m &= m - 1;
         const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+        // All of the record is read here, from one address register: left to
+        // itself the compiler sinks each read into the branch that uses it
+        // and rebuilds the address there (a move per read).
+        __asm__ volatile("" ::"v"(B.y), "v"(B.z), "v"(B.w), "v"(Cc.x));
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
           const float power = -0.5f * (A.z * dx * dx + A.w * dx * dy + B.x * dy * dy);
           // power > 0, a finished pixel (NaN), or below the alpha >= 1/255 bound
-          if (!(power <= 0.0f) || power < Cc.w) continue;
+          if (!(power <= 0.0f) || power < Cc.y) continue;
           const float g = nts_exp_blend(power);
           const float ar = B.y * g;
           const float alpha = fminf(ar, NTS_ALPHA_MAX);  // min(alpha_raw, 0.99): a NaN gives 0.99
@@ -282,7 +288,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
           C2[j] = C2[j] + ta * Cc.x;
           T[j] = T[j] * (1.0f - alpha);
           last[j] = base + k - begin + 1;
-          if (T[j] < NTS_T_MIN) pxf[j] = __builtin_nanf("");
+          pxf[j] = T[j] < NTS_T_MIN ? __builtin_nanf("") : pxf[j];
         }
       }
     }
@@ -537,7 +543,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
           }
           // the exact complement of the forward's skip tests (a NaN threshold
           // or alpha must be kept here as it is there)
-          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && !(power < Cc.w) && !(alpha < NTS_ALPHA_MIN);
+          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && !(power < Cc.y) && !(alpha < NTS_ALPHA_MIN);
           if (RECOMP && j == 0) {
             // The first pixel of the quad without a branch: a lane that fails
             // the tests adds exact zeros and keeps T (acc needs no zeroing
