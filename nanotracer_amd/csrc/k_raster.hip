@@ -508,9 +508,11 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
         //   2      sum dL/dalpha G                  -> dL/dopacity
         //   3-5    sum gpow dx^2, dx dy, dy^2       -> dL/dconic (-1/2, -1, -1/2)
         //   6-8    sum dL/dC_i T alpha              -> dL/dcolour
-        float acc[9];
+        float acc[9];  // recompute mode: set by the quad's first pixel
+        if (!RECOMP) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
+          for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
+        }
         bool contrib = false;
         const float sigma = B.y, c0 = B.z, c1 = B.w, c2 = Cc.x;
 #pragma unroll
@@ -546,8 +548,9 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
           const bool ok = rel0 + k < lastj[j] && power <= 0.0f && !(power < Cc.y) && !(alpha < NTS_ALPHA_MIN);
           if (RECOMP && j == 0) {
             // The first pixel of the quad without a branch: a lane that fails
-            // the tests adds exact zeros and keeps T (acc needs no zeroing
-            // on a path around it, which costs nine moves per join).
+            // the tests sets exact zeros and keeps T (acc needs no zeroing
+            // on a path around it, which costs nine moves per join, and
+            // starts from this pixel's terms rather than from zero).
             const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
             const float Tn = T[j] * r, Un = U[j] * r;
             const float gc = __builtin_fmaf(G2[j], c2, __builtin_fmaf(G1[j], c1, G0[j] * c0));
@@ -557,15 +560,15 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
             const float og = ok ? gar * g : 0.0f;
             const float ta = ok ? Tn * alpha : 0.0f;
             const float gx = gpow * dx, gy = gpow * dy;
-            acc[0] += gx;
-            acc[1] += gy;
-            acc[2] += og;
-            acc[3] = __builtin_fmaf(gx, dx, acc[3]);
-            acc[4] = __builtin_fmaf(gx, dy, acc[4]);
-            acc[5] = __builtin_fmaf(gy, dy, acc[5]);
-            acc[6] = __builtin_fmaf(G0[j], ta, acc[6]);
-            acc[7] = __builtin_fmaf(G1[j], ta, acc[7]);
-            acc[8] = __builtin_fmaf(G2[j], ta, acc[8]);
+            acc[0] = gx;
+            acc[1] = gy;
+            acc[2] = og;
+            acc[3] = gx * dx;
+            acc[4] = gx * dy;
+            acc[5] = gy * dy;
+            acc[6] = G0[j] * ta;
+            acc[7] = G1[j] * ta;
+            acc[8] = G2[j] * ta;
             U[j] = __builtin_fmaf(ta, gc, U[j]);
             T[j] = ok ? Tn : T[j];
             contrib = contrib || ok;

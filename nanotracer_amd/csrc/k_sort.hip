@@ -156,19 +156,40 @@ __global__ void k_sort_parts(const uint64_t* __restrict__ seg_begin, uint32_t nu
   part_begin[num_segs] = p;
 }
 
-// Digit histograms of every pass for every segment.  A fixed grid walks the
-// partitions (the live count is only known on the device), so a launch
-// sized for the capacity costs nothing for partitions that are empty.
+// Digit histograms of every pass for every segment.  A fixed grid takes
+// contiguous runs of partitions (the live count is only known on the device,
+// so a launch sized for the capacity costs nothing for empty partitions) and
+// keeps its counts in LDS until the segment changes: the global atomics are
+// one set per (block, segment) rather than one per partition.
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                         const uint32_t* __restrict__ part_begin, uint32_t num_segs,
                                                         uint32_t passes, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_hist[8 * RS_RADIX];
   const uint32_t total_parts = part_begin[num_segs];
-  for (uint32_t p = blockIdx.x; p < total_parts; p += gridDim.x) {
-    for (uint32_t i = threadIdx.x; i < passes * RS_RADIX; i += RS_BLOCK) s_hist[i] = 0;
+  const uint32_t per = (total_parts + gridDim.x - 1) / gridDim.x;
+  const uint32_t p0 = blockIdx.x * per;
+  const uint32_t p1 = p0 + per < total_parts ? p0 + per : total_parts;
+  if (p0 >= p1) return;
+  const uint32_t nbins = passes * RS_RADIX;
+  const auto flush = [&](uint32_t seg) {
     __syncthreads();
-    const uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p);
+    uint32_t* h = hist + (size_t)seg * nbins;
+    for (uint32_t i = threadIdx.x; i < nbins; i += RS_BLOCK) {
+      const uint32_t c = s_hist[i];
+      if (c) atomicAdd(&h[i], c);
+      s_hist[i] = 0;
+    }
+    __syncthreads();
+  };
+  for (uint32_t i = threadIdx.x; i < nbins; i += RS_BLOCK) s_hist[i] = 0;
+  __syncthreads();
+  uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p0);
+  for (uint32_t p = p0; p < p1; ++p) {
+    while (p >= part_begin[s + 1]) {  // next non-empty segment
+      flush(s);
+      ++s;
+    }
     const uint64_t base = seg_begin[s] + (uint64_t)(p - part_begin[s]) * RS_TILE;
     const uint64_t end = seg_begin[s + 1];
 #pragma unroll 4
@@ -180,14 +201,8 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
           atomicAdd(&s_hist[ps * RS_RADIX + (uint32_t)((key >> (8 * ps)) & 255u)], 1u);
       }
     }
-    __syncthreads();
-    uint32_t* h = hist + (size_t)s * passes * RS_RADIX;
-    for (uint32_t i = threadIdx.x; i < passes * RS_RADIX; i += RS_BLOCK) {
-      const uint32_t c = s_hist[i];
-      if (c) atomicAdd(&h[i], c);
-    }
-    __syncthreads();
   }
+  flush(s);
 }
 
 // Clears the look-back status words of the live partitions of every pass

@@ -64,7 +64,10 @@ def test_sort_random_u32_with_payload(ctx):
     assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
 
 
-@pytest.mark.parametrize("n,bits", [(5, 32), (4096, 32), (4097, 20), (1_000_003, 32), (2_500_000, 28)])
+# key widths that leave the top 8-bit digit partly empty, or exactly full
+@pytest.mark.parametrize("n,bits", [(5, 32), (4096, 32), (4097, 20), (1_000_003, 32), (2_500_000, 28), (4097, 25),
+                                    (1_000_003, 27), (300_000, 26), (70_001, 17), (1_000_003, 30), (5000, 29),
+                                    (600_000, 24), (4096, 16)])
 def test_sort_u32_stable_vs_oracle(ctx, n, bits):
     rng = np.random.default_rng(n + bits)
     keys = rng.integers(0, 1 << bits, n, dtype=np.uint64).astype(np.uint32)
