@@ -280,13 +280,18 @@ __global__ void __launch_bounds__(RS_BLOCK) k_onesweep(const K* __restrict__ key
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
       const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
-      uint64_t match = ~0ull;
+      // lanes with this lane's digit: the AND over the digit's bits of the
+      // bit's ballot, complemented where this lane's bit is clear, kept as
+      // two 32-bit halves (neg is the bit sign-extended: 0 or all ones)
+      uint32_t mlo = ~0u, mhi = ~0u;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        match &= bit ? bal : ~bal;
+        const uint32_t neg = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t bal = __ballot(neg != 0u);
+        mlo &= ~((uint32_t)bal ^ neg);
+        mhi &= ~((uint32_t)(bal >> 32) ^ neg);
       }
+      const uint64_t match = ((uint64_t)mhi << 32) | mlo;
       const uint32_t leader = (uint32_t)__ffsll((unsigned long long)match) - 1u;
       uint32_t old = 0;
       if (lane == leader) {
@@ -319,22 +324,43 @@ __global__ void __launch_bounds__(RS_BLOCK) k_onesweep(const K* __restrict__ key
     } else {
       __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_AGG | (tot & VALUE_MASK), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t j = p - 1;
+      // The words of up to LB_WIN predecessors are read together (one round
+      // trip to the device-coherent level per window, not per partition)
+      // and consumed in order until an inclusive one or the segment start.
+      constexpr uint32_t LB_WIN = 4;
+      uint32_t j = p - 1;  // next predecessor to consume, >= first_part
       uint32_t spins = 0;
       while (true) {
-        const uint32_t w = __hip_atomic_load(&st[(size_t)j * RS_RADIX + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t flag = w & FLAG_MASK;
-        if (flag == 0u) {
+        uint32_t w[LB_WIN];
+#pragma unroll
+        for (uint32_t q = 0; q < LB_WIN; ++q)
+          w[q] = (q <= j - first_part)
+                     ? __hip_atomic_load(&st[(size_t)(j - q) * RS_RADIX + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : 0u;
+        bool done = false, stalled = false;
+#pragma unroll
+        for (uint32_t q = 0; q < LB_WIN; ++q) {
+          if (done || stalled) continue;
+          const uint32_t flag = w[q] & FLAG_MASK;
+          if (flag == 0u) {  // not published yet: wait for this one
+            stalled = true;
+            continue;
+          }
+          prefix += w[q] & VALUE_MASK;
+          if (flag == FLAG_INC || j == first_part) {
+            done = true;
+            continue;
+          }
+          --j;
+        }
+        if (done) break;
+        if (stalled) {
           if (++spins > (1u << 22)) {
             atomicOr(err, 1u);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          continue;
         }
-        prefix += w & VALUE_MASK;
-        if (flag == FLAG_INC || j == first_part) break;
-        --j;
       }
       __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_INC | ((prefix + tot) & VALUE_MASK), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
