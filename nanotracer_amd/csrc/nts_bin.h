@@ -103,6 +103,10 @@ template <typename K>
 __global__ void __launch_bounds__(256) k_tile_ranges(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                      uint32_t num_segs, const uint32_t* __restrict__ tile_base,
                                                      uint2* __restrict__ ranges) {
+  // Each thread takes E consecutive keys (two 16-byte loads) and the key
+  // before them: a tile starts where its key differs from the previous one,
+  // which also ends the previous tile; a segment's last key ends its tile.
+  constexpr uint32_t E = 32 / sizeof(K);
   __shared__ uint64_t s_sb[NTS_MAX_VIEW_TILES + 1];
   __shared__ uint32_t s_tb[NTS_MAX_VIEW_TILES + 1];
   for (uint32_t k = threadIdx.x; k <= num_segs; k += blockDim.x) {
@@ -111,17 +115,38 @@ __global__ void __launch_bounds__(256) k_tile_ranges(const K* __restrict__ keys,
   }
   __syncthreads();
   const uint64_t total = s_sb[num_segs];
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * E;
   uint32_t s = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    while (s + 1 < num_segs && i >= s_sb[s + 1]) ++s;
-    const uint64_t sb = s_sb[s], se = s_sb[s + 1];
-    const K tile = keys[i] >> 16;
-    const bool starts = (i == sb) || ((keys[i - 1] >> 16) != tile);
-    const bool ends = (i + 1 == se) || ((keys[i + 1] >> 16) != tile);
-    const uint32_t tb = s_tb[s], tn = s_tb[s + 1] - tb;
-    if (tile >= (K)tn) continue;
-    if (starts) ranges[tb + (uint32_t)tile].x = (uint32_t)i;
-    if (ends) ranges[tb + (uint32_t)tile].y = (uint32_t)(i + 1);
+  for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * E; i0 < total; i0 += stride) {
+    K k[E + 1];  // k[e + 1] = keys[i0 + e], k[0] = keys[i0 - 1]
+    if (i0 + E <= total) {
+      const uint4 v0 = *reinterpret_cast<const uint4*>(keys + i0);
+      const uint4 v1 = *reinterpret_cast<const uint4*>(keys + i0 + E / 2);
+      const K* a = reinterpret_cast<const K*>(&v0);
+      const K* b = reinterpret_cast<const K*>(&v1);
+#pragma unroll
+      for (uint32_t e = 0; e < E / 2; ++e) {
+        k[e + 1] = a[e];
+        k[e + 1 + E / 2] = b[e];
+      }
+    } else {
+#pragma unroll
+      for (uint32_t e = 0; e < E; ++e) k[e + 1] = i0 + e < total ? keys[i0 + e] : (K)0;
+    }
+    k[0] = i0 > 0 ? keys[i0 - 1] : (K)0;
+#pragma unroll
+    for (uint32_t e = 0; e < E; ++e) {
+      const uint64_t i = i0 + e;
+      if (i >= total) break;
+      while (s + 1 < num_segs && i >= s_sb[s + 1]) ++s;
+      const uint64_t sb = s_sb[s], se = s_sb[s + 1];
+      const uint32_t tb = s_tb[s], tn = s_tb[s + 1] - tb;
+      const K tile = k[e + 1] >> 16, prev = k[e] >> 16;
+      const bool starts = i == sb || prev != tile;
+      if (starts && i > sb && prev < (K)tn) ranges[tb + (uint32_t)prev].y = (uint32_t)i;
+      if (tile >= (K)tn) continue;
+      if (starts) ranges[tb + (uint32_t)tile].x = (uint32_t)i;
+      if (i + 1 == se) ranges[tb + (uint32_t)tile].y = (uint32_t)(i + 1);
+    }
   }
 }
