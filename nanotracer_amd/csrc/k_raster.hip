@@ -90,16 +90,22 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
 // and gradients are unchanged.  The margin covers the float evaluation of
 // the power for kappa < NTS_CULL_KAPPA_MAX (nts_cull_conic_ok); other
 // conics, and a NaN or infinite opacity, are not culled.  Records:
-// a = (mean, conic a, 2b), b = (conic c, opacity, red, green),
+// a = (mean, -a/2, -b), b = (-c/2, opacity, red, green),
 // c = (blue, power threshold, box half extents x / y); the fourth record
 // holds what the per-wave test needs that depends on the splat alone
 // (computed once here, not once per wave): the threshold of that test with
-// its float-error margin (+inf where nothing is culled), the edge
-// minimisers' slopes -b/c and -b/a, and c.
+// its float-error margin (-inf where nothing is culled), the edge
+// minimisers' slopes -b/c and -b/a, and -c/2; the test's quadratic form and
+// threshold carry the power's factor -1/2 too.
 __device__ __forceinline__ void stage_splat(float4 pa, float4 pb, float4 pc, float4& a, float4& b, float4& c,
                                             float4& d) {
-  a = make_float4(pa.x, pa.y, pb.x, 2.0f * pb.y);  // 2b: exact, the blend's dx dy coefficient
-  b = make_float4(pb.z, pc.w, pc.x, pc.y);
+  // The conic is staged as the power's own coefficients: power =
+  // -a/2 dx^2 - b dx dy - c/2 dy^2 rounds at every step exactly as
+  // -0.5 (a dx^2 + 2b dx dy + c dy^2) does (scaling by a power of two
+  // commutes with rounding; where an intermediate is subnormal the power is
+  // within 2^-126 of zero and exp gives 1 either way), one multiply fewer.
+  a = make_float4(pa.x, pa.y, -0.5f * pb.x, -pb.y);
+  b = make_float4(-0.5f * pb.z, pc.w, pc.x, pc.y);
   const float o = pc.w, qa = pb.x, qb = pb.y, qc = pb.z;
   const float det = qa * qc - qb * qb;
   float ex = 3.4e38f, ey = 3.4e38f, pmin, R = __builtin_inff(), rbc = 0.0f, rba = 0.0f;
@@ -123,7 +129,7 @@ __device__ __forceinline__ void stage_splat(float4 pa, float4 pb, float4 pc, flo
     }
   }
   c = make_float4(pc.z, pmin, ex, ey);
-  d = make_float4(R, rbc, rba, qc);
+  d = make_float4(-0.5f * R, rbc, rba, -0.5f * qc);  // the test's threshold and c, scaled as the power
 }
 __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float4& b, float4& c, float4& d) {
   stage_splat(p.a, p.b, p.c, a, b, c, d);
@@ -148,17 +154,17 @@ __device__ __forceinline__ void stage_splat(const Projected& p, float4& a, float
 // splats).
 __device__ __forceinline__ bool wave_hits(const WaveRect& r, float4 A, float4 Cc, float4 D) {
   if (fabsf(r.cx - A.x) > r.hw + Cc.z || fabsf(r.cy - A.y) > r.hh + Cc.w) return false;
-  const float a = A.z, b = 0.5f * A.w, c = D.w;
+  // in the staged scale: -q/2 >= -R/2 (the same decision, bit for bit)
   const float u0 = r.x0 - A.x, u1 = r.x1 - A.x, v0 = r.y0 - A.y, v1 = r.y1 - A.y;
   const bool out_x = u0 > 0.0f || u1 < 0.0f, out_y = v0 > 0.0f || v1 < 0.0f;
   const float U = u0 > 0.0f ? u0 : u1;
   const float dy = __builtin_amdgcn_fmed3f(U * D.y, v0, v1);
-  const float qx = a * U * U + 2.0f * b * U * dy + c * dy * dy;
+  const float qx = A.z * U * U + A.w * U * dy + D.w * dy * dy;
   const float V = v0 > 0.0f ? v0 : v1;
   const float dx = __builtin_amdgcn_fmed3f(V * D.z, u0, u1);
-  const float qy = a * dx * dx + 2.0f * b * dx * V + c * V * V;
-  const float q = fminf(out_x ? qx : 3.4e38f, out_y ? qy : 3.4e38f);
-  return (!out_x && !out_y) || q <= D.x;
+  const float qy = A.z * dx * dx + A.w * dx * V + D.w * V * V;
+  const float q = fmaxf(out_x ? qx : -3.4e38f, out_y ? qy : -3.4e38f);
+  return (!out_x && !out_y) || q >= D.x;
 }
 
 // Per-wave hit masks of the staged batch: bit b of mk[i] is set when slot
@@ -275,7 +281,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
-          const float power = -0.5f * (A.z * dx * dx + A.w * dx * dy + B.x * dy * dy);
+          const float power = A.z * dx * dx + A.w * dx * dy + B.x * dy * dy;  // the staged -q/2
           // power > 0, a finished pixel (NaN), or below the alpha >= 1/255 bound
           if (!(power <= 0.0f) || power < Cc.y) continue;
           const float g = nts_exp_blend(power);
@@ -519,7 +525,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
         for (int j = 0; j < PPT; ++j) {
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
           // the forward's power and thresholds, bit for bit
-          const float power = -0.5f * (A.z * dx * dx + A.w * dx * dy + B.x * dy * dy);
+          const float power = A.z * dx * dx + A.w * dx * dy + B.x * dy * dy;  // the staged -q/2
           // alpha for every lane, then one branch on all the forward's tests
           // recompute mode: hardware 2^x (error well under 1e-6 relative),
           // and where alpha lies that close to 1/255 or 0.99 the forward's
@@ -639,8 +645,8 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
         }
       if (q < 2u) {
         const float4 A = s_a[k];
-        const float cb = 0.5f * A.w;
-        sum = q == 0u ? __builtin_fmaf(A.z, sum, cb * sum2) : __builtin_fmaf(cb, sum, s_b[k].x * sum2);
+        const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * s_b[k].x;  // the conic, from the staged -q/2
+        sum = q == 0u ? __builtin_fmaf(qa, sum, qb * sum2) : __builtin_fmaf(qb, sum, qc * sum2);
       } else if (q == 3u || q == 5u) {
         sum = -0.5f * sum;
       } else if (q == 4u) {
