@@ -9,11 +9,12 @@
 //
 // nts_expf: Cody & Waite (1980) argument reduction x = n*ln2 + r with a
 // two-word ln2 (the high word has 9 significant bits, so n*ln2_hi is exact
-// for |n| <= 150), then the degree-7 Taylor polynomial of e^r on
-// |r| <= ln2/2 in Horner form with fused multiply-adds (truncation error
-// 5e-9 relative, below half an ulp), then scaling by 2^n built from the
-// exponent field.  Accuracy: <= 1 ulp over the normal range (checked against
-// a float64 exp in the test suite).
+// for |n| <= 150), then e^r on |r| <= ln2/2 as 1 + r + r^2 Q(r), Q the
+// degree-4 minimax fit of (e^r - 1 - r) / r^2 in relative error (3.8e-9 with
+// the float coefficients below, under 0.07 ulp), in Horner form with fused
+// multiply-adds, then scaling by 2^n built from the exponent field.
+// Accuracy: at most 0.90 ulp over every float argument with a normal result
+// (checked exhaustively against a float64 exp; the test suite samples it).
 #pragma once
 
 #include <stdint.h>
@@ -34,6 +35,17 @@ NTS_HD float nts_fmaf(float a, float b, float c) { return __builtin_fmaf(a, b, c
 // 2^k for k in [-126, 127] as an exact float.
 NTS_HD float nts_pow2i(int k) { return nts_bits_to_f32((uint32_t)(k + 127) << 23); }
 
+// e^r for the reduced argument |r| <= ln2/2: six fused multiply-adds.
+NTS_HD float nts_exp_poly(float r) {
+  float p = 0.0013814572f;
+  p = nts_fmaf(p, r, 0.008368721f);
+  p = nts_fmaf(p, r, 0.04166839f);
+  p = nts_fmaf(p, r, 0.16666521f);
+  p = nts_fmaf(p, r, 0.49999994f);
+  p = nts_fmaf(p, r, 1.0f);
+  return nts_fmaf(p, r, 1.0f);
+}
+
 NTS_HD float nts_expf(float x) {
   if (!(x > -104.0f)) {            // deep underflow (and NaN)
     return (x != x) ? x : 0.0f;
@@ -42,14 +54,7 @@ NTS_HD float nts_expf(float x) {
   const float n = __builtin_rintf(x * 1.44269504088896341f);
   float r = nts_fmaf(n, -0.693359375f, x);          // ln2_hi = 355/512
   r = nts_fmaf(n, 2.12194440e-4f, r);               // ln2 - ln2_hi
-  float p = 1.98412698e-4f;                         // 1/7!
-  p = nts_fmaf(p, r, 1.38888889e-3f);               // 1/6!
-  p = nts_fmaf(p, r, 8.33333333e-3f);               // 1/5!
-  p = nts_fmaf(p, r, 4.16666667e-2f);               // 1/4!
-  p = nts_fmaf(p, r, 1.66666667e-1f);               // 1/3!
-  p = nts_fmaf(p, r, 0.5f);
-  p = nts_fmaf(p, r, 1.0f);
-  p = nts_fmaf(p, r, 1.0f);
+  float p = nts_exp_poly(r);
   int k = (int)n;
   if (k < -125) {                                   // result near/below FLT_MIN
     p = p * nts_pow2i(-64);

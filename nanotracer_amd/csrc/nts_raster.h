@@ -51,14 +51,7 @@ __device__ __forceinline__ float nts_expf_nonpos(float x) {
   const float n = __builtin_rintf(x * 1.44269504088896341f);
   float r = nts_fmaf(n, -0.693359375f, x);
   r = nts_fmaf(n, 2.12194440e-4f, r);
-  float p = 1.98412698e-4f;
-  p = nts_fmaf(p, r, 1.38888889e-3f);
-  p = nts_fmaf(p, r, 8.33333333e-3f);
-  p = nts_fmaf(p, r, 4.16666667e-2f);
-  p = nts_fmaf(p, r, 1.66666667e-1f);
-  p = nts_fmaf(p, r, 0.5f);
-  p = nts_fmaf(p, r, 1.0f);
-  p = nts_fmaf(p, r, 1.0f);
+  float p = nts_exp_poly(r);
   int k = (int)n;
   if (k < -125) {
     p = p * nts_pow2i(-64);
@@ -78,15 +71,7 @@ __device__ __forceinline__ float nts_exp_blend(float x) {
   const float n = __builtin_rintf(x * 1.44269504088896341f);
   float r = nts_fmaf(n, -0.693359375f, x);
   r = nts_fmaf(n, 2.12194440e-4f, r);
-  float p = 1.98412698e-4f;
-  p = nts_fmaf(p, r, 1.38888889e-3f);
-  p = nts_fmaf(p, r, 8.33333333e-3f);
-  p = nts_fmaf(p, r, 4.16666667e-2f);
-  p = nts_fmaf(p, r, 1.66666667e-1f);
-  p = nts_fmaf(p, r, 0.5f);
-  p = nts_fmaf(p, r, 1.0f);
-  p = nts_fmaf(p, r, 1.0f);
-  return __builtin_ldexpf(p, (int)n);
+  return __builtin_ldexpf(nts_exp_poly(r), (int)n);
 }
 
 // Splats staged through LDS per batch when the caller does not choose.
