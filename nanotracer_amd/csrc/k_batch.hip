@@ -163,6 +163,7 @@ struct AdamCfgs {
 // their Adam step as soon as their sums are complete), then the position,
 // rotation, scale and opacity planes: no more than 23 parameters and sums
 // are live across a view loop, so several waves share each SIMD.
+#define ADAM_LDS_VIEWS 16
 template <bool FULL>
 __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__ P, float* __restrict__ G,
                                                              float* __restrict__ M, float* __restrict__ V,
@@ -204,6 +205,11 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
 #pragma unroll
   for (int k = 0; k < 11; ++k) gq[k] = 0.0f;
   uint32_t kept = 0;  // views that keep the splat and passed it a gradient
+  // Up to ADAM_LDS_VIEWS views, the colour terms read here are kept in LDS
+  // for the colour pass (each thread reads back only its own words), which
+  // otherwise reads them again once per channel.
+  __shared__ float s_col[3][ADAM_LDS_VIEWS][256];
+  const bool stash = n_views <= ADAM_LDS_VIEWS;
   {
     ProjectedGrad next = pgrad[i];  // the next view's record loads while this one is chained back
     for (uint32_t v = 0; v < n_views; ++v) {
@@ -212,7 +218,14 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
       if (pg.a.x == 0.0f && pg.a.y == 0.0f && pg.a.z == 0.0f && pg.b.x == 0.0f && pg.b.y == 0.0f &&
           pg.b.z == 0.0f && pg.c.x == 0.0f && pg.c.y == 0.0f && pg.c.z == 0.0f)
         continue;  // the view gave this splat no gradient: its terms are all zero
-      if (project_backward_geo<FULL>(q, cams[v], pg.a, pg.b, gq)) kept |= 1u << v;
+      if (project_backward_geo<FULL>(q, cams[v], pg.a, pg.b, gq)) {
+        kept |= 1u << v;
+        if (stash) {
+          s_col[0][v][threadIdx.x] = pg.c.x;
+          s_col[1][v][threadIdx.x] = pg.c.y;
+          s_col[2][v][threadIdx.x] = pg.c.z;
+        }
+      }
     }
   }
 
@@ -230,8 +243,13 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
     float gdc = 0.0f;
     for (uint32_t m = kept; m; m &= m - 1) {
       const uint32_t v = (uint32_t)__builtin_ctz(m);
-      const float4 pc = pgrad[(size_t)v * n + i].c;
-      const float dcol = ch == 0 ? pc.x : (ch == 1 ? pc.y : pc.z);
+      float dcol;
+      if (stash) {
+        dcol = s_col[ch][v][threadIdx.x];
+      } else {
+        const float4 pc = pgrad[(size_t)v * n + i].c;
+        dcol = ch == 0 ? pc.x : (ch == 1 ? pc.y : pc.z);
+      }
       project_backward_colour<FULL>(q[0], q[1], q[2], dc, rest, cams[v], dcol, gdc, grest, psh);
     }
     P[(size_t)(P_DC + ch) * c + i] = step(P_DC + ch, dc, gdc);
