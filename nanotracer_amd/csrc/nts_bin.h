@@ -48,8 +48,11 @@ __device__ __forceinline__ uint32_t rec_view(const RecTable& rt, uint32_t f) {
 // last record whose first pair is <= j (binary search in LDS), so every
 // store is coalesced whatever the footprints.  offsets is the exclusive
 // scan of the per-record tile counts; *total the pair total.
-// CULL: only the tiles the record's alpha >= 1/255 ellipse can reach
+// CULL: only the tiles the record's alpha >= 1/255 box can reach
 // (record_tiles), in the same order; the counts must come from the same rule.
+// A pair of such a tile that the alpha >= 1/255 region itself cannot reach
+// (nts_tile_cull_hits) is written with the all-ones key: it sorts after the
+// view's live pairs and lies beyond every tile, so no tile range holds it.
 template <typename K, bool CULL = false>
 __global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restrict__ proj, RecTable rt,
                                                        TileTable tt, const uint32_t* __restrict__ offsets,
@@ -57,6 +60,9 @@ __global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restri
                                                        K* __restrict__ keys, uint32_t* __restrict__ payloads) {
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_tile0[256], s_bw[256], s_tx[256], s_d[256], s_pay[256];
+  __shared__ NtsTileCull s_cull[CULL ? 256 : 1];
+  __shared__ int32_t s_x0[CULL ? 256 : 1], s_y0[CULL ? 256 : 1];
+  __shared__ float s_ts[CULL ? 256 : 1];
   const uint32_t t = threadIdx.x;
   const uint32_t nrec = rt.base[rt.n];
   const uint32_t f0 = blockIdx.x * 256u, f = f0 + t;
@@ -73,6 +79,12 @@ __global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restri
     s_tx[t] = vt.tiles_x;
     s_d[t] = depth16(p.a.z, vt.depth_max);
     s_pay[t] = i;
+    if (CULL) {
+      s_cull[t] = nts_tile_cull_setup(p.a.x, p.a.y, p.b.x, p.b.y, p.b.z, p.c.w);
+      s_x0[t] = x0;
+      s_y0[t] = y0;
+      s_ts[t] = (float)vt.tile_size;
+    }
   }
   s_off[t] = off;
   if (t == 0) s_off[256] = f0 + 256u < nrec ? offsets[f0 + 256u] : (uint32_t)*total;
@@ -86,10 +98,16 @@ __global__ void __launch_bounds__(256) k_emit_balanced(const Projected* __restri
       if (s_off[mid] <= j) lo = mid; else hi = mid;
     }
     const uint32_t r = j - s_off[lo], bw = s_bw[lo];
-    const uint32_t row = r / bw;
-    const uint32_t tile = s_tile0[lo] + row * s_tx[lo] + (r - row * bw);
+    const uint32_t row = r / bw, col = r - row * bw;
+    const uint32_t tile = s_tile0[lo] + row * s_tx[lo] + col;
+    bool live = true;
+    if (CULL) {
+      float x0, x1, y0, y1;
+      nts_tile_centres(s_x0[lo] + (int32_t)col, s_y0[lo] + (int32_t)row, s_ts[lo], &x0, &x1, &y0, &y1);
+      live = nts_tile_cull_hits(s_cull[lo], x0, x1, y0, y1);
+    }
     if ((uint64_t)j < capacity) {
-      keys[j] = ((K)tile << 16) | (K)s_d[lo];
+      keys[j] = live ? (((K)tile << 16) | (K)s_d[lo]) : ~(K)0;
       payloads[j] = s_pay[lo];
     }
   }

@@ -254,3 +254,52 @@ NTS_HD bool nts_alpha_tile_rect(float mx, float my, float qa, float qb, float qc
   *y1 = nts_mini(*y1, nts_f2i_sat(__builtin_floorf(nts_div_tile(my + ey, ts))));
   return !(*x1 < *x0 || *y1 < *y0);
 }
+
+// Culled binning, per (splat, tile): whether the splat's alpha >= 1/255
+// region can reach a pixel centre of the tile's rectangle
+// [x0, x1] x [y0, y1].  The test is the exact minimum of
+// q(d) = qa dx^2 + 2 qb dx dy + qc dy^2 over the rectangle. It is 0 when
+// the mean lies inside; otherwise the minimum is a clamped 1-D minimum on
+// an edge facing the mean, at dy = -qb U / qc on dx = U and dx = -qb V / qa
+// on dy = V. The result is compared with R = 2 ln(255 o) + 2e-3, which
+// carries a float-error margin of 4e-6 kappa + 1e-5 (the float evaluation
+// of q is within 6 u kappa q, u = 2^-24).  A failing pair
+// fails every pixel's alpha test.  Set up once per record; a splat the
+// alpha box keeps whole (nts_alpha_tile_rect: degenerate or ill-conditioned
+// conic, NaN or infinite opacity) gets R = +inf and keeps every tile.
+struct NtsTileCull {
+  float mx, my, qa, qb2, qc, R, rbc, rba;
+};
+NTS_HD NtsTileCull nts_tile_cull_setup(float mx, float my, float qa, float qb, float qc, float o) {
+  NtsTileCull c{mx, my, qa, 2.0f * qb, qc, __builtin_inff(), 0.0f, 0.0f};
+  const float det = qa * qc - qb * qb;
+  if (!(o * 255.0f >= 0.999f) || !(o * 255.0f <= 3.4e38f) || !nts_cull_conic_ok(qa, qb, qc, det)) return c;
+  const float pmin = -nts_logf(o * 255.0f) - 1e-3f;
+  const float s = __builtin_sqrtf(qa * qc) + __builtin_fabsf(qb);
+  const float kap = s * s / det;
+  c.R = -2.0f * pmin * (1.0f + 1e-5f + 4e-6f * kap);
+  c.rbc = -qb / qc;
+  c.rba = -qb / qa;
+  return c;
+}
+NTS_HD bool nts_tile_cull_hits(const NtsTileCull& c, float x0, float x1, float y0, float y1) {
+  const float u0 = x0 - c.mx, u1 = x1 - c.mx, v0 = y0 - c.my, v1 = y1 - c.my;
+  const bool out_x = u0 > 0.0f || u1 < 0.0f, out_y = v0 > 0.0f || v1 < 0.0f;
+  if (!out_x && !out_y) return true;
+  const float U = u0 > 0.0f ? u0 : u1;
+  const float dy = nts_minf(nts_maxf(U * c.rbc, v0), v1);
+  const float qx = c.qa * U * U + c.qb2 * U * dy + c.qc * dy * dy;
+  const float V = v0 > 0.0f ? v0 : v1;
+  const float dx = nts_minf(nts_maxf(V * c.rba, u0), u1);
+  const float qy = c.qa * dx * dx + c.qb2 * dx * V + c.qc * V * V;
+  const float q = nts_minf(out_x ? qx : 3.4e38f, out_y ? qy : 3.4e38f);
+  return !(q > c.R);
+}
+
+// The pixel-centre rectangle of tile (tx, ty) of size ts (exact in float).
+NTS_HD void nts_tile_centres(int32_t tx, int32_t ty, float ts, float* x0, float* x1, float* y0, float* y1) {
+  *x0 = (float)tx * ts + 0.5f;
+  *x1 = *x0 + (ts - 1.0f);
+  *y0 = (float)ty * ts + 0.5f;
+  *y1 = *y0 + (ts - 1.0f);
+}

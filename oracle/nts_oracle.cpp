@@ -441,11 +441,20 @@ static void tile_emit_ex(const float* projected, uint32_t n, const uint32_t* off
     float dn = p[2] / t->depth_max;
     dn = dn < 0.0f ? 0.0f : (dn > 1.0f ? 1.0f : dn);
     const uint32_t d16 = (uint32_t)(dn * 65535.0f);
+    // with cull, a tile the alpha >= 1/255 region cannot reach gets the
+    // all-ones key (after every live pair of the view, in no tile's range)
+    const NtsTileCull tc = nts_tile_cull_setup(p[0], p[1], p[4], p[5], p[6], p[11]);
     uint32_t w = offsets[i];
     for (int32_t ty = y0; ty <= y1; ++ty)
       for (int32_t tx = x0; tx <= x1; ++tx) {
         const uint32_t tile = (uint32_t)ty * tiles_x + (uint32_t)tx;
-        keys[w] = (tile << 16) | d16;
+        bool live = true;
+        if (cull) {
+          float cx0, cx1, cy0, cy1;
+          nts_tile_centres(tx, ty, (float)t->tile_size, &cx0, &cx1, &cy0, &cy1);
+          live = nts_tile_cull_hits(tc, cx0, cx1, cy0, cy1);
+        }
+        keys[w] = live ? ((tile << 16) | d16) : 0xFFFFFFFFu;
         payloads[w] = (uint32_t)i;
         ++w;
       }

@@ -68,7 +68,8 @@ def test_render_views_bit_exact_per_view(ctx, tile):
         assert np.array_equal(keys, ok) and np.array_equal(pay, opay) and np.array_equal(rng, orr)
         # the culled pairs are reference pairs, in the reference's order
         rk, rp, rr = O.bin(op, t)
-        assert len(ok) < len(rk) and _is_subsequence(ok, opay, rk, rp)
+        live = ok != 0xFFFFFFFF  # pairs of tiles the alpha >= 1/255 region misses close the list
+        assert live.sum() < len(rk) and _is_subsequence(ok[live], opay[live], rk, rp)
         img, ft, li = O.composite(op, opay, orr, t, with_state=True)
         assert np.array_equal(img.view(np.uint32), O.composite(op, rp, rr, t).view(np.uint32))
         gimg = b.image(k)
@@ -219,7 +220,8 @@ def test_single_view_batch_equals_plain_path(ctx):
     b.render(g, [spec], 16)
     assert np.array_equal(b.image(0).view(np.uint32), plain.view(np.uint32))
     keys, pay, rng = b.bins(0, 16)
-    assert _is_subsequence(keys, pay, bins.sorted_keys(), bins.sorted_payloads())
+    live = keys != 0xFFFFFFFF
+    assert _is_subsequence(keys[live], pay[live], bins.sorted_keys(), bins.sorted_payloads())
     b.free()
 
 

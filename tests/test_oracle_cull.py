@@ -1,8 +1,10 @@
-"""The batched and serving paths bin only the tiles a splat's alpha >= 1/255
-ellipse can reach (nts_alpha_tile_rect).  On the host restatement: the
-culled pairs are reference pairs in the reference's order, nothing with
-opacity below 1/255 is binned, and the image, final transmittance and last
-contributing splat of every pixel equal those of the reference binning."""
+"""The batched and serving paths bin only the tiles of a splat's alpha >=
+1/255 box (nts_alpha_tile_rect), and mark the pairs of those tiles its
+alpha >= 1/255 region cannot reach (nts_tile_cull_hits) with the all-ones
+key.  On the host restatement: the live culled pairs are reference pairs in
+the reference's order, nothing with opacity below 1/255 is binned, and the
+image, final transmittance and last contributing splat of every pixel equal
+those of the reference binning."""
 import math
 
 import numpy as np
@@ -42,11 +44,17 @@ def test_culled_binning_keeps_every_pixel(tile):
     k0, p0, r0 = O.bin(p, t)
     k1, p1, r1 = O.bin(p, t, cull=True)
     assert 0 < len(k1) < len(k0)
-    # order-preserving subset of the reference pairs
+    # the pairs of tiles the alpha >= 1/255 region cannot reach carry the
+    # all-ones key: they close the list, and no tile range reaches them
+    live = k1 != 0xFFFFFFFF
+    nl = int(live.sum())
+    assert 0 < nl < len(k1) and live[:nl].all() and not live[nl:].any()
+    assert r1.reshape(-1, 2)[:, 1].max() == nl
+    # the live pairs: an order-preserving subset of the reference pairs
     pos = {}
     for i, pair in enumerate(zip(k0.tolist(), p0.tolist())):
         pos[pair] = i
-    idx = np.array([pos[pair] for pair in zip(k1.tolist(), p1.tolist())])
+    idx = np.array([pos[pair] for pair in zip(k1[:nl].tolist(), p1[:nl].tolist())])
     assert np.all(np.diff(idx) > 0)
     # splats that cannot reach alpha 1/255 anywhere are not binned
     faint = np.nonzero(p[:, 11] * np.float32(255.0) < np.float32(0.999))[0]
@@ -78,7 +86,7 @@ def test_culled_binning_edge_records(tile):
     assert np.array_equal(ft0.view(np.uint32), ft1.view(np.uint32))
     assert np.array_equal(_last_splat(p0, r0, li0, tile, w, h), _last_splat(p1, r1, li1, tile, w, h))
     c0 = np.bincount(p0, minlength=len(p))
-    c1 = np.bincount(p1, minlength=len(p))
+    c1 = np.bincount(p1[k1 != 0xFFFFFFFF], minlength=len(p))  # live pairs
     kap = conic_kappa(p)
     with np.errstate(invalid="ignore"):
         keep = ~(kap < 2e4) | ~(p[:, 11] * np.float32(255.0) <= np.float32(3.4e38))
