@@ -161,8 +161,9 @@ def valu_of(kernels, cfg, pairs):
     return out
 
 
-def render_latency(ctx, g, n, tile):
-    """Single-view 1920x1080 forward (project, bin, blend) latency."""
+def render_latency(ctx, g, n, tile, lds=None):
+    """Single-view 1920x1080 forward (project, bin, blend) latency, at the
+    training step's tile and (when given) its forward LDS batch."""
     from nanotracer_amd import synth
     from nanotracer_amd.api import CameraUniform, ViewBatch, ViewSpec
 
@@ -170,6 +171,8 @@ def render_latency(ctx, g, n, tile):
     eye = synth.orbit_eye(6.0, 2.0 * math.pi * float(u[0]), 0.3)
     spec = ViewSpec(CameraUniform.from_pose(eye, (0, 0, 0), (0, 1, 0), math.pi / 3, 1920, 1080, n), 1920, 1080, 12.0)
     b = ViewBatch(ctx, 1, n, 1920 * 1080, 20_000_000)
+    if lds is not None:
+        b.set_lds_batch(*lds)
     lat = []
     for _ in range(40):
         ctx.sync()
@@ -378,7 +381,7 @@ def run_batched(args, ctx):
         views_per_s=round(iters_s * N_VIEWS, 2), train_mpx_s=round(iters_s * pixels / 1e6, 2),
         final_loss=float(losses.sum()),
     )
-    out.update(render_latency(ctx, g, s.n, tile))
+    out.update(render_latency(ctx, g, s.n, tile, lds))
     out["roofline"] = roofline_of(kernels, (tile, lds[0], lds[1]), pairs)
     out["valu"] = valu_of(kernels, (tile, lds[0], lds[1]), pairs)
     if ssim:
