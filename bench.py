@@ -237,15 +237,16 @@ def serving_scenes(seed):
         yield synth.HostSplats(*arrs)
 
 
-def run_serving(rate, duration, seed=SEED + 5):
+def run_serving(rate, duration, seed=SEED + 5, tile=32, lds=512):
     """Poisson arrivals at `rate` req/s for `duration` s, each one camera of a
     random resident scene at a random resolution; latency = arrival to the
-    tonemapped image in host memory."""
+    tonemapped image in host memory.  The server renders at the given tile
+    size and forward LDS batch (the training step's tuned ones)."""
     from nanotracer_amd import synth
     from nanotracer_amd.api import CameraUniform, RenderServer, ViewSpec
 
     srv = RenderServer(slots=3, max_views=32, max_pixels=48 << 20, max_records=64 << 20,
-                       pair_capacity=600_000_000, tile_size=16, lds_batch=128)
+                       pair_capacity=600_000_000, tile_size=tile, lds_batch=lds)
     sizes = []
     t_setup = time.perf_counter()
     for sc in serving_scenes(seed):
@@ -391,7 +392,7 @@ def run_batched(args, ctx):
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
                               gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
     if args.serve_rate > 0:
-        out["serving"] = run_serving(args.serve_rate, args.serve_seconds)
+        out["serving"] = run_serving(args.serve_rate, args.serve_seconds, tile=tile, lds=lds[0])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_batched(s, views, targets, tile)
     return out

@@ -561,7 +561,7 @@ class RenderServer:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 2, max_views: int = 32,
                  max_pixels: int = 16 << 20, max_records: int = 16 << 20, pair_capacity: int = 120_000_000,
-                 tile_size: int = 16, lds_batch: int = 128):
+                 tile_size: int = 32, lds_batch: int = 512):
         cfg = L.ServerCfg(int(hbm_budget), int(slots), int(max_views), int(max_pixels), int(max_records),
                           int(pair_capacity), int(tile_size), int(lds_batch))
         h = C.c_void_p()

@@ -58,16 +58,20 @@ def test_mixed_batches_equal_solo_renders(ctx, fmt):
     srv.close()
 
 
-def test_float_results_match_the_host_forward(ctx):
+@pytest.mark.parametrize("tile,lds", [(16, 128), (32, 512)])
+def test_float_results_match_the_host_forward(ctx, tile, lds):
+    # images depend on the tile size (the reference bins by tile rectangles):
+    # the server and the oracle use the same one
     scenes = _scenes()
-    srv = RenderServer(slots=3, max_views=6, max_pixels=1 << 20, max_records=200_000, pair_capacity=4_000_000)
+    srv = RenderServer(slots=3, max_views=6, max_pixels=1 << 20, max_records=200_000, pair_capacity=4_000_000,
+                       tile_size=tile, lds_batch=lds)
     for s in scenes:
         srv.add_scene(s)
     reqs = _requests(scenes, 12, 2)
     imgs, _ = _serve(srv, reqs, L.OUT_F32)
     for (sid, v), img in zip(reqs, imgs):
         ocam = O.camera_like(v.camera)
-        t = O.tiling(v.width, v.height, 16, v.depth_max)
+        t = O.tiling(v.width, v.height, tile, v.depth_max)
         op = O.project(scenes[sid], ocam)
         k, pay, rg = O.bin(op, t)
         ref = O.composite(op, pay, rg, t)
