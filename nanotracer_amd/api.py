@@ -496,7 +496,7 @@ class ViewBatch:
                                              C.byref(cfg_pos.c()), C.byref(cfg_attr.c()), int(t), C.byref(cons),
                                              flags))
 
-    def set_lds_batch(self, fwd_batch: int = 256, bwd_batch: int = 256) -> None:
+    def set_lds_batch(self, fwd_batch: int = 256, bwd_batch: int = 128) -> None:
         """Splats staged per LDS batch by the blend forward / backward."""
         L.check(L.lib().nts_batch_set_lds_batch(self.ctx.h, self.h, int(fwd_batch), int(bwd_batch)))
 

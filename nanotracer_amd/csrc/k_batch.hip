@@ -62,7 +62,7 @@ struct nts_batch {
   uint32_t last_key_bytes = 4;
   uint32_t last_tile_base[NTS_MAX_VIEWS + 1] = {};
   // splats staged per LDS batch by the blend kernels
-  uint32_t fwd_batch = NTS_DEFAULT_LDS_BATCH, bwd_batch = NTS_DEFAULT_LDS_BATCH;
+  uint32_t fwd_batch = NTS_DEFAULT_FWD_BATCH, bwd_batch = NTS_DEFAULT_BWD_BATCH;
   // combined MSE + SSIM loss of NTS_LOSS_SSIM steps (planes allocated by the setter)
   float ssim_lambda = 0.2f;
   void* ssim_base = nullptr;

@@ -778,7 +778,7 @@ extern "C" int nts_composite_ex(nts_ctx* ctx, const nts_buffer* projected, const
   ViewTable vt;
   int rc = fill_single_view(bins, tiling, vt);
   if (rc) return rc;
-  return nts_blend_forward(ctx, tiling->tile_size, NTS_DEFAULT_LDS_BATCH, (const Projected*)projected->ptr,
+  return nts_blend_forward(ctx, tiling->tile_size, NTS_DEFAULT_FWD_BATCH, (const Projected*)projected->ptr,
                            bins->payloads, bins->ranges, vt,
                            (float4*)image->ptr, final_t ? (float*)final_t->ptr : nullptr,
                            last_idx ? (uint32_t*)last_idx->ptr : nullptr, nullptr);
@@ -804,7 +804,7 @@ extern "C" int nts_composite_backward_ex(nts_ctx* ctx, const nts_buffer* project
   ViewTable vt;
   int rc = fill_single_view(bins, tiling, vt);
   if (rc) return rc;
-  return nts_blend_backward(ctx, tiling->tile_size, NTS_DEFAULT_LDS_BATCH, mode, (const Projected*)projected->ptr,
+  return nts_blend_backward(ctx, tiling->tile_size, NTS_DEFAULT_BWD_BATCH, mode, (const Projected*)projected->ptr,
                             bins->payloads,
                             bins->ranges, vt, forward_out ? (const float4*)forward_out->ptr : nullptr,
                             final_t ? (const float*)final_t->ptr : nullptr,

@@ -74,8 +74,12 @@ __device__ __forceinline__ float nts_exp_blend(float x) {
   return __builtin_ldexpf(nts_exp_poly(r), (int)n);
 }
 
-// Splats staged through LDS per batch when the caller does not choose.
-#define NTS_DEFAULT_LDS_BATCH 256
+// Splats staged through LDS per batch when the caller does not choose: the
+// forward's is within a few percent of the best at tiles 16 and 32; a
+// backward batch of 256 is the slowest there (its per-wave partial sums
+// grow with the batch), 128 the best or near it.
+#define NTS_DEFAULT_FWD_BATCH 256
+#define NTS_DEFAULT_BWD_BATCH 128
 
 // Host-side launchers of the blend kernels (k_raster.hip).  tile_size in
 // {8, 16, 32}; lds_batch in {64, 128, 256, 512} forward, {64, 128, 256}
