@@ -804,7 +804,7 @@ extern "C" int nts_composite_backward_ex(nts_ctx* ctx, const nts_buffer* project
   ViewTable vt;
   int rc = fill_single_view(bins, tiling, vt);
   if (rc) return rc;
-  return nts_blend_backward(ctx, tiling->tile_size, NTS_DEFAULT_BWD_BATCH, mode, (const Projected*)projected->ptr,
+  return nts_blend_backward(ctx, tiling->tile_size, nts_default_bwd_batch(tiling->tile_size), mode, (const Projected*)projected->ptr,
                             bins->payloads,
                             bins->ranges, vt, forward_out ? (const float4*)forward_out->ptr : nullptr,
                             final_t ? (const float*)final_t->ptr : nullptr,
