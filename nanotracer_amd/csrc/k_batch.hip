@@ -83,7 +83,9 @@ __global__ void k_set_u32s(uint32_t* __restrict__ dst, U32Table t) {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_project_views(const float* __restrict__ P, uint32_t cap, uint32_t n,
+// Held to 4 waves per SIMD (125 VGPRs, no spills; the compiler's own choice
+// is 139 VGPRs and 3 waves): 6% faster over 1M splats x 16 views.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_project_views(const float* __restrict__ P, uint32_t cap, uint32_t n,
                                                        const DevCamera* __restrict__ cams, TileTable tt,
                                                        Projected* __restrict__ proj, uint32_t* __restrict__ counts,
                                                        uint32_t* __restrict__ visible) {
