@@ -8,7 +8,8 @@
 //                          per view;
 //   onesweep sort          segmented by view;
 //   ranges                 global tile ids (view tile base + local tile);
-//   blend forward          one workgroup per (view, tile), loss fused in the
+//   tile order             tiles by pair count, heaviest first;
+//   blend forward          one workgroup per (view, tile) in that order, loss fused in the
 //                          epilogue (dL/dC and per-view loss);
 //   blend backward         per (view, tile), per-view projected gradients;
 //   project_backward+Adam  one thread per splat: the views' parameter
@@ -40,6 +41,7 @@ struct nts_batch {
   uint32_t* vals_a = nullptr;
   uint32_t* vals_b = nullptr;
   uint2* ranges = nullptr;
+  uint32_t* order = nullptr;  // the blend kernels' tile order (most pairs first)
   float4* image = nullptr;
   float4* dl_dc = nullptr;
   float* final_t = nullptr;
@@ -113,6 +115,45 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) vis += __shfl_xor(vis, off, 64);
   if ((threadIdx.x & 63u) == 0 && vis) atomicAdd(visible, vis);
+}
+
+// The order in which the blend kernels take their tiles: most pairs first.
+// The hardware hands out a launch's workgroups in grid order, so a heavy
+// tile late in the grid would run on alone at the end of the launch; taken
+// first, the light tiles fill in behind it.  Tiles are bucketed by pair count,
+// four buckets per octave, in descending bucket order (within a bucket the
+// order is what the LDS atomics give; each tile's image and gradient terms
+// do not depend on it).  One workgroup: count, exclusive scan from the top
+// bucket down, scatter.
+#define NTS_ORDER_BUCKETS 132
+__device__ __forceinline__ uint32_t tile_order_bucket(uint32_t pairs) {
+  if (pairs == 0) return 0;
+  const uint32_t e = 31u - (uint32_t)__builtin_clz(pairs);  // floor(log2(pairs))
+  const uint32_t f = e >= 2 ? (pairs >> (e - 2)) & 3u : (pairs << (2 - e)) & 3u;
+  return 1u + 4u * e + f;  // <= 128
+}
+
+__global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ ranges, uint32_t n_tiles,
+                                                     uint32_t* __restrict__ order) {
+  __shared__ uint32_t s_cnt[NTS_ORDER_BUCKETS];
+  __shared__ uint32_t s_pos[NTS_ORDER_BUCKETS];
+  for (uint32_t i = threadIdx.x; i < NTS_ORDER_BUCKETS; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < n_tiles; t += blockDim.x) {
+    const uint2 r = ranges[t];
+    atomicAdd(&s_cnt[tile_order_bucket(r.y - r.x)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < NTS_ORDER_BUCKETS) {
+    uint32_t above = 0;
+    for (uint32_t j = threadIdx.x + 1; j < NTS_ORDER_BUCKETS; ++j) above += s_cnt[j];
+    s_pos[threadIdx.x] = above;
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < n_tiles; t += blockDim.x) {
+    const uint2 r = ranges[t];
+    order[atomicAdd(&s_pos[tile_order_bucket(r.y - r.x)], 1u)] = t;
+  }
 }
 
 // Segment bounds per view from the scanned counts, clamped to the pair
@@ -318,6 +359,7 @@ extern "C" int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_s
   const uint64_t o_va = take(pair_capacity * 4);
   const uint64_t o_vb = take(pair_capacity * 4);
   const uint64_t o_rg = take(b->max_tiles * 8);
+  const uint64_t o_ord = take(b->max_tiles * 4);
   const uint64_t o_img = take(max_pixels * 16);
   const uint64_t o_dl = take(max_pixels * 16);
   const uint64_t o_ft = take(max_pixels * 4);
@@ -346,6 +388,7 @@ extern "C" int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_s
   b->vals_a = (uint32_t*)(p + o_va);
   b->vals_b = (uint32_t*)(p + o_vb);
   b->ranges = (uint2*)(p + o_rg);
+  b->order = (uint32_t*)(p + o_ord);
   b->image = (float4*)(p + o_img);
   b->dl_dc = (float4*)(p + o_dl);
   b->final_t = (float*)(p + o_ft);
@@ -434,6 +477,7 @@ static int plan_batch(nts_batch* b, const nts_scene* scene, const nts_view_desc*
                                             (unsigned long long)tiles, (unsigned long long)b->max_tiles);
   if ((uint64_t)n_views * scene->n >= (1ull << 32)) return nts_fail(NTS_E_CAPACITY, "views x splats >= 2^32");
   pl.vt.total_tiles = (uint32_t)tiles;
+  pl.vt.order = b->order;
   pl.pixels = pix;
   uint32_t tb = 1;
   while (tb < 32 && (1ull << tb) < max_tiles_view) ++tb;
@@ -498,6 +542,8 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
   else
     NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)keys,
                (const uint64_t*)b->seg_begin, V, (const uint32_t*)b->tile_base, b->ranges);
+  NTS_LAUNCH(ctx, "tile_order", k_tile_order, dim3(1), dim3(1024), 0, (const uint2*)b->ranges, pl.vt.total_tiles,
+             b->order);
   rc = nts_blend_forward(ctx, pl.tile_size, b->fwd_batch, b->proj, pay, b->ranges, pl.vt, b->image,
                          keep_state ? b->final_t : nullptr, keep_state ? b->last_idx : nullptr, loss);
   if (rc) return rc;

@@ -224,7 +224,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
   __shared__ float4 s_c[BATCH];  // b, power threshold, cull half-extents x / y
   __shared__ float4 s_d[BATCH];  // per-wave test: threshold, -b/c, -b/a, c
-  const uint32_t gt = blockIdx.x;
+  const uint32_t gt = vt.order ? vt.order[blockIdx.x] : blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
   const ViewDesc vd = vt.v[vi];
   const uint32_t lt = gt - vd.tile_base;
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
   __shared__ float s_part[NW][BATCH][9];  // per-wave sums of each splat's terms
   __shared__ uint64_t s_cm[NW][NM];      // which waves wrote s_part for a slot
   __shared__ uint32_t s_max;
-  const uint32_t gt = blockIdx.x;
+  const uint32_t gt = vt.order ? vt.order[blockIdx.x] : blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
   const ViewDesc vd = vt.v[vi];
   const uint32_t lt = gt - vd.tile_base;
@@ -534,24 +534,34 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
           // 1.0001 bail on the divided-back T is sensitive to the last bit.
           // alpha = min(alpha_raw, 0.99), a NaN giving 0.99, taken next to each
           // product (a product is canonical: the min needs no quieting first)
+          // The alpha >= 1/255 decision of the hardware value is taken as
+          // final only outside the band [1 - 4e-6, 1 + 4e-6] 1/255; inside
+          // it, and from 0.99 - 4e-6 up (the clamp), the exact exponential
+          // decides.  Three compares: alpha above the band, alpha in or above
+          // it, ar at or above the clamp's band.  A NaN ar gives alpha 0.99:
+          // kept, no exact pass, as in the forward.
           float g, ar, alpha;
+          bool aok;
           if (RECOMP) {
             g = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
             ar = sigma * g;
             alpha = fminf(ar, NTS_ALPHA_MAX);
-            if (fabsf(ar - NTS_ALPHA_MIN) <= 4e-6f * NTS_ALPHA_MIN || fabsf(ar - NTS_ALPHA_MAX) <= 4e-6f) {
+            aok = alpha > NTS_ALPHA_MIN * (1.0f + 4e-6f);
+            if ((!aok && alpha >= NTS_ALPHA_MIN * (1.0f - 4e-6f)) || ar >= NTS_ALPHA_MAX - 4e-6f) {
               g = nts_exp_blend(power);
               ar = sigma * g;
               alpha = fminf(ar, NTS_ALPHA_MAX);
+              aok = !(alpha < NTS_ALPHA_MIN);
             }
           } else {
             g = nts_exp_blend(power);
             ar = sigma * g;
             alpha = fminf(ar, NTS_ALPHA_MAX);
+            aok = !(alpha < NTS_ALPHA_MIN);
           }
           // the exact complement of the forward's skip tests (a NaN threshold
           // or alpha must be kept here as it is there)
-          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && !(power < Cc.y) && !(alpha < NTS_ALPHA_MIN);
+          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && !(power < Cc.y) && aok;
           if (RECOMP && j == 0) {
             // The first pixel of the quad without a branch: a lane that fails
             // the tests sets exact zeros and keeps T (acc needs no zeroing

@@ -30,7 +30,7 @@ struct LossArgs {
 struct ViewTable {
   uint32_t n;
   uint32_t total_tiles;
-  uint32_t pad[2];
+  const uint32_t* order;  // global tile of each workgroup (nullptr: grid order)
   ViewDesc v[NTS_MAX_VIEWS];
 };
 
