@@ -336,7 +336,8 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
 // the gradient arithmetic uses fused multiply-adds, and the recompute mode
 // the hardware exponential (within 1e-6 relative of the exact forms).
 // Each splat's nine per-pixel terms are reduce-scattered across the wave,
-// the waves' partial sums meet in LDS, and one set of float atomics per
+// the waves' sums are added in LDS (float atomics into one slot table, which
+// keeps the workgroup's LDS small), and one set of global float atomics per
 // (tile, splat) commits them.
 // ---------------------------------------------------------------------------
 
@@ -413,20 +414,23 @@ __device__ __forceinline__ uint32_t grad_term_of_quad(uint32_t q) {
   return vi < 9u ? vi : 9u;
 }
 
+// A 256-thread workgroup is one wave per SIMD; 80 VGPRs (6 waves per SIMD)
+// hold without spills, and the LDS of a 128- or 256-splat batch allows that
+// many workgroups per CU.
 template <int TS, int BATCH, bool RECOMP>
-__global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
+__global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT))
+__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? 6 : 1))) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
     ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
     const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
   using BL = Blend<TS, NTS_BWD_THREADS>;
-  constexpr int NT = BL::NT, PPT = BL::PPT, NW = NT / 64, NM = BATCH / 64;
+  constexpr int NT = BL::NT, PPT = BL::PPT, NM = BATCH / 64;
   __shared__ float4 s_a[BATCH];
   __shared__ float4 s_b[BATCH];
   __shared__ float4 s_c[BATCH];
   __shared__ float4 s_d[BATCH];
   __shared__ uint32_t s_idx[BATCH];
-  __shared__ float s_part[NW][BATCH][9];  // per-wave sums of each splat's terms
-  __shared__ uint64_t s_cm[NW][NM];      // which waves wrote s_part for a slot
+  __shared__ float s_acc[BATCH][9];  // each slot's terms summed over the waves (LDS float atomics)
   __shared__ uint32_t s_max;
   const uint32_t gt = vt.order ? vt.order[blockIdx.x] : blockIdx.x;
   const uint32_t vi = view_of_tile(vt, gt);
@@ -435,7 +439,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
   const uint32_t tx = lt % vd.tiles_x, ty = lt / vd.tiles_x;
   const uint2 rg = ranges[gt];
   const uint32_t begin = rg.x, end = rg.y;
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t my_term = (lane & 3u) == 0u ? grad_term_of_quad(lane >> 2) : 9u;
   const bool term_lane = my_term < 9u;
   const WaveRect wr = wave_rect<TS, NTS_BWD_THREADS>(tx, ty);
@@ -494,15 +498,15 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
       stage_splat(proj[vd.proj_base + sidx], s_a[k], s_b[k], s_c[k], s_d[k]);
       s_idx[k] = sidx;
     }
+    for (uint32_t i = threadIdx.x; i < take * 9u; i += NT) (&s_acc[0][0])[i] = 0.0f;
     __syncthreads();
     uint32_t klim = 0;
     if (__ballot(live) && wave_last > rel0) klim = wave_last - rel0 < take ? wave_last - rel0 : take;
-    uint64_t mk[NM], cm[NM];
+    uint64_t mk[NM];
     wave_masks<NM>(wr, s_a, s_c, s_d, klim, mk);
 #pragma unroll
     for (int i = NM - 1; i >= 0; --i) {
       uint64_t m = mk[i];
-      cm[i] = 0;
       while (m) {
         const int bt = 63 - __builtin_clzll(m);
         m &= ~(1ull << bt);
@@ -629,30 +633,19 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT)) k_blend_bwd(
         }
         if (__ballot(contrib)) {
           const float r = wave_reduce9(acc);
-          if (term_lane) s_part[wave][k][my_term] = r;
-          cm[i] |= 1ull << bt;
+          if (term_lane) atomicAdd(&s_acc[k][my_term], r);
         }
       }
     }
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < NM; ++i) s_cm[wave][i] = cm[i];
-    }
     __syncthreads();
-    // one set of float atomics per (tile, splat): each thread sums one
-    // (slot, term) over the waves that wrote a partial for the slot, so the
-    // nine terms of a splat are consecutive lanes and consecutive words
+    // one set of float atomics per (tile, splat): each thread maps one
+    // (slot, term) of the waves' sums, so the nine terms of a splat are
+    // consecutive lanes and consecutive words
     for (uint32_t it = threadIdx.x; it < take * 9u; it += NT) {
       const uint32_t k = it / 9u, q = it - k * 9u;
-      const uint64_t bit = 1ull << (k & 63u);
-      const uint32_t qs = q < 2u ? 0u : q;  // dL/dmean needs both first moments
-      float sum = 0.0f, sum2 = 0.0f;
-#pragma unroll
-      for (int w = 0; w < NW; ++w)
-        if (s_cm[w][k >> 6] & bit) {
-          sum += s_part[w][k][qs];
-          if (q < 2u) sum2 += s_part[w][k][1];
-        }
+      // dL/dmean needs both first moments
+      float sum = s_acc[k][q < 2u ? 0u : q];
+      const float sum2 = s_acc[k][1];
       if (q < 2u) {
         const float4 A = s_a[k];
         const float qa = -2.0f * A.z, qb = -A.w, qc = -2.0f * s_b[k].x;  // the conic, from the staged -q/2
