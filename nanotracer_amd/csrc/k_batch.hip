@@ -156,6 +156,12 @@ __global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ r
   }
 }
 
+int nts_tile_order(nts_ctx* ctx, const uint2* ranges, uint32_t n_tiles, uint32_t* order) {
+  if (n_tiles == 0) return NTS_OK;
+  NTS_LAUNCH(ctx, "tile_order", k_tile_order, dim3(1), dim3(1024), 0, ranges, n_tiles, order);
+  return NTS_OK;
+}
+
 // Segment bounds per view from the scanned counts, clamped to the pair
 // capacity (overflow is flagged, never written past).
 __global__ void k_view_segments(const uint32_t* __restrict__ offsets, RecTable rt, const uint64_t* __restrict__ d_total,
@@ -542,8 +548,8 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
   else
     NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)keys,
                (const uint64_t*)b->seg_begin, V, (const uint32_t*)b->tile_base, b->ranges);
-  NTS_LAUNCH(ctx, "tile_order", k_tile_order, dim3(1), dim3(1024), 0, (const uint2*)b->ranges, pl.vt.total_tiles,
-             b->order);
+  rc = nts_tile_order(ctx, b->ranges, pl.vt.total_tiles, b->order);
+  if (rc) return rc;
   rc = nts_blend_forward(ctx, pl.tile_size, b->fwd_batch, b->proj, pay, b->ranges, pl.vt, b->image,
                          keep_state ? b->final_t : nullptr, keep_state ? b->last_idx : nullptr, loss);
   if (rc) return rc;
