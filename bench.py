@@ -100,20 +100,22 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 # HBM bytes per launch measured with rocprofv3 (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors
 # read on a 1 GiB copy kernel) at the (kernel, tile size, that kernel's LDS batch) it was profiled at,
 # with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.7024e+09, 45_319_956),
-                    ("blend_fwd_loss", 32, 512): (3.5579e+09, 45_319_956)}
+PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.8567e+09, 45_319_956),
+                    ("blend_fwd_loss", 32, 512): (3.8726e+09, 45_319_956)}
 # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
-PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.9687e9, 45_319_956),
-                 ("blend_fwd_loss", 32, 512): (2.2059e9, 45_319_956)}
+PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.7587e9, 45_319_956),
+                 ("blend_fwd_loss", 32, 512): (2.1923e9, 45_319_956)}
 # LDS bytes per launch moved by the lanes (rocprofv3 SQ_INSTS_LDS_LOAD_BANDWIDTH + _STORE_BANDWIDTH,
 # 64-byte units) and the share of the kernel's cycles the LDS spends on indexed operations
 # (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((846562531.4 + 11953172.3) * 64, 45_319_956, 0.089),
-                ("blend_fwd_loss", 32, 512): ((1960380152.8 + 15178916.0) * 64, 45_319_956, 0.374)}
+PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((792269317.5 + 15983137.5) * 64, 45_319_956, 0.234),
+                ("blend_fwd_loss", 32, 512): ((1957194823.6 + 13700604.4) * 64, 45_319_956, 0.389)}
 LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 128 B per clock per CU
 N_SIMDS, CLOCK_HZ = 1024, 2.4e9
 # Independent FP32 FMA chains at full occupancy issue 0.244 wave instructions per SIMD per cycle
-# (a wave64 instruction occupies a SIMD for 4 cycles); mixed code co-issues slightly above it.
+# (a wave64 FMA occupies a SIMD for 4 cycles). The blend kernels' mixed code issues above that
+# rate, and fewer VALU instructions alone did not shorten them: they also wait on LDS, barriers
+# and memory (the stall counters in the same profile).
 FMA_PROBE_RATE = 0.244
 
 
