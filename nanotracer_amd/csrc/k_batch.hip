@@ -583,11 +583,22 @@ static int record_step(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const Batch
   const uint32_t n = pl.n, V = pl.n_views;
   int rc = nts_fill_u32(ctx, b->losses, 0, 2 * (uint64_t)V);
   if (rc) return rc;
-  rc = nts_fill_u32(ctx, b->pgrad, 0, (uint64_t)V * n * (sizeof(ProjectedGrad) / 4));
-  if (rc) return rc;
   // NTS_LOSS_SSIM at lambda 0 is the fused MSE (loss.rs:398-402)
   const bool ssim = loss_kind == NTS_LOSS_SSIM && b->ssim_lambda > 0.0f;
-  LossArgs la{(const float4*)targets->ptr, b->dl_dc, b->losses, loss_kind == NTS_LOSS_L1 ? NTS_LOSS_L1 : NTS_LOSS_MSE};
+  LossArgs la{(const float4*)targets->ptr, b->dl_dc, b->losses, loss_kind == NTS_LOSS_L1 ? NTS_LOSS_L1 : NTS_LOSS_MSE,
+              nullptr, 0};
+  // The projected gradients start at zero: cleared by the forward blend behind
+  // its tiles (its HBM is mostly idle), or by a fill where that launch has no
+  // loss epilogue or no tiles.
+  const uint64_t pgrad_n4 = (uint64_t)V * n * (sizeof(ProjectedGrad) / 16);
+  static_assert(sizeof(ProjectedGrad) % 16 == 0, "projected gradient records are whole float4s");
+  if (!ssim && pl.vt.total_tiles > 0) {
+    la.clear = (float4*)b->pgrad;
+    la.clear_n4 = pgrad_n4;
+  } else {
+    rc = nts_fill_u32(ctx, b->pgrad, 0, pgrad_n4 * 4);
+    if (rc) return rc;
+  }
   bool alt = false;
   bool* altp = &alt;
   rc = launch_forward(ctx, scene, b, pl, ssim ? nullptr : &la, mode == NTS_BWD_RECOMPUTE, &altp);

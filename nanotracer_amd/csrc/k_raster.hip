@@ -323,6 +323,12 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
       unsafeAtomicAdd(la.losses + vi, sum);
     }
   }
+  if (la.clear) {
+    const uint64_t per = (la.clear_n4 + gridDim.x - 1) / gridDim.x;
+    const uint64_t c0 = (uint64_t)blockIdx.x * per;
+    const uint64_t c1 = c0 + per < la.clear_n4 ? c0 + per : la.clear_n4;
+    for (uint64_t i = c0 + threadIdx.x; i < c1; i += NT) la.clear[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
 }
 
 // ---------------------------------------------------------------------------

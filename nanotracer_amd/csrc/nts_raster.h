@@ -23,6 +23,11 @@ struct LossArgs {
   float4* dl_dc;
   double* losses;        // per view, accumulated
   int kind;              // NTS_LOSS_L1 or NTS_LOSS_MSE
+  // Optional buffer the launch clears to zero behind the blend, a slice per
+  // workgroup after its last barrier (the training step's projected
+  // gradients, which the backward then accumulates into); nullptr = none.
+  float4* clear;
+  uint64_t clear_n4;     // float4 words of `clear`
 };
 
 // Passed by value as a kernel argument (kernarg segment), so launches need
