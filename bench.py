@@ -74,7 +74,8 @@ def alg_bytes(name: str, N: int, Nv: int, K: int, P: int, T: int, V: int, kb: in
         "sort_onesweep": 2 * (kb + 4) * K,
         "tile_ranges": kb * K + 8 * T,
         "blend_fwd": 8 * T + 40 * K + 16 * P,
-        "blend_fwd_loss": 8 * T + 40 * K + 16 * P + 16 * P + 16 * P + 8 * P,
+        # the batched step's forward also clears the V x N projected-gradient records (48 B)
+        "blend_fwd_loss": 8 * T + 40 * K + 16 * P + 16 * P + 16 * P + 8 * P + 48 * N * V,
         "loss": 48 * P,
         "blend_bwd": 8 * T + 40 * K + 32 * P + 36 * Nv,
         "blend_bwd_recompute": 8 * T + 40 * K + 24 * P + 36 * Nv,
@@ -101,7 +102,7 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 # read on a 1 GiB copy kernel) at the (kernel, tile size, that kernel's LDS batch) it was profiled at,
 # with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
 PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.8567e+09, 45_319_956),
-                    ("blend_fwd_loss", 32, 512): (3.8726e+09, 45_319_956)}
+                    ("blend_fwd_loss", 32, 512): (4.6451e+09, 45_319_956)}
 # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
 PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.7587e9, 45_319_956),
                  ("blend_fwd_loss", 32, 512): (2.1923e9, 45_319_956)}
