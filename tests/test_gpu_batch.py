@@ -255,7 +255,7 @@ def test_lds_batch_keeps_images_bit_exact(ctx, tile, fwd_batch):
     b.free()
 
 
-@pytest.mark.parametrize("tile,bwd_batch", [(32, 64), (8, 128), (16, 64)])
+@pytest.mark.parametrize("tile,bwd_batch", [(32, 64), (32, 128), (32, 256), (8, 128), (16, 64)])
 def test_lds_batch_backward_matches_oracle(ctx, tile, bwd_batch):
     s = synth.make_scene(20_000, 0x5EED0002, 3.0)
     views = _views(s.n)
