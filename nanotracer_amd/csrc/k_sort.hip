@@ -192,13 +192,29 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
     }
     const uint64_t base = seg_begin[s] + (uint64_t)(p - part_begin[s]) * RS_TILE;
     const uint64_t end = seg_begin[s + 1];
+    // Neighbouring pairs mostly come from one record (same depth, neighbouring
+    // tiles), so a wave's digits come in runs: each run's first lane adds the
+    // run's length, one LDS atomic per run instead of one per key.  The valid
+    // lanes of a wave are a prefix (consecutive indices).
+    const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll 4
     for (int k = 0; k < RS_ITEMS; ++k) {
       const uint64_t i = base + (uint64_t)k * RS_BLOCK + threadIdx.x;
-      if (i < end) {
-        const K key = keys[i];
-        for (uint32_t ps = 0; ps < passes; ++ps)
-          atomicAdd(&s_hist[ps * RS_RADIX + (uint32_t)((key >> (8 * ps)) & 255u)], 1u);
+      const bool valid = i < end;
+      const K key = valid ? keys[i] : (K)0;
+      const uint64_t vmask = __ballot(valid);
+      if (vmask == 0) continue;
+      const uint32_t n_valid = (uint32_t)__popcll(vmask);
+      for (uint32_t ps = 0; ps < passes; ++ps) {
+        const uint32_t d = (uint32_t)((key >> (8 * ps)) & 255u);
+        const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, 64);
+        const bool head = valid && (lane == 0 || prev != d);
+        const uint64_t heads = __ballot(head);
+        if (head) {
+          const uint64_t above = lane < 63u ? heads & (~0ull << (lane + 1u)) : 0ull;
+          const uint32_t next = above ? (uint32_t)__builtin_ctzll(above) : n_valid;
+          atomicAdd(&s_hist[ps * RS_RADIX + d], next - lane);
+        }
       }
     }
   }
