@@ -1,6 +1,7 @@
 // Tile binning: per-splat tile counts, exclusive scan, key emission, segmented
 // onesweep sort and per-tile ranges (TileBinner::bin, tile_binner.rs:181-382).
 #include "nts_bin.h"
+#include "nts_raster.h"
 
 // tile_count.wgsl:28-55 (the total comes from the scan instead of an atomic).
 __global__ void __launch_bounds__(256) k_tile_count(const Projected* __restrict__ proj, uint32_t n, ViewTiles vt,
@@ -93,7 +94,8 @@ extern "C" int nts_bin(nts_ctx* ctx, const nts_buffer* projected, uint32_t n, co
   const uint64_t off_vals = off_keys2 + al(k1 * kb);
   const uint64_t off_vals2 = off_vals + al(k1 * 4);
   const uint64_t off_ranges = off_vals2 + al(k1 * 4);
-  const uint64_t off_meta = off_ranges + al(num_tiles * 8);
+  const uint64_t off_order = off_ranges + al(num_tiles * 8);
+  const uint64_t off_meta = off_order + al(num_tiles * 4);
   b->bytes = off_meta + 256;
   rc = nts_dev_alloc(ctx, &b->base, b->bytes);
   if (rc) {
@@ -158,6 +160,11 @@ extern "C" int nts_bin(nts_ctx* ctx, const nts_buffer* projected, uint32_t n, co
   else
     NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)b->keys,
                (const uint64_t*)seg_begin, 1u, (const uint32_t*)tile_base, b->ranges);
+  // the blend order is published only once written (empty bins keep grid order)
+  uint32_t* order = (uint32_t*)(base + off_order);
+  rc = nts_tile_order(ctx, b->ranges, num_tiles, order);
+  if (rc) return rc;
+  b->order = order;
   return NTS_OK;
 }
 

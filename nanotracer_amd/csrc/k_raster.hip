@@ -684,6 +684,7 @@ static int fill_single_view(const nts_bins* bins, const nts_tiling* t, ViewTable
   vt.v[0].tile_base = 0;
   vt.v[0].proj_base = 0;
   vt.v[0].pix_base = 0;
+  vt.order = bins->order;
   if (bins->tile_size != t->tile_size || bins->tiles_x != (t->width + t->tile_size - 1) / t->tile_size ||
       bins->tiles_y != (t->height + t->tile_size - 1) / t->tile_size)
     return nts_fail(NTS_E_INVALID, "tiling does not match the bins");

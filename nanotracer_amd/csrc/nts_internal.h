@@ -104,6 +104,7 @@ struct nts_bins {
   void* keys = nullptr;           // sorted
   uint32_t* payloads = nullptr;   // sorted
   uint2* ranges = nullptr;        // per tile
+  uint32_t* order = nullptr;      // blend order of the tiles, most pairs first (nullptr: grid order)
   void* base = nullptr;
   uint64_t bytes = 0;
 };
