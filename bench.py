@@ -222,24 +222,6 @@ SERVE_SCENES = 64
 SERVE_RES = ((512, 512), (1280, 720), (1920, 1080), (3840, 2160))
 
 
-def serving_scenes(seed):
-    """64 resident scenes, splat counts log-uniform in [50k, 3M]: windows of
-    one 3M-splat pool (per-scene offset and scale), so that setting up 46M
-    resident splats costs seconds."""
-    from nanotracer_amd import synth
-
-    pool = synth.fast_scene(3_000_000, seed, 3.0)
-    rng = np.random.default_rng(seed + 1)
-    counts = np.exp(rng.uniform(math.log(50_000), math.log(3_000_000), SERVE_SCENES)).astype(np.int64)
-    for k, n in enumerate(counts):
-        start = int(rng.integers(0, pool.n))
-        idx = (start + np.arange(n)) % pool.n
-        sc = float(rng.uniform(0.8, 1.2))
-        arrs = [np.ascontiguousarray(a[idx]) for a in pool.arrays()]
-        arrs[0] *= np.float32(sc)
-        yield synth.HostSplats(*arrs)
-
-
 def run_serving(rate, duration, seed=SEED + 5, tile=32, lds=512):
     """Poisson arrivals at `rate` req/s for `duration` s, each one camera of a
     random resident scene at a random resolution; latency = arrival to the
@@ -252,7 +234,7 @@ def run_serving(rate, duration, seed=SEED + 5, tile=32, lds=512):
                        pair_capacity=600_000_000, tile_size=tile, lds_batch=lds)
     sizes = []
     t_setup = time.perf_counter()
-    for sc in serving_scenes(seed):
+    for sc in synth.serving_scenes(SERVE_SCENES, seed):
         sizes.append(sc.n)
         srv.add_scene(sc)
     t_setup = time.perf_counter() - t_setup

@@ -113,12 +113,25 @@ int nts_buffer_zero(nts_ctx* ctx, nts_buffer* buf);
 /* ---- scene: parameters + gradients + Adam moments + grad accumulator
  *      (replaces GpuSplatBuffer, GradSplatBuffers, AdamMomentBuffers,
  *      splat_gpu.rs:36-384) ---------------------------------------------- */
-int nts_scene_create(nts_ctx* ctx, uint32_t capacity, nts_scene** out);
+#define NTS_SCENE_TRAINABLE 1u /* also hold gradients, both Adam moments and grad_acc */
+/* A scene of up to `capacity` splats.  With NTS_SCENE_TRAINABLE it holds
+ * GpuSplatBuffer + GradSplatBuffers + AdamMomentBuffers + the grad_acc slab
+ * (train.rs:94-96, 236 B x 4 + 4 B per splat); without it only the
+ * parameters (236 B per splat), as a viewer's GpuSplatBuffer
+ * (nano-view/src/lib.rs:642-652): calls that need gradients, moments or
+ * grad_acc then fail with NTS_E_STATE. */
+int nts_scene_create(nts_ctx* ctx, uint32_t capacity, uint32_t flags, nts_scene** out);
 int nts_scene_destroy(nts_ctx* ctx, nts_scene* scene);
 uint32_t nts_scene_count(const nts_scene* scene);
-/* GpuSplatBuffer::upload / sync_from (splat_gpu.rs:94-150). Resets moments,
- * gradients and grad_acc to zero. */
+/* GpuSplatBuffer::upload (splat_gpu.rs:94-120): sets the splat count and the
+ * parameters, and resets gradients, moments and grad_acc to zero (a fresh
+ * upload has fresh optimiser state, train.rs:94-96). */
 int nts_scene_upload(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* host);
+/* GpuSplatBuffer::sync_from (splat_gpu.rs:124-150): rewrites the parameters
+ * of a scene holding host->n splats in place; gradients, moments and
+ * grad_acc are kept.  NTS_E_INVALID when the count differs (the reference
+ * reallocates on densify / prune instead). */
+int nts_scene_sync_params(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* host);
 /* GpuSplatBuffer::readback (splat_gpu.rs:152-187); host->n is set. */
 int nts_scene_download(nts_ctx* ctx, const nts_scene* scene, nts_splats_host* host);
 /* GradSplatBuffers readback (train.rs:370-389 reads them for norms). */
@@ -141,8 +154,14 @@ int nts_camera_from_pose(const float eye[3], const float target[3], const float 
 int nts_project(nts_ctx* ctx, const nts_scene* scene, const nts_camera* cam, nts_buffer* projected);
 
 /* ---- binning (TileBinner::bin, tile_binner.rs:181-382) ---------------- */
-/* Count, scan, emit 64-bit (tile << 16 | depth16) keys, onesweep radix sort,
- * per-tile ranges.  Like the reference it reads the pair total back. */
+/* Count, scan, emit (tile << 16 | depth16) keys, onesweep radix sort,
+ * per-tile ranges.  Like the reference it reads the pair total back.  Keys
+ * are u32, the reference's format, when the tile id fits 16 bits (tiles <=
+ * 65,536, every view up to 4096x4096 at tile 16); u64 otherwise, where the
+ * reference's key wraps (tile_emit.wgsl:60-61).  The bins also hold the
+ * order in which composite and composite_backward take the tiles, most
+ * pairs first (4 B per tile; grid order when there are no pairs): it
+ * changes no image or gradient term, only the order workgroups start. */
 int nts_bin(nts_ctx* ctx, const nts_buffer* projected, uint32_t n_splats, const nts_tiling* tiling,
             nts_bins** out);
 int nts_bins_destroy(nts_ctx* ctx, nts_bins* bins);
@@ -213,6 +232,19 @@ int nts_project_backward_ex(nts_ctx* ctx, nts_scene* scene, const nts_buffer* pr
 /* AdamGpu::step on a flat slab of n floats (adam_step.wgsl). */
 int nts_adam_step_flat(nts_ctx* ctx, nts_buffer* params, const nts_buffer* grads, nts_buffer* m,
                        nts_buffer* v, uint64_t n, const nts_adam_cfg* cfg, uint64_t t);
+/* AdamGpu::step_all (adam_gpu.rs:334-427): Adam on every attribute plane
+ * of the scene (positions with cfg_pos, the rest with cfg_attr, bias
+ * corrections with step t).  Gradients come from the scene. */
+int nts_adam_step_all(nts_ctx* ctx, nts_scene* scene, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr,
+                      uint64_t t);
+/* AdamGpu::apply_constraints (adam_gpu.rs:431-489, apply_constraints.wgsl):
+ * quaternion renormalisation (max(|q|, 1e-8)), log-scales clamped to
+ * [log_scale_min, log_scale_max], logits to +-logit_max.  Works on
+ * parameter-only scenes too. */
+int nts_apply_constraints(nts_ctx* ctx, nts_scene* scene, const nts_constraints* cons);
+/* AdamGpu::accumulate_grad (adam_gpu.rs:492-541, accumulate_grad_pos.wgsl):
+ * grad_acc += |dx| + |dy| + |dz| of the scene's position gradient. */
+int nts_accumulate_grad(nts_ctx* ctx, nts_scene* scene);
 /* accumulate_grad + step_all + apply_constraints fused in one kernel:
  * grad_acc += |dpos|_1, Adam on every attribute (positions use cfg_pos,
  * the rest cfg_attr, bias corrections with step t), then quaternion
@@ -252,7 +284,11 @@ int nts_batch_destroy(nts_ctx* ctx, nts_batch* batch);
  * Nothing synchronises with the host; with NTS_STEP_GRAPH the step is
  * captured in a hipGraph on first use and replayed while the views and
  * settings stay the same and t advances by one.  Pairs beyond
- * pair_capacity are dropped and flagged (nts_batch_stats). */
+ * pair_capacity are dropped and flagged (nts_batch_stats); a step whose
+ * pairs overflowed, or whose sort look-back timed out, skips its update on
+ * the device (parameters, moments, gradients and grad_acc keep their values;
+ * the step number t is still used up) and latches the error for
+ * nts_batch_stats. */
 int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* batch, const nts_view_desc* views,
                          uint32_t n_views, const nts_buffer* targets, uint32_t tile_size, int loss_kind,
                          int bwd_mode, const nts_adam_cfg* cfg_pos, const nts_adam_cfg* cfg_attr, uint64_t t,
@@ -274,6 +310,11 @@ int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* batch, uint32_t fwd_batch, 
  * combined per-view loss. */
 int nts_batch_set_ssim_lambda(nts_ctx* ctx, nts_batch* batch, float lambda);
 int nts_batch_losses(nts_ctx* ctx, const nts_batch* batch, double* losses);
+/* Pairs and visible (splat, view) records of the last call.  *overflow is
+ * nonzero when any call since the previous nts_batch_stats overflowed the
+ * pair capacity: the number of training steps whose update was skipped for
+ * it (1 for a forward-only overflow).  A sort look-back timeout since then
+ * is NTS_E_HIP.  Reading clears the latched errors. */
 int nts_batch_stats(nts_ctx* ctx, const nts_batch* batch, uint64_t* total_pairs, uint32_t* visible,
                     uint32_t* overflow);
 int nts_batch_download_image(nts_ctx* ctx, const nts_batch* batch, uint32_t view, float* host);

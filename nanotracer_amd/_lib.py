@@ -17,6 +17,7 @@ NTS_OK = 0
 NTS_E_INVALID, NTS_E_HIP, NTS_E_NOMEM, NTS_E_CAPACITY, NTS_E_STATE = -1, -2, -3, -4, -5
 ERRORS = {-1: "NTS_E_INVALID", -2: "NTS_E_HIP", -3: "NTS_E_NOMEM", -4: "NTS_E_CAPACITY", -5: "NTS_E_STATE"}
 LOSS_L1, LOSS_MSE, LOSS_SSIM = 0, 1, 2
+SCENE_TRAINABLE = 1
 BWD_REF_PARITY, BWD_RECOMPUTE = 0, 1
 
 
@@ -124,7 +125,8 @@ _SIGS = {
     "nts_buffer_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]),
     "nts_buffer_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64]),
     "nts_buffer_zero": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "nts_scene_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "nts_scene_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "nts_scene_sync_params": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SplatsHost)]),
     "nts_scene_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nts_scene_count": (C.c_uint32, [C.c_void_p]),
     "nts_scene_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SplatsHost)]),
@@ -165,6 +167,9 @@ _SIGS = {
     "nts_project_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Camera)]),
     "nts_adam_step_flat": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                      C.POINTER(AdamCfg), C.c_uint64]),
+    "nts_adam_step_all": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AdamCfg), C.POINTER(AdamCfg), C.c_uint64]),
+    "nts_apply_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Constraints)]),
+    "nts_accumulate_grad": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nts_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AdamCfg), C.POINTER(AdamCfg), C.c_uint64,
                                 C.POINTER(Constraints)]),
     "nts_tonemap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),

@@ -141,27 +141,42 @@ def write_ply(path, splats: HostSplats) -> None:
 
 class GpuSplatBuffer:
     """GpuSplatBuffer + GradSplatBuffers + AdamMomentBuffers + grad_acc
-    (splat_gpu.rs:36-384) as one device scene of attribute planes."""
+    (splat_gpu.rs:36-384) as one device scene of attribute planes.  With
+    trainable=False only the parameters are held (a viewer's scene), and the
+    calls that need gradients, moments or grad_acc raise NTS_E_STATE."""
 
-    def __init__(self, ctx: GpuCtx, capacity: int):
+    def __init__(self, ctx: GpuCtx, capacity: int, trainable: bool = True):
         self.ctx = ctx
         h = C.c_void_p()
-        L.check(L.lib().nts_scene_create(ctx.h, max(1, int(capacity)), C.byref(h)))
+        flags = L.SCENE_TRAINABLE if trainable else 0
+        L.check(L.lib().nts_scene_create(ctx.h, max(1, int(capacity)), flags, C.byref(h)))
         self.h = h
         self.capacity = capacity
+        self.trainable = trainable
 
     @classmethod
-    def upload(cls, ctx: GpuCtx, splats: HostSplats, capacity: int | None = None) -> "GpuSplatBuffer":
-        g = cls(ctx, capacity or max(1, splats.n))
-        g.sync_from(splats)
+    def upload(cls, ctx: GpuCtx, splats: HostSplats, capacity: int | None = None,
+               trainable: bool = True) -> "GpuSplatBuffer":
+        """GpuSplatBuffer::upload (splat_gpu.rs:94-120): a new scene, fresh
+        optimiser state."""
+        g = cls(ctx, capacity or max(1, splats.n), trainable)
+        g.upload_from(splats)
         return g
 
     @property
     def n(self) -> int:
         return int(L.lib().nts_scene_count(self.h))
 
-    def sync_from(self, splats: HostSplats) -> None:
+    def upload_from(self, splats: HostSplats) -> None:
+        """Set the splat count and parameters of this scene; gradients,
+        moments and grad_acc restart at zero (the state of a fresh upload)."""
         L.check(L.lib().nts_scene_upload(self.ctx.h, self.h, C.byref(_host_struct(splats))))
+
+    def sync_from(self, splats: HostSplats) -> None:
+        """GpuSplatBuffer::sync_from (splat_gpu.rs:124-150): parameters of
+        the same splat count rewritten in place; moments, gradients and
+        grad_acc are kept."""
+        L.check(L.lib().nts_scene_sync_params(self.ctx.h, self.h, C.byref(_host_struct(splats))))
 
     def readback(self) -> HostSplats:
         out = HostSplats.zeros(self.n)
@@ -433,6 +448,24 @@ class AdamGpu:
              cfg: AdamConfig) -> None:
         c = AdamConfig(lr, cfg.beta1, cfg.beta2, cfg.eps).c()
         L.check(L.lib().nts_adam_step_flat(ctx.h, params.h, grads.h, m.h, v.h, int(n), C.byref(c), int(t)))
+
+    def step_all(self, ctx: GpuCtx, splats: GpuSplatBuffer, t: int, cfg_pos: AdamConfig,
+                 cfg_attr: AdamConfig) -> None:
+        """AdamGpu::step_all (adam_gpu.rs:334-427): Adam on every attribute
+        of the scene from its gradients."""
+        L.check(L.lib().nts_adam_step_all(ctx.h, splats.h, C.byref(cfg_pos.c()), C.byref(cfg_attr.c()), int(t)))
+
+    def apply_constraints(self, ctx: GpuCtx, splats: GpuSplatBuffer, log_scale_min: float = LOG_SCALE_MIN,
+                          log_scale_max: float = LOG_SCALE_MAX,
+                          opacity_logit_max: float = OPACITY_LOGIT_MAX) -> None:
+        """AdamGpu::apply_constraints (adam_gpu.rs:431-489)."""
+        cons = L.Constraints(log_scale_min, log_scale_max, opacity_logit_max)
+        L.check(L.lib().nts_apply_constraints(ctx.h, splats.h, C.byref(cons)))
+
+    def accumulate_grad(self, ctx: GpuCtx, splats: GpuSplatBuffer) -> None:
+        """AdamGpu::accumulate_grad (adam_gpu.rs:492-541): grad_acc +=
+        |dpos|_1 of the scene's position gradient."""
+        L.check(L.lib().nts_accumulate_grad(ctx.h, splats.h))
 
     def step_all_fused(self, ctx: GpuCtx, splats: GpuSplatBuffer, t: int, cfg_pos: AdamConfig,
                        cfg_attr: AdamConfig, log_scale_min: float = LOG_SCALE_MIN,

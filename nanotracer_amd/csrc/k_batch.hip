@@ -35,6 +35,10 @@ struct nts_batch {
   uint64_t* seg_begin = nullptr;
   uint32_t* tile_base = nullptr;
   uint32_t* flags = nullptr;    // [0] overflow, [1] visible count, [2..3] total copy
+  // Errors latched since the last nts_batch_stats: bit 0 a pair overflow,
+  // bit 1 a sort look-back timeout; [1] counts the steps whose Adam update
+  // was skipped for either.  Only nts_batch_stats clears it.
+  uint32_t* sticky = nullptr;
   uint64_t* d_step = nullptr;   // Adam step counter for captured replays
   void* keys_a = nullptr;
   void* keys_b = nullptr;
@@ -80,6 +84,20 @@ struct U32Table {
 __global__ void k_set_u32s(uint32_t* __restrict__ dst, U32Table t) {
   const uint32_t i = threadIdx.x;
   if (blockIdx.x == 0 && i < t.n) dst[i] = t.v[i];
+}
+
+// k_set_u32s that also latches this step's pair overflow (flags[0]) and
+// sort look-back timeout (*sort_err) into the sticky error word; it runs
+// after the sort, so both are final.
+__global__ void k_set_u32s_latch(uint32_t* __restrict__ dst, U32Table t, const uint32_t* __restrict__ flags,
+                                 const uint32_t* __restrict__ sort_err, uint32_t* __restrict__ sticky) {
+  const uint32_t i = threadIdx.x;
+  if (blockIdx.x != 0) return;
+  if (i < t.n) dst[i] = t.v[i];
+  if (i == 0) {
+    const uint32_t bits = (flags[0] ? 1u : 0u) | (*sort_err ? 2u : 0u);
+    if (bits) atomicOr(sticky, bits);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -220,8 +238,17 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
                                                              uint32_t n_views, const DevCamera* __restrict__ cams,
                                                              const ProjectedGrad* __restrict__ pgrad,
                                                              const uint64_t* __restrict__ d_step, AdamCfgs cfg,
-                                                             int write_grads) {
+                                                             int write_grads, const uint32_t* __restrict__ flags,
+                                                             const uint32_t* __restrict__ sort_err,
+                                                             uint32_t* __restrict__ sticky) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  // A step whose pairs overflowed the capacity or whose sort timed out has
+  // truncated or misordered pairs: its update is skipped (parameters, moments,
+  // gradients and grad_acc keep their values) and counted in the sticky word.
+  if (flags[0] | *sort_err) {
+    if (i == 0) atomicAdd(sticky + 1, 1u);
+    return;
+  }
   if (i >= n) return;
   const size_t c = cap;
   const uint64_t t = *d_step;
@@ -359,6 +386,7 @@ extern "C" int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_s
   const uint64_t o_seg = take(8 * (max_views + 1));
   const uint64_t o_tb = take(4 * (max_views + 1));
   const uint64_t o_flags = take(64);
+  const uint64_t o_sticky = take(8);
   const uint64_t o_step = take(8);
   const uint64_t o_ka = take(pair_capacity * 8);
   const uint64_t o_kb = take(pair_capacity * 8);
@@ -388,6 +416,7 @@ extern "C" int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_s
   b->seg_begin = (uint64_t*)(p + o_seg);
   b->tile_base = (uint32_t*)(p + o_tb);
   b->flags = (uint32_t*)(p + o_flags);
+  b->sticky = (uint32_t*)(p + o_sticky);
   b->d_step = (uint64_t*)(p + o_step);
   b->keys_a = p + o_ka;
   b->keys_b = p + o_kb;
@@ -538,7 +567,8 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
     memset(&tb, 0, sizeof tb);
     tb.n = V + 1;
     for (uint32_t v = 0; v <= V; ++v) tb.v[v] = v < V ? pl.vt.v[v].tile_base : pl.vt.total_tiles;
-    NTS_LAUNCH(ctx, "set_tile_base", k_set_u32s, dim3(1), dim3(64), 0, b->tile_base, tb);
+    NTS_LAUNCH(ctx, "set_tile_base", k_set_u32s_latch, dim3(1), dim3(64), 0, b->tile_base, tb,
+               (const uint32_t*)b->flags, (const uint32_t*)b->sort_err, b->sticky);
   }
   uint32_t grid = nts_div_up(b->pair_cap, 256);
   if (grid > 8192) grid = 8192;
@@ -633,12 +663,12 @@ static int record_step(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const Batch
       NTS_LAUNCH(ctx, "backward_adam_views", k_backward_adam_views<true>, dim3(nts_div_up(n, 256)), dim3(256), 0,
                  scene->params, scene->grads, scene->m, scene->v, scene->grad_acc, scene->cap, n, V,
                  (const DevCamera*)b->d_cams, (const ProjectedGrad*)b->pgrad, (const uint64_t*)b->d_step, cfg,
-                 write_grads);
+                 write_grads, (const uint32_t*)b->flags, (const uint32_t*)b->sort_err, b->sticky);
     else
       NTS_LAUNCH(ctx, "backward_adam_views", k_backward_adam_views<false>, dim3(nts_div_up(n, 256)), dim3(256), 0,
                  scene->params, scene->grads, scene->m, scene->v, scene->grad_acc, scene->cap, n, V,
                  (const DevCamera*)b->d_cams, (const ProjectedGrad*)b->pgrad, (const uint64_t*)b->d_step, cfg,
-                 write_grads);
+                 write_grads, (const uint32_t*)b->flags, (const uint32_t*)b->sort_err, b->sticky);
   }
   return NTS_OK;
 }
@@ -650,6 +680,7 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
   NTS_CHECK_ARG(ctx && scene && b && views && targets && cfg_pos && cfg_attr && cons && t > 0);
   NTS_CHECK_ARG(loss_kind == NTS_LOSS_L1 || loss_kind == NTS_LOSS_MSE || loss_kind == NTS_LOSS_SSIM);
   NTS_CHECK_ARG(bwd_mode == NTS_BWD_REF_PARITY || bwd_mode == NTS_BWD_RECOMPUTE);
+  NTS_NEED_TRAINABLE(scene);
   if (loss_kind == NTS_LOSS_SSIM && b->ssim_lambda > 0.0f && !b->ssim_base)
     return nts_fail(NTS_E_INVALID, "NTS_LOSS_SSIM needs nts_batch_set_ssim_lambda first (it allocates the planes)");
   BatchPlan pl;
@@ -675,6 +706,10 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
   };
   const void* ids[3] = {scene, targets->ptr, scene->params};
   put(ids, sizeof ids);
+  // the plane strides baked into the kernels follow the capacity; the
+  // creation number tells apart scenes that reuse freed addresses
+  put(&scene->gen, sizeof scene->gen);
+  put(&scene->cap, 4);
   put(&scene->n, 4);
   put(views, sizeof(nts_view_desc) * n_views);
   int lam_bits;
@@ -770,15 +805,29 @@ extern "C" int nts_batch_losses(nts_ctx* ctx, const nts_batch* b, double* losses
 extern "C" int nts_batch_stats(nts_ctx* ctx, const nts_batch* b, uint64_t* total_pairs, uint32_t* visible,
                                uint32_t* overflow) {
   NTS_CHECK_ARG(ctx && b);
-  uint32_t f[4] = {0, 0, 0, 0};
-  NTS_HIP_TRY(hipMemcpyAsync(f, b->flags, 16, hipMemcpyDeviceToHost, ctx->stream));
-  uint32_t err = 0;
-  if (b->sort_err) NTS_HIP_TRY(hipMemcpyAsync(&err, b->sort_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  // Everything queued before this call has finished before the words are
+  // read (into the context's pinned words), and the latch is cleared before
+  // the call returns.
+  uint32_t* hw = ctx->host_word;  // [0..3] flags, [4..5] sticky, [6] sort error
   NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  if (err) return nts_fail(NTS_E_HIP, "radix sort look-back timed out");
+  hw[6] = 0;
+  NTS_HIP_TRY(hipMemcpyAsync(hw, b->flags, 16, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipMemcpyAsync(hw + 4, b->sticky, 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (b->sort_err) NTS_HIP_TRY(hipMemcpyAsync(hw + 6, b->sort_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  uint32_t f[4], st[2];
+  memcpy(f, hw, 16);
+  memcpy(st, hw + 4, 8);
+  const uint32_t err = hw[6];
+  // the latched errors are reported once
+  int rc = nts_fill_u32(ctx, b->sticky, 0, 2);
+  if (rc) return rc;
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (err || (st[0] & 2u))
+    return nts_fail(NTS_E_HIP, "radix sort look-back timed out (%u step updates skipped)", st[1]);
   if (total_pairs) *total_pairs = (uint64_t)f[2] | ((uint64_t)f[3] << 32);
   if (visible) *visible = f[1];
-  if (overflow) *overflow = f[0];
+  if (overflow) *overflow = (st[0] & 1u) ? (st[1] > 0 ? st[1] : 1u) : 0u;
   return NTS_OK;
 }
 

@@ -40,6 +40,82 @@ static inline uint32_t bits_for(uint32_t n) {  // ceil(log2(n)), at least 1
   return b;
 }
 
+// Offsets of the bins' single allocation.
+struct BinLayout {
+  uint64_t keys, keys2, vals, vals2, ranges, order, meta;
+};
+
+// Emit, sort, ranges and tile order into an allocated bins object.
+static int bin_fill(nts_ctx* ctx, nts_bins* b, const nts_buffer* projected, uint32_t n, const ViewTiles& vt,
+                    const uint32_t* counts, const BinLayout& lay, uint32_t key_bits) {
+  const uint64_t total = b->total_pairs, num_tiles = b->num_tiles;
+  const uint32_t kb = b->key_bytes;
+  int rc = NTS_OK;
+  char* base = (char*)b->base;
+  void* keys_a = base + lay.keys;
+  void* keys_b = base + lay.keys2;
+  uint32_t* vals_a = (uint32_t*)(base + lay.vals);
+  uint32_t* vals_b = (uint32_t*)(base + lay.vals2);
+  b->ranges = (uint2*)(base + lay.ranges);
+  uint64_t* seg_begin = (uint64_t*)(base + lay.meta);
+  uint32_t* tile_base = (uint32_t*)(base + lay.meta + 64);
+  b->keys = keys_a;
+  b->payloads = vals_a;
+
+  rc = nts_fill_u32(ctx, b->ranges, 0, num_tiles * 2);
+  if (rc) return rc;
+  {
+    const uint64_t hs[2] = {0, total};
+    const uint32_t ht[2] = {0, (uint32_t)num_tiles};
+    NTS_HIP_TRY(hipMemcpyAsync(seg_begin, hs, 16, hipMemcpyHostToDevice, ctx->stream));
+    NTS_HIP_TRY(hipMemcpyAsync(tile_base, ht, 8, hipMemcpyHostToDevice, ctx->stream));
+    NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  if (total == 0) return NTS_OK;  // empty fast path: ranges all (0, 0)
+
+  bool alt = false;
+  TileTable tt;
+  memset(&tt, 0, sizeof tt);
+  tt.n = 1;
+  tt.t[0] = vt;
+  RecTable rt;
+  memset(&rt, 0, sizeof rt);
+  rt.n = 1;
+  rt.base[1] = n;
+  if (kb == 4) {
+    NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint32_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
+               (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
+               total, (uint32_t*)keys_a, vals_a);
+    rc = nts_sort_pairs32_dev(ctx, (uint32_t*)keys_a, vals_a, (uint32_t*)keys_b, vals_b, seg_begin, 1, total,
+                              key_bits, &alt);
+  } else {
+    NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint64_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
+               (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
+               total, (uint64_t*)keys_a, vals_a);
+    rc = nts_sort_pairs_dev(ctx, (uint64_t*)keys_a, vals_a, (uint64_t*)keys_b, vals_b, seg_begin, 1, total,
+                            key_bits, &alt);
+  }
+  if (rc) return rc;
+  if (alt) {
+    b->keys = keys_b;
+    b->payloads = vals_b;
+  }
+  uint32_t grid = nts_div_up(total, 256);
+  if (grid > 65535) grid = 65535;
+  if (kb == 4)
+    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint32_t>, dim3(grid), dim3(256), 0, (const uint32_t*)b->keys,
+               (const uint64_t*)seg_begin, 1u, (const uint32_t*)tile_base, b->ranges);
+  else
+    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)b->keys,
+               (const uint64_t*)seg_begin, 1u, (const uint32_t*)tile_base, b->ranges);
+  // the blend order is published only once written (empty bins keep grid order)
+  uint32_t* order = (uint32_t*)(base + lay.order);
+  rc = nts_tile_order(ctx, b->ranges, num_tiles, order);
+  if (rc) return rc;
+  b->order = order;
+  return NTS_OK;
+}
+
 extern "C" int nts_bin(nts_ctx* ctx, const nts_buffer* projected, uint32_t n, const nts_tiling* tiling,
                        nts_bins** out) {
   NTS_CHECK_ARG(ctx && projected && tiling && out);
@@ -102,69 +178,13 @@ extern "C" int nts_bin(nts_ctx* ctx, const nts_buffer* projected, uint32_t n, co
     delete b;
     return rc;
   }
-  char* base = (char*)b->base;
-  void* keys_a = base + off_keys;
-  void* keys_b = base + off_keys2;
-  uint32_t* vals_a = (uint32_t*)(base + off_vals);
-  uint32_t* vals_b = (uint32_t*)(base + off_vals2);
-  b->ranges = (uint2*)(base + off_ranges);
-  uint64_t* seg_begin = (uint64_t*)(base + off_meta);
-  uint32_t* tile_base = (uint32_t*)(base + off_meta + 64);
-  b->keys = keys_a;
-  b->payloads = vals_a;
+  const BinLayout lay{off_keys, off_keys2, off_vals, off_vals2, off_ranges, off_order, off_meta};
+  rc = bin_fill(ctx, b, projected, n, vt, counts, lay, key_bits);
+  if (rc) {  // never hand out a half-built bins object
+    nts_bins_destroy(ctx, b);
+    return rc;
+  }
   *out = b;
-
-  rc = nts_fill_u32(ctx, b->ranges, 0, num_tiles * 2);
-  if (rc) return rc;
-  {
-    const uint64_t hs[2] = {0, total};
-    const uint32_t ht[2] = {0, (uint32_t)num_tiles};
-    NTS_HIP_TRY(hipMemcpyAsync(seg_begin, hs, 16, hipMemcpyHostToDevice, ctx->stream));
-    NTS_HIP_TRY(hipMemcpyAsync(tile_base, ht, 8, hipMemcpyHostToDevice, ctx->stream));
-    NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  }
-  if (total == 0) return NTS_OK;  // empty fast path: ranges all (0, 0)
-
-  bool alt = false;
-  TileTable tt;
-  memset(&tt, 0, sizeof tt);
-  tt.n = 1;
-  tt.t[0] = vt;
-  RecTable rt;
-  memset(&rt, 0, sizeof rt);
-  rt.n = 1;
-  rt.base[1] = n;
-  if (kb == 4) {
-    NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint32_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
-               (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
-               total, (uint32_t*)keys_a, vals_a);
-    rc = nts_sort_pairs32_dev(ctx, (uint32_t*)keys_a, vals_a, (uint32_t*)keys_b, vals_b, seg_begin, 1, total,
-                              key_bits, &alt);
-  } else {
-    NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint64_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
-               (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
-               total, (uint64_t*)keys_a, vals_a);
-    rc = nts_sort_pairs_dev(ctx, (uint64_t*)keys_a, vals_a, (uint64_t*)keys_b, vals_b, seg_begin, 1, total,
-                            key_bits, &alt);
-  }
-  if (rc) return rc;
-  if (alt) {
-    b->keys = keys_b;
-    b->payloads = vals_b;
-  }
-  uint32_t grid = nts_div_up(total, 256);
-  if (grid > 65535) grid = 65535;
-  if (kb == 4)
-    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint32_t>, dim3(grid), dim3(256), 0, (const uint32_t*)b->keys,
-               (const uint64_t*)seg_begin, 1u, (const uint32_t*)tile_base, b->ranges);
-  else
-    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)b->keys,
-               (const uint64_t*)seg_begin, 1u, (const uint32_t*)tile_base, b->ranges);
-  // the blend order is published only once written (empty bins keep grid order)
-  uint32_t* order = (uint32_t*)(base + off_order);
-  rc = nts_tile_order(ctx, b->ranges, num_tiles, order);
-  if (rc) return rc;
-  b->order = order;
   return NTS_OK;
 }
 

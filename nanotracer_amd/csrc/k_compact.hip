@@ -206,11 +206,13 @@ int remove_flagged(nts_ctx* ctx, nts_scene* s, Work& w, uint32_t N, uint32_t m) 
 
 extern "C" int nts_scene_zero_grad_acc(nts_ctx* ctx, nts_scene* s) {
   NTS_CHECK_ARG(ctx && s);
+  NTS_NEED_TRAINABLE(s);
   return nts_fill_u32(ctx, s->grad_acc, 0, s->cap);
 }
 
 extern "C" int nts_prune(nts_ctx* ctx, nts_scene* s, float logit_threshold, uint32_t* removed) {
   NTS_CHECK_ARG(ctx && s);
+  NTS_NEED_TRAINABLE(s);
   if (removed) *removed = 0;
   const uint32_t n = s->n;
   if (n == 0) return NTS_OK;
@@ -237,6 +239,7 @@ extern "C" int nts_prune(nts_ctx* ctx, nts_scene* s, float logit_threshold, uint
 extern "C" int nts_densify(nts_ctx* ctx, nts_scene* s, uint32_t grad_count, float grad_threshold,
                            float split_log_scale, uint32_t max_splats, uint32_t iter, uint32_t counts[3]) {
   NTS_CHECK_ARG(ctx && s);
+  NTS_NEED_TRAINABLE(s);
   if (counts) counts[0] = counts[1] = counts[2] = 0;
   const uint32_t n = s->n;
   if (n == 0) return NTS_OK;

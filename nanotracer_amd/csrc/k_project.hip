@@ -41,6 +41,7 @@ extern "C" int nts_project_backward_ex(nts_ctx* ctx, nts_scene* scene, const nts
                                        const nts_camera* cam, int mode) {
   NTS_CHECK_ARG(ctx && scene && cam && projected_grad);
   NTS_CHECK_ARG(mode == NTS_BWD_REF_PARITY || mode == NTS_BWD_RECOMPUTE);
+  NTS_NEED_TRAINABLE(scene);
   NTS_CHECK_ARG(projected_grad->bytes >= (uint64_t)scene->n * sizeof(ProjectedGrad));
   if (scene->n == 0) return NTS_OK;
   DevCamera dc;

@@ -4,6 +4,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+
 #include "nts_internal.h"
 
 static thread_local char g_err[1024] = "";
@@ -271,9 +273,15 @@ extern "C" int nts_buffer_zero(nts_ctx* ctx, nts_buffer* buf) {
 static const int kAttrPlane[6] = {P_POS, P_ROT, P_SCALE, P_LOGIT, P_DC, P_REST};
 static const int kAttrWidth[6] = {3, 4, 3, 1, 3, 45};
 
-extern "C" int nts_scene_create(nts_ctx* ctx, uint32_t capacity, nts_scene** out) {
-  NTS_CHECK_ARG(ctx && out && capacity > 0);
+uint64_t nts_scene_next_gen() {
+  static std::atomic<uint64_t> next{1};
+  return next.fetch_add(1);
+}
+
+extern "C" int nts_scene_create(nts_ctx* ctx, uint32_t capacity, uint32_t flags, nts_scene** out) {
+  NTS_CHECK_ARG(ctx && out && capacity > 0 && (flags & ~NTS_SCENE_TRAINABLE) == 0);
   *out = nullptr;
+  if (!(flags & NTS_SCENE_TRAINABLE)) return nts_scene_alloc_params(ctx, capacity, out);
   nts_scene* s = new nts_scene();
   // pad each plane to a multiple of 64 floats (256 B) so every plane starts aligned
   const uint64_t cap = ((uint64_t)capacity + 63) & ~63ull;
@@ -285,13 +293,19 @@ extern "C" int nts_scene_create(nts_ctx* ctx, uint32_t capacity, nts_scene** out
     return rc;
   }
   s->cap = (uint32_t)cap;
+  s->gen = nts_scene_next_gen();
+  s->trainable = true;
   s->params = (float*)s->base;
   s->grads = s->params + cap * P_COUNT;
   s->m = s->grads + cap * P_COUNT;
   s->v = s->m + cap * P_COUNT;
   s->grad_acc = s->v + cap * P_COUNT;
   rc = nts_fill_u32(ctx, s->base, 0, s->bytes / 4);
-  if (rc) return rc;
+  if (rc) {
+    nts_dev_free(ctx, s->base, s->bytes);
+    delete s;
+    return rc;
+  }
   *out = s;
   return NTS_OK;
 }
@@ -309,6 +323,8 @@ int nts_scene_alloc_params(nts_ctx* ctx, uint32_t capacity, nts_scene** out) {
     return rc;
   }
   s->cap = (uint32_t)cap;
+  s->gen = nts_scene_next_gen();
+  s->trainable = false;
   s->params = (float*)s->base;
   *out = s;
   return NTS_OK;
@@ -370,9 +386,18 @@ extern "C" int nts_scene_upload(nts_ctx* ctx, nts_scene* scene, const nts_splats
   NTS_CHECK_ARG(ctx && scene && host);
   if (host->n > scene->cap)
     return nts_fail(NTS_E_CAPACITY, "scene capacity %u < %u splats", scene->cap, host->n);
-  int rc = nts_fill_u32(ctx, scene->grads, 0, (uint64_t)scene->cap * (3 * P_COUNT + 1));
-  if (rc) return rc;
+  if (scene->trainable) {
+    int rc = nts_fill_u32(ctx, scene->grads, 0, (uint64_t)scene->cap * (3 * P_COUNT + 1));
+    if (rc) return rc;
+  }
   scene->n = host->n;
+  return nts_upload_planes(ctx, scene, scene->params, host);
+}
+
+extern "C" int nts_scene_sync_params(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* host) {
+  NTS_CHECK_ARG(ctx && scene && host);
+  if (host->n != scene->n)
+    return nts_fail(NTS_E_INVALID, "sync_params needs the scene's splat count (%u), got %u", scene->n, host->n);
   return nts_upload_planes(ctx, scene, scene->params, host);
 }
 
@@ -383,22 +408,26 @@ extern "C" int nts_scene_download(nts_ctx* ctx, const nts_scene* scene, nts_spla
 
 extern "C" int nts_scene_download_grads(nts_ctx* ctx, const nts_scene* scene, nts_splats_host* grads) {
   NTS_CHECK_ARG(ctx && scene && grads);
+  NTS_NEED_TRAINABLE(scene);
   return download_planes(ctx, scene, scene->grads, grads);
 }
 
 extern "C" int nts_scene_upload_grads(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* grads) {
   NTS_CHECK_ARG(ctx && scene && grads && grads->n == scene->n);
+  NTS_NEED_TRAINABLE(scene);
   return nts_upload_planes(ctx, scene, scene->grads, grads);
 }
 
 extern "C" int nts_scene_zero_grads(nts_ctx* ctx, nts_scene* scene) {
   NTS_CHECK_ARG(ctx && scene);
+  NTS_NEED_TRAINABLE(scene);
   return nts_fill_u32(ctx, scene->grads, 0, (uint64_t)scene->cap * P_COUNT);
 }
 
 extern "C" int nts_scene_download_moments(nts_ctx* ctx, const nts_scene* scene, nts_splats_host* m,
                                           nts_splats_host* v) {
   NTS_CHECK_ARG(ctx && scene && m && v);
+  NTS_NEED_TRAINABLE(scene);
   int rc = download_planes(ctx, scene, scene->m, m);
   if (rc) return rc;
   return download_planes(ctx, scene, scene->v, v);
@@ -407,6 +436,7 @@ extern "C" int nts_scene_download_moments(nts_ctx* ctx, const nts_scene* scene, 
 extern "C" int nts_scene_upload_moments(nts_ctx* ctx, nts_scene* scene, const nts_splats_host* m,
                                         const nts_splats_host* v) {
   NTS_CHECK_ARG(ctx && scene && m && v && m->n == scene->n && v->n == scene->n);
+  NTS_NEED_TRAINABLE(scene);
   int rc = nts_upload_planes(ctx, scene, scene->m, m);
   if (rc) return rc;
   return nts_upload_planes(ctx, scene, scene->v, v);
@@ -414,6 +444,7 @@ extern "C" int nts_scene_upload_moments(nts_ctx* ctx, nts_scene* scene, const nt
 
 extern "C" int nts_scene_download_grad_acc(nts_ctx* ctx, const nts_scene* scene, float* host) {
   NTS_CHECK_ARG(ctx && scene && host);
+  NTS_NEED_TRAINABLE(scene);
   if (scene->n == 0) return NTS_OK;
   NTS_HIP_TRY(hipMemcpyAsync(host, scene->grad_acc, (size_t)scene->n * 4, hipMemcpyDeviceToHost, ctx->stream));
   NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -422,6 +453,7 @@ extern "C" int nts_scene_download_grad_acc(nts_ctx* ctx, const nts_scene* scene,
 
 extern "C" int nts_scene_upload_grad_acc(nts_ctx* ctx, nts_scene* scene, const float* host) {
   NTS_CHECK_ARG(ctx && scene && host);
+  NTS_NEED_TRAINABLE(scene);
   if (scene->n == 0) return NTS_OK;
   NTS_HIP_TRY(hipMemcpyAsync(scene->grad_acc, host, (size_t)scene->n * 4, hipMemcpyHostToDevice, ctx->stream));
   NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -92,7 +92,19 @@ struct nts_scene {
   float* grad_acc = nullptr; // 1 plane
   void* base = nullptr;      // single allocation
   uint64_t bytes = 0;
+  uint64_t gen = 0;          // creation number, unique per process (keys captured graphs)
+  bool trainable = true;     // holds gradients, moments and grad_acc
 };
+// Next scene creation number.
+uint64_t nts_scene_next_gen();
+// Calls that read or write gradients, moments or grad_acc need a scene
+// created with NTS_SCENE_TRAINABLE.
+#define NTS_NEED_TRAINABLE(s)                                                               \
+  do {                                                                                      \
+    if (!(s)->trainable)                                                                    \
+      return nts_fail(NTS_E_STATE, "scene holds parameters only (created without "          \
+                                   "NTS_SCENE_TRAINABLE): no gradients, moments or grad_acc"); \
+  } while (0)
 
 struct nts_bins {
   uint64_t total_pairs = 0;

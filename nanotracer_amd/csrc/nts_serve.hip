@@ -129,7 +129,9 @@ struct Slot {
   uint8_t* h_out = nullptr;
   hipEvent_t done = nullptr;
   std::vector<Request*> reqs;
-  std::vector<uint64_t> out_off, pix;
+  // per request: pixel offset in the batch image, pixel count, and byte
+  // offset of its result in h_out (results of both formats packed end to end)
+  std::vector<uint64_t> out_off, pix, stage_off;
   bool busy = false;
 };
 
@@ -249,6 +251,8 @@ int launch_batch(nts_server* sv, Slot& sl) {
   uint64_t pix = 0;
   sl.out_off.assign(V, 0);
   sl.pix.assign(V, 0);
+  sl.stage_off.assign(V, 0);
+  uint64_t stage = 0;
   for (uint32_t v = 0; v < V; ++v) {
     const Request& r = *sl.reqs[v];
     const nts_scene* sc = sv->scenes[r.scene].scene;
@@ -269,6 +273,8 @@ int launch_batch(nts_server* sv, Slot& sl) {
     tb.v[v] = tiles;
     sl.out_off[v] = pix;
     sl.pix[v] = (uint64_t)w * h;
+    sl.stage_off[v] = stage;
+    stage += sl.pix[v] * (r.format == NTS_OUT_RGBA8 ? 4 : 16);
     rec += sc->n;
     tiles += tx * ty;
     pix += (uint64_t)w * h;
@@ -333,10 +339,10 @@ int launch_batch(nts_server* sv, Slot& sl) {
   }
   for (uint32_t v = 0; v < V; ++v) {
     if (sl.reqs[v]->format == NTS_OUT_RGBA8)
-      NTS_HIP_TRY(hipMemcpyAsync(sl.h_out + sl.out_off[v] * 4, sl.rgba + sl.out_off[v], sl.pix[v] * 4,
+      NTS_HIP_TRY(hipMemcpyAsync(sl.h_out + sl.stage_off[v], sl.rgba + sl.out_off[v], sl.pix[v] * 4,
                                  hipMemcpyDeviceToHost, ctx->stream));
     else
-      NTS_HIP_TRY(hipMemcpyAsync(sl.h_out + sl.out_off[v] * 16, sl.image + sl.out_off[v], sl.pix[v] * 16,
+      NTS_HIP_TRY(hipMemcpyAsync(sl.h_out + sl.stage_off[v], sl.image + sl.out_off[v], sl.pix[v] * 16,
                                  hipMemcpyDeviceToHost, ctx->stream));
   }
   sl.h_meta[1] = 0;  // the slot is idle: no copy into h_meta is in flight
@@ -436,7 +442,7 @@ void retire(nts_server* sv, Slot& sl) {
     const double ppx = (double)r->pairs / ((double)r->view.width * r->view.height);
     rs.pairs_per_px = rs.pairs_per_px > 0.0 ? (ppx > 0.9 * rs.pairs_per_px ? ppx : 0.9 * rs.pairs_per_px) : ppx;
     const size_t bytes = sl.pix[v] * (r->format == NTS_OUT_RGBA8 ? 4 : 16);
-    memcpy(r->out, sl.h_out + sl.out_off[v] * (r->format == NTS_OUT_RGBA8 ? 4 : 16), bytes);
+    memcpy(r->out, sl.h_out + sl.stage_off[v], bytes);
     finish(sv, r, REQ_DONE, NTS_OK);
   }
   sl.reqs.clear();

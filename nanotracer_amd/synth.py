@@ -149,3 +149,30 @@ def orbit_views(n_views: int, seed: int, radius: float, resolutions=RESOLUTIONS_
         w, h = resolutions[min(int(u[k, 2] * len(resolutions)), len(resolutions) - 1)]
         views.append((orbit_eye(radius, az, el), w, h))
     return views
+
+
+RESOLUTIONS_E = ((512, 512), (1280, 720), (1920, 1080), (3840, 2160))
+
+
+def serving_scene_sizes(count: int, seed: int, lo: int = 50_000, hi: int = 3_000_000) -> np.ndarray:
+    """Splat counts log-uniform in [lo, hi] (the resident serving scenes)."""
+    rng = np.random.default_rng(seed + 1)
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), count)).astype(np.int64)
+
+
+def serving_scenes(count: int, seed: int, lo: int = 50_000, hi: int = 3_000_000):
+    """`count` resident scenes with splat counts log-uniform in [lo, hi]:
+    windows of one `hi`-splat pool (a per-scene start and scale), so that
+    setting up tens of millions of resident splats costs seconds.  Yields
+    HostSplats in order."""
+    pool = fast_scene(hi, seed, 3.0)
+    counts = serving_scene_sizes(count, seed, lo, hi)
+    rng = np.random.default_rng(seed + 1)
+    rng.uniform(size=count)  # the draws of serving_scene_sizes
+    for n in counts:
+        start = int(rng.integers(0, pool.n))
+        idx = (start + np.arange(n)) % pool.n
+        sc = float(rng.uniform(0.8, 1.2))
+        arrs = [np.ascontiguousarray(a[idx]) for a in pool.arrays()]
+        arrs[0] *= np.float32(sc)
+        yield HostSplats(*arrs)

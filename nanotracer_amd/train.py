@@ -174,7 +174,7 @@ def _norm(a) -> float:
 
 
 def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None = None, on_iter=None,
-          resume: str | None = None, log=print, on_snapshot=None, snapshot_every: int = 1):
+          resume: str | None = None, log=print, on_snapshot=None, snapshot_every: int = 1, on_event=None):
     """The training loop of train.rs:155-401 on the device: view iter %
     len(views) each iteration, one fused Adam step, prune every
     `prune_every` iterations from `prune_warmup`, densify every
@@ -184,8 +184,10 @@ def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None 
     start) or ignored when `resume` names a checkpoint.  on_iter(iter, loss,
     n) is called after every iteration; on_snapshot(iter, HostSplats, loss),
     the reference's on_iter with the splats (train.rs:55-64), every
-    `snapshot_every` iterations and after the last.  Returns the final
-    HostSplats."""
+    `snapshot_every` iterations and after the last.  on_event(iter,
+    stage, scene) is called with stage "before" and "after" around each
+    prune / densify checkpoint (train.rs:259-362), the device scene in
+    hand.  Returns the final HostSplats."""
     cfg = cfg or TrainConfig()
     cap = max(cfg.max_splats, splats.n if splats is not None else 0) + cfg.max_splats
     if resume:
@@ -200,6 +202,8 @@ def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None 
         st.grad_count += 1
         do_prune = it >= cfg.prune_warmup and it % cfg.prune_every == 0
         do_densify = it >= cfg.densify_warmup and it % cfg.densify_every == 0
+        if on_event is not None and (do_prune or do_densify):
+            on_event(it, "before", g)
         if do_prune:
             before = g.n
             removed = g.prune(cfg.prune_opacity_logit)
@@ -213,6 +217,8 @@ def train(ctx: GpuCtx, splats, views, depth_max: float, cfg: TrainConfig | None 
                     f"-> {g.n} splats")
             g.zero_grad_acc()
             st.grad_count = 0
+        if on_event is not None and (do_prune or do_densify):
+            on_event(it, "after", g)
         st.iteration = it + 1
         if on_iter is not None:
             on_iter(it, res.loss, g.n)

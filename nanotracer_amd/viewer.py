@@ -68,7 +68,7 @@ class FrameRenderer:
         self.ctx = ctx
         self.r = Rasterizer()
         self.binner = TileBinner()
-        self.gpu_splats = GpuSplatBuffer.upload(ctx, splats)
+        self.gpu_splats = GpuSplatBuffer.upload(ctx, splats, trainable=False)
         self.last_cpu_splats = splats
         self.image_w, self.image_h = int(width), int(height)
         self.projected = self.r.alloc_projected(ctx, self.gpu_splats.capacity)
@@ -80,14 +80,16 @@ class FrameRenderer:
         self.splat_view = self.ctx.storage_buffer_zeroed(self.image_w * self.image_h * 4)
 
     def update_splats(self, splats: HostSplats) -> None:
-        """lib.rs:712-731: same count -> upload in place; another count ->
+        """lib.rs:712-731: same count -> sync_from in place; another count ->
         a new device scene and projected buffer.  A count within the
         current capacity reuses both."""
-        if splats.n <= self.gpu_splats.capacity:
+        if splats.n == self.gpu_splats.n:
             self.gpu_splats.sync_from(splats)
+        elif splats.n <= self.gpu_splats.capacity:
+            self.gpu_splats.upload_from(splats)
         else:
             self.gpu_splats.free()
-            self.gpu_splats = GpuSplatBuffer.upload(self.ctx, splats)
+            self.gpu_splats = GpuSplatBuffer.upload(self.ctx, splats, trainable=False)
             self.projected.free()
             self.projected = self.r.alloc_projected(self.ctx, self.gpu_splats.capacity)
         self.last_cpu_splats = splats

@@ -81,7 +81,7 @@ def test_config_d_gradient_by_central_differences(ctx):
     scale = 3.0 * W * H  # compare on the summed loss
     rng = np.random.default_rng(0x5EED0005)
     vis = np.nonzero(fdc.proj32[:, 7] != 0)[0]
-    report = {}
+    report, outliers, samples = {}, {}, {}
     for name, attr, width, count in SPLIT:
         ga = getattr(grads, name)
         idx = rng.choice(vis, count, replace=False).astype(np.uint32)
@@ -101,17 +101,36 @@ def test_config_d_gradient_by_central_differences(ctx):
                             pass_h1e3=int((ok3 & c3).sum()), pass_mse=int((okm & cm).sum()),
                             continuous_mse=int(cm.sum()))
         bad = np.nonzero(c5 & ~ok5)[0]
+        outliers[name] = [int(k) for k in bad]
+        samples[name] = (idx, comp, gv, fd5, c5)
         if bad.size:
             print(name, "outside the tolerance:", [(int(idx[k]), int(comp[k]), float(gv[k]), float(fd5[k]))
                                                   for k in bad[:8]])
     print("\nconfig D central differences:", report)
-    # The rare parameter just outside 1e-4 (an opacity at 2e-4 in the seeded
-    # set) is reproduced to 7 digits by the float32 host restatement: the
-    # float32 dL/dalpha = G (T c - S / (1 - alpha)) cancels when a splat's
-    # colour matches the colour behind it.  Per attribute: 99% of the
-    # continuous parameters within the per-parameter acceptance, and the
-    # norm-wise error within 3e-4.
-    for name, rep in report.items():
+    # A continuous parameter outside the per-parameter acceptance is accepted
+    # only where the float32 host restatement of the same backward (the
+    # oracle: recompute backward + full chain rule over the same binning and
+    # dL/dC) misses the float64 differences too, by at least half the
+    # acceptance (the method in float32 cannot meet 1e-4 there: dL/dalpha =
+    # G (T c - S / (1 - alpha)) and the conic's moment sums cancel where
+    # a splat's colour matches the colour behind it), and the device stays
+    # within 5e-4 of the differences.  Over the remaining continuous
+    # parameters the norm-wise error per attribute is below 1e-4.
+    if any(outliers.values()):
+        _, ft32, li32 = O.composite(fdc.proj32, fdc.pay, fdc.rg, fdc.t, with_state=True)
+        pg32 = O.composite_backward_recompute(fdc.proj32, fdc.pay, fdc.rg, fdc.t, ft32, li32, weights)
+        g32 = O.project_backward(s, ocam, pg32, full=True)
+    for name, attr, width, count in SPLIT:
+        rep, (idx, comp, gv, fd5, c5) = report[name], samples[name]
         assert rep["continuous"] >= 0.5 * rep["params"], (name, rep)
-        assert rep["pass_vjp"] >= 0.99 * rep["continuous"], (name, rep)
-        assert rep["normwise_vjp"] < 3e-4, (name, rep)
+        keep = c5.copy()
+        for k in outliers[name]:
+            a32 = getattr(g32, name)
+            g_host = float(a32[idx[k]] if a32.ndim == 1 else a32[idx[k], comp[k]]) * scale
+            dev_err, host_err = abs(gv[k] - fd5[k]) / abs(fd5[k]), abs(g_host - fd5[k]) / abs(fd5[k])
+            print(f"{name} outlier {int(idx[k])}.{int(comp[k])}: device {gv[k]:.9g} ({dev_err:.1e}), float32 host "
+                  f"{g_host:.9g} ({host_err:.1e}), float64 differences {fd5[k]:.9g}")
+            assert host_err >= 0.5e-4 and dev_err <= 5e-4, (name, int(idx[k]), int(comp[k]), dev_err, host_err)
+            keep[k] = False
+        nrm = float(np.linalg.norm((gv - fd5)[keep]) / max(np.linalg.norm(fd5[keep]), 1e-300))
+        assert nrm < 1e-4, (name, nrm, rep)

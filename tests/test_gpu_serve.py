@@ -58,6 +58,25 @@ def test_mixed_batches_equal_solo_renders(ctx, fmt):
     srv.close()
 
 
+def test_mixed_formats_in_one_batch(ctx):
+    # rgba8 (4 B per pixel) and float (16 B per pixel) results of one batch
+    # share the pinned staging buffer: each must come back whole
+    scenes = _scenes()
+    srv = RenderServer(slots=1, max_views=16, max_pixels=1 << 21, max_records=200_000, pair_capacity=8_000_000)
+    for s in scenes:
+        srv.add_scene(s)
+    reqs = _requests(scenes, 16, 4)
+    fmts = [L.OUT_F32 if k % 3 == 0 else L.OUT_RGBA8 for k in range(16)]
+    ids = [srv.submit(sid, v, f)[0] for (sid, v), f in zip(reqs, fmts)]
+    res = [srv.wait(i) for i in ids]
+    assert max(info["batch_views"] for _, info in res) > 1
+    for (sid, v), f, (img, _) in zip(reqs, fmts, res):
+        solo, info = srv.wait(srv.submit(sid, v, f)[0])
+        assert info["batch_views"] == 1
+        assert np.array_equal(img.view(np.uint8), solo.view(np.uint8))
+    srv.close()
+
+
 @pytest.mark.parametrize("tile,lds", [(16, 128), (32, 512)])
 def test_float_results_match_the_host_forward(ctx, tile, lds):
     # images depend on the tile size (the reference bins by tile rectangles):
