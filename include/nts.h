@@ -180,6 +180,19 @@ int nts_scan_u32(nts_ctx* ctx, nts_buffer* data, uint64_t n);
  * payloads, in place.  key_bits limits the sorted low bits (32 = all). */
 int nts_sort_pairs_u32(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits);
 int nts_sort_pairs_u64(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  nts_sort_pairs_u32 with the same result and errors, but every
+ * buffer beyond keys and vals (the ping-pong pairs, digit histograms and
+ * offsets, look-back status words, counters) lives in one static device
+ * array of 18.9 MB per GPU, held until the process ends: nothing is
+ * allocated.  At most NTS_SORT_NOALLOC_MAX_PAIRS pairs (NTS_E_CAPACITY
+ * beyond).  All calls on one GPU share that array, so calls that overlap on
+ * one GPU (several contexts or threads) race; sequential calls are safe.
+ * Like nts_sort_pairs_u32 it waits for the sort to finish (it reads the
+ * look-back error word into the context's pinned word). */
+#define NTS_SORT_NOALLOC_MAX_PAIRS (1u << 21)
+int nts_sort_pairs_u32_noalloc(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits);
 
 /* ---- composite (Rasterizer::composite, raster.rs:468-529) ------------- */
 /* image: 16 B per pixel (r, g, b, 1 - T).  Any tile size in {8, 16, 32};

@@ -18,6 +18,7 @@ NTS_E_INVALID, NTS_E_HIP, NTS_E_NOMEM, NTS_E_CAPACITY, NTS_E_STATE = -1, -2, -3,
 ERRORS = {-1: "NTS_E_INVALID", -2: "NTS_E_HIP", -3: "NTS_E_NOMEM", -4: "NTS_E_CAPACITY", -5: "NTS_E_STATE"}
 LOSS_L1, LOSS_MSE, LOSS_SSIM = 0, 1, 2
 SCENE_TRAINABLE = 1
+SORT_NOALLOC_MAX_PAIRS = 1 << 21
 BWD_REF_PARITY, BWD_RECOMPUTE = 0, 1
 
 
@@ -151,6 +152,7 @@ _SIGS = {
     "nts_scan_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "nts_sort_pairs_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32]),
     "nts_sort_pairs_u64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32]),
+    "nts_sort_pairs_u32_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32]),
     "nts_composite": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Tiling)]),
     "nts_composite_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Tiling), C.c_void_p,
                                    C.c_void_p]),

@@ -353,6 +353,12 @@ class RadixSort:
     def sort_u64(self, ctx: GpuCtx, keys: Buffer, vals: Buffer, n: int, key_bits: int = 64) -> None:
         L.check(L.lib().nts_sort_pairs_u64(ctx.h, keys.h, vals.h, int(n), int(key_bits)))
 
+    def sort_noalloc(self, ctx: GpuCtx, keys: Buffer, vals: Buffer, n: int, key_bits: int = 32) -> None:
+        """sort() with no device allocation: the library's static per-GPU
+        workspace, at most SORT_NOALLOC_MAX_PAIRS pairs; concurrent calls on
+        one GPU race."""
+        L.check(L.lib().nts_sort_pairs_u32_noalloc(ctx.h, keys.h, vals.h, int(n), int(key_bits)))
+
 
 class Rasterizer:
     """Rasterizer (raster.rs:114-804)."""

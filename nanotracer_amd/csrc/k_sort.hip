@@ -1,4 +1,6 @@
 // Exclusive u32 scan and the segmented onesweep radix sort.
+#include <mutex>
+
 #include "nts_sort.h"
 
 // ---------------------------------------------------------------------------
@@ -553,4 +555,107 @@ extern "C" int nts_sort_pairs_u32(nts_ctx* ctx, nts_buffer* keys, nts_buffer* va
 
 extern "C" int nts_sort_pairs_u64(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits) {
   return sort_api<uint64_t>(ctx, keys, vals, n, key_bits);
+}
+
+// ---------------------------------------------------------------------------
+// nts_sort_pairs_u32_noalloc: nts_sort_pairs_u32 with every buffer beyond
+// the caller's keys and values in one file-scope device array, for callers
+// that cannot allocate device memory.  The array holds the segment bounds,
+// the ping-pong keys and values and the sort workspace (histograms, digit
+// offsets, look-back status words, partition counters, the error word) for
+// up to NTS_SORT_NOALLOC_MAX_PAIRS pairs: 18.9 MB, held on each GPU the
+// library runs on until the process ends.  Every call on a GPU uses that
+// GPU's one copy, so calls that overlap on one GPU race; whatever must start
+// at zero is cleared by each call (fill kernels), nothing is allocated.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint64_t na_al(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint64_t kNaMax = NTS_SORT_NOALLOC_MAX_PAIRS;
+constexpr uint64_t kNaParts = (kNaMax + RS_TILE - 1) / RS_TILE + 1;
+constexpr uint64_t kNaWs = 4 * (256 + 2 * (4 * RS_RADIX) + 64 + 4 * kNaParts * RS_RADIX);  // nts_sort_ws_bytes(1, max, 4)
+constexpr uint64_t kNaOffKeys = 256;
+constexpr uint64_t kNaOffVals = kNaOffKeys + na_al(4 * kNaMax);
+constexpr uint64_t kNaOffWs = kNaOffVals + na_al(4 * kNaMax);
+constexpr uint64_t kNaBytes = kNaOffWs + na_al(kNaWs);
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+std::mutex g_na_mu;
+void* g_na_ptr[64] = {};  // the device address of `scratch` on each device
+
+// The address of this device's copy of `scratch`, looked up once per device
+// with that device current.
+int noalloc_scratch(nts_ctx* ctx, char** out) {
+  if (ctx->device < 0 || ctx->device >= 64) return nts_fail(NTS_E_INVALID, "device %d", ctx->device);
+  std::lock_guard<std::mutex> g(g_na_mu);
+  if (!g_na_ptr[ctx->device]) {
+    int prev = 0;
+    NTS_HIP_TRY(hipGetDevice(&prev));
+    NTS_HIP_TRY(hipSetDevice(ctx->device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p)
+      return nts_fail(NTS_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_na_ptr[ctx->device] = p;
+  }
+  *out = (char*)g_na_ptr[ctx->device];
+  return NTS_OK;
+}
+}  // namespace
+
+__global__ void k_set_one_segment(uint64_t* __restrict__ seg, uint64_t n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    seg[0] = 0;
+    seg[1] = n;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_copy_pairs_u32(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ vs,
+                                                        uint32_t* __restrict__ kd, uint32_t* __restrict__ vd,
+                                                        uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    kd[i] = ks[i];
+    vd[i] = vs[i];
+  }
+}
+
+extern "C" int nts_sort_pairs_u32_noalloc(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n,
+                                          uint32_t key_bits) {
+  NTS_CHECK_ARG(ctx && keys && vals);
+  NTS_CHECK_ARG(n * 4 <= keys->bytes && n * 4 <= vals->bytes);
+  NTS_CHECK_ARG(key_bits <= 32);
+  if (n > kNaMax)
+    return nts_fail(NTS_E_CAPACITY, "nts_sort_pairs_u32_noalloc sorts at most %llu pairs (got %llu)",
+                    (unsigned long long)kNaMax, (unsigned long long)n);
+  if (n <= 1) return NTS_OK;
+  char* base = nullptr;
+  int rc = noalloc_scratch(ctx, &base);
+  if (rc) return rc;
+  uint64_t* seg = (uint64_t*)base;
+  uint32_t* k_alt = (uint32_t*)(base + kNaOffKeys);
+  uint32_t* v_alt = (uint32_t*)(base + kNaOffVals);
+  void* ws = base + kNaOffWs;
+  NTS_LAUNCH(ctx, "sort_noalloc_segment", k_set_one_segment, dim3(1), dim3(64), 0, seg, n);
+  bool alt = false;
+  uint32_t* err = nullptr;
+  rc = sort_segmented_ws<uint32_t>(ctx, (uint32_t*)keys->ptr, (uint32_t*)vals->ptr, k_alt, v_alt, seg, 1, n,
+                                   key_bits, ws, &alt, &err);
+  if (rc) return rc;
+  if (alt) {
+    uint32_t grid = nts_div_up(n, 256);
+    if (grid > 4096) grid = 4096;
+    NTS_LAUNCH(ctx, "sort_noalloc_copy", k_copy_pairs_u32, dim3(grid), dim3(256), 0, (const uint32_t*)k_alt,
+               (const uint32_t*)v_alt, (uint32_t*)keys->ptr, (uint32_t*)vals->ptr, n);
+  }
+  // the look-back error word, read like nts_sort_pairs_u32 reads it (the
+  // context's pinned word: no allocation)
+  uint32_t* hw = ctx->host_word;
+  NTS_HIP_TRY(hipMemcpyAsync(hw, err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (*hw) return nts_fail(NTS_E_HIP, "radix sort look-back timed out");
+  return NTS_OK;
 }
