@@ -362,7 +362,8 @@ def run_batched(args, ctx):
         config=dict(workload="batched training iteration: 1M splats, 16 views of mixed resolution per launch, "
                              "hipGraph replay", splats=s.n, views=N_VIEWS, pixels=pixels, tile=tile,
                     lds_batch=dict(forward=lds[0], backward=lds[1]),
-                    tuning="autotuned during warm-up" if tune else "fixed", pairs=pairs,
+                    tuning="autotuned during warm-up" if tune else "fixed",
+                    scheduler=tune.scheduler if tune else "grid", pairs=pairs,
                     visible_records=visible, backward=args.backward, loss="L1 summed over views"),
         views_per_s=round(iters_s * N_VIEWS, 2), train_mpx_s=round(iters_s * pixels / 1e6, 2),
         final_loss=float(losses.sum()),
@@ -373,7 +374,8 @@ def run_batched(args, ctx):
     if ssim:
         out["ssim_step"] = ssim
     if tune:
-        out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), table=tune.table)
+        out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), blend_ms_by_scheduler=tune.blend_ms,
+                               table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
                               gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
     if args.serve_rate > 0:

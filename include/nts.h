@@ -314,6 +314,17 @@ int nts_render_views(nts_ctx* ctx, const nts_scene* scene, nts_batch* batch, con
  * default.  No reference counterpart: the reference stages 256
  * (rasterize.wgsl:40). */
 int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* batch, uint32_t fwd_batch, uint32_t bwd_batch);
+/* How the batch's blend kernels take their tiles.  Both take them most pairs
+ * first (the order TileBinner-style ranges give, 4 buckets per octave).
+ * NTS_SCHED_GRID (the default): one workgroup per (view, tile), the
+ * reference's dispatch (raster.rs:526,721) in that order.
+ * NTS_SCHED_PERSISTENT: as many workgroups as the device holds at once,
+ * each taking the next tile from a device counter until none is left.
+ * Images and gradient terms do not depend on the choice.  No reference
+ * counterpart. */
+#define NTS_SCHED_GRID 0
+#define NTS_SCHED_PERSISTENT 1
+int nts_batch_set_scheduler(nts_ctx* ctx, nts_batch* batch, int scheduler);
 /* Blend factor lambda of NTS_LOSS_SSIM steps, (1 - lambda) MSE + lambda
  * (1 - SSIM) (TrainConfig::ssim_lambda, train.rs:48-52; 0.2 by default,
  * main.rs:189-190), clamped to [0, 1].  The first lambda > 0 allocates the

@@ -18,6 +18,7 @@ NTS_E_INVALID, NTS_E_HIP, NTS_E_NOMEM, NTS_E_CAPACITY, NTS_E_STATE = -1, -2, -3,
 ERRORS = {-1: "NTS_E_INVALID", -2: "NTS_E_HIP", -3: "NTS_E_NOMEM", -4: "NTS_E_CAPACITY", -5: "NTS_E_STATE"}
 LOSS_L1, LOSS_MSE, LOSS_SSIM = 0, 1, 2
 SCENE_TRAINABLE = 1
+SCHED_GRID, SCHED_PERSISTENT = 0, 1
 SORT_NOALLOC_MAX_PAIRS = 1 << 21
 BWD_REF_PARITY, BWD_RECOMPUTE = 0, 1
 
@@ -201,6 +202,7 @@ _SIGS = {
                               C.POINTER(C.c_uint32)]),
     "nts_batch_set_lds_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "nts_batch_set_ssim_lambda": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
+    "nts_batch_set_scheduler": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "nts_batch_losses": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nts_batch_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                   C.POINTER(C.c_uint32)]),

@@ -539,6 +539,12 @@ class ViewBatch:
         """Splats staged per LDS batch by the blend forward / backward."""
         L.check(L.lib().nts_batch_set_lds_batch(self.ctx.h, self.h, int(fwd_batch), int(bwd_batch)))
 
+    def set_scheduler(self, scheduler: int = L.SCHED_GRID) -> None:
+        """Tile scheduler of the blend kernels: SCHED_GRID (one workgroup per
+        tile) or SCHED_PERSISTENT (resident workgroups take tiles from a
+        device counter)."""
+        L.check(L.lib().nts_batch_set_scheduler(self.ctx.h, self.h, int(scheduler)))
+
     def set_ssim_lambda(self, lam: float = 0.2) -> None:
         """Blend factor of LOSS_SSIM steps: (1 - lam) MSE + lam (1 - SSIM)
         (train.rs:48-52); allocates the SSIM planes on first use."""

@@ -69,6 +69,8 @@ struct nts_batch {
   uint32_t last_tile_base[NTS_MAX_VIEWS + 1] = {};
   // splats staged per LDS batch by the blend kernels
   uint32_t fwd_batch = NTS_DEFAULT_FWD_BATCH, bwd_batch = NTS_DEFAULT_BWD_BATCH;
+  // tile scheduler of the blend kernels (NTS_SCHED_*)
+  int scheduler = NTS_SCHED_GRID;
   // combined MSE + SSIM loss of NTS_LOSS_SSIM steps (planes allocated by the setter)
   float ssim_lambda = 0.2f;
   void* ssim_base = nullptr;
@@ -513,6 +515,9 @@ static int plan_batch(nts_batch* b, const nts_scene* scene, const nts_view_desc*
   if ((uint64_t)n_views * scene->n >= (1ull << 32)) return nts_fail(NTS_E_CAPACITY, "views x splats >= 2^32");
   pl.vt.total_tiles = (uint32_t)tiles;
   pl.vt.order = b->order;
+  // the persistent scheduler's tile counters: flags[4] (forward) and
+  // flags[5] (backward), zeroed with the flags at the start of every launch
+  pl.vt.tickets = b->scheduler == NTS_SCHED_PERSISTENT ? b->flags + 4 : nullptr;
   pl.pixels = pix;
   uint32_t tb = 1;
   while (tb < 32 && (1ull << tb) < max_tiles_view) ++tb;
@@ -714,8 +719,8 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
   put(views, sizeof(nts_view_desc) * n_views);
   int lam_bits;
   memcpy(&lam_bits, &b->ssim_lambda, 4);
-  const int meta[7] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch,
-                       loss_kind == NTS_LOSS_SSIM ? lam_bits : 0};
+  const int meta[8] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch,
+                       loss_kind == NTS_LOSS_SSIM ? lam_bits : 0, b->scheduler};
   put(meta, sizeof meta);
   put(cfg_pos, sizeof *cfg_pos);
   put(cfg_attr, sizeof *cfg_attr);
@@ -775,6 +780,12 @@ extern "C" int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* b, uint32_t fwd_
                     fwd_batch, bwd_batch);
   b->fwd_batch = fwd_batch;
   b->bwd_batch = bwd_batch;
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_set_scheduler(nts_ctx* ctx, nts_batch* b, int scheduler) {
+  NTS_CHECK_ARG(ctx && b && (scheduler == NTS_SCHED_GRID || scheduler == NTS_SCHED_PERSISTENT));
+  b->scheduler = scheduler;
   return NTS_OK;
 }
 

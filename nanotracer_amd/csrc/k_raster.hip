@@ -7,6 +7,9 @@
 // alpha >= 1/255 region misses the wave's pixels, a pixel skips the
 // exponential below the splat's power threshold, and a block-wide vote ends
 // the tile once every pixel is saturated.
+#include <map>
+#include <mutex>
+
 #include "nts_raster.h"
 
 // Thread layout of a (view, tile) workgroup with at most MAXT threads.
@@ -211,7 +214,7 @@ __device__ __forceinline__ float4 pixel_loss_grad(float4 p, float4 t, float inv_
 // A 1024-thread workgroup is 4 waves per SIMD: two of them share a CU only
 // within 64 VGPRs (8 waves per SIMD), which the attribute holds the compiler
 // to; one per CU leaves half the SIMD slots idle at every barrier.
-template <int TS, int BATCH, bool LOSS>
+template <int TS, int BATCH, bool LOSS, bool PERSIST>
 __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT))
 __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 1))) k_blend_fwd(const Projected* __restrict__ proj,
                                                              const uint32_t* __restrict__ payloads,
@@ -224,7 +227,24 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
   __shared__ float4 s_c[BATCH];  // b, power threshold, cull half-extents x / y
   __shared__ float4 s_d[BATCH];  // per-wave test: threshold, -b/c, -b/a, c
-  const uint32_t gt = vt.order ? vt.order[blockIdx.x] : blockIdx.x;
+  __shared__ double s_loss[NT / 64];
+  __shared__ uint32_t s_ticket;
+  // One pass per tile; `slot` is the tile's place in the blend order.  The
+  // persistent form takes tiles from the forward counter until it passes the
+  // last, so every workgroup leaves once the tiles are handed out (the
+  // barrier first lets the previous tile's LDS reads finish).
+  for (uint32_t slot = blockIdx.x;;) {
+  if (PERSIST) {
+    __syncthreads();  // the previous tile's LDS reads are done
+    if (threadIdx.x == 0) s_ticket = atomicAdd(vt.tickets, 1u);
+    // The ticket's LDS write completes before the barrier (lgkmcnt(0)), so
+    // no wave can read the previous tile's ticket after it.
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __syncthreads();
+    slot = __builtin_amdgcn_readfirstlane(s_ticket);  // uniform: the tile's values stay scalar
+    if (slot >= vt.total_tiles) break;
+  }
+  const uint32_t gt = vt.order ? vt.order[slot] : slot;
   const uint32_t vi = view_of_tile(vt, gt);
   const ViewDesc vd = vt.v[vi];
   const uint32_t lt = gt - vd.tile_base;
@@ -311,7 +331,6 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
     if (LOSS) la.dl_dc[pix] = pixel_loss_grad(px, la.target[pix], vd.inv_n, la.kind, lacc);
   }
   if (LOSS) {
-    __shared__ double s_loss[NT / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) lacc += __shfl_xor(lacc, off, 64);
     if ((threadIdx.x & 63u) == 0) s_loss[threadIdx.x >> 6] = lacc;
@@ -322,6 +341,8 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
       for (int w = 0; w < NT / 64; ++w) sum += s_loss[w];
       unsafeAtomicAdd(la.losses + vi, sum);
     }
+  }
+  if (!PERSIST) break;
   }
   if (la.clear) {
     const uint64_t per = (la.clear_n4 + gridDim.x - 1) / gridDim.x;
@@ -422,10 +443,12 @@ __device__ __forceinline__ uint32_t grad_term_of_quad(uint32_t q) {
 
 // A 256-thread workgroup is one wave per SIMD; 80 VGPRs (6 waves per SIMD)
 // hold without spills, and the LDS of a 128- or 256-splat batch allows that
-// many workgroups per CU.
-template <int TS, int BATCH, bool RECOMP>
+// many workgroups per CU.  The persistent form keeps some per-thread setup
+// live across its tile loop and would spill at 80: it is held to 5 waves per
+// SIMD (96 VGPRs).
+template <int TS, int BATCH, bool RECOMP, bool PERSIST>
 __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT))
-__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? 6 : 1))) k_blend_bwd(
+__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (PERSIST ? 5 : 6) : 1))) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
     ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
     const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
@@ -438,7 +461,20 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? 6 : 1
   __shared__ uint32_t s_idx[BATCH];
   __shared__ float s_acc[BATCH][9];  // each slot's terms summed over the waves (LDS float atomics)
   __shared__ uint32_t s_max;
-  const uint32_t gt = vt.order ? vt.order[blockIdx.x] : blockIdx.x;
+  __shared__ uint32_t s_ticket;
+  // One pass per tile, as the forward (with the backward's counter).
+  for (uint32_t slot = blockIdx.x;;) {
+  if (PERSIST) {
+    __syncthreads();  // the previous tile's LDS reads are done
+    if (threadIdx.x == 0) s_ticket = atomicAdd(vt.tickets + 1, 1u);
+    // The ticket's LDS write completes before the barrier (lgkmcnt(0)), so
+    // no wave can read the previous tile's ticket after it.
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __syncthreads();
+    slot = __builtin_amdgcn_readfirstlane(s_ticket);  // uniform: the tile's values stay scalar
+    if (slot >= vt.total_tiles) break;
+  }
+  const uint32_t gt = vt.order ? vt.order[slot] : slot;
   const uint32_t vi = view_of_tile(vt, gt);
   const ViewDesc vd = vt.v[vi];
   const uint32_t lt = gt - vd.tile_base;
@@ -668,6 +704,8 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? 6 : 1
     }
     rem -= take;
   }
+  if (!PERSIST) break;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -698,17 +736,57 @@ bool nts_blend_config_ok(uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_ba
   return ts && fb && bb;
 }
 
+// Workgroups of `kernel` (block threads) resident on the device at once:
+// the persistent scheduler's grid.  Looked up once per kernel and device.
+template <typename KernelT>
+static uint32_t resident_workgroups(KernelT kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, uint32_t> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  const auto key = std::make_pair(reinterpret_cast<const void*>(kernel), dev);
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  const uint32_t r = (uint32_t)((per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1));
+  cache[key] = r;
+  return r;
+}
+
+template <int TS, int B, bool LOSS>
+static int launch_fwd_kernel(nts_ctx* ctx, const char* name, const Projected* proj, const uint32_t* payloads,
+                             const uint2* ranges, const ViewTable& vt, float4* out, float* final_t,
+                             uint32_t* last_idx, const LossArgs& la) {
+  constexpr int NT = Blend<TS, NTS_FWD_THREADS>::NT;
+  if (vt.tickets) {
+    const uint32_t r = resident_workgroups(k_blend_fwd<TS, B, LOSS, true>, NT);
+    const dim3 grid(r < vt.total_tiles ? r : vt.total_tiles);
+    NTS_LAUNCH(ctx, name, (k_blend_fwd<TS, B, LOSS, true>), grid, dim3(NT), 0, proj, payloads, ranges, vt, out,
+               final_t, last_idx, la);
+  } else {
+    NTS_LAUNCH(ctx, name, (k_blend_fwd<TS, B, LOSS, false>), dim3(vt.total_tiles), dim3(NT), 0, proj, payloads,
+               ranges, vt, out, final_t, last_idx, la);
+  }
+  return NTS_OK;
+}
+
 template <int TS, int B>
 static int launch_fwd(nts_ctx* ctx, dim3 grid, bool loss, const Projected* proj, const uint32_t* payloads,
                        const uint2* ranges, const ViewTable& vt, float4* out, float* final_t, uint32_t* last_idx,
                        const LossArgs& la) {
   if (loss)
-    NTS_LAUNCH(ctx, "blend_fwd_loss", (k_blend_fwd<TS, B, true>), grid, dim3(Blend<TS, NTS_FWD_THREADS>::NT), 0, proj, payloads,
-               ranges, vt, out, final_t, last_idx, la);
-  else
-    NTS_LAUNCH(ctx, "blend_fwd", (k_blend_fwd<TS, B, false>), grid, dim3(Blend<TS, NTS_FWD_THREADS>::NT), 0, proj, payloads, ranges,
-               vt, out, final_t, last_idx, la);
-  return NTS_OK;
+    return launch_fwd_kernel<TS, B, true>(ctx, "blend_fwd_loss", proj, payloads, ranges, vt, out, final_t, last_idx,
+                                          la);
+  return launch_fwd_kernel<TS, B, false>(ctx, "blend_fwd", proj, payloads, ranges, vt, out, final_t, last_idx, la);
 }
 
 template <int TS>
@@ -739,17 +817,32 @@ int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, cons
   }
 }
 
+template <int TS, int B, bool RECOMP>
+static int launch_bwd_kernel(nts_ctx* ctx, const char* name, const Projected* proj, const uint32_t* payloads,
+                             const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
+                             const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
+  constexpr int NT = Blend<TS, NTS_BWD_THREADS>::NT;
+  if (vt.tickets) {
+    const uint32_t r = resident_workgroups(k_blend_bwd<TS, B, RECOMP, true>, NT);
+    const dim3 grid(r < vt.total_tiles ? r : vt.total_tiles);
+    NTS_LAUNCH(ctx, name, (k_blend_bwd<TS, B, RECOMP, true>), grid, dim3(NT), 0, proj, payloads, ranges, vt,
+               fwd_out, final_t, last_idx, dl_dc, pgrad);
+  } else {
+    NTS_LAUNCH(ctx, name, (k_blend_bwd<TS, B, RECOMP, false>), dim3(vt.total_tiles), dim3(NT), 0, proj, payloads,
+               ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+  }
+  return NTS_OK;
+}
+
 template <int TS, int B>
 static int launch_bwd(nts_ctx* ctx, dim3 grid, int mode, const Projected* proj, const uint32_t* payloads,
                        const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
                        const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
   if (mode == NTS_BWD_RECOMPUTE)
-    NTS_LAUNCH(ctx, "blend_bwd_recompute", (k_blend_bwd<TS, B, true>), grid, dim3(Blend<TS, NTS_BWD_THREADS>::NT), 0, proj, payloads,
-               ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
-  else
-    NTS_LAUNCH(ctx, "blend_bwd", (k_blend_bwd<TS, B, false>), grid, dim3(Blend<TS, NTS_BWD_THREADS>::NT), 0, proj, payloads, ranges,
-               vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
-  return NTS_OK;
+    return launch_bwd_kernel<TS, B, true>(ctx, "blend_bwd_recompute", proj, payloads, ranges, vt, fwd_out, final_t,
+                                          last_idx, dl_dc, pgrad);
+  return launch_bwd_kernel<TS, B, false>(ctx, "blend_bwd", proj, payloads, ranges, vt, fwd_out, final_t, last_idx,
+                                         dl_dc, pgrad);
 }
 
 template <int TS>

@@ -36,6 +36,10 @@ struct ViewTable {
   uint32_t n;
   uint32_t total_tiles;
   const uint32_t* order;  // global tile of each workgroup (nullptr: grid order)
+  // Persistent tile scheduler (nullptr: one workgroup per tile): a grid of
+  // resident workgroups takes tiles, in `order`, from these two counters
+  // (forward, backward), which must be zero when the launch starts.
+  uint32_t* tickets;
   ViewDesc v[NTS_MAX_VIEWS];
 };
 

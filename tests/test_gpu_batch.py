@@ -322,9 +322,55 @@ def test_autotuner_picks_a_candidate_and_trains(ctx):
     b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 4_000_000)
     r = autotune_batch(ctx, b, g, specs, tbuf, 1, tiles=(16, 32), fwd_batches=(128, 256), bwd_batches=(64, 128))
     assert r.tile in (16, 32) and r.fwd_batch in (128, 256) and r.bwd_batch in (64, 128)
-    assert r.next_t == 1 + 2 * 2 * 2 and r.step_ms > 0
-    assert len([row for row in r.table if "fwd_ms" in row]) == 4
+    # two steps per timed configuration, then two timed pairs per scheduler
+    assert r.next_t == 1 + 2 * 2 * 2 + 2 * 2 * 2 and r.step_ms > 0
+    assert len([row for row in r.table if "fwd_ms" in row and "scheduler" not in row]) == 4
+    assert r.scheduler in ("grid", "persistent") and set(r.blend_ms) == {"grid", "persistent"}
     b.train_step(g, specs, tbuf, r.next_t, r.tile, graph=True)
     ctx.sync()
     assert np.isfinite(b.losses()).all()
     b.free()
+
+
+@pytest.mark.parametrize("tile,fb,bb", [(8, 64, 64), (16, 256, 128), (32, 512, 128), (32, 128, 256)])
+def test_persistent_scheduler_matches_grid(ctx, tile, fb, bb):
+    """The persistent tile scheduler (resident workgroups taking tiles from a
+    counter) against one workgroup per tile: identical images, final T and
+    last contributors; on the same scene the same losses and gradients (the
+    float atomics sum in another order); after graph-replayed steps the
+    same parameters within Adam's step (a near-zero gradient summed to the
+    other sign moves a parameter by lr the other way)."""
+    s = synth.make_scene(20_000, 0x5EED0002, 3.0)
+    views = _views(s.n)
+    specs = [x[0] for x in views]
+    targets = _targets(synth.perturb_means(s, 81), views, tile)
+    tbuf = ctx.storage_buffer(np.concatenate([x.reshape(-1) for x in targets]))
+    out = {}
+    for sched in (L.SCHED_GRID, L.SCHED_PERSISTENT):
+        g = GpuSplatBuffer.upload(ctx, s)
+        b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 2_000_000)
+        b.set_lds_batch(fb, bb)
+        b.set_scheduler(sched)
+        b.render(g, specs, tile)
+        imgs = [b.image(k) for k in range(len(specs))]
+        states = [b.state(k) for k in range(len(specs))]
+        b.train_step(g, specs, tbuf, 1, tile, L.LOSS_L1, L.BWD_RECOMPUTE, write_grads=True)
+        first = (b.losses(), g.grads())
+        for t in (2, 3, 4):
+            b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, L.BWD_RECOMPUTE, graph=True)
+        ctx.sync()
+        assert b.stats()[2] == 0
+        out[sched] = (imgs, states, first, b.losses(), g.readback())
+        b.free()
+        g.free()
+    (ia, sa, (la, ga), la4, pa), (ib, sb, (lb, gb), lb4, pb) = out[L.SCHED_GRID], out[L.SCHED_PERSISTENT]
+    for x, y in zip(ia, ib):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    for (fa, lia), (fb_, lib_) in zip(sa, sb):
+        assert np.array_equal(fa.view(np.uint32), fb_.view(np.uint32)) and np.array_equal(lia, lib_)
+    assert np.allclose(la, lb, rtol=1e-9, atol=0)
+    for name, x, y in zip(("pos", "rot", "scale", "logit", "dc", "rest"), ga.arrays(), gb.arrays()):
+        assert rel_norm_err(x, y) < 1e-5, (name, rel_norm_err(x, y))
+    assert np.allclose(la4, lb4, rtol=1e-4, atol=0)
+    for x, y in zip(pa.arrays(), pb.arrays()):
+        assert np.abs(x.astype(np.float64) - y).max() <= 2 * 3 * 1.6e-4 + 1e-6
