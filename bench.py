@@ -16,12 +16,15 @@ plain: the single-view path (one 1920x1080 view per iteration, reference
   call sequence).
 
 Also reported: single-view 1080p render throughput and p50/p99 latency at
-1M splats; the render server (64 resident scenes of 50k-3M splats, Poisson
-arrivals at --serve-rate req/s, resolutions 512^2 to 3840x2160, up to 32
-views per batch, 3 batches in flight) with p50/p99 arrival-to-host latency
-and Mpx/s; per-kernel HIP-event timings (library stream) with the dominant
-kernel against the HBM roofline; and the CPU baseline (the host
-restatement of the reference path, run on this host's cores on the same
+1M splats; the plain single-view training path's views/s; the render server
+(64 resident scenes of 50k-3M splats under an HBM budget from hipMemGetInfo,
+batches bounded by the slot workspaces, up to 32 views per batch, 3 batches
+in flight) swept over Poisson rates of 200-3200 req/s with p50/p99
+arrival-to-host latency, Mpx/s and views per batch; per-kernel HIP-event
+timings with the dominant kernel and the whole step against the HBM
+roofline; peaks measured on the device (copy bandwidth, LDS read bandwidth,
+shader clock); and the CPU baseline (the host restatement of the reference
+path, timed in C++ on this host's cores on a bounded sample of the same
 workload).
 """
 from __future__ import annotations
@@ -111,8 +114,7 @@ PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.7587e9, 45_319_956),
 # (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
 PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((792269317.5 + 15983137.5) * 64, 45_319_956, 0.234),
                 ("blend_fwd_loss", 32, 512): ((1957194823.6 + 13700604.4) * 64, 45_319_956, 0.389)}
-LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 128 B per clock per CU
-N_SIMDS, CLOCK_HZ = 1024, 2.4e9
+N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 # Independent FP32 FMA chains at full occupancy issue 0.244 wave instructions per SIMD per cycle
 # (a wave64 FMA occupies a SIMD for 4 cycles). The blend kernels' mixed code issues above that
 # rate, and fewer VALU instructions alone did not shorten them: they also wait on LDS, barriers
@@ -141,10 +143,13 @@ def roofline_of(kernels, cfg=None, pairs=None):
     return out
 
 
-def valu_of(kernels, cfg, pairs):
+def valu_of(kernels, cfg, pairs, peaks):
     """VALU issue rate and LDS bandwidth of the blend kernels: the profiled instruction count and LDS
-    bytes, scaled by this run's pairs, over this run's HIP-event kernel time; the issue rate beside
-    the FMA-chain probe's, the LDS rate beside the LDS peak."""
+    bytes, scaled by this run's pairs, over this run's HIP-event kernel time; the issue rate (at the
+    shader clock measured on this device) beside the FMA-chain probe's, the LDS rate beside the LDS
+    read bandwidth measured on this device."""
+    CLOCK_HZ = peaks["clock_ghz"] * 1e9
+    LDS_PEAK_TBS = peaks["lds_read_tbs"]
     out = {}
     tile, fb, bb = cfg
     for name in ("blend_bwd_recompute", "blend_fwd_loss"):
@@ -191,56 +196,99 @@ def render_latency(ctx, g, n, tile, lds=None):
 
 
 def cpu_baseline_batched(s, views, targets, tile):
-    """The host restatement of the reference path on one full iteration of
-    the same 16-view workload (recompute backward as on the GPU)."""
-    from nanotracer_amd import synth
+    """The host restatement of the reference path (oracle/nts_oracle.cpp,
+    steady_clock inside C++) on a bounded sample of the same 16-view
+    workload: one view of each resolution class (project, bin, blend with
+    the saved state, L1, recompute backward, project_backward), median of 5
+    runs after 1 warm-up on every core, plus the iteration's one Adam step.
+    The 16-view iteration is the sum, over its views, of the median of the
+    sampled view of the same resolution, plus Adam.  Single-thread: the
+    smallest class's view, median of 3 after the all-core runs."""
+    import platform
     from oracle import oracle as O
 
+    classes = {}
+    for k, (spec, _) in enumerate(views):
+        classes.setdefault((spec.width, spec.height), k)
+    picked = sorted(classes.values())
+    cams = [O.camera_like(views[k][0].camera) for k in picked]
+    tils = [O.tiling(views[k][0].width, views[k][0].height, tile, views[k][0].depth_max) for k in picked]
+    tg = [targets[k] for k in picked]
     t0 = time.perf_counter()
-    total = None
-    for (spec, eye), tgt in zip(views, targets):
-        cam = O.camera_like(spec.camera)
-        t = O.tiling(spec.width, spec.height, tile, spec.depth_max)
-        op = O.project(s, cam)
-        keys, pay, rg = O.bin(op, t)
-        img, ft, li = O.composite(op, pay, rg, t, with_state=True)
-        loss, dl = O.loss_grad(img, tgt, 0)
-        pg = O.composite_backward_recompute(op, pay, rg, t, ft, li, dl)
-        gv = O.project_backward(s, cam, pg, full=True)
-        total = gv if total is None else synth.HostSplats(*[a + b for a, b in zip(total.arrays(), gv.arrays())])
-    for p, gr in zip([a.reshape(-1).copy() for a in s.arrays()], total.arrays()):
-        m = np.zeros_like(p)
-        v = np.zeros_like(p)
-        O.adam_step(p, gr.reshape(-1), m, v, 1.6e-4, 0.9, 0.999, 1e-15, 1)
+    vs, ad = O.time_step_views(s, cams, tils, tg, warmup=1, reps=5, threads=0)
+    med = np.median(vs, axis=0)
+    per_class = {(views[k][0].width, views[k][0].height): float(m) for k, m in zip(picked, med)}
+    t_iter = sum(per_class[(spec.width, spec.height)] for spec, _ in views) + float(np.median(ad))
+    small = min(per_class, key=lambda wh: wh[0] * wh[1])
+    ks = picked[[views[k][0].width * views[k][0].height for k in picked].index(small[0] * small[1])]
+    vs1, _ = O.time_step_views(s, [O.camera_like(views[ks][0].camera)],
+                               [O.tiling(small[0], small[1], tile, views[ks][0].depth_max)], [targets[ks]],
+                               warmup=0, reps=3, threads=1)
+    single = float(np.median(vs1[:, 0]))
+    cpu_wall = time.perf_counter() - t0
+    pixels = sum(spec.width * spec.height for spec, _ in views)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=round(1.0 / t_iter, 5), unit="iters/s", cores=O.num_threads(), kind="port",
+                sample=f"one view of each resolution class of the 16-view iteration ({', '.join(f'{w}x{h}' for w, h in sorted(per_class))}), "
+                       f"median of 5 after 1 warm-up on all cores, + one Adam step; iteration = sum over the 16 "
+                       f"views of their class's median + Adam ({cpu_wall:.1f} s of CPU work in this run)",
+                ms_per_iter=round(t_iter * 1e3, 1), views_per_s=round(len(views) / t_iter, 3),
+                mpx_s=round(pixels / t_iter / 1e6, 3),
+                view_ms_by_class={f"{w}x{h}": round(v * 1e3, 1) for (w, h), v in sorted(per_class.items())},
+                adam_ms=round(float(np.median(ad)) * 1e3, 1),
+                single_thread=dict(view=f"{small[0]}x{small[1]}", ms=round(single * 1e3, 1),
+                                   all_core_ms=round(per_class[small] * 1e3, 1),
+                                   speedup=round(single / per_class[small], 2)),
+                cpu_model=model or platform.processor(), hardware_concurrency=os.cpu_count(),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
+def plain_views_per_s(ctx, s, steps=10, warmup=3):
+    """The plain single-view path (the reference's call sequence, one
+    1920x1080 view per training step, reference-parity backward) on the same
+    scene: GPU views per second."""
+    from nanotracer_amd import synth
+    from nanotracer_amd.api import GpuSplatBuffer
+    from nanotracer_amd.train import Trainer, View
+
+    u = synth.uniform((SEED << 8) + 31, 1)
+    eye = synth.orbit_eye(6.0, 2.0 * math.pi * float(u[0]), 0.3)
+    g = GpuSplatBuffer.upload(ctx, s)
+    tr = Trainer(ctx, g, [View(eye, 1920, 1080, np.zeros((1080, 1920, 4), np.float32))], 12.0)
+    for it in range(warmup):
+        tr.step(it)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for it in range(steps):
+        tr.step(warmup + it)
+    ctx.sync()
     dt = time.perf_counter() - t0
-    return dict(value=round(1.0 / dt, 5), unit="iters/s", cores=O.num_threads(), kind="port",
-                sample=f"one full 16-view training iteration of the same workload on the host restatement "
-                       f"(project, bin, blend, L1, recompute backward, project_backward, Adam): {dt:.2f} s")
+    g.free()
+    return round(steps / dt, 2)
 
 
 SERVE_SCENES = 64
 SERVE_RES = ((512, 512), (1280, 720), (1920, 1080), (3840, 2160))
 
 
-def run_serving(rate, duration, seed=SEED + 5, tile=32, lds=512):
-    """Poisson arrivals at `rate` req/s for `duration` s, each one camera of a
-    random resident scene at a random resolution; latency = arrival to the
-    tonemapped image in host memory.  The server renders at the given tile
-    size and forward LDS batch (the training step's tuned ones)."""
-    from nanotracer_amd import synth
-    from nanotracer_amd.api import CameraUniform, RenderServer, ViewSpec
+SERVE_RATES = (200.0, 400.0, 600.0, 800.0, 1200.0, 1600.0, 3200.0)
+SERVE_BUDGET_CAP = 24 << 30  # the server's share of the device
 
-    srv = RenderServer(slots=3, max_views=32, max_pixels=48 << 20, max_records=64 << 20,
-                       pair_capacity=600_000_000, tile_size=tile, lds_batch=lds)
-    sizes = []
-    t_setup = time.perf_counter()
-    for sc in synth.serving_scenes(SERVE_SCENES, seed):
-        sizes.append(sc.n)
-        srv.add_scene(sc)
-    t_setup = time.perf_counter() - t_setup
-    rng = np.random.default_rng(seed + 2)
-    out = dict(rate_req_s=rate, scenes=len(sizes), resident_splats=int(sum(sizes)))
-    n_req = max(1, int(rate * duration))
+
+def _serve_rate(srv, sizes, rate, n_req, rng):
+    """Poisson arrivals at `rate` req/s, each one camera of a random resident
+    scene at a random resolution; latency = arrival to the tonemapped image
+    in host memory."""
+    from nanotracer_amd import synth
+    from nanotracer_amd.api import CameraUniform, ViewSpec
+
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     reqs = []
     for k in range(n_req):
@@ -250,10 +298,7 @@ def run_serving(rate, duration, seed=SEED + 5, tile=32, lds=512):
                               float(rng.uniform(-0.5, 0.5)))
         cam = CameraUniform.from_pose(eye, (0, 0, 0), (0, 1, 0), math.pi / 3, w, h, sizes[sid])
         reqs.append((sid, ViewSpec(cam, w, h, 24.0)))
-    # warm-up: one request per resolution
-    for w, h in SERVE_RES:
-        cam = CameraUniform.from_pose((0, 0, 8), (0, 0, 0), (0, 1, 0), math.pi / 3, w, h, sizes[0])
-        srv.wait(srv.submit(0, ViewSpec(cam, w, h, 24.0))[0])
+    before = srv.stats()
     lat, pix, views = [], 0, []
     inflight = {}
     t0 = time.perf_counter()
@@ -278,15 +323,55 @@ def run_serving(rate, duration, seed=SEED + 5, tile=32, lds=512):
             time.sleep(0.0002)
     wall = time.perf_counter() - t0
     st = srv.stats()
-    srv.close()
+    d = {k: st[k] - before[k] for k in ("batches", "overflow_reruns", "failed", "memory_limited")}
     lat = np.array(lat)
-    out.update(requests=n_req, p50_ms=round(float(np.percentile(lat, 50)), 3),
-               p99_ms=round(float(np.percentile(lat, 99)), 3), mean_ms=round(float(lat.mean()), 3),
-               mpx_s=round(pix / wall / 1e6, 1), mean_views_per_batch=round(float(np.mean(views)), 2),
-               batches=int(st["batches"]), overflow_reruns=int(st["overflow_reruns"]),
-               max_in_flight=int(st["max_in_flight"]), failed=int(st["failed"]),
-               hbm_gb=round(st["hbm_in_use"] / 1e9, 1), setup_s=round(t_setup, 1))
-    return out
+    return dict(rate_req_s=rate, requests=n_req, p50_ms=round(float(np.percentile(lat, 50)), 3),
+                p99_ms=round(float(np.percentile(lat, 99)), 3), mean_ms=round(float(lat.mean()), 3),
+                mpx_s=round(pix / wall / 1e6, 1), mean_views_per_batch=round(float(np.mean(views)), 2),
+                max_views_per_batch=int(max(views)), batches=int(d["batches"]),
+                batches_closed_by_memory=int(d["memory_limited"]), overflow_reruns=int(d["overflow_reruns"]),
+                failed=int(d["failed"]), max_in_flight_so_far=int(st["max_in_flight"]))
+
+
+def run_serving(rates=SERVE_RATES, n_req=1500, seed=SEED + 5, tile=32, lds=512):
+    """The render server over 64 resident scenes of 50k-3M splats, swept over
+    Poisson arrival rates until p99 passes 20 ms.  HBM budget: the smaller of
+    24 GiB and 90% of the free memory hipMemGetInfo reports; the scenes take
+    their share and the rest is three slot workspaces, which bound every
+    batch's working set (admission by memory)."""
+    from nanotracer_amd import synth
+    from nanotracer_amd.api import CameraUniform, RenderServer, ViewSpec, device_memory
+
+    sizes = [int(n) for n in synth.serving_scene_sizes(SERVE_SCENES, seed)]
+    resident = sum(((n + 63) // 64) * 64 * 4 * 59 for n in sizes)
+    free, total = device_memory(0)
+    budget = min(SERVE_BUDGET_CAP, int(0.9 * free))
+    slot_bytes = (budget - resident) // 3 - (1 << 20)
+    srv = RenderServer(hbm_budget=budget, slots=3, max_views=32, max_pixels=96 << 20, max_records=0,
+                       pair_capacity=0, tile_size=tile, lds_batch=lds, slot_bytes=slot_bytes)
+    t_setup = time.perf_counter()
+    for sc in synth.serving_scenes(SERVE_SCENES, seed):
+        srv.add_scene(sc)
+    t_setup = time.perf_counter() - t_setup
+    # warm-up: one request per resolution
+    for w, h in SERVE_RES:
+        cam = CameraUniform.from_pose((0, 0, 8), (0, 0, 0), (0, 1, 0), math.pi / 3, w, h, sizes[0])
+        srv.wait(srv.submit(0, ViewSpec(cam, w, h, 24.0))[0])
+    rng = np.random.default_rng(seed + 2)
+    sweep = []
+    for rate in rates:
+        sweep.append(_serve_rate(srv, sizes, rate, n_req, rng))
+        print(f"[serving] {rate:.0f} req/s: p50 {sweep[-1]['p50_ms']} ms, p99 {sweep[-1]['p99_ms']} ms, "
+              f"{sweep[-1]['mean_views_per_batch']} views/batch", file=sys.stderr, flush=True)
+        if sweep[-1]["p99_ms"] > 20.0:
+            break
+    st = srv.stats()
+    srv.close()
+    ok = [r["rate_req_s"] for r in sweep if r["p99_ms"] < 20.0 and r["failed"] == 0]
+    return dict(scenes=len(sizes), resident_splats=int(sum(sizes)), hbm_free_gb=round(free / 1e9, 1),
+                hbm_budget_gb=round(budget / 1e9, 1), resident_gb=round(resident / 1e9, 1),
+                slot_gb=round(slot_bytes / 1e9, 2), hbm_in_use_gb=round(st["hbm_in_use"] / 1e9, 1),
+                setup_s=round(t_setup, 1), sweep=sweep, max_rate_p99_under_20ms=max(ok) if ok else None)
 
 
 def run_batched(args, ctx):
@@ -369,8 +454,22 @@ def run_batched(args, ctx):
         final_loss=float(losses.sum()),
     )
     out.update(render_latency(ctx, g, s.n, tile, lds))
+    peaks = ctx.probe_peaks(4 << 30)
     out["roofline"] = roofline_of(kernels, (tile, lds[0], lds[1]), pairs)
-    out["valu"] = valu_of(kernels, (tile, lds[0], lds[1]), pairs)
+    out["roofline"]["frac_of_measured_copy"] = round(out["roofline"]["achieved"] / peaks["hbm_copy_gbs"], 4)
+    out["peaks"] = dict(hbm_copy_gbs=round(peaks["hbm_copy_gbs"], 1), lds_read_tbs=round(peaks["lds_read_tbs"], 2),
+                        clock_ghz=round(peaks["clock_ghz"], 3), hbm_vendor_gbs=HBM_PEAK_GBS,
+                        how="copy of 4 GiB (read + write bytes); conflict-free ds_read_b128 over all CUs; "
+                            "shader cycle counter over the wall clock during the LDS run")
+    # the whole step against HBM: compulsory bytes of the step (params, Adam, records, pairs, pixels, tiles)
+    b_step = 2368.0 * s.n + 132.0 * visible + 116.0 * pairs + 76.0 * pixels + 8.0 * n_tiles
+    ach = b_step / (out["ms_per_step"] * 1e-3) / 1e9
+    out["step_roofline"] = dict(alg_bytes=b_step, achieved_gbs=round(ach, 1), peak_gbs=HBM_PEAK_GBS,
+                                frac=round(ach / HBM_PEAK_GBS, 4),
+                                frac_of_measured_copy=round(ach / peaks["hbm_copy_gbs"], 4),
+                                formula="2368 N + sum_v (132 N_v + 116 K_v + 76 P_v + 8 T_v)",
+                                counters=dict(N=s.n, N_v=visible, K=pairs, P=pixels, T=n_tiles))
+    out["valu"] = valu_of(kernels, (tile, lds[0], lds[1]), pairs, peaks)
     if ssim:
         out["ssim_step"] = ssim
     if tune:
@@ -378,10 +477,21 @@ def run_batched(args, ctx):
                                table=tune.table)
     out["kernels"] = {k: dict(avg_ms=round(v["avg_ms"], 4), launches=v["launches"],
                               gbs=None if v["gbs"] is None else round(v["gbs"], 1)) for k, v in kernels.items()}
+    out["plain_single_view"] = dict(views_per_s=plain_views_per_s(ctx, s), view="1920x1080, tile 16",
+                                    sequence="reference call sequence, reference-parity backward")
     if args.serve_rate > 0:
-        out["serving"] = run_serving(args.serve_rate, args.serve_seconds, tile=tile, lds=lds[0])
+        out["serving"] = run_serving(tile=tile, lds=lds[0])
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_batched(s, views, targets, tile)
+        cb = cpu_baseline_batched(s, views, targets, tile)
+        out["cpu_baseline"] = cb
+        out["vs_cpu"] = dict(batched_views_per_s=out["views_per_s"], cpu_views_per_s=cb["views_per_s"],
+                             batched_over_cpu=round(out["views_per_s"] / cb["views_per_s"], 1),
+                             plain_1080p_views_per_s=out["plain_single_view"]["views_per_s"],
+                             cpu_1080p_views_per_s=round(1e3 / cb["view_ms_by_class"]["1920x1080"], 3)
+                             if "1920x1080" in cb["view_ms_by_class"] else None)
+        if out["vs_cpu"]["cpu_1080p_views_per_s"]:
+            out["vs_cpu"]["plain_over_cpu_1080p"] = round(out["vs_cpu"]["plain_1080p_views_per_s"]
+                                                          / out["vs_cpu"]["cpu_1080p_views_per_s"], 1)
     return out
 
 
@@ -477,9 +587,8 @@ def main():
     ap.add_argument("--tile", choices=("auto", "8", "16", "32"), default="auto",
                     help="blend tile size, or autotuned during warm-up")
     ap.add_argument("--lds-batch", default="auto", help="blend forward,backward LDS batch, or auto")
-    ap.add_argument("--serve-rate", type=float, default=200.0,
-                    help="render-server Poisson arrival rate (req/s); 0 skips the serving leg")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=1.0,
+                    help="0 skips the render-server sweep (Poisson rates 200 to 3200 req/s, until p99 > 20 ms)")
     ap.add_argument("--ssim-lambda", type=float, default=0.2,
                     help="also time the iteration with the combined MSE + SSIM loss; 0 skips it")
     args = ap.parse_args()

@@ -100,6 +100,13 @@ int nts_prof_reset(nts_ctx* ctx);
 int nts_prof_query(nts_ctx* ctx, const char* prefix, double* total_ms, uint64_t* launches);
 /* Newline-separated "name launches total_ms" report into buf. */
 int nts_prof_report(nts_ctx* ctx, char* buf, size_t cap);
+/* Ceilings measured on the device, for roofline figures (no reference
+ * counterpart): the bandwidth of a plain copy of copy_bytes (read + write
+ * bytes over time, GB/s), the LDS bandwidth of conflict-free 128-bit reads
+ * over all CUs (TB/s), and the shader clock during that LDS run (GHz, from
+ * the cycle counter over the wall clock).  Allocates 2 x copy_bytes. */
+int nts_probe_peaks(nts_ctx* ctx, uint64_t copy_bytes, double* hbm_copy_gbs, double* lds_read_tbs,
+                    double* clock_ghz);
 
 /* ---- device buffers (replaces WgpuCtx::storage_buffer[_zeroed] and
  *      readback, gpu.rs:79-152) ------------------------------------------ */
@@ -361,20 +368,30 @@ int nts_batch_download_bins(nts_ctx* ctx, const nts_batch* batch, uint32_t view,
  * slots from the FIFO of pending requests (continuous batching).  Every
  * result is bit-identical to rendering its request alone. */
 typedef struct nts_server nts_server;
+/* Admission by memory: each slot owns one device workspace of slot_bytes,
+ * and a batch lays out its working set inside it (projected records, the
+ * (splat, tile) pairs it expects, their sort workspace, its pixels).  A
+ * request joins the batch being formed while that working set still fits
+ * (pairs estimated from the scene's earlier renders) and the caps hold, so
+ * a 4K view of a large scene takes most of a slot while many small views
+ * share one; the rest wait for the next free slot.  With slot_bytes = 0 the
+ * workspace is sized to hold every cap at once (then all caps must be set). */
 typedef struct nts_server_cfg {
   uint64_t hbm_budget;     /* bytes for resident scenes + slot workspaces (0 = no limit) */
   uint32_t slots;          /* batches in flight (1..8) */
   uint32_t max_views;      /* requests per batch (<= 32) */
-  uint64_t max_pixels;     /* per batch */
-  uint64_t max_records;    /* per batch: sum of the requested scenes' splat counts */
-  uint64_t pair_capacity;  /* per batch (splat, tile) pairs */
+  uint64_t max_pixels;     /* per batch (sizes the pinned result staging) */
+  uint64_t max_records;    /* per batch: sum of the requested scenes' splat counts (0 = no cap) */
+  uint64_t pair_capacity;  /* per batch (splat, tile) pairs (0 = what the workspace holds) */
   uint32_t tile_size;      /* 8, 16 or 32 */
   uint32_t lds_batch;      /* blend forward LDS batch: 64, 128, 256 or 512 */
+  uint64_t slot_bytes;     /* each slot's workspace (0 = sized from the caps) */
 } nts_server_cfg;
 typedef struct nts_serve_stats {
   uint64_t submitted, completed, failed;
   uint64_t batches, views, overflow_reruns;
-  uint32_t max_in_flight, scenes, pending, pad;
+  uint32_t max_in_flight, scenes, pending;
+  uint32_t memory_limited; /* batches closed because the next request's working set did not fit */
   uint64_t hbm_in_use;
 } nts_serve_stats;
 typedef struct nts_request_info {
@@ -385,6 +402,10 @@ typedef struct nts_request_info {
 } nts_request_info;
 #define NTS_OUT_RGBA8 0  /* tonemapped (tonemap.wgsl), 4 B per pixel */
 #define NTS_OUT_F32 1    /* (r, g, b, 1 - T) float, 16 B per pixel */
+/* Free and total memory of a device (hipMemGetInfo), for sizing a server's
+ * hbm_budget and slot_bytes (the reference's gpu-mem probe,
+ * crates/gpu-mem/src/lib.rs:70, reads sysfs instead). */
+int nts_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 int nts_server_create(int device, const nts_server_cfg* cfg, nts_server** out);
 int nts_server_destroy(nts_server* server);
 /* Uploads a resident scene (parameters only); NTS_E_NOMEM over the budget. */

@@ -87,7 +87,7 @@ class ServerCfg(C.Structure):
 
     _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("max_views", C.c_uint32),
                 ("max_pixels", C.c_uint64), ("max_records", C.c_uint64), ("pair_capacity", C.c_uint64),
-                ("tile_size", C.c_uint32), ("lds_batch", C.c_uint32)]
+                ("tile_size", C.c_uint32), ("lds_batch", C.c_uint32), ("slot_bytes", C.c_uint64)]
 
 
 class ServeStats(C.Structure):
@@ -96,7 +96,7 @@ class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("batches", C.c_uint64), ("views", C.c_uint64), ("overflow_reruns", C.c_uint64),
                 ("max_in_flight", C.c_uint32), ("scenes", C.c_uint32), ("pending", C.c_uint32),
-                ("pad", C.c_uint32), ("hbm_in_use", C.c_uint64)]
+                ("memory_limited", C.c_uint32), ("hbm_in_use", C.c_uint64)]
 
 
 class RequestInfo(C.Structure):
@@ -121,6 +121,8 @@ _SIGS = {
     "nts_prof_reset": (C.c_int, [C.c_void_p]),
     "nts_prof_query": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "nts_prof_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "nts_probe_peaks": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
     "nts_buffer_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "nts_buffer_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nts_buffer_size": (C.c_uint64, [C.c_void_p]),
@@ -184,6 +186,7 @@ _SIGS = {
                                        C.POINTER(AdamCfg), C.c_uint64, C.POINTER(Constraints), C.c_int]),
     "nts_render_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(ViewDesc), C.c_uint32,
                                    C.c_uint32]),
+    "nts_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "nts_server_create": (C.c_int, [C.c_int, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "nts_server_destroy": (C.c_int, [C.c_void_p]),
     "nts_server_add_scene": (C.c_int, [C.c_void_p, C.POINTER(SplatsHost), C.POINTER(C.c_uint32)]),

@@ -106,6 +106,13 @@ class GpuCtx:
         L.check(L.lib().nts_prof_query(self.h, prefix.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def probe_peaks(self, copy_bytes: int = 4 << 30) -> dict:
+        """Measured HBM copy bandwidth (GB/s), LDS 128-bit read bandwidth
+        (TB/s) and shader clock (GHz) of this device."""
+        h, l, c = C.c_double(), C.c_double(), C.c_double()
+        L.check(L.lib().nts_probe_peaks(self.h, int(copy_bytes), C.byref(h), C.byref(l), C.byref(c)))
+        return dict(hbm_copy_gbs=h.value, lds_read_tbs=l.value, clock_ghz=c.value)
+
     def prof_report(self) -> dict:
         buf = C.create_string_buffer(1 << 16)
         L.check(L.lib().nts_prof_report(self.h, buf, len(buf)))
@@ -597,6 +604,13 @@ class ViewBatch:
         return keys[:k], pay[:k], rng
 
 
+def device_memory(device: int = 0):
+    """(free, total) bytes of a device (hipMemGetInfo)."""
+    f, t = C.c_uint64(), C.c_uint64()
+    L.check(L.lib().nts_device_memory(int(device), C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
 class RenderServer:
     """The render server (nts_server): scenes resident in HBM, requests
     batched as they arrive, `slots` batches in flight on their own streams.
@@ -606,9 +620,13 @@ class RenderServer:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 2, max_views: int = 32,
                  max_pixels: int = 16 << 20, max_records: int = 16 << 20, pair_capacity: int = 120_000_000,
-                 tile_size: int = 32, lds_batch: int = 512):
+                 tile_size: int = 32, lds_batch: int = 512, slot_bytes: int = 0):
+        """slot_bytes > 0: each slot's device workspace; a batch's working set
+        must fit it (admission by memory), and 0 for max_records or
+        pair_capacity means no cap.  slot_bytes = 0: the workspace holds every
+        cap at once."""
         cfg = L.ServerCfg(int(hbm_budget), int(slots), int(max_views), int(max_pixels), int(max_records),
-                          int(pair_capacity), int(tile_size), int(lds_batch))
+                          int(pair_capacity), int(tile_size), int(lds_batch), int(slot_bytes))
         h = C.c_void_p()
         L.check(L.lib().nts_server_create(int(device), C.byref(cfg), C.byref(h)))
         self.h = h
@@ -665,4 +683,4 @@ class RenderServer:
     def stats(self) -> dict:
         st = L.ServeStats()
         L.check(L.lib().nts_server_stats(self.h, C.byref(st)))
-        return {name: getattr(st, name) for name, _ in L.ServeStats._fields_ if name != "pad"}
+        return {name: getattr(st, name) for name, _ in L.ServeStats._fields_}

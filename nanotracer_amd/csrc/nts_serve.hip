@@ -107,7 +107,8 @@ struct Resident {
 struct Slot {
   nts_ctx* ctx = nullptr;
   void* base = nullptr;
-  uint64_t bytes = 0;
+  uint64_t bytes = 0;       // the slot's workspace; each batch lays out its working set inside it
+  uint64_t pair_cap = 0;    // pairs the current batch has room for
   DevCamera* d_cams = nullptr;
   Projected* proj = nullptr;
   uint32_t* counts = nullptr;
@@ -141,6 +142,9 @@ struct nts_server {
   int device = 0;
   nts_server_cfg cfg{};
   uint64_t max_tiles = 0;
+  uint64_t slot_bytes = 0;   // each slot's workspace
+  uint64_t max_records = 0;  // per batch (the caps of cfg, 0 = no cap)
+  uint64_t pair_cap = 0;
   nts_ctx* ctx0 = nullptr;  // uploads
   std::vector<Resident> scenes;
   std::vector<Slot> slots;
@@ -159,52 +163,69 @@ namespace {
 
 uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
 
+// Working set of one batch of V views (rec projected records, px pixels,
+// tiles tiles, keys of kb bytes, room for pair_cap pairs), laid out from the
+// start of its slot's workspace.
+struct BatchLayout {
+  uint64_t cams, proj, counts, total, seg, tb, flags, ka, kb, va, vb, rg, img, rgba, scan, sort, end;
+};
+
+BatchLayout layout_batch(uint32_t V, uint64_t rec, uint64_t px, uint64_t tiles, uint32_t kb, uint64_t pair_cap) {
+  BatchLayout L{};
+  uint64_t off = 0;
+  auto take = [&](uint64_t bytes) { const uint64_t o = off; off += al256(bytes); return o; };
+  const uint64_t r1 = rec > 0 ? rec : 1, p1 = pair_cap > 0 ? pair_cap : 1;
+  L.cams = take(sizeof(DevCamera) * V);
+  L.proj = take(r1 * sizeof(Projected));
+  L.counts = take(r1 * 4);
+  L.total = take(8);
+  L.seg = take(8 * (V + 1));
+  L.tb = take(4 * (V + 1));
+  L.flags = take(64);
+  L.ka = take(p1 * kb);
+  L.kb = take(p1 * kb);
+  L.va = take(p1 * 4);
+  L.vb = take(p1 * 4);
+  L.rg = take((tiles > 0 ? tiles : 1) * 8);
+  L.img = take(px * 16);
+  L.rgba = take(px * 4);
+  L.scan = take(nts_scan_ws_bytes(r1));
+  L.sort = take(nts_sort_ws_bytes(V, p1, kb));
+  L.end = off;
+  return L;
+}
+
+// The most pairs (below 2^30) a batch of this shape can hold in `bytes`;
+// 0 when not even its fixed part fits.
+uint64_t pairs_that_fit(uint64_t bytes, uint32_t V, uint64_t rec, uint64_t px, uint64_t tiles, uint32_t kb) {
+  if (layout_batch(V, rec, px, tiles, kb, 1).end > bytes) return 0;
+  uint64_t lo = 1, hi = (1ull << 30) - 1;
+  if (layout_batch(V, rec, px, tiles, kb, hi).end <= bytes) return hi;
+  while (hi - lo > 1) {  // invariant: lo fits, hi does not
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (layout_batch(V, rec, px, tiles, kb, mid).end <= bytes) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Key bytes of a batch whose largest view has max_tiles_view tiles.
+uint32_t key_bytes_for(uint64_t max_tiles_view) {
+  uint32_t tbits = 1;
+  while (tbits < 32 && (1ull << tbits) < max_tiles_view) ++tbits;
+  return 16 + tbits <= 32 ? 4u : 8u;
+}
+
 int slot_create(nts_server* sv, Slot& sl) {
   const nts_server_cfg& c = sv->cfg;
   int rc = nts_ctx_create(sv->device, 0, &sl.ctx);
   if (rc) return rc;
-  uint64_t off = 0;
-  auto take = [&](uint64_t bytes) { const uint64_t o = off; off += al256(bytes); return o; };
-  const uint64_t V = c.max_views;
-  const uint64_t o_cams = take(sizeof(DevCamera) * V);
-  const uint64_t o_proj = take(c.max_records * sizeof(Projected));
-  const uint64_t o_counts = take(c.max_records * 4);
-  const uint64_t o_total = take(8);
-  const uint64_t o_seg = take(8 * (V + 1));
-  const uint64_t o_tb = take(4 * (V + 1));
-  const uint64_t o_flags = take(64);
-  const uint64_t o_ka = take(c.pair_capacity * 8);
-  const uint64_t o_kb = take(c.pair_capacity * 8);
-  const uint64_t o_va = take(c.pair_capacity * 4);
-  const uint64_t o_vb = take(c.pair_capacity * 4);
-  const uint64_t o_rg = take(sv->max_tiles * 8);
-  const uint64_t o_img = take(c.max_pixels * 16);
-  const uint64_t o_rgba = take(c.max_pixels * 4);
-  const uint64_t o_scan = take(nts_scan_ws_bytes(c.max_records));
-  const uint64_t o_sort = take(nts_sort_ws_bytes((uint32_t)V, c.pair_capacity, 8));
-  sl.bytes = off;
+  sl.bytes = sv->slot_bytes;
   if (sv->cfg.hbm_budget && sv->used + sl.bytes > sv->cfg.hbm_budget)
     return nts_fail(NTS_E_NOMEM, "server slot of %llu bytes exceeds the HBM budget", (unsigned long long)sl.bytes);
   rc = nts_dev_alloc(sl.ctx, &sl.base, sl.bytes);
   if (rc) return rc;
   sv->used += sl.bytes;
-  char* p = (char*)sl.base;
-  sl.d_cams = (DevCamera*)(p + o_cams);
-  sl.proj = (Projected*)(p + o_proj);
-  sl.counts = (uint32_t*)(p + o_counts);
-  sl.d_total = (uint64_t*)(p + o_total);
-  sl.seg_begin = (uint64_t*)(p + o_seg);
-  sl.tile_base = (uint32_t*)(p + o_tb);
-  sl.flags = (uint32_t*)(p + o_flags);
-  sl.keys_a = p + o_ka;
-  sl.keys_b = p + o_kb;
-  sl.vals_a = (uint32_t*)(p + o_va);
-  sl.vals_b = (uint32_t*)(p + o_vb);
-  sl.ranges = (uint2*)(p + o_rg);
-  sl.image = (float4*)(p + o_img);
-  sl.rgba = (uint32_t*)(p + o_rgba);
-  sl.scan_ws = p + o_scan;
-  sl.sort_ws = p + o_sort;
+  const uint64_t V = c.max_views;
   NTS_HIP_TRY(hipHostMalloc((void**)&sl.h_cams, sizeof(DevCamera) * V, hipHostMallocDefault));
   NTS_HIP_TRY(hipHostMalloc((void**)&sl.h_meta, 8 * (V + 4), hipHostMallocDefault));
   NTS_HIP_TRY(hipHostMalloc((void**)&sl.h_out, c.max_pixels * 16, hipHostMallocDefault));
@@ -287,6 +308,34 @@ int launch_batch(nts_server* sv, Slot& sl) {
   uint32_t tbits = 1;
   while (tbits < 32 && (1ull << tbits) < max_tiles_view) ++tbits;
   const uint32_t key_bits = 16 + tbits, kb = key_bits <= 32 ? 4 : 8;
+  // the batch's working set, laid out in the slot's workspace with every
+  // byte left over given to the (splat, tile) pairs
+  uint64_t cap = pairs_that_fit(sl.bytes, V, rec, pix, tiles, kb);
+  if (cap > sv->pair_cap) cap = sv->pair_cap;
+  if (cap == 0)
+    return nts_fail(NTS_E_CAPACITY, "batch of %u views does not fit a slot's %llu bytes", V,
+                    (unsigned long long)sl.bytes);
+  {
+    const BatchLayout L = layout_batch(V, rec, pix, tiles, kb, cap);
+    char* p = (char*)sl.base;
+    sl.d_cams = (DevCamera*)(p + L.cams);
+    sl.proj = (Projected*)(p + L.proj);
+    sl.counts = (uint32_t*)(p + L.counts);
+    sl.d_total = (uint64_t*)(p + L.total);
+    sl.seg_begin = (uint64_t*)(p + L.seg);
+    sl.tile_base = (uint32_t*)(p + L.tb);
+    sl.flags = (uint32_t*)(p + L.flags);
+    sl.keys_a = p + L.ka;
+    sl.keys_b = p + L.kb;
+    sl.vals_a = (uint32_t*)(p + L.va);
+    sl.vals_b = (uint32_t*)(p + L.vb);
+    sl.ranges = (uint2*)(p + L.rg);
+    sl.image = (float4*)(p + L.img);
+    sl.rgba = (uint32_t*)(p + L.rgba);
+    sl.scan_ws = p + L.scan;
+    sl.sort_ws = p + L.sort;
+    sl.pair_cap = cap;
+  }
   NTS_HIP_TRY(hipMemcpyAsync(sl.d_cams, sl.h_cams, sizeof(DevCamera) * V, hipMemcpyHostToDevice, ctx->stream));
   int rc = nts_fill_u32(ctx, sl.flags, 0, 16);
   if (rc) return rc;
@@ -298,28 +347,28 @@ int launch_batch(nts_server* sv, Slot& sl) {
   rc = nts_scan_u32_ws(ctx, sl.counts, sl.counts, rec > 0 ? rec : 1, sl.d_total, sl.scan_ws);
   if (rc) return rc;
   NTS_LAUNCH(ctx, "serve_segments", k_request_segments, dim3(1), dim3(64), 0, (const uint32_t*)sl.counts, rt,
-             (const uint64_t*)sl.d_total, sv->cfg.pair_capacity, sl.seg_begin, sl.tile_base, tb, sl.flags);
+             (const uint64_t*)sl.d_total, sl.pair_cap, sl.seg_begin, sl.tile_base, tb, sl.flags);
   bool alt = false;
   if (rec > 0) {
     if (kb == 4)
       NTS_LAUNCH(ctx, "serve_emit", (k_emit_balanced<uint32_t, true>), dim3(nts_div_up(rec, 256)), dim3(256), 0,
                  (const Projected*)sl.proj, rt, tt, (const uint32_t*)sl.counts, (const uint64_t*)sl.d_total,
-                 sv->cfg.pair_capacity, (uint32_t*)sl.keys_a, sl.vals_a);
+                 sl.pair_cap, (uint32_t*)sl.keys_a, sl.vals_a);
     else
       NTS_LAUNCH(ctx, "serve_emit", (k_emit_balanced<uint64_t, true>), dim3(nts_div_up(rec, 256)), dim3(256), 0,
                  (const Projected*)sl.proj, rt, tt, (const uint32_t*)sl.counts, (const uint64_t*)sl.d_total,
-                 sv->cfg.pair_capacity, (uint64_t*)sl.keys_a, sl.vals_a);
+                 sl.pair_cap, (uint64_t*)sl.keys_a, sl.vals_a);
   }
   if (kb == 4)
     rc = nts_sort_ws_launch32(ctx, (uint32_t*)sl.keys_a, sl.vals_a, (uint32_t*)sl.keys_b, sl.vals_b, sl.seg_begin, V,
-                              sv->cfg.pair_capacity, key_bits, sl.sort_ws, &alt, &sl.sort_err);
+                              sl.pair_cap, key_bits, sl.sort_ws, &alt, &sl.sort_err);
   else
     rc = nts_sort_ws_launch64(ctx, (uint64_t*)sl.keys_a, sl.vals_a, (uint64_t*)sl.keys_b, sl.vals_b, sl.seg_begin, V,
-                              sv->cfg.pair_capacity, key_bits, sl.sort_ws, &alt, &sl.sort_err);
+                              sl.pair_cap, key_bits, sl.sort_ws, &alt, &sl.sort_err);
   if (rc) return rc;
   void* keys = alt ? sl.keys_b : sl.keys_a;
   const uint32_t* pay = alt ? sl.vals_b : sl.vals_a;
-  uint32_t grid = nts_div_up(sv->cfg.pair_capacity, 256);
+  uint32_t grid = nts_div_up(sl.pair_cap, 256);
   if (grid > 8192) grid = 8192;
   if (kb == 4)
     NTS_LAUNCH(ctx, "serve_ranges", k_tile_ranges<uint32_t>, dim3(grid), dim3(256), 0, (const uint32_t*)keys,
@@ -374,14 +423,20 @@ double est_pairs(nts_server* sv, const Request& r) {
 
 bool fits_alone(nts_server* sv, const Request& r) {
   const uint64_t px = (uint64_t)r.view.width * r.view.height;
-  return px <= sv->cfg.max_pixels && sv->scenes[r.scene].scene->n <= sv->cfg.max_records &&
-         tiles_of(r.view, 8) <= sv->max_tiles;
+  const uint64_t n = sv->scenes[r.scene].scene->n;
+  const uint64_t t = tiles_of(r.view, sv->cfg.tile_size);
+  return px <= sv->cfg.max_pixels && n <= sv->max_records && tiles_of(r.view, 8) <= sv->max_tiles &&
+         pairs_that_fit(sv->slot_bytes, 1, n, px, t, key_bytes_for(t)) > 0;
 }
 
 // Form a batch for a free slot from the FIFO head.  Caller holds the lock.
+// A request joins while the batch's working set (records, estimated pairs,
+// sort workspace, pixels: layout_batch) still fits the slot's workspace and
+// the per-batch caps hold; the first request that does not fit waits for the
+// next slot (FIFO).
 void form_batch(nts_server* sv, Slot& sl) {
   sl.reqs.clear();
-  uint64_t px = 0, rec = 0, tiles = 0;
+  uint64_t px = 0, rec = 0, tiles = 0, max_view_tiles = 1;
   double pairs = 0.0;
   while (!sv->pending.empty() && sl.reqs.size() < sv->cfg.max_views) {
     Request* r = sv->pending.front();
@@ -395,9 +450,19 @@ void form_batch(nts_server* sv, Slot& sl) {
     const uint64_t rrec = sv->scenes[r->scene].scene->n;
     const uint64_t rt = tiles_of(r->view, sv->cfg.tile_size);
     const double rp = est_pairs(sv, *r);
-    const bool fits = px + rpx <= sv->cfg.max_pixels && rec + rrec <= sv->cfg.max_records &&
-                      tiles + rt <= sv->max_tiles && pairs + rp <= 0.9 * (double)sv->cfg.pair_capacity;
-    if (!fits && !sl.reqs.empty()) break;  // FIFO: the head waits for the next slot
+    if (!sl.reqs.empty()) {
+      const bool caps = px + rpx <= sv->cfg.max_pixels && rec + rrec <= sv->max_records &&
+                        tiles + rt <= sv->max_tiles;
+      const uint64_t mvt = rt > max_view_tiles ? rt : max_view_tiles;
+      uint64_t room = pairs_that_fit(sv->slot_bytes, (uint32_t)sl.reqs.size() + 1, rec + rrec, px + rpx,
+                                     tiles + rt, key_bytes_for(mvt));
+      if (room > sv->pair_cap) room = sv->pair_cap;
+      const bool memory = pairs + rp <= 0.9 * (double)room;
+      if (!caps || !memory) {  // FIFO: the head waits for the next slot
+        if (caps) sv->stats.memory_limited++;
+        break;
+      }
+    }
     sv->pending.pop_front();
     r->state = REQ_RUNNING;
     sl.reqs.push_back(r);
@@ -405,6 +470,7 @@ void form_batch(nts_server* sv, Slot& sl) {
     rec += rrec;
     tiles += rt;
     pairs += rp;
+    if (rt > max_view_tiles) max_view_tiles = rt;
     if (r->solo) break;
   }
 }
@@ -489,14 +555,21 @@ void run(nts_server* sv) {
 extern "C" int nts_server_create(int device, const nts_server_cfg* cfg, nts_server** out) {
   NTS_CHECK_ARG(cfg && out && cfg->slots >= 1 && cfg->slots <= 8);
   NTS_CHECK_ARG(cfg->max_views >= 1 && cfg->max_views <= NTS_MAX_VIEWS);
-  NTS_CHECK_ARG(cfg->max_pixels > 0 && cfg->max_records > 0 && cfg->max_records < (1ull << 32));
-  NTS_CHECK_ARG(cfg->pair_capacity > 0 && cfg->pair_capacity < (1ull << 30));
+  NTS_CHECK_ARG(cfg->max_pixels > 0 && cfg->max_records < (1ull << 32) && cfg->pair_capacity < (1ull << 30));
+  // without slot_bytes every cap sizes the workspace; with it, 0 = no cap
+  NTS_CHECK_ARG(cfg->slot_bytes > 0 || (cfg->max_records > 0 && cfg->pair_capacity > 0));
   NTS_CHECK_ARG(nts_blend_config_ok(cfg->tile_size, cfg->lds_batch, 64));
   *out = nullptr;
   nts_server* sv = new nts_server();
   sv->device = device;
   sv->cfg = *cfg;
   sv->max_tiles = cfg->max_pixels / 64 + (uint64_t)cfg->max_views * 1024 + 64;
+  sv->max_records = cfg->max_records ? cfg->max_records : 0xffffffffull;
+  sv->pair_cap = cfg->pair_capacity ? cfg->pair_capacity : (1ull << 30) - 1;
+  // a workspace that holds the caps at once (widest keys), unless given
+  sv->slot_bytes = cfg->slot_bytes ? al256(cfg->slot_bytes)
+                                   : layout_batch(cfg->max_views, cfg->max_records, cfg->max_pixels, sv->max_tiles,
+                                                  8, cfg->pair_capacity).end;
   int rc = nts_ctx_create(device, 0, &sv->ctx0);
   if (rc == NTS_OK) {
     sv->slots.resize(cfg->slots);
@@ -534,8 +607,8 @@ extern "C" int nts_server_destroy(nts_server* sv) {
 
 extern "C" int nts_server_add_scene(nts_server* sv, const nts_splats_host* host, uint32_t* scene_id) {
   NTS_CHECK_ARG(sv && host && scene_id && host->n > 0);
-  if (host->n > sv->cfg.max_records) return nts_fail(NTS_E_CAPACITY, "scene of %u splats exceeds a slot's %llu records",
-                                                     host->n, (unsigned long long)sv->cfg.max_records);
+  if (host->n > sv->max_records) return nts_fail(NTS_E_CAPACITY, "scene of %u splats exceeds a slot's %llu records",
+                                                 host->n, (unsigned long long)sv->max_records);
   const uint64_t need = (((uint64_t)host->n + 63) & ~63ull) * 4 * P_COUNT;
   std::lock_guard<std::mutex> g(sv->mu);
   if (sv->cfg.hbm_budget && sv->used + need > sv->cfg.hbm_budget)
@@ -649,5 +722,20 @@ extern "C" int nts_server_stats(nts_server* sv, nts_serve_stats* out) {
   out->hbm_in_use = sv->used;
   out->scenes = (uint32_t)sv->scenes.size();
   out->pending = (uint32_t)sv->pending.size();
+  return NTS_OK;
+}
+
+// Free and total device memory (hipMemGetInfo): the serving budget's source.
+extern "C" int nts_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
+  NTS_CHECK_ARG(free_bytes && total_bytes);
+  int prev = 0;
+  NTS_HIP_TRY(hipGetDevice(&prev));
+  NTS_HIP_TRY(hipSetDevice(device));
+  size_t f = 0, t = 0;
+  const hipError_t e = hipMemGetInfo(&f, &t);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) return nts_fail(NTS_E_HIP, "hipMemGetInfo: %s", hipGetErrorString(e));
+  *free_bytes = f;
+  *total_bytes = t;
   return NTS_OK;
 }

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <numeric>
 #include <vector>
 
@@ -1443,6 +1444,84 @@ void orc_tonemap(const float* image, uint32_t w, uint32_t h, uint8_t* out) {
     const float a = nts_minf(nts_maxf(image[4 * i + 3], 0.0f), 1.0f);
     out[4 * i + 3] = (uint8_t)lrintf(a * 255.0f);
   }
+}
+
+
+// Timing driver of the CPU baseline: the reference step sequence of
+// train.rs:155-250 over n_views views of one scene (project, TileBinner::bin
+// with its per-call allocation, composite with the saved state, L1 dL/dC,
+// recompute backward, project_backward with the full chain rule), the views'
+// gradients summed, then one Adam step on every attribute slab +
+// constraints.  `warmup` untimed runs, then `reps` timed ones; each view's
+// wall time of each timed run (steady_clock) goes to
+// view_secs[rep * n_views + v], the Adam step's to adam_secs[rep].  threads >
+// 0 sets the OpenMP thread count for the call (restored after).
+int orc_time_step_views(const orc_splats_f32* s, uint32_t n, const orc_camera* cams, const orc_tiling* tilings,
+                        const float* const* targets, uint32_t n_views, int warmup, int reps, int threads,
+                        double* view_secs, double* adam_secs) {
+  if (!s || !cams || !tilings || !targets || n_views == 0 || reps <= 0) return -1;
+#ifdef _OPENMP
+  const int prev_threads = omp_get_max_threads();
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  const size_t widths[6] = {3, 4, 3, 1, 3, 45};
+  std::vector<float> total[6], g[6], params[6], m[6], v[6];
+  const float* src[6] = {s->pos, s->rot, s->log_scale, s->logit, s->sh_dc, s->sh_rest};
+  for (int a = 0; a < 6; ++a) {
+    total[a].assign(widths[a] * n, 0.0f);
+    g[a].assign(widths[a] * n, 0.0f);
+    params[a].assign(src[a], src[a] + widths[a] * n);
+    m[a].assign(widths[a] * n, 0.0f);
+    v[a].assign(widths[a] * n, 0.0f);
+  }
+  orc_grads_f32 gg{g[0].data(), g[1].data(), g[2].data(), g[3].data(), g[4].data(), g[5].data()};
+  std::vector<float> proj(12 * (size_t)n), pgrad(12 * (size_t)n);
+  typedef std::chrono::steady_clock clk;
+  for (int run = 0; run < warmup + reps; ++run) {
+    for (int a = 0; a < 6; ++a) std::fill(total[a].begin(), total[a].end(), 0.0f);
+    for (uint32_t vi = 0; vi < n_views; ++vi) {
+      const auto t0 = clk::now();
+      const orc_tiling& t = tilings[vi];
+      const uint64_t np = (uint64_t)t.width * t.height;
+      const uint32_t ntiles = ((t.width + t.tile_size - 1) / t.tile_size) * ((t.height + t.tile_size - 1) / t.tile_size);
+      orc_project_f32(s, n, &cams[vi], proj.data());
+      const uint64_t k = orc_bin(proj.data(), n, &t, nullptr, nullptr, nullptr);
+      std::vector<uint32_t> keys(k > 0 ? k : 1), pay(k > 0 ? k : 1), ranges(2 * (size_t)ntiles);
+      orc_bin(proj.data(), n, &t, keys.data(), pay.data(), ranges.data());
+      std::vector<float> img(4 * np), ft(np), dl(4 * np);
+      std::vector<uint32_t> li(np);
+      orc_composite(proj.data(), pay.data(), ranges.data(), &t, img.data(), ft.data(), li.data());
+      (void)orc_loss_grad(img.data(), targets[vi], t.width, t.height, 0, dl.data());
+      orc_composite_backward_recompute(proj.data(), pay.data(), ranges.data(), &t, n, ft.data(), li.data(), dl.data(),
+                                       pgrad.data());
+      orc_project_backward_full(s, n, &cams[vi], pgrad.data(), &gg);
+      for (int a = 0; a < 6; ++a) {
+        float* dst = total[a].data();
+        const float* gs = g[a].data();
+        const int64_t cnt = (int64_t)(widths[a] * n);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < cnt; ++i) dst[i] = dst[i] + gs[i];
+      }
+      if (run >= warmup) view_secs[(size_t)(run - warmup) * n_views + vi] =
+          std::chrono::duration<double>(clk::now() - t0).count();
+    }
+    const auto t0 = clk::now();
+    for (int a = 0; a < 6; ++a) {
+      const int64_t cnt = (int64_t)(widths[a] * n), chunk = 1 << 16;
+#pragma omp parallel for schedule(static)
+      for (int64_t c = 0; c < cnt; c += chunk) {
+        const uint64_t len = (uint64_t)std::min<int64_t>(chunk, cnt - c);
+        orc_adam_step(params[a].data() + c, total[a].data() + c, m[a].data() + c, v[a].data() + c, len, 1.6e-4f,
+                      0.9f, 0.999f, 1e-15f, (uint64_t)run + 1);
+      }
+    }
+    orc_apply_constraints(params[1].data(), params[2].data(), params[3].data(), n, -10.0f, 5.0f, 10.0f);
+    if (run >= warmup) adam_secs[run - warmup] = std::chrono::duration<double>(clk::now() - t0).count();
+  }
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(prev_threads);
+#endif
+  return 0;
 }
 
 }  // extern "C"

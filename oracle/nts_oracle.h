@@ -179,6 +179,13 @@ uint32_t orc_densify(orc_splats_mut* s, uint32_t n, uint32_t grad_count, float g
 // Reinhard + sRGB + rgba8 (tonemap.wgsl:27-41).
 void orc_tonemap(const float* image, uint32_t w, uint32_t h, uint8_t* out);
 
+// Timing driver of the CPU baseline (see nts_oracle.cpp): the reference step
+// over n_views views of one scene, `reps` timed runs after `warmup`, per-view
+// and Adam wall times in seconds.  threads > 0 sets the OpenMP thread count.
+int orc_time_step_views(const orc_splats_f32* s, uint32_t n, const orc_camera* cams, const orc_tiling* tilings,
+                        const float* const* targets, uint32_t n_views, int warmup, int reps, int threads,
+                        double* view_secs, double* adam_secs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -457,3 +457,22 @@ def tonemap(image: np.ndarray) -> np.ndarray:
     out = np.zeros((h, w, 4), np.uint8)
     lib().orc_tonemap(_p(image), C.c_uint32(w), C.c_uint32(h), _p(out))
     return out
+
+
+def time_step_views(s, cams, tilings, targets, warmup=1, reps=5, threads=0):
+    """The CPU baseline's timed step (orc_time_step_views): per-view and Adam
+    wall times in seconds, shape (reps, n_views) and (reps,), measured by
+    steady_clock inside the restatement."""
+    st, keep = _splats(s, np.float32)
+    nv = len(cams)
+    cam_arr = (Camera * nv)(*cams)
+    til_arr = (Tiling * nv)(*tilings)
+    tg = [np.ascontiguousarray(t, np.float32) for t in targets]
+    ptrs = (C.c_void_p * nv)(*[t.ctypes.data for t in tg])
+    vs = np.zeros((reps, nv), np.float64)
+    ad = np.zeros(reps, np.float64)
+    rc = lib().orc_time_step_views(C.byref(st), C.c_uint32(s.n), cam_arr, til_arr, ptrs, C.c_uint32(nv),
+                                   C.c_int(warmup), C.c_int(reps), C.c_int(threads), _p(vs), _p(ad))
+    if rc != 0:
+        raise ValueError("orc_time_step_views: bad arguments")
+    return vs, ad

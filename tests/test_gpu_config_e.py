@@ -1,7 +1,8 @@
 """The render server at its full size against solo renders and the oracle.
 
 64 resident scenes with splat counts log-uniform in [50k, 3M] (SH order 3),
-a finite HBM budget, up to 32 views per launch, resolutions drawn from
+a 24 GiB HBM budget (the scenes, then three slot workspaces that each
+batch's working set must fit), up to 32 views per launch, resolutions drawn from
 {512^2, 1280x720, 1920x1080, 3840x2160}: the bench's serving workload.  A
 burst of requests arrives at once, so batches form while earlier ones are
 on the device.  Then:
@@ -20,7 +21,7 @@ import pytest
 from oracle import oracle as O
 from nanotracer_amd import _lib as L
 from nanotracer_amd import synth
-from nanotracer_amd.api import CameraUniform, RenderServer, ViewSpec
+from nanotracer_amd.api import CameraUniform, RenderServer, ViewSpec, device_memory
 
 pytestmark = pytest.mark.gpu
 
@@ -41,18 +42,19 @@ def test_config_e_full_size_burst(ctx):
     scenes = list(synth.serving_scenes(64, SEED))
     sizes = [s.n for s in scenes]
     resident = sum(((n + 63) // 64) * 64 * 4 * 59 for n in sizes)
-    # budget: the resident scenes plus three slot workspaces, with 10% to spare
-    slot_px, slot_rec, slot_pairs = 48 << 20, 64 << 20, 600_000_000
-    slot_bytes = slot_rec * 52 + slot_pairs * 24 + slot_px * 20 + (2 << 30)
-    budget = int((resident + 3 * slot_bytes) * 1.1)
-    srv = RenderServer(hbm_budget=budget, slots=3, max_views=32, max_pixels=slot_px, max_records=slot_rec,
-                       pair_capacity=slot_pairs, tile_size=TILE, lds_batch=LDS)
+    # the serving budget: 24 GiB of what the device reports free; the scenes
+    # first, the rest split into three slot workspaces that bound each batch
+    free, _ = device_memory(0)
+    budget = min(24 << 30, int(0.9 * free))
+    slot_bytes = (budget - resident) // 3 - (1 << 20)
+    srv = RenderServer(hbm_budget=budget, slots=3, max_views=32, max_pixels=96 << 20, max_records=0,
+                       pair_capacity=0, tile_size=TILE, lds_batch=LDS, slot_bytes=slot_bytes)
     for s in scenes:
         srv.add_scene(s)
     st = srv.stats()
     assert st["scenes"] == 64 and st["hbm_in_use"] <= budget
-    print(f"{sum(sizes)} resident splats, {st['hbm_in_use'] / 1e9:.1f} GB of a {budget / 1e9:.1f} GB budget",
-          flush=True)
+    print(f"{sum(sizes)} resident splats, {st['hbm_in_use'] / 1e9:.1f} GB of a {budget / 1e9:.1f} GB budget "
+          f"({slot_bytes / 1e9:.2f} GB per slot)", flush=True)
 
     rng = np.random.default_rng(SEED + 3)
     big = int(np.argmax(sizes))
@@ -66,9 +68,11 @@ def test_config_e_full_size_burst(ctx):
     st = srv.stats()
     views = [info["batch_views"] for _, info in results]
     print(f"burst of {len(reqs)}: {st['batches']} batches, mean {np.mean(views):.2f} views per batch "
-          f"(max {max(views)}), max in flight {st['max_in_flight']}, {st['overflow_reruns']} reruns", flush=True)
+          f"(max {max(views)}), max in flight {st['max_in_flight']}, {st['overflow_reruns']} reruns, "
+          f"{st['memory_limited']} batches closed by memory", flush=True)
     assert st["completed"] >= len(reqs) and st["failed"] == 0
     assert np.mean(views) > 1.0 and max(views) >= 8
+    assert st["memory_limited"] > 0  # the budget, not the view cap, bounds batches
     for k in sample:
         sid, v = reqs[k]
         img = results[k][0]
