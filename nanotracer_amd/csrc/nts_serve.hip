@@ -11,7 +11,9 @@
 // thread retires finished slots (event query), copies each result to the
 // caller's buffer, and refills free slots from the FIFO of pending requests
 // while the other slots' batches still run: continuous batching with
-// overlap on the device.
+// overlap on the device.  The copies out of the pinned staging run on a pool
+// of copier threads in pieces of 4 MB; a slot takes its next batch once its
+// results are out.
 //
 // Admission: a request joins the batch being formed while the batch's
 // views, pixels, projected records, tiles and estimated pairs fit the slot
@@ -97,6 +99,7 @@ struct Request {
   double t_submit = 0.0, t_done = 0.0;
   uint32_t batch_views = 0;
   uint64_t pairs = 0;
+  uint32_t chunks_left = 0;  // result copies still to do (copier threads)
 };
 
 struct Resident {
@@ -134,6 +137,17 @@ struct Slot {
   // offset of its result in h_out (results of both formats packed end to end)
   std::vector<uint64_t> out_off, pix, stage_off;
   bool busy = false;
+  uint32_t copies_left = 0;  // result copies out of h_out still to do: the slot waits for them
+};
+
+// One piece of a result: copied from a slot's pinned staging into the
+// caller's buffer by a copier thread.
+struct CopyJob {
+  Request* r;
+  Slot* slot;
+  const uint8_t* src;
+  uint8_t* dst;
+  size_t bytes;
 };
 
 }  // namespace
@@ -156,6 +170,11 @@ struct nts_server {
   std::condition_variable cv, done_cv;
   std::thread th;
   bool stop = false;
+  // result copies, off the dispatcher thread
+  std::vector<std::thread> copiers;
+  std::deque<CopyJob> jobs;
+  std::condition_variable job_cv;
+  bool stop_copiers = false;
   nts_serve_stats stats{};
 };
 
@@ -507,11 +526,42 @@ void retire(nts_server* sv, Slot& sl) {
     Resident& rs = sv->scenes[r->scene];
     const double ppx = (double)r->pairs / ((double)r->view.width * r->view.height);
     rs.pairs_per_px = rs.pairs_per_px > 0.0 ? (ppx > 0.9 * rs.pairs_per_px ? ppx : 0.9 * rs.pairs_per_px) : ppx;
+    // the result leaves the pinned staging in pieces, on the copier
+    // threads; the request is done with its last piece, the slot free with
+    // the batch's last
     const size_t bytes = sl.pix[v] * (r->format == NTS_OUT_RGBA8 ? 4 : 16);
-    memcpy(r->out, sl.h_out + sl.stage_off[v], bytes);
-    finish(sv, r, REQ_DONE, NTS_OK);
+    const size_t chunk = (size_t)4 << 20;
+    const uint32_t pieces = (uint32_t)((bytes + chunk - 1) / chunk);
+    r->chunks_left = pieces;
+    sl.copies_left += pieces;
+    for (uint32_t c = 0; c < pieces; ++c) {
+      const size_t off = (size_t)c * chunk;
+      sv->jobs.push_back(CopyJob{r, &sl, sl.h_out + sl.stage_off[v] + off, (uint8_t*)r->out + off,
+                                 bytes - off < chunk ? bytes - off : chunk});
+    }
   }
   sl.reqs.clear();
+  sv->job_cv.notify_all();
+}
+
+// A copier thread: result pieces from the pinned staging to the callers'
+// buffers, outside the lock.
+void copier(nts_server* sv) {
+  std::unique_lock<std::mutex> lk(sv->mu);
+  for (;;) {
+    sv->job_cv.wait(lk, [&] { return sv->stop_copiers || !sv->jobs.empty(); });
+    if (sv->jobs.empty()) return;  // stopping, nothing left
+    const CopyJob j = sv->jobs.front();
+    sv->jobs.pop_front();
+    lk.unlock();
+    memcpy(j.dst, j.src, j.bytes);
+    lk.lock();
+    if (--j.r->chunks_left == 0) {
+      finish(sv, j.r, REQ_DONE, NTS_OK);
+      sv->done_cv.notify_all();
+    }
+    if (--j.slot->copies_left == 0) sv->cv.notify_one();  // the slot is free again
+  }
 }
 
 void run(nts_server* sv) {
@@ -528,7 +578,7 @@ void run(nts_server* sv) {
       }
     }
     for (Slot& sl : sv->slots) {
-      if (sl.busy || sv->pending.empty()) continue;
+      if (sl.busy || sl.copies_left || sv->pending.empty()) continue;
       form_batch(sv, sl);
       if (sl.reqs.empty()) {
         sv->done_cv.notify_all();  // requests failed admission
@@ -544,7 +594,7 @@ void run(nts_server* sv) {
       sl.busy = true;
       progress = true;
     }
-    for (Slot& sl : sv->slots) in_flight += sl.busy ? 1u : 0u;
+    for (Slot& sl : sv->slots) in_flight += (sl.busy || sl.copies_left) ? 1u : 0u;
     if (in_flight > sv->stats.max_in_flight) sv->stats.max_in_flight = in_flight;
     if (!progress) sv->cv.wait_for(lk, std::chrono::microseconds(in_flight ? 20 : 200));
   }
@@ -585,6 +635,11 @@ extern "C" int nts_server_create(int device, const nts_server_cfg* cfg, nts_serv
     return rc;
   }
   sv->th = std::thread(run, sv);
+  // result copy-out: enough threads for a host's memory bandwidth, not more
+  // than the slots can keep busy
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nc = hw >= 16 ? 8u : (hw >= 4 ? hw / 2 : 1u);
+  for (unsigned k = 0; k < nc; ++k) sv->copiers.emplace_back(copier, sv);
   *out = sv;
   return NTS_OK;
 }
@@ -597,6 +652,13 @@ extern "C" int nts_server_destroy(nts_server* sv) {
   }
   sv->cv.notify_all();
   if (sv->th.joinable()) sv->th.join();
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->stop_copiers = true;  // the copiers finish the queued pieces first
+  }
+  sv->job_cv.notify_all();
+  for (std::thread& t : sv->copiers)
+    if (t.joinable()) t.join();
   for (Slot& sl : sv->slots) slot_destroy(sv, sl);
   for (Resident& r : sv->scenes) nts_scene_destroy(sv->ctx0, r.scene);
   for (auto& kv : sv->reqs) delete kv.second;
