@@ -187,6 +187,17 @@ int nts_scan_u32(nts_ctx* ctx, nts_buffer* data, uint64_t n);
  * payloads, in place.  key_bits limits the sorted low bits (32 = all). */
 int nts_sort_pairs_u32(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits);
 int nts_sort_pairs_u64(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits);
+/* The binning half of TileBinner::bin (tile_binner.rs:181-382: RadixSort::sort
+ * of the (tile_id << 16 | depth16) keys, then tile_ranges.wgsl) in one call:
+ * sorts n pairs in place into the order nts_sort_pairs_u32/u64 give over the
+ * low 16 + tile_bits bits, and writes ranges[t] = [begin, end) (2 u32) for
+ * each tile t < num_tiles ((0, 0) for a tile with no pair).  Keys are below
+ * 2^(16 + tile_bits) or all ones (a dropped pair: it sorts last and lies in
+ * no range); with any all-ones key, every tile id is below 2^tile_bits - 1.
+ * key_bytes 4 or 8.  Only the tile bits are sorted device-wide; the depth
+ * order inside each group of tiles is made in LDS. */
+int nts_sort_tile_pairs(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bytes,
+                        uint32_t tile_bits, nts_buffer* ranges, uint32_t num_tiles);
 /* No reference counterpart: for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  nts_sort_pairs_u32 with the same result and errors, but every

@@ -366,6 +366,16 @@ class RadixSort:
         one GPU race."""
         L.check(L.lib().nts_sort_pairs_u32_noalloc(ctx.h, keys.h, vals.h, int(n), int(key_bits)))
 
+    def sort_tiles(self, ctx: GpuCtx, keys: Buffer, vals: Buffer, n: int, tile_bits: int, ranges: Buffer,
+                   num_tiles: int, key_bytes: int = 4) -> None:
+        """The binning sort: sort() over the low 16 + tile_bits bits of
+        (tile << 16 | depth16) keys, plus ranges (2 u32 per tile, [begin,
+        end)) of the tiles below num_tiles.  Keys are below
+        2^(16 + tile_bits) or all ones (a dropped pair); with any all-ones
+        key, tile ids stay below 2^tile_bits - 1."""
+        L.check(L.lib().nts_sort_tile_pairs(ctx.h, keys.h, vals.h, int(n), int(key_bytes), int(tile_bits),
+                                            ranges.h, int(num_tiles)))
+
 
 class Rasterizer:
     """Rasterizer (raster.rs:114-804)."""

@@ -6,9 +6,10 @@
 //   scan                   over the (view, splat) counts;
 //   emit                   (view-local tile << 16 | depth16) keys, one segment
 //                          per view;
-//   onesweep sort          segmented by view;
-//   ranges                 global tile ids (view tile base + local tile);
-//   tile order             tiles by pair count, heaviest first;
+//   binning sort           onesweep over the tile bits, segmented by view,
+//                          the tile ranges (global tile ids: view tile base +
+//                          local tile), the tile order (by pair count,
+//                          heaviest first), then each tile by depth in LDS;
 //   blend forward          one workgroup per (view, tile) in that order, loss fused in the
 //                          epilogue (dL/dC and per-view loss);
 //   blend backward         per (view, tile), per-view projected gradients;
@@ -88,14 +89,13 @@ __global__ void k_set_u32s(uint32_t* __restrict__ dst, U32Table t) {
   if (blockIdx.x == 0 && i < t.n) dst[i] = t.v[i];
 }
 
-// k_set_u32s that also latches this step's pair overflow (flags[0]) and
-// sort look-back timeout (*sort_err) into the sticky error word; it runs
-// after the sort, so both are final.
-__global__ void k_set_u32s_latch(uint32_t* __restrict__ dst, U32Table t, const uint32_t* __restrict__ flags,
-                                 const uint32_t* __restrict__ sort_err, uint32_t* __restrict__ sticky) {
+// Latches this step's pair overflow (flags[0]) and sort look-back timeout
+// (*sort_err) into the sticky error word; it runs after the sort, so both
+// are final.
+__global__ void k_latch_errors(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ sort_err,
+                               uint32_t* __restrict__ sticky) {
   const uint32_t i = threadIdx.x;
   if (blockIdx.x != 0) return;
-  if (i < t.n) dst[i] = t.v[i];
   if (i == 0) {
     const uint32_t bits = (flags[0] ? 1u : 0u) | (*sort_err ? 2u : 0u);
     if (bits) atomicOr(sticky, bits);
@@ -468,7 +468,7 @@ struct BatchPlan {
   uint32_t n_views, tile_size, n;
   ViewTable vt;
   TileTable tt;
-  uint32_t key_bits, key_bytes;
+  uint32_t tile_bits, key_bits, key_bytes;
   uint64_t pixels;
 };
 
@@ -519,9 +519,9 @@ static int plan_batch(nts_batch* b, const nts_scene* scene, const nts_view_desc*
   // flags[5] (backward), zeroed with the flags at the start of every launch
   pl.vt.tickets = b->scheduler == NTS_SCHED_PERSISTENT ? b->flags + 4 : nullptr;
   pl.pixels = pix;
-  uint32_t tb = 1;
-  while (tb < 32 && (1ull << tb) < max_tiles_view) ++tb;
-  pl.key_bits = 16 + tb;
+  // culled pairs carry the all-ones key: tile ids below 2^tb - 1
+  pl.tile_bits = nts_tile_bits_strict(max_tiles_view);
+  pl.key_bits = 16 + pl.tile_bits;
   pl.key_bytes = pl.key_bits <= 32 ? 4 : 8;
   return NTS_OK;
 }
@@ -556,35 +556,27 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
                  (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
                  b->pair_cap, (uint64_t*)b->keys_a, b->vals_a);
   }
-  if (pl.key_bytes == 4)
-    rc = nts_sort_ws_launch32(ctx, (uint32_t*)b->keys_a, b->vals_a, (uint32_t*)b->keys_b, b->vals_b, b->seg_begin, V,
-                              b->pair_cap, pl.key_bits, b->sort_ws, &alt, &b->sort_err);
-  else
-    rc = nts_sort_ws_launch64(ctx, (uint64_t*)b->keys_a, b->vals_a, (uint64_t*)b->keys_b, b->vals_b, b->seg_begin, V,
-                              b->pair_cap, pl.key_bits, b->sort_ws, &alt, &b->sort_err);
-  if (rc) return rc;
-  void* keys = alt ? b->keys_b : b->keys_a;
-  uint32_t* pay = alt ? b->vals_b : b->vals_a;
+  // tile ranges (zero for tiles with no pair) and tile bases before the sort,
+  // which writes the ranges and the tiles' blend order
   rc = nts_fill_u32(ctx, b->ranges, 0, (uint64_t)pl.vt.total_tiles * 2);
   if (rc) return rc;
-  {
-    U32Table tb;
-    memset(&tb, 0, sizeof tb);
-    tb.n = V + 1;
-    for (uint32_t v = 0; v <= V; ++v) tb.v[v] = v < V ? pl.vt.v[v].tile_base : pl.vt.total_tiles;
-    NTS_LAUNCH(ctx, "set_tile_base", k_set_u32s_latch, dim3(1), dim3(64), 0, b->tile_base, tb,
-               (const uint32_t*)b->flags, (const uint32_t*)b->sort_err, b->sticky);
-  }
-  uint32_t grid = nts_div_up(b->pair_cap, 256);
-  if (grid > 8192) grid = 8192;
+  U32Table tb;
+  memset(&tb, 0, sizeof tb);
+  tb.n = V + 1;
+  for (uint32_t v = 0; v <= V; ++v) tb.v[v] = v < V ? pl.vt.v[v].tile_base : pl.vt.total_tiles;
+  NTS_LAUNCH(ctx, "set_tile_base", k_set_u32s, dim3(1), dim3(64), 0, b->tile_base, tb);
   if (pl.key_bytes == 4)
-    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint32_t>, dim3(grid), dim3(256), 0, (const uint32_t*)keys,
-               (const uint64_t*)b->seg_begin, V, (const uint32_t*)b->tile_base, b->ranges);
+    rc = nts_bin_sort_ws_launch32(ctx, (uint32_t*)b->keys_a, b->vals_a, (uint32_t*)b->keys_b, b->vals_b, b->seg_begin,
+                                  V, b->pair_cap, pl.tile_bits, b->tile_base, pl.vt.total_tiles, b->ranges, b->order,
+                                  b->sort_ws, &alt, &b->sort_err);
   else
-    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)keys,
-               (const uint64_t*)b->seg_begin, V, (const uint32_t*)b->tile_base, b->ranges);
-  rc = nts_tile_order(ctx, b->ranges, pl.vt.total_tiles, b->order);
+    rc = nts_bin_sort_ws_launch64(ctx, (uint64_t*)b->keys_a, b->vals_a, (uint64_t*)b->keys_b, b->vals_b, b->seg_begin,
+                                  V, b->pair_cap, pl.tile_bits, b->tile_base, pl.vt.total_tiles, b->ranges, b->order,
+                                  b->sort_ws, &alt, &b->sort_err);
   if (rc) return rc;
+  uint32_t* pay = alt ? b->vals_b : b->vals_a;
+  NTS_LAUNCH(ctx, "sort_latch", k_latch_errors, dim3(1), dim3(64), 0, (const uint32_t*)b->flags,
+             (const uint32_t*)b->sort_err, b->sticky);
   rc = nts_blend_forward(ctx, pl.tile_size, b->fwd_batch, b->proj, pay, b->ranges, pl.vt, b->image,
                          keep_state ? b->final_t : nullptr, keep_state ? b->last_idx : nullptr, loss);
   if (rc) return rc;

@@ -1,5 +1,6 @@
-// Tile binning: per-splat tile counts, exclusive scan, key emission, segmented
-// onesweep sort and per-tile ranges (TileBinner::bin, tile_binner.rs:181-382).
+// Tile binning: per-splat tile counts, exclusive scan, key emission, and the
+// binning sort with its per-tile ranges (TileBinner::bin,
+// tile_binner.rs:181-382).
 #include "nts_bin.h"
 #include "nts_raster.h"
 
@@ -59,6 +60,7 @@ static int bin_fill(nts_ctx* ctx, nts_bins* b, const nts_buffer* projected, uint
   b->ranges = (uint2*)(base + lay.ranges);
   uint64_t* seg_begin = (uint64_t*)(base + lay.meta);
   uint32_t* tile_base = (uint32_t*)(base + lay.meta + 64);
+  uint32_t* order = (uint32_t*)(base + lay.order);
   b->keys = keys_a;
   b->payloads = vals_a;
 
@@ -86,32 +88,21 @@ static int bin_fill(nts_ctx* ctx, nts_bins* b, const nts_buffer* projected, uint
     NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint32_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
                (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
                total, (uint32_t*)keys_a, vals_a);
-    rc = nts_sort_pairs32_dev(ctx, (uint32_t*)keys_a, vals_a, (uint32_t*)keys_b, vals_b, seg_begin, 1, total,
-                              key_bits, &alt);
+    rc = nts_bin_sort32_dev(ctx, (uint32_t*)keys_a, vals_a, (uint32_t*)keys_b, vals_b, seg_begin, 1, total,
+                            key_bits - 16, tile_base, (uint32_t)num_tiles, b->ranges, order, &alt);
   } else {
     NTS_LAUNCH(ctx, "tile_emit", k_emit_balanced<uint64_t>, dim3(nts_div_up(n, 256)), dim3(256), 0,
                (const Projected*)projected->ptr, rt, tt, (const uint32_t*)counts, (const uint64_t*)seg_begin + 1,
                total, (uint64_t*)keys_a, vals_a);
-    rc = nts_sort_pairs_dev(ctx, (uint64_t*)keys_a, vals_a, (uint64_t*)keys_b, vals_b, seg_begin, 1, total,
-                            key_bits, &alt);
+    rc = nts_bin_sort64_dev(ctx, (uint64_t*)keys_a, vals_a, (uint64_t*)keys_b, vals_b, seg_begin, 1, total,
+                            key_bits - 16, tile_base, (uint32_t)num_tiles, b->ranges, order, &alt);
   }
   if (rc) return rc;
   if (alt) {
     b->keys = keys_b;
     b->payloads = vals_b;
   }
-  uint32_t grid = nts_div_up(total, 256);
-  if (grid > 65535) grid = 65535;
-  if (kb == 4)
-    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint32_t>, dim3(grid), dim3(256), 0, (const uint32_t*)b->keys,
-               (const uint64_t*)seg_begin, 1u, (const uint32_t*)tile_base, b->ranges);
-  else
-    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)b->keys,
-               (const uint64_t*)seg_begin, 1u, (const uint32_t*)tile_base, b->ranges);
-  // the blend order is published only once written (empty bins keep grid order)
-  uint32_t* order = (uint32_t*)(base + lay.order);
-  rc = nts_tile_order(ctx, b->ranges, num_tiles, order);
-  if (rc) return rc;
+  // the blend order (written by the sort) is published only on success
   b->order = order;
   return NTS_OK;
 }

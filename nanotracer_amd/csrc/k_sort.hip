@@ -1,6 +1,8 @@
 // Exclusive u32 scan and the segmented onesweep radix sort.
 #include <mutex>
 
+#include "nts_bin.h"
+#include "nts_raster.h"
 #include "nts_sort.h"
 
 // ---------------------------------------------------------------------------
@@ -158,7 +160,8 @@ __global__ void k_sort_parts(const uint64_t* __restrict__ seg_begin, uint32_t nu
   part_begin[num_segs] = p;
 }
 
-// Digit histograms of every pass for every segment.  A fixed grid takes
+// Digit histograms of every pass for every segment (pass ps sorts bits
+// [lo_bit + 8 ps, lo_bit + 8 ps + 8)).  A fixed grid takes
 // contiguous runs of partitions (the live count is only known on the device,
 // so a launch sized for the capacity costs nothing for empty partitions) and
 // keeps its counts in LDS until the segment changes: the global atomics are
@@ -166,7 +169,7 @@ __global__ void k_sort_parts(const uint64_t* __restrict__ seg_begin, uint32_t nu
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                         const uint32_t* __restrict__ part_begin, uint32_t num_segs,
-                                                        uint32_t passes, uint32_t* __restrict__ hist) {
+                                                        uint32_t lo_bit, uint32_t passes, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_hist[8 * RS_RADIX];
   const uint32_t total_parts = part_begin[num_segs];
   const uint32_t per = (total_parts + gridDim.x - 1) / gridDim.x;
@@ -208,7 +211,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
       if (vmask == 0) continue;
       const uint32_t n_valid = (uint32_t)__popcll(vmask);
       for (uint32_t ps = 0; ps < passes; ++ps) {
-        const uint32_t d = (uint32_t)((key >> (8 * ps)) & 255u);
+        const uint32_t d = (uint32_t)((key >> (lo_bit + 8 * ps)) & 255u);
         const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, 64);
         const bool head = valid && (lane == 0 || prev != d);
         const uint64_t heads = __ballot(head);
@@ -421,21 +424,23 @@ uint64_t nts_sort_ws_bytes(uint32_t num_segs, uint64_t max_pairs, uint32_t key_b
   return 4 * (256 + 2 * n_hist + 64 + (uint64_t)key_bytes * max_parts * RS_RADIX);
 }
 
-// Launch the sort of pairs of segments (result in keys/vals, or in the alt
-// buffers when *result_in_alt is set) with no host synchronisation.
-// seg_begin (device, num_segs + 1 entries) bounds the segments; max_pairs
-// bounds seg_begin[num_segs].  Only the low ceil(key_bits / 8) * 8 bits are
-// sorted.  A look-back that never completes sets *err (checked by callers).
+// Launch the onesweep passes over bits [lo_bit, lo_bit + 8 passes) of pairs
+// of segments (result in keys/vals, or in the alt buffers when
+// *result_in_alt is set) with no host synchronisation.  seg_begin (device,
+// num_segs + 1 entries) bounds the segments; max_pairs bounds
+// seg_begin[num_segs].  A look-back that never completes sets *err (checked
+// by callers).  *offsets_out receives the digit starts: offsets[(s * passes +
+// ps) * 256 + d] is where digit d of pass ps begins in segment s.
 template <typename K>
-static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
-                             const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
-                             void* ws, bool* result_in_alt, uint32_t** err_out) {
+static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                       const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t lo_bit,
+                       uint32_t passes, void* ws, bool* result_in_alt, uint32_t** err_out,
+                       const uint32_t** offsets_out = nullptr) {
   *result_in_alt = false;
   if (num_segs == 0 || num_segs > RS_MAX_SEGS) return nts_fail(NTS_E_INVALID, "segments %u", num_segs);
   if (max_pairs >= (1ull << 30)) return nts_fail(NTS_E_CAPACITY, "sort supports < 2^30 pairs (%llu)",
                                                  (unsigned long long)max_pairs);
-  uint32_t passes = (key_bits + 7) / 8;
-  if (passes > sizeof(K)) passes = sizeof(K);
+  if (lo_bit + 8 * passes > 8 * sizeof(K)) return nts_fail(NTS_E_INVALID, "sort bits beyond the key");
   const uint32_t max_parts = nts_div_up(max_pairs, RS_TILE) + num_segs;
   const uint64_t n_hist = (uint64_t)num_segs * sizeof(K) * RS_RADIX;
   uint32_t* part_begin = (uint32_t*)ws;
@@ -445,6 +450,7 @@ static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt,
   uint32_t* err = counters + 32;
   uint32_t* status = counters + 64;
   if (err_out) *err_out = err;
+  if (offsets_out) *offsets_out = offsets;
   int rc = nts_fill_u32(ctx, hist, 0, n_hist);
   if (rc) return rc;
   rc = nts_fill_u32(ctx, counters, 0, 64);
@@ -455,7 +461,7 @@ static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt,
              num_segs, passes, max_parts);
   const uint32_t grid = max_parts < RS_GRID ? max_parts : RS_GRID;
   NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
-             (const uint32_t*)part_begin, num_segs, passes, hist);
+             (const uint32_t*)part_begin, num_segs, lo_bit, passes, hist);
   NTS_LAUNCH(ctx, "sort_offsets", k_sort_offsets, dim3(num_segs * passes), dim3(RS_RADIX), 0, (const uint32_t*)hist,
              d_seg_begin, passes, offsets);
   K* src_k = keys;
@@ -464,13 +470,25 @@ static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt,
   uint32_t* dst_v = vals_alt;
   for (uint32_t ps = 0; ps < passes; ++ps) {
     NTS_LAUNCH(ctx, "sort_onesweep", k_onesweep<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
-               (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs, 8 * ps,
-               passes, ps, (const uint32_t*)offsets, status + (size_t)ps * max_parts * RS_RADIX, counters + ps, err);
+               (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs,
+               lo_bit + 8 * ps, passes, ps, (const uint32_t*)offsets, status + (size_t)ps * max_parts * RS_RADIX,
+               counters + ps, err);
     K* tk = src_k; src_k = dst_k; dst_k = tk;
     uint32_t* tv = src_v; src_v = dst_v; dst_v = tv;
   }
   *result_in_alt = (passes & 1u) != 0;
   return NTS_OK;
+}
+
+// The full sort: the low ceil(key_bits / 8) * 8 bits.
+template <typename K>
+static int sort_segmented_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                             const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
+                             void* ws, bool* result_in_alt, uint32_t** err_out) {
+  uint32_t passes = (key_bits + 7) / 8;
+  if (passes > sizeof(K)) passes = sizeof(K);
+  return onesweep_ws<K>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, 0, passes, ws,
+                        result_in_alt, err_out);
 }
 
 template <typename K>
@@ -521,6 +539,496 @@ int nts_sort_ws_launch64(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t*
                                      ws, result_in_alt, err);
 }
 
+// ---------------------------------------------------------------------------
+// Binning sort of (tile << 16 | depth16) keys, segment by segment, into the
+// order of the full-key stable sort (ascending key, ties in emit order).
+//
+// The onesweep passes sort only the tile bits from bit 16 + g up.  Each group
+// of 2^g consecutive tiles is then contiguous and, inside it, still in emit
+// order, so sorting every group stably by its low 16 + g bits completes the
+// order.  One workgroup sorts a group in LDS (stable LSD, 8-bit digits ranked
+// with wave64 ballots as the onesweep blocks rank theirs) and writes the
+// group's tile ranges from the sorted keys.  A group beyond the LDS capacity
+// goes in chunks through the other ping-pong buffer: one stable pass by its g
+// tile bits, then each tile by depth.  A group is 1/256 of its view's pairs,
+// which outgrows LDS for views of a few million pairs, so groups span at most
+// 2^BS_GMAX tiles; views with more tiles take the onesweep over every tile
+// bit, the tile ranges from the keys, and one workgroup per tile for the
+// depth (k_tile_sort: 8192 pairs in LDS).
+//
+// The bits sorted are [0, 16 + tile_bits): keys must be below
+// 2^(16 + tile_bits) or all ones (a culled pair), and with culled pairs the
+// tile ids stay below 2^tile_bits - 1, so the all-ones key sorts last.
+// ---------------------------------------------------------------------------
+#define BS_GMAX 0         // tile bits a group takes in LDS (0: a group is one tile)
+#define GS_GROUP_NT 1024  // threads of a group workgroup
+#define GS_TILE_NT 256    // threads of a tile workgroup
+
+// Ranks lane digits within a wave: returns whist[d] (the wave's running count
+// of digit d) plus the lanes below this one with the same digit, and adds the
+// wave's lanes of each digit to whist[d].  Digits have w <= 8 bits; lanes
+// with valid false take no part (their result is meaningless).
+__device__ __forceinline__ uint32_t gs_wave_rank(uint32_t d, uint32_t w, bool valid, uint32_t* whist) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t vb = __ballot(valid);
+  uint32_t mlo = (uint32_t)vb, mhi = (uint32_t)(vb >> 32);
+#pragma unroll
+  for (uint32_t b = 0; b < 8; ++b) {
+    if (b < w) {
+      const uint32_t neg = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+      const uint64_t bal = __ballot(neg != 0u);
+      mlo &= ~((uint32_t)bal ^ neg);
+      mhi &= ~((uint32_t)(bal >> 32) ^ neg);
+    }
+  }
+  const uint64_t match = valid ? (((uint64_t)mhi << 32) | mlo) : 0ull;
+  const uint32_t leader = match ? (uint32_t)__builtin_ctzll(match) : 64u;
+  uint32_t old = 0;
+  if (lane == leader) {
+    old = whist[d];
+    whist[d] = old + (uint32_t)__popcll(match);
+  }
+  old = __shfl(old, (int)(leader & 63u), 64);
+  return old + (uint32_t)__popcll(match & ((1ull << lane) - 1ull));
+}
+
+// After ranking: the per-wave rows of s_hist become exclusive prefixes over
+// the waves, and the thread of digit t (t < 256) gets the digit's total.
+template <int NT>
+__device__ __forceinline__ uint32_t gs_wave_prefix(uint32_t* s_hist) {
+  uint32_t tot = 0;
+  if (threadIdx.x < 256) {
+#pragma unroll 4
+    for (int w = 0; w < NT / 64; ++w) {
+      const uint32_t c = s_hist[w * 256 + threadIdx.x];
+      s_hist[w * 256 + threadIdx.x] = tot;
+      tot += c;
+    }
+  }
+  return tot;
+}
+
+// Sorts n <= NT * ITEMS pairs of (in_k, in_v) by bits [0, bits) into (out_k,
+// out_v) through LDS; in and out may be the same arrays (every pair is read
+// before any is written).  Wave w ranks the R * 64 consecutive pairs from
+// w * R * 64 (R = ceil(n / NT) rounds); the slots past n hold all-ones keys,
+// which stay behind every pair.  With ranges set, the ranges of the sorted
+// tiles below tn go to ranges[tbase + tile], positions offset by gpos.
+template <typename K, int NT, int ITEMS>
+__device__ void gs_lds_sort(const K* in_k, const uint32_t* in_v, K* out_k, uint32_t* out_v, uint32_t n,
+                            uint32_t bits, K* s_keys, uint32_t* s_vals, uint32_t* s_hist, uint32_t* s_excl,
+                            uint32_t* s_scan, uint2* ranges, uint32_t tbase, uint32_t tn, uint64_t gpos) {
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t R = (n + NT - 1) / NT;
+  const uint32_t wbase = wave * R * 64u;
+  K key[ITEMS];
+  uint32_t val[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) {
+    if ((uint32_t)r < R) {
+      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+      const bool valid = i < n;
+      key[r] = valid ? in_k[i] : ~(K)0;
+      val[r] = valid ? in_v[i] : 0u;
+    }
+  }
+  for (uint32_t shift = 0; shift < bits; shift += 8) {
+    const uint32_t w = bits - shift < 8u ? bits - shift : 8u;
+    const uint32_t mask = (1u << w) - 1u;
+    for (uint32_t i = t; i < (NT / 64) * 256; i += NT) s_hist[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      if ((uint32_t)r < R) rank[r] = gs_wave_rank((uint32_t)(key[r] >> shift) & mask, w, true, s_hist + wave * 256);
+    __syncthreads();
+    const uint32_t tot = gs_wave_prefix<NT>(s_hist);
+    uint32_t all;
+    const uint32_t ex = block_excl_scan<NT>(tot, s_scan, all);
+    if (t < 256) s_excl[t] = ex;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      if ((uint32_t)r < R) {
+        const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
+        const uint32_t lp = s_excl[d] + s_hist[wave * 256 + d] + rank[r];
+        s_keys[lp] = key[r];
+        s_vals[lp] = val[r];
+      }
+    }
+    __syncthreads();
+    if (shift + 8 < bits) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) {
+        if ((uint32_t)r < R) {
+          const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+          key[r] = s_keys[i];
+          val[r] = s_vals[i];
+        }
+      }
+    }
+  }
+  for (uint32_t j = t; j < n; j += NT) {
+    const K k = s_keys[j];
+    out_k[j] = k;
+    out_v[j] = s_vals[j];
+    if (ranges) {
+      const K tile = k >> 16;
+      if (tile < (K)tn) {
+        if (j == 0 || (s_keys[j - 1] >> 16) != tile) ranges[tbase + (uint32_t)tile].x = (uint32_t)(gpos + j);
+        if (j + 1 == n || (s_keys[j + 1] >> 16) != tile) ranges[tbase + (uint32_t)tile].y = (uint32_t)(gpos + j + 1);
+      }
+    }
+  }
+  __syncthreads();  // the caller reuses the LDS
+}
+
+// One stable counting pass by digit (key >> shift) & (2^w - 1) of n pairs of
+// (in_k, in_v) into (out_k, out_v) (other arrays), NT * ITEMS pairs at a
+// time.  With s_start set, digit d's run is left at [s_start[d], s_start[d] +
+// s_count[d]).
+template <typename K, int NT, int ITEMS>
+__device__ void gs_chunk_pass(const K* __restrict__ in_k, const uint32_t* __restrict__ in_v, K* __restrict__ out_k,
+                              uint32_t* __restrict__ out_v, uint32_t n, uint32_t shift, uint32_t w, uint32_t* s_hist,
+                              uint32_t* s_run, uint32_t* s_scan, uint32_t* s_start, uint32_t* s_count) {
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t mask = (1u << w) - 1u;
+  uint32_t* wh = s_hist + wave * 256;
+  for (uint32_t i = t; i < (NT / 64) * 256; i += NT) s_hist[i] = 0;
+  __syncthreads();
+  // digit counts, a chunk at a time (its loads issued together)
+  for (uint32_t c0 = 0; c0 < n; c0 += NT * ITEMS) {
+    const uint32_t wbase = c0 + wave * ITEMS * 64u;
+    uint32_t dg[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+      dg[r] = i < n ? (uint32_t)(in_k[i] >> shift) & mask : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      if (wbase + (uint32_t)r * 64u < n) (void)gs_wave_rank(dg[r], w, wbase + (uint32_t)r * 64u + lane < n, wh);
+  }
+  __syncthreads();
+  const uint32_t tot = gs_wave_prefix<NT>(s_hist);
+  uint32_t all;
+  const uint32_t ex = block_excl_scan<NT>(tot, s_scan, all);
+  if (t < 256) {
+    s_run[t] = ex;
+    if (s_start) {
+      s_start[t] = ex;
+      s_count[t] = tot;
+    }
+  }
+  __syncthreads();
+  K key[ITEMS];
+  uint32_t val[ITEMS], rank[ITEMS];
+  for (uint32_t c0 = 0; c0 < n; c0 += NT * ITEMS) {
+    for (uint32_t i = t; i < (NT / 64) * 256; i += NT) s_hist[i] = 0;
+    __syncthreads();
+    const uint32_t wbase = c0 + wave * ITEMS * 64u;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      rank[r] = 0;
+      if (wbase + (uint32_t)r * 64u < n) {  // the wave has pairs in this round
+        const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+        const bool valid = i < n;
+        key[r] = valid ? in_k[i] : (K)0;
+        val[r] = valid ? in_v[i] : 0u;
+        rank[r] = gs_wave_rank((uint32_t)(key[r] >> shift) & mask, w, valid, wh);
+      }
+    }
+    __syncthreads();
+    const uint32_t ctot = gs_wave_prefix<NT>(s_hist);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+      if (i < n) {
+        const uint32_t d = (uint32_t)(key[r] >> shift) & mask;
+        const uint32_t pos = s_run[d] + s_hist[wave * 256 + d] + rank[r];
+        out_k[pos] = key[r];
+        out_v[pos] = val[r];
+      }
+    }
+    __syncthreads();
+    if (t < 256) s_run[t] += ctot;
+  }
+  __syncthreads();
+}
+
+template <typename K, int NT>
+__device__ void gs_copy(const K* __restrict__ in_k, const uint32_t* __restrict__ in_v, K* __restrict__ out_k,
+                        uint32_t* __restrict__ out_v, uint32_t n) {
+  for (uint32_t j = threadIdx.x; j < n; j += NT) {
+    out_k[j] = in_k[j];
+    out_v[j] = in_v[j];
+  }
+  __syncthreads();
+}
+
+// One workgroup per group, sorting it in place in (keys, vals) with (tmp_k,
+// tmp_v) as scratch over the same positions.
+// BY_DIGIT: group blockIdx.x = (segment s, digit d) of the onesweep pass,
+// [digit_start[s * 256 + d], the next start or the segment end); the sort
+// takes bits [0, 16 + g) and writes the group's tile ranges (tile ids below
+// the segment's tile count, ranges[tile_base[s] + tile]).
+// Otherwise: group blockIdx.x = groups[blockIdx.x] (one tile), depth bits only.
+template <typename K, bool BY_DIGIT, int NT, int ITEMS>
+__global__ void __launch_bounds__(NT) k_group_sort(K* keys, uint32_t* vals, K* tmp_k, uint32_t* tmp_v,
+                                                   const uint32_t* __restrict__ digit_start,
+                                                   const uint64_t* __restrict__ seg_begin,
+                                                   const uint2* __restrict__ groups, uint32_t g,
+                                                   const uint32_t* __restrict__ tile_base, uint2* ranges) {
+  constexpr uint32_t CAP = NT * ITEMS;
+  __shared__ K s_keys[CAP];
+  __shared__ uint32_t s_vals[CAP];
+  __shared__ uint32_t s_hist[(NT / 64) * 256];
+  __shared__ uint32_t s_excl[256], s_run[256], s_start[256], s_count[256];
+  __shared__ uint32_t s_scan[NT / 64];
+  uint64_t gb, ge;
+  uint32_t d = 0, tbase = 0, tn = 0;
+  if (BY_DIGIT) {
+    const uint32_t s = blockIdx.x >> 8;
+    d = blockIdx.x & 255u;
+    gb = digit_start[blockIdx.x];
+    ge = d < 255u ? (uint64_t)digit_start[blockIdx.x + 1] : seg_begin[s + 1];
+    tbase = tile_base[s];
+    tn = tile_base[s + 1] - tbase;
+  } else {
+    const uint2 r = groups[blockIdx.x];
+    gb = r.x;
+    ge = r.y;
+  }
+  if (ge <= gb) return;
+  const uint32_t n = (uint32_t)(ge - gb);
+  K* rk = keys + gb;
+  uint32_t* rv = vals + gb;
+  K* sk = tmp_k + gb;
+  uint32_t* sv = tmp_v + gb;
+  if (n <= CAP) {
+    gs_lds_sort<K, NT, ITEMS>(rk, rv, rk, rv, n, 16 + g, s_keys, s_vals, s_hist, s_excl, s_scan,
+                              BY_DIGIT ? ranges : nullptr, tbase, tn, gb);
+    return;
+  }
+  if (BY_DIGIT && g > 0) {
+    // by tile within the group, then each tile by depth
+    gs_chunk_pass<K, NT, ITEMS>(rk, rv, sk, sv, n, 16, g, s_hist, s_run, s_scan, s_start, s_count);
+    for (uint32_t b = 0; b < (1u << g); ++b) {
+      const uint32_t bs = s_start[b], bc = s_count[b];
+      if (bc == 0) continue;
+      if (bc <= CAP) {
+        gs_lds_sort<K, NT, ITEMS>(sk + bs, sv + bs, rk + bs, rv + bs, bc, 16, s_keys, s_vals, s_hist, s_excl, s_scan,
+                                  nullptr, 0, 0, 0);
+      } else {
+        gs_chunk_pass<K, NT, ITEMS>(sk + bs, sv + bs, rk + bs, rv + bs, bc, 0, 8, s_hist, s_run, s_scan, nullptr,
+                                    nullptr);
+        gs_chunk_pass<K, NT, ITEMS>(rk + bs, rv + bs, sk + bs, sv + bs, bc, 8, 8, s_hist, s_run, s_scan, nullptr,
+                                    nullptr);
+        gs_copy<K, NT>(sk + bs, sv + bs, rk + bs, rv + bs, bc);
+      }
+      const uint32_t tile = (d << g) | b;
+      if (tile < tn && threadIdx.x == 0)
+        ranges[tbase + tile] = make_uint2((uint32_t)(gb + bs), (uint32_t)(gb + bs + bc));
+    }
+  } else {
+    // one tile: its two depth digits, ending in place
+    gs_chunk_pass<K, NT, ITEMS>(rk, rv, sk, sv, n, 0, 8, s_hist, s_run, s_scan, nullptr, nullptr);
+    gs_chunk_pass<K, NT, ITEMS>(sk, sv, rk, rv, n, 8, 8, s_hist, s_run, s_scan, nullptr, nullptr);
+    if (BY_DIGIT && d < tn && threadIdx.x == 0) ranges[tbase + d] = make_uint2((uint32_t)gb, (uint32_t)ge);
+  }
+}
+
+// One workgroup per tile, heaviest first (ranges[order[blockIdx.x]]: a large
+// tile started late would run on alone at the end), sorting its pairs in place by
+// depth (the low 16 key bits; the pairs of a tile share the bits above).
+// Up to NT * ITEMS pairs go through LDS as one u32 each, depth << 16 | the
+// pair's index in the tile: the stable LSD over the two depth digits leaves
+// them in (depth, index) order, and the payloads are then gathered by index
+// (from L2: the tile was just read).  4 B of LDS per pair keeps several tiles
+// in flight per CU.  Larger tiles take two chunked passes through (tmp_k,
+// tmp_v), CITEMS pairs per thread per chunk.
+template <typename K, int NT, int ITEMS, int CITEMS>
+__global__ void __launch_bounds__(NT) k_tile_sort(K* keys, uint32_t* vals, K* tmp_k, uint32_t* tmp_v,
+                                                  const uint2* __restrict__ ranges, const uint32_t* __restrict__ order) {
+  constexpr uint32_t CAP = NT * ITEMS;
+  static_assert(CAP <= 65536, "the pair index takes 16 bits");
+  __shared__ uint32_t s_pack[CAP];
+  __shared__ uint32_t s_hist[(NT / 64) * 256];
+  __shared__ uint32_t s_excl[256], s_run[256];
+  __shared__ uint32_t s_scan[NT / 64];
+  const uint2 rg = ranges[order[blockIdx.x]];
+  if (rg.y <= rg.x + 1u) return;  // zero or one pair: in order
+  const uint32_t n = rg.y - rg.x;
+  K* rk = keys + rg.x;
+  uint32_t* rv = vals + rg.x;
+  if (n > CAP) {
+    gs_chunk_pass<K, NT, CITEMS>(rk, rv, tmp_k + rg.x, tmp_v + rg.x, n, 0, 8, s_hist, s_run, s_scan, nullptr,
+                                 nullptr);
+    gs_chunk_pass<K, NT, CITEMS>(tmp_k + rg.x, tmp_v + rg.x, rk, rv, n, 8, 8, s_hist, s_run, s_scan, nullptr,
+                                 nullptr);
+    return;
+  }
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t R = (n + NT - 1) / NT;
+  const uint32_t wbase = wave * R * 64u;
+  const K hi = rk[0] >> 16;  // the tile's key bits above the depth
+  uint32_t p[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) {
+    if ((uint32_t)r < R) {
+      const uint32_t i = wbase + (uint32_t)r * 64u + lane;
+      p[r] = i < n ? ((uint32_t)(rk[i] & (K)0xFFFFu) << 16) | i : 0xFFFFFFFFu;
+    }
+  }
+#pragma unroll
+  for (uint32_t shift = 16; shift < 32; shift += 8) {
+    for (uint32_t i = t; i < (NT / 64) * 256; i += NT) s_hist[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      if ((uint32_t)r < R) rank[r] = gs_wave_rank((p[r] >> shift) & 255u, 8, true, s_hist + wave * 256);
+    __syncthreads();
+    const uint32_t tot = gs_wave_prefix<NT>(s_hist);
+    uint32_t all;
+    const uint32_t ex = block_excl_scan<NT>(tot, s_scan, all);
+    if (t < 256) s_excl[t] = ex;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      if ((uint32_t)r < R) {
+        const uint32_t d = (p[r] >> shift) & 255u;
+        s_pack[s_excl[d] + s_hist[wave * 256 + d] + rank[r]] = p[r];
+      }
+    }
+    __syncthreads();
+    if (shift == 16) {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r)
+        if ((uint32_t)r < R) p[r] = s_pack[wbase + (uint32_t)r * 64u + lane];
+    }
+  }
+  // gather every payload before any is overwritten
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) {
+    const uint32_t j = (uint32_t)r * NT + t;
+    if ((uint32_t)r < R && j < n) {
+      p[r] = s_pack[j];
+      rank[r] = rv[p[r] & 0xFFFFu];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) {
+    const uint32_t j = (uint32_t)r * NT + t;
+    if ((uint32_t)r < R && j < n) {
+      rk[j] = (hi << 16) | (K)(p[r] >> 16);
+      rv[j] = rank[r];
+    }
+  }
+}
+
+// Launch the binning sort (result in keys/vals, or in the alt buffers when
+// *result_in_alt is set) and the tile ranges, with no host synchronisation.
+// ranges (total_tiles entries, zero-filled by the caller) receives [begin,
+// end) of every tile; d_tile_base (num_segs + 1 entries) holds each
+// segment's first tile.  ws as for the full sort.
+template <typename K>
+static int bin_sort_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                       const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                       const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                       void* ws, bool* result_in_alt, uint32_t** err_out) {
+  if (tile_bits == 0 || 16 + tile_bits > 8 * sizeof(K))
+    return nts_fail(NTS_E_INVALID, "%u tile bits do not fit %u-byte keys", tile_bits, (unsigned)sizeof(K));
+  const bool by_digit = tile_bits <= 8 + BS_GMAX;
+  const uint32_t g = by_digit && tile_bits > 8 ? tile_bits - 8 : 0;
+  uint32_t passes = by_digit ? 1 : (tile_bits + 7) / 8;
+  if (16 + 8 * passes > 8 * sizeof(K)) passes = (8 * (uint32_t)sizeof(K) - 16) / 8;
+  const uint32_t* offsets = nullptr;
+  bool alt = false;
+  int rc = onesweep_ws<K>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, 16 + g, passes, ws,
+                          &alt, err_out, &offsets);
+  if (rc) return rc;
+  *result_in_alt = alt;
+  if (max_pairs == 0) return nts_tile_order(ctx, ranges, total_tiles, order);
+  K* rk = alt ? keys_alt : keys;
+  uint32_t* rv = alt ? vals_alt : vals;
+  K* sk = alt ? keys : keys_alt;
+  uint32_t* sv = alt ? vals : vals_alt;
+  if (by_digit) {
+    constexpr int IA = sizeof(K) == 4 ? 16 : 8;
+    NTS_LAUNCH(ctx, "bin_group_sort", (k_group_sort<K, true, GS_GROUP_NT, IA>), dim3(num_segs * 256),
+               dim3(GS_GROUP_NT), 0, rk, rv, sk, sv, offsets, d_seg_begin, (const uint2*)nullptr, g, d_tile_base,
+               ranges);
+    return nts_tile_order(ctx, ranges, total_tiles, order);
+  } else {
+    uint32_t grid = nts_div_up(max_pairs, 256);
+    if (grid > 8192) grid = 8192;
+    NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<K>, dim3(grid), dim3(256), 0, (const K*)rk, d_seg_begin, num_segs,
+               d_tile_base, ranges);
+    rc = nts_tile_order(ctx, ranges, total_tiles, order);
+    if (rc) return rc;
+    if (total_tiles > 0)
+      NTS_LAUNCH(ctx, "bin_tile_sort", (k_tile_sort<K, GS_TILE_NT, 32, 16>), dim3(total_tiles), dim3(GS_TILE_NT), 0,
+                 rk, rv, sk, sv, (const uint2*)ranges, (const uint32_t*)order);
+  }
+  return NTS_OK;
+}
+
+int nts_bin_sort_ws_launch32(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                             const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                             const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                             void* ws, bool* result_in_alt, uint32_t** err) {
+  return bin_sort_ws<uint32_t>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, tile_bits,
+                               d_tile_base, total_tiles, ranges, order, ws, result_in_alt, err);
+}
+
+int nts_bin_sort_ws_launch64(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                             const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                             const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                             void* ws, bool* result_in_alt, uint32_t** err) {
+  return bin_sort_ws<uint64_t>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, tile_bits,
+                               d_tile_base, total_tiles, ranges, order, ws, result_in_alt, err);
+}
+
+// The binning sort with its workspace from the context scratch, checked:
+// returns once the sort has finished.
+template <typename K>
+static int bin_sort_sync(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
+                         const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                         const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                         bool* result_in_alt) {
+  *result_in_alt = false;
+  if (max_pairs == 0) return nts_tile_order(ctx, ranges, total_tiles, order);
+  void* scr;
+  int rc = nts_scratch(ctx, nts_sort_ws_bytes(num_segs, max_pairs, sizeof(K)), &scr);
+  if (rc) return rc;
+  uint32_t* err = nullptr;
+  rc = bin_sort_ws<K>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, tile_bits, d_tile_base,
+                      total_tiles, ranges, order, scr, result_in_alt, &err);
+  if (rc) return rc;
+  uint32_t* hw = ctx->host_word;
+  NTS_HIP_TRY(hipMemcpyAsync(hw, err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (*hw) return nts_fail(NTS_E_HIP, "radix sort look-back timed out");
+  return NTS_OK;
+}
+
+int nts_bin_sort32_dev(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                       const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                       const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                       bool* result_in_alt) {
+  return bin_sort_sync<uint32_t>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, tile_bits,
+                                 d_tile_base, total_tiles, ranges, order, result_in_alt);
+}
+
+int nts_bin_sort64_dev(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                       const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                       const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                       bool* result_in_alt) {
+  return bin_sort_sync<uint64_t>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, tile_bits,
+                                 d_tile_base, total_tiles, ranges, order, result_in_alt);
+}
+
 template <typename K>
 static int sort_api(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits) {
   NTS_CHECK_ARG(ctx && keys && vals);
@@ -555,6 +1063,55 @@ extern "C" int nts_sort_pairs_u32(nts_ctx* ctx, nts_buffer* keys, nts_buffer* va
 
 extern "C" int nts_sort_pairs_u64(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bits) {
   return sort_api<uint64_t>(ctx, keys, vals, n, key_bits);
+}
+
+template <typename K>
+static int tile_sort_api(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t tile_bits,
+                         nts_buffer* ranges, uint32_t num_tiles) {
+  NTS_CHECK_ARG(n * sizeof(K) <= keys->bytes && n * 4 <= vals->bytes && (uint64_t)num_tiles * 8 <= ranges->bytes);
+  NTS_CHECK_ARG(tile_bits >= 1 && 16 + tile_bits <= 8 * sizeof(K) && (uint64_t)num_tiles <= (1ull << tile_bits));
+  if (n >= (1ull << 30)) return nts_fail(NTS_E_CAPACITY, "sort supports < 2^30 pairs (%llu)", (unsigned long long)n);
+  int rc = nts_fill_u32(ctx, ranges->ptr, 0, 2ull * num_tiles);
+  if (rc) return rc;
+  if (n == 0) {
+    NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return NTS_OK;
+  }
+  // ping-pong buffers, the one segment bound and the tile base
+  void* tmp = nullptr;
+  const uint64_t kbytes = (n * sizeof(K) + 255) & ~255ull;
+  const uint64_t vbytes = (n * 4 + 255) & ~255ull;
+  const uint64_t bytes = 256 + kbytes + vbytes + (uint64_t)num_tiles * 4 + 4;
+  rc = nts_dev_alloc(ctx, &tmp, bytes);
+  if (rc) return rc;
+  uint64_t* segb = (uint64_t*)tmp;
+  uint32_t* tbase = (uint32_t*)((char*)tmp + 64);
+  K* k_alt = (K*)((char*)tmp + 256);
+  uint32_t* v_alt = (uint32_t*)((char*)tmp + 256 + kbytes);
+  uint32_t* order = (uint32_t*)((char*)tmp + 256 + kbytes + vbytes);
+  uint64_t hb[2] = {0, n};
+  uint32_t ht[2] = {0, num_tiles};
+  bool alt = false;
+  if (hipMemcpyAsync(segb, hb, 16, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(tbase, ht, 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = nts_fail(NTS_E_HIP, "tile sort: bounds upload failed");
+  if (rc == NTS_OK)
+    rc = bin_sort_sync<K>(ctx, (K*)keys->ptr, (uint32_t*)vals->ptr, k_alt, v_alt, segb, 1, n, tile_bits, tbase,
+                          num_tiles, (uint2*)ranges->ptr, order, &alt);
+  if (rc == NTS_OK && alt) {
+    (void)hipMemcpyAsync(keys->ptr, k_alt, n * sizeof(K), hipMemcpyDeviceToDevice, ctx->stream);
+    (void)hipMemcpyAsync(vals->ptr, v_alt, n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  nts_dev_free(ctx, tmp, bytes);
+  return rc;
+}
+
+extern "C" int nts_sort_tile_pairs(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64_t n, uint32_t key_bytes,
+                                   uint32_t tile_bits, nts_buffer* ranges, uint32_t num_tiles) {
+  NTS_CHECK_ARG(ctx && keys && vals && ranges && (key_bytes == 4 || key_bytes == 8));
+  if (key_bytes == 4) return tile_sort_api<uint32_t>(ctx, keys, vals, n, tile_bits, ranges, num_tiles);
+  return tile_sort_api<uint64_t>(ctx, keys, vals, n, tile_bits, ranges, num_tiles);
 }
 
 // ---------------------------------------------------------------------------

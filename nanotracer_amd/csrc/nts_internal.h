@@ -188,3 +188,34 @@ int nts_sort_ws_launch32(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t*
 int nts_sort_ws_launch64(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                          const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t key_bits,
                          void* ws, bool* result_in_alt, uint32_t** err);
+// The binning sort of (tile << 16 | depth16) keys: the order of the full
+// sort over bits [0, 16 + tile_bits), the tile ranges (ranges holds
+// total_tiles entries, zero-filled by the caller; d_tile_base the first tile
+// of each segment, num_segs + 1 entries) and the blend order of the tiles
+// (order, total_tiles entries: most pairs first).  Keys are below 2^(16 + tile_bits)
+// or all ones; with all-ones keys, tile ids stay below 2^tile_bits - 1.
+// The launch variants take the full sort's workspace and do not
+// synchronise; the _dev variants wait and check the look-back error.
+int nts_bin_sort_ws_launch32(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                             const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                             const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                             void* ws, bool* result_in_alt, uint32_t** err);
+int nts_bin_sort_ws_launch64(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                             const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                             const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                             void* ws, bool* result_in_alt, uint32_t** err);
+int nts_bin_sort32_dev(nts_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                       const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                       const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                       bool* result_in_alt);
+int nts_bin_sort64_dev(nts_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                       const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t tile_bits,
+                       const uint32_t* d_tile_base, uint32_t total_tiles, uint2* ranges, uint32_t* order,
+                       bool* result_in_alt);
+// Smallest b >= 1 with 2^b > tiles: tile ids then stay below 2^b - 1, as the
+// binning sort needs where culled pairs carry the all-ones key.
+inline uint32_t nts_tile_bits_strict(uint64_t tiles) {
+  uint32_t b = 1;
+  while (b < 48 && (1ull << b) <= tiles) ++b;
+  return b;
+}

@@ -122,6 +122,7 @@ struct Slot {
   void *keys_a = nullptr, *keys_b = nullptr;
   uint32_t *vals_a = nullptr, *vals_b = nullptr;
   uint2* ranges = nullptr;
+  uint32_t* order = nullptr;  // the blend order of the tiles (most pairs first)
   float4* image = nullptr;
   uint32_t* rgba = nullptr;
   void* scan_ws = nullptr;
@@ -186,7 +187,7 @@ uint64_t al256(uint64_t x) { return (x + 255) & ~255ull; }
 // tiles tiles, keys of kb bytes, room for pair_cap pairs), laid out from the
 // start of its slot's workspace.
 struct BatchLayout {
-  uint64_t cams, proj, counts, total, seg, tb, flags, ka, kb, va, vb, rg, img, rgba, scan, sort, end;
+  uint64_t cams, proj, counts, total, seg, tb, flags, ka, kb, va, vb, rg, order, img, rgba, scan, sort, end;
 };
 
 BatchLayout layout_batch(uint32_t V, uint64_t rec, uint64_t px, uint64_t tiles, uint32_t kb, uint64_t pair_cap) {
@@ -206,6 +207,7 @@ BatchLayout layout_batch(uint32_t V, uint64_t rec, uint64_t px, uint64_t tiles, 
   L.va = take(p1 * 4);
   L.vb = take(p1 * 4);
   L.rg = take((tiles > 0 ? tiles : 1) * 8);
+  L.order = take((tiles > 0 ? tiles : 1) * 4);
   L.img = take(px * 16);
   L.rgba = take(px * 4);
   L.scan = take(nts_scan_ws_bytes(r1));
@@ -229,9 +231,7 @@ uint64_t pairs_that_fit(uint64_t bytes, uint32_t V, uint64_t rec, uint64_t px, u
 
 // Key bytes of a batch whose largest view has max_tiles_view tiles.
 uint32_t key_bytes_for(uint64_t max_tiles_view) {
-  uint32_t tbits = 1;
-  while (tbits < 32 && (1ull << tbits) < max_tiles_view) ++tbits;
-  return 16 + tbits <= 32 ? 4u : 8u;
+  return 16 + nts_tile_bits_strict(max_tiles_view) <= 32 ? 4u : 8u;
 }
 
 int slot_create(nts_server* sv, Slot& sl) {
@@ -324,9 +324,9 @@ int launch_batch(nts_server* sv, Slot& sl) {
   rt.base[V] = rec;
   vt.total_tiles = tiles;
   tb.v[V] = tiles;
-  uint32_t tbits = 1;
-  while (tbits < 32 && (1ull << tbits) < max_tiles_view) ++tbits;
-  const uint32_t key_bits = 16 + tbits, kb = key_bits <= 32 ? 4 : 8;
+  // culled pairs carry the all-ones key: tile ids below 2^tbits - 1
+  const uint32_t tbits = nts_tile_bits_strict(max_tiles_view);
+  const uint32_t kb = key_bytes_for(max_tiles_view);
   // the batch's working set, laid out in the slot's workspace with every
   // byte left over given to the (splat, tile) pairs
   uint64_t cap = pairs_that_fit(sl.bytes, V, rec, pix, tiles, kb);
@@ -349,6 +349,7 @@ int launch_batch(nts_server* sv, Slot& sl) {
     sl.vals_a = (uint32_t*)(p + L.va);
     sl.vals_b = (uint32_t*)(p + L.vb);
     sl.ranges = (uint2*)(p + L.rg);
+    sl.order = (uint32_t*)(p + L.order);
     sl.image = (float4*)(p + L.img);
     sl.rgba = (uint32_t*)(p + L.rgba);
     sl.scan_ws = p + L.scan;
@@ -378,23 +379,19 @@ int launch_batch(nts_server* sv, Slot& sl) {
                  (const Projected*)sl.proj, rt, tt, (const uint32_t*)sl.counts, (const uint64_t*)sl.d_total,
                  sl.pair_cap, (uint64_t*)sl.keys_a, sl.vals_a);
   }
+  // the sort writes the tile ranges (zero-filled above; tile bases set by
+  // the segments kernel) and the tiles' blend order
   if (kb == 4)
-    rc = nts_sort_ws_launch32(ctx, (uint32_t*)sl.keys_a, sl.vals_a, (uint32_t*)sl.keys_b, sl.vals_b, sl.seg_begin, V,
-                              sl.pair_cap, key_bits, sl.sort_ws, &alt, &sl.sort_err);
+    rc = nts_bin_sort_ws_launch32(ctx, (uint32_t*)sl.keys_a, sl.vals_a, (uint32_t*)sl.keys_b, sl.vals_b, sl.seg_begin,
+                                  V, sl.pair_cap, tbits, sl.tile_base, tiles, sl.ranges, sl.order, sl.sort_ws, &alt,
+                                  &sl.sort_err);
   else
-    rc = nts_sort_ws_launch64(ctx, (uint64_t*)sl.keys_a, sl.vals_a, (uint64_t*)sl.keys_b, sl.vals_b, sl.seg_begin, V,
-                              sl.pair_cap, key_bits, sl.sort_ws, &alt, &sl.sort_err);
+    rc = nts_bin_sort_ws_launch64(ctx, (uint64_t*)sl.keys_a, sl.vals_a, (uint64_t*)sl.keys_b, sl.vals_b, sl.seg_begin,
+                                  V, sl.pair_cap, tbits, sl.tile_base, tiles, sl.ranges, sl.order, sl.sort_ws, &alt,
+                                  &sl.sort_err);
   if (rc) return rc;
-  void* keys = alt ? sl.keys_b : sl.keys_a;
   const uint32_t* pay = alt ? sl.vals_b : sl.vals_a;
-  uint32_t grid = nts_div_up(sl.pair_cap, 256);
-  if (grid > 8192) grid = 8192;
-  if (kb == 4)
-    NTS_LAUNCH(ctx, "serve_ranges", k_tile_ranges<uint32_t>, dim3(grid), dim3(256), 0, (const uint32_t*)keys,
-               (const uint64_t*)sl.seg_begin, V, (const uint32_t*)sl.tile_base, sl.ranges);
-  else
-    NTS_LAUNCH(ctx, "serve_ranges", k_tile_ranges<uint64_t>, dim3(grid), dim3(256), 0, (const uint64_t*)keys,
-               (const uint64_t*)sl.seg_begin, V, (const uint32_t*)sl.tile_base, sl.ranges);
+  vt.order = sl.order;
   rc = nts_blend_forward(ctx, ts, sv->cfg.lds_batch, sl.proj, pay, sl.ranges, vt, sl.image, nullptr, nullptr, nullptr);
   if (rc) return rc;
   // outputs: rgba8 (tonemapped) for the requests that asked for it, float

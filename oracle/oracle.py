@@ -176,6 +176,16 @@ def sort_pairs(keys: np.ndarray, vals: np.ndarray):
     return k, v
 
 
+def tile_ranges(sorted_keys: np.ndarray, num_tiles: int) -> np.ndarray:
+    """tile_ranges.wgsl over sorted u32 (tile << 16 | depth16) keys: 2 u32
+    per tile, [begin, end), (0, 0) for a tile with no key; keys whose tile is
+    num_tiles or more lie in no range."""
+    k = np.ascontiguousarray(sorted_keys, np.uint32)
+    ranges = np.zeros(2 * max(num_tiles, 1), np.uint32)
+    lib().orc_tile_ranges(_p(k), C.c_uint64(k.size), C.c_uint32(num_tiles), _p(ranges))
+    return ranges[: 2 * num_tiles]
+
+
 def bin(projected: np.ndarray, t: Tiling, cull: bool = False):
     """Full TileBinner::bin: (sorted u32 keys, payloads, ranges[2*tiles]).
     cull: only the tiles each record's alpha >= 1/255 ellipse can reach, as

@@ -89,6 +89,118 @@ def test_sort_u64_stable_vs_oracle(ctx, n, bits):
     assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
 
 
+def _tile_bits_strict(num_tiles):
+    b = 1
+    while (1 << b) <= num_tiles:
+        b += 1
+    return b
+
+
+def _tile_keys(rng, n, num_tiles, hot, cull, dtype):
+    """(tile << 16 | depth16) keys: a fraction `hot` of them in three tiles (so
+    groups and single tiles outgrow a workgroup's LDS), depths from a small
+    set for a third of them (ties, which must keep their input order), and a
+    fraction `cull` all ones (dropped pairs)."""
+    tiles = rng.integers(0, num_tiles, n).astype(np.uint64)
+    nh = int(hot * n)
+    tiles[:nh] = rng.choice(np.array([0, num_tiles // 2, num_tiles - 1], np.uint64), nh)
+    rng.shuffle(tiles)
+    depth = rng.integers(0, 1 << 16, n).astype(np.uint64)
+    tie = rng.uniform(size=n) < 0.33
+    depth[tie] = rng.choice(np.array([0, 7, 0xFFFF], np.uint64), int(tie.sum()))
+    keys = (tiles << np.uint64(16)) | depth
+    keys[rng.uniform(size=n) < cull] = np.uint64((1 << (8 * np.dtype(dtype).itemsize)) - 1)
+    return keys.astype(dtype)
+
+
+def _tile_sort(ctx, keys, vals, tile_bits, num_tiles):
+    kb, vb = ctx.storage_buffer(keys), ctx.storage_buffer(vals)
+    rb = ctx.storage_buffer(np.full(2 * max(num_tiles, 1), 0xDEAD, np.uint32))
+    RadixSort().sort_tiles(ctx, kb, vb, keys.size, tile_bits, rb, num_tiles, keys.dtype.itemsize)
+    return (kb.download(keys.dtype, keys.size), vb.download(np.uint32, keys.size),
+            rb.download(np.uint32, 2 * num_tiles))
+
+
+def _ranges_u64(sorted_keys, num_tiles):
+    """[begin, end) of each tile by binary search over the sorted tile ids."""
+    tiles = (sorted_keys >> np.uint64(16)).astype(np.int64)
+    ids = np.arange(num_tiles)
+    lo = np.searchsorted(tiles, ids, "left")
+    hi = np.searchsorted(tiles, ids, "right")
+    empty = lo == hi
+    lo[empty] = 0
+    hi[empty] = 0
+    return np.stack([lo, hi], 1).reshape(-1).astype(np.uint32)
+
+
+# tile counts: 200 (8 tile bits, one tile per group), 2040 (1920x1080 at
+# 32-px tiles: groups of 8 tiles), 8160 (1920x1080 at 16 px: groups of 32),
+# 32400 (3840x2160 at 16 px: every tile bit device-wide, then per tile)
+@pytest.mark.parametrize("num_tiles,n,hot,cull", [
+    (200, 60_000, 0.0, 0.0), (200, 300_000, 0.5, 0.05),
+    (2040, 400_000, 0.0, 0.05), (2040, 400_000, 0.6, 0.0),
+    (8160, 300_000, 0.3, 0.05), (32400, 300_000, 0.3, 0.05), (32400, 50_000, 0.0, 0.0),
+])
+def test_tile_sort_equals_full_sort_u32(ctx, num_tiles, n, hot, cull):
+    rng = np.random.default_rng(num_tiles * 7 + n + int(100 * hot))
+    tb = _tile_bits_strict(num_tiles)
+    keys = _tile_keys(rng, n, num_tiles, hot, cull, np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv, sr = _tile_sort(ctx, keys, vals, tb, num_tiles)
+    ok, ov = O.sort_pairs(keys, vals)
+    assert np.array_equal(sk, ok)
+    assert np.array_equal(sv, ov)
+    assert np.array_equal(sr, O.tile_ranges(ok, num_tiles))
+
+
+def test_tile_sort_all_tile_bits_used_without_dropped_pairs(ctx):
+    # 2048 tiles in 11 bits: tile 2047 exists, allowed with no all-ones key
+    rng = np.random.default_rng(11)
+    keys = _tile_keys(rng, 200_000, 2048, 0.2, 0.0, np.uint32)
+    keys[:100] = np.uint32((2047 << 16) | 0xFFFF)
+    vals = rng.integers(0, 2**32, keys.size, dtype=np.uint64).astype(np.uint32)
+    sk, sv, sr = _tile_sort(ctx, keys, vals, 11, 2048)
+    ok, ov = O.sort_pairs(keys, vals)
+    assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
+    assert np.array_equal(sr, O.tile_ranges(ok, 2048))
+
+
+def test_tile_sort_u64_keys(ctx):
+    # 3840x2160 at 8-px tiles: 129,600 tiles, 17 tile bits, 64-bit keys
+    rng = np.random.default_rng(17)
+    num_tiles, n = 129_600, 400_000
+    keys = _tile_keys(rng, n, num_tiles, 0.3, 0.05, np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv, sr = _tile_sort(ctx, keys, vals, 17, num_tiles)
+    ok, ov = O.sort_pairs(keys, vals)
+    assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
+    live = ok[ok != np.uint64(2**64 - 1)]
+    assert np.array_equal(sr, _ranges_u64(live, num_tiles))
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 4097])
+def test_tile_sort_small_and_edge_sizes(ctx, n):
+    rng = np.random.default_rng(n)
+    keys = _tile_keys(rng, max(n, 1), 100, 0.0, 0.1, np.uint32)[:n]
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv, sr = _tile_sort(ctx, keys, vals, 7, 100)
+    ok, ov = O.sort_pairs(keys, vals)
+    assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
+    assert np.array_equal(sr, O.tile_ranges(ok, 100))
+
+
+def test_tile_sort_rejects_bad_widths(ctx):
+    keys = np.zeros(16, np.uint32)
+    vals = np.zeros(16, np.uint32)
+    kb, vb = ctx.storage_buffer(keys), ctx.storage_buffer(vals)
+    rb = ctx.storage_buffer(np.zeros(8, np.uint32))
+    for bits in (0, 17):
+        with pytest.raises(Exception):
+            RadixSort().sort_tiles(ctx, kb, vb, 16, bits, rb, 4, 4)
+    with pytest.raises(Exception):  # 5 tiles do not fit 2 tile bits
+        RadixSort().sort_tiles(ctx, kb, vb, 16, 2, rb, 5, 4)
+
+
 def _bin_gpu(ctx, proj, params):
     buf = ctx.storage_buffer(np.ascontiguousarray(proj, np.float32))
     res = TileBinner().bin(ctx, buf, proj.shape[0], params)
