@@ -154,7 +154,7 @@ __device__ __forceinline__ uint32_t tile_order_bucket(uint32_t pairs) {
 }
 
 __global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ ranges, uint32_t n_tiles,
-                                                     uint32_t* __restrict__ order) {
+                                                     uint32_t* __restrict__ order, uint32_t* __restrict__ n_heavy) {
   __shared__ uint32_t s_cnt[NTS_ORDER_BUCKETS];
   __shared__ uint32_t s_pos[NTS_ORDER_BUCKETS];
   for (uint32_t i = threadIdx.x; i < NTS_ORDER_BUCKETS; i += blockDim.x) s_cnt[i] = 0;
@@ -168,6 +168,9 @@ __global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ r
     uint32_t above = 0;
     for (uint32_t j = threadIdx.x + 1; j < NTS_ORDER_BUCKETS; ++j) above += s_cnt[j];
     s_pos[threadIdx.x] = above;
+    // tiles of NTS_TILE_HEAVY pairs or more: the buckets from its own up
+    // (a power of two starts a bucket)
+    if (n_heavy && threadIdx.x == tile_order_bucket(NTS_TILE_HEAVY)) *n_heavy = above + s_cnt[threadIdx.x];
   }
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < n_tiles; t += blockDim.x) {
@@ -176,9 +179,9 @@ __global__ void __launch_bounds__(1024) k_tile_order(const uint2* __restrict__ r
   }
 }
 
-int nts_tile_order(nts_ctx* ctx, const uint2* ranges, uint32_t n_tiles, uint32_t* order) {
-  if (n_tiles == 0) return NTS_OK;
-  NTS_LAUNCH(ctx, "tile_order", k_tile_order, dim3(1), dim3(1024), 0, ranges, n_tiles, order);
+int nts_tile_order(nts_ctx* ctx, const uint2* ranges, uint32_t n_tiles, uint32_t* order, uint32_t* n_heavy) {
+  if (n_tiles == 0) return n_heavy ? nts_fill_u32(ctx, n_heavy, 0, 1) : NTS_OK;
+  NTS_LAUNCH(ctx, "tile_order", k_tile_order, dim3(1), dim3(1024), 0, ranges, n_tiles, order, n_heavy);
   return NTS_OK;
 }
 

@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(RS_RADIX) k_sort_offsets(const uint32_t* __res
 }
 
 template <typename K>
-__global__ void __launch_bounds__(RS_BLOCK) k_onesweep(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+__global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(RS_WAVES_PER_EU))) k_onesweep(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                                                        K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                        const uint64_t* __restrict__ seg_begin,
                                                        const uint32_t* __restrict__ part_begin, uint32_t num_segs,
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_onesweep(const K* __restrict__ key
       // The words of up to LB_WIN predecessors are read together (one round
       // trip to the device-coherent level per window, not per partition)
       // and consumed in order until an inclusive one or the segment start.
-      constexpr uint32_t LB_WIN = 4;
+      constexpr uint32_t LB_WIN = RS_LB_WIN;
       uint32_t j = p - 1;  // next predecessor to consume, >= first_part
       uint32_t spins = 0;
       while (true) {
@@ -435,7 +435,7 @@ template <typename K>
 static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
                        const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t lo_bit,
                        uint32_t passes, void* ws, bool* result_in_alt, uint32_t** err_out,
-                       const uint32_t** offsets_out = nullptr) {
+                       const uint32_t** offsets_out = nullptr, uint32_t** counters_out = nullptr) {
   *result_in_alt = false;
   if (num_segs == 0 || num_segs > RS_MAX_SEGS) return nts_fail(NTS_E_INVALID, "segments %u", num_segs);
   if (max_pairs >= (1ull << 30)) return nts_fail(NTS_E_CAPACITY, "sort supports < 2^30 pairs (%llu)",
@@ -451,6 +451,7 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   uint32_t* status = counters + 64;
   if (err_out) *err_out = err;
   if (offsets_out) *offsets_out = offsets;
+  if (counters_out) *counters_out = counters;  // words 40-63 are free for the caller
   int rc = nts_fill_u32(ctx, hist, 0, n_hist);
   if (rc) return rc;
   rc = nts_fill_u32(ctx, counters, 0, 64);
@@ -838,26 +839,110 @@ __global__ void __launch_bounds__(NT) k_group_sort(K* keys, uint32_t* vals, K* t
   }
 }
 
-// One workgroup per tile, heaviest first (ranges[order[blockIdx.x]]: a large
-// tile started late would run on alone at the end), sorting its pairs in place by
-// depth (the low 16 key bits; the pairs of a tile share the bits above).
-// Up to NT * ITEMS pairs go through LDS as one u32 each, depth << 16 | the
-// pair's index in the tile: the stable LSD over the two depth digits leaves
+// Compiler barrier between LDS accesses of one wave that different lanes
+// make: the hardware runs a wave's LDS instructions in order, so only the
+// compiler could reorder them.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One wave sorts the n <= 64 * WITEMS pairs of a tile in place by depth, as
+// the workgroup path of k_tile_sort does but with no barrier: the packed
+// words go through the wave's own LDS (s_pack: 64 * WITEMS words, wh: 256
+// digit counters), and one wave scan turns the counts into digit offsets.
+template <typename K, int WITEMS>
+__device__ void gs_wave_sort(K* rk, uint32_t* rv, uint32_t n, uint32_t* s_pack, uint32_t* wh) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t R = (n + 63u) / 64u;
+  const K hi = rk[0] >> 16;
+  uint32_t p[WITEMS], rank[WITEMS];
+#pragma unroll
+  for (int r = 0; r < WITEMS; ++r) {
+    if ((uint32_t)r < R) {
+      const uint32_t i = (uint32_t)r * 64u + lane;
+      p[r] = i < n ? ((uint32_t)(rk[i] & (K)0xFFFFu) << 16) | i : 0xFFFFFFFFu;
+    }
+  }
+#pragma unroll
+  for (uint32_t shift = 16; shift < 32; shift += 8) {
+    reinterpret_cast<uint4*>(wh)[lane] = make_uint4(0u, 0u, 0u, 0u);
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < WITEMS; ++r)
+      if ((uint32_t)r < R) rank[r] = gs_wave_rank((p[r] >> shift) & 255u, 8, true, wh);
+    wave_lds_order();
+    // digit offsets: lane l owns digits 4 l .. 4 l + 3
+    const uint4 c = reinterpret_cast<const uint4*>(wh)[lane];
+    const uint32_t sum = c.x + c.y + c.z + c.w;
+    const uint32_t ex = wave_incl_scan(sum) - sum;
+    reinterpret_cast<uint4*>(wh)[lane] = make_uint4(ex, ex + c.x, ex + c.x + c.y, ex + c.x + c.y + c.z);
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < WITEMS; ++r)
+      if ((uint32_t)r < R) s_pack[wh[(p[r] >> shift) & 255u] + rank[r]] = p[r];
+    wave_lds_order();
+    if (shift == 16) {
+#pragma unroll
+      for (int r = 0; r < WITEMS; ++r)
+        if ((uint32_t)r < R) p[r] = s_pack[(uint32_t)r * 64u + lane];
+      wave_lds_order();
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < WITEMS; ++r) {
+    const uint32_t j = (uint32_t)r * 64u + lane;
+    if ((uint32_t)r < R && j < n) {
+      p[r] = s_pack[j];
+      rank[r] = rv[p[r] & 0xFFFFu];
+    }
+  }
+  // every payload gathered (vmcnt(0)) before any is overwritten
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+#pragma unroll
+  for (int r = 0; r < WITEMS; ++r) {
+    const uint32_t j = (uint32_t)r * 64u + lane;
+    if ((uint32_t)r < R && j < n) {
+      rk[j] = (hi << 16) | (K)(p[r] >> 16);
+      rv[j] = rank[r];
+    }
+  }
+}
+
+// Sorts every tile's pairs in place by depth (the low 16 key bits; the pairs
+// of a tile share the bits above), heaviest tiles first (order: a large tile
+// started late would run on alone at the end).  The first *n_heavy tiles of
+// the order, those of NTS_TILE_HEAVY pairs or more, take one workgroup each:
+// up to NT * ITEMS pairs go through LDS as one u32 each, depth << 16 | the
+// pair's index in the tile, the stable LSD over the two depth digits leaves
 // them in (depth, index) order, and the payloads are then gathered by index
-// (from L2: the tile was just read).  4 B of LDS per pair keeps several tiles
+// (from L2: the tile was just read); 4 B of LDS per pair keeps several tiles
 // in flight per CU.  Larger tiles take two chunked passes through (tmp_k,
-// tmp_v), CITEMS pairs per thread per chunk.
+// tmp_v), CITEMS pairs per thread per chunk.  The lighter tiles take one wave
+// each (gs_wave_sort), NT / 64 per workgroup, with no barrier.  The grid has
+// one workgroup per tile, which covers both.
 template <typename K, int NT, int ITEMS, int CITEMS>
-__global__ void __launch_bounds__(NT) k_tile_sort(K* keys, uint32_t* vals, K* tmp_k, uint32_t* tmp_v,
-                                                  const uint2* __restrict__ ranges, const uint32_t* __restrict__ order) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) k_tile_sort(K* keys, uint32_t* vals, K* tmp_k, uint32_t* tmp_v,
+                                                  const uint2* __restrict__ ranges, const uint32_t* __restrict__ order,
+                                                  const uint32_t* __restrict__ n_heavy, uint32_t n_tiles) {
   constexpr uint32_t CAP = NT * ITEMS;
   static_assert(CAP <= 65536, "the pair index takes 16 bits");
+  static_assert(64u * ITEMS == NTS_TILE_HEAVY, "a wave sorts every tile below the heavy class");
   __shared__ uint32_t s_pack[CAP];
-  __shared__ uint32_t s_hist[(NT / 64) * 256];
+  __shared__ __align__(16) uint32_t s_hist[(NT / 64) * 256];  // the wave path reads it as uint4
   __shared__ uint32_t s_excl[256], s_run[256];
   __shared__ uint32_t s_scan[NT / 64];
+  const uint32_t heavy = *n_heavy;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  if (blockIdx.x >= heavy) {
+    const uint32_t k = heavy + (blockIdx.x - heavy) * (NT / 64) + wave;
+    if (k >= n_tiles) return;
+    const uint2 rg = ranges[order[k]];
+    if (rg.y <= rg.x + 1u) return;  // zero or one pair: in order
+    gs_wave_sort<K, ITEMS>(keys + rg.x, vals + rg.x, rg.y - rg.x, s_pack + wave * 64u * ITEMS, s_hist + wave * 256u);
+    return;
+  }
   const uint2 rg = ranges[order[blockIdx.x]];
-  if (rg.y <= rg.x + 1u) return;  // zero or one pair: in order
   const uint32_t n = rg.y - rg.x;
   K* rk = keys + rg.x;
   uint32_t* rv = vals + rg.x;
@@ -868,7 +953,6 @@ __global__ void __launch_bounds__(NT) k_tile_sort(K* keys, uint32_t* vals, K* tm
                                  nullptr);
     return;
   }
-  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   const uint32_t R = (n + NT - 1) / NT;
   const uint32_t wbase = wave * R * 64u;
   const K hi = rk[0] >> 16;  // the tile's key bits above the depth
@@ -945,8 +1029,9 @@ static int bin_sort_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   if (16 + 8 * passes > 8 * sizeof(K)) passes = (8 * (uint32_t)sizeof(K) - 16) / 8;
   const uint32_t* offsets = nullptr;
   bool alt = false;
+  uint32_t* counters = nullptr;
   int rc = onesweep_ws<K>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, 16 + g, passes, ws,
-                          &alt, err_out, &offsets);
+                          &alt, err_out, &offsets, &counters);
   if (rc) return rc;
   *result_in_alt = alt;
   if (max_pairs == 0) return nts_tile_order(ctx, ranges, total_tiles, order);
@@ -965,11 +1050,13 @@ static int bin_sort_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
     if (grid > 8192) grid = 8192;
     NTS_LAUNCH(ctx, "tile_ranges", k_tile_ranges<K>, dim3(grid), dim3(256), 0, (const K*)rk, d_seg_begin, num_segs,
                d_tile_base, ranges);
-    rc = nts_tile_order(ctx, ranges, total_tiles, order);
+    uint32_t* n_heavy = counters + 40;
+    rc = nts_tile_order(ctx, ranges, total_tiles, order, n_heavy);
     if (rc) return rc;
     if (total_tiles > 0)
       NTS_LAUNCH(ctx, "bin_tile_sort", (k_tile_sort<K, GS_TILE_NT, 32, 16>), dim3(total_tiles), dim3(GS_TILE_NT), 0,
-                 rk, rv, sk, sv, (const uint2*)ranges, (const uint32_t*)order);
+                 rk, rv, sk, sv, (const uint2*)ranges, (const uint32_t*)order, (const uint32_t*)n_heavy,
+                 total_tiles);
   }
   return NTS_OK;
 }

@@ -100,8 +100,12 @@ bool nts_blend_config_ok(uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_ba
 // tonemap.wgsl:27-41 over np pixels (rgba8 out).
 int nts_tonemap_dev(nts_ctx* ctx, const float4* image, uint32_t* rgba8, uint64_t np);
 // The blend order of n_tiles tiles (ViewTable::order): most pairs first,
-// from the tile ranges.
-int nts_tile_order(nts_ctx* ctx, const uint2* ranges, uint32_t n_tiles, uint32_t* order);
+// from the tile ranges.  With n_heavy set, the number of tiles of at least
+// NTS_TILE_HEAVY pairs (the first *n_heavy of the order) goes to that device
+// word.
+#define NTS_TILE_HEAVY 2048u
+int nts_tile_order(nts_ctx* ctx, const uint2* ranges, uint32_t n_tiles, uint32_t* order,
+                   uint32_t* n_heavy = nullptr);
 int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, const Projected* proj,
                       const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, float4* out, float* final_t,
                       uint32_t* last_idx, const LossArgs* loss);

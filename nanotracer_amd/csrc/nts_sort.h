@@ -14,6 +14,12 @@
 #define RS_ITEMS 16
 #endif
 #define RS_TILE (RS_BLOCK * RS_ITEMS)
+#ifndef RS_WAVES_PER_EU
+#define RS_WAVES_PER_EU 3  // onesweep occupancy floor per SIMD (bounds its VGPRs)
+#endif
+#ifndef RS_LB_WIN
+#define RS_LB_WIN 4  // predecessors whose look-back words are read per round trip
+#endif
 #define RS_RADIX 256
 #define RS_MAX_SEGS 64
 #define RS_GRID 2048  // blocks of the partition-walking sort kernels
