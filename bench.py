@@ -122,6 +122,34 @@ N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 FMA_PROBE_RATE = 0.244
 
 
+def blend_evals_of(ctx, evals, kernels, cfg, pairs):
+    """The blend kernels' work counted on the device (one counted step) and the recompute
+    backward against the ceiling of its own arithmetic: a probe replays the backward's
+    per-(splat, wave) code (bwd_hit, the nine-term wave reduce, the LDS slot atomic) at the
+    kernel's occupancy with every pixel taking part and no memory traffic or barriers; the
+    counted evaluations at the probe's rate give the time the arithmetic alone needs.  The
+    probe's evaluation runs all four pixel branches, the dearest path, so that time is an upper
+    estimate of the arithmetic's share."""
+    tile, fb, bb = cfg
+    lanes_px_bwd = 64 * (4 if tile == 32 else 1)  # pixels a backward wave covers (2x2 per lane at tile 32)
+    out = dict(evals)
+    out["fwd_evaluated_pixel_splats"] = evals["fwd_wave_evals"] * 64
+    out["bwd_evaluated_pixel_splats"] = evals["bwd_wave_evals"] * lanes_px_bwd
+    for name, key, ev in (("blend_fwd_loss", (tile, fb), "fwd_wave_evals"),
+                          ("blend_bwd_recompute", (tile, bb), "bwd_wave_evals")):
+        pk = (name,) + key
+        if pk in PROFILED_VALU and evals[ev] and pairs:
+            instr = PROFILED_VALU[pk][0] * pairs / PROFILED_VALU[pk][1]
+            out[name.split("_")[1] + "_valu_per_wave_eval"] = round(instr / evals[ev], 1)
+    if "blend_bwd_recompute" in kernels and evals["bwd_wave_evals"]:
+        rate = ctx.probe_blend_mix(tile, bb)
+        meas = kernels["blend_bwd_recompute"]["avg_ms"]
+        ceil_ms = evals["bwd_wave_evals"] / rate * 1e3
+        out["bwd_mix_probe"] = dict(wave_evals_per_s=round(rate, 1), arithmetic_ms=round(ceil_ms, 4),
+                                    measured_ms=round(meas, 4), frac=round(ceil_ms / meas, 3))
+    return out
+
+
 def roofline_of(kernels, cfg=None, pairs=None):
     """cfg = (tile, forward LDS batch, backward LDS batch) of the run."""
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
@@ -436,6 +464,17 @@ def run_batched(args, ctx):
     ctx.prof_enable(False)
     n_tiles = sum((-(-v.width // tile)) * (-(-v.height // tile)) for v in specs)
     kernels = kernel_table(ctx.prof_report(), s.n, visible, pairs, pixels, n_tiles, N_VIEWS)
+    # evaluation counts of one more step, taken by the counting form of the
+    # blend kernels (not timed; the grid scheduler, which the form requires)
+    if tune and tune.scheduler != "grid":
+        b.set_scheduler(L.SCHED_GRID)
+    b.set_eval_counting(True)
+    b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, mode, graph=False)
+    t += 1
+    evals = b.eval_counts()
+    b.set_eval_counting(False)
+    if tune and tune.scheduler != "grid":
+        b.set_scheduler(L.SCHED_PERSISTENT)
     ssim = ssim_step(ctx, b, g, specs, tbuf, t, tile, mode, min(args.steps, 10), s.n, visible, pairs, pixels,
                      n_tiles, args.ssim_lambda) if args.ssim_lambda > 0 else None
     b.free()
@@ -470,6 +509,7 @@ def run_batched(args, ctx):
                                 formula="2368 N + sum_v (132 N_v + 116 K_v + 76 P_v + 8 T_v)",
                                 counters=dict(N=s.n, N_v=visible, K=pairs, P=pixels, T=n_tiles))
     out["valu"] = valu_of(kernels, (tile, lds[0], lds[1]), pairs, peaks)
+    out["blend_evals"] = blend_evals_of(ctx, evals, kernels, (tile, lds[0], lds[1]), pairs)
     if ssim:
         out["ssim_step"] = ssim
     if tune:

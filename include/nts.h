@@ -107,6 +107,14 @@ int nts_prof_report(nts_ctx* ctx, char* buf, size_t cap);
  * the cycle counter over the wall clock).  Allocates 2 x copy_bytes. */
 int nts_probe_peaks(nts_ctx* ctx, uint64_t copy_bytes, double* hbm_copy_gbs, double* lds_read_tbs,
                     double* clock_ghz);
+/* Instruction-mix ceiling of the recompute blend backward (no reference
+ * counterpart): the (splat, wave) evaluations per second a launch of the
+ * backward's own per-evaluation code reaches at the kernel's workgroup size,
+ * LDS and occupancy, with every pixel taking part and no memory traffic or
+ * barriers (tile_size 8, 16 or 32; bwd_batch 64, 128 or 256).  With the
+ * evaluation count of a step (nts_batch_eval_counts) it bounds the
+ * backward's time from below by its arithmetic alone. */
+int nts_probe_blend_mix(nts_ctx* ctx, uint32_t tile_size, uint32_t bwd_batch, double* bwd_hits_per_s);
 
 /* ---- device buffers (replaces WgpuCtx::storage_buffer[_zeroed] and
  *      readback, gpu.rs:79-152) ------------------------------------------ */
@@ -343,6 +351,18 @@ int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* batch, uint32_t fwd_batch, 
 #define NTS_SCHED_GRID 0
 #define NTS_SCHED_PERSISTENT 1
 int nts_batch_set_scheduler(nts_ctx* ctx, nts_batch* batch, int scheduler);
+/* Evaluation counting (no reference counterpart; a measurement aid, off by
+ * default).  While on, the batch's steps run the grid-scheduler blend
+ * kernels in a form that also counts their work, and nts_batch_eval_counts
+ * returns the last step's counts: [0] (splat, wave) evaluations of the
+ * forward (a wave of 64 lanes testing one staged splat against its pixels),
+ * [1] (pixel, splat) pairs the forward blended, [2] (splat, wave)
+ * evaluations of the backward, [3] (pixel, splat) pairs that took part in
+ * the backward.  Images and gradients are unchanged; the counting form is
+ * slower and is not the one timed.  With NTS_SCHED_PERSISTENT nothing is
+ * counted (the counts read 0). */
+int nts_batch_set_eval_counting(nts_ctx* ctx, nts_batch* batch, int on);
+int nts_batch_eval_counts(nts_ctx* ctx, const nts_batch* batch, uint64_t counts[4]);
 /* Blend factor lambda of NTS_LOSS_SSIM steps, (1 - lambda) MSE + lambda
  * (1 - SSIM) (TrainConfig::ssim_lambda, train.rs:48-52; 0.2 by default,
  * main.rs:189-190), clamped to [0, 1].  The first lambda > 0 allocates the

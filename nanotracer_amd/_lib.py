@@ -123,6 +123,7 @@ _SIGS = {
     "nts_prof_report": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "nts_probe_peaks": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
+    "nts_probe_blend_mix": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     "nts_buffer_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "nts_buffer_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nts_buffer_size": (C.c_uint64, [C.c_void_p]),
@@ -208,6 +209,8 @@ _SIGS = {
     "nts_batch_set_lds_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "nts_batch_set_ssim_lambda": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
     "nts_batch_set_scheduler": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "nts_batch_set_eval_counting": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "nts_batch_eval_counts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nts_batch_losses": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nts_batch_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                   C.POINTER(C.c_uint32)]),

@@ -113,6 +113,14 @@ class GpuCtx:
         L.check(L.lib().nts_probe_peaks(self.h, int(copy_bytes), C.byref(h), C.byref(l), C.byref(c)))
         return dict(hbm_copy_gbs=h.value, lds_read_tbs=l.value, clock_ghz=c.value)
 
+    def probe_blend_mix(self, tile_size: int = 32, bwd_batch: int = 128) -> float:
+        """(splat, wave) evaluations per second of the recompute backward's
+        own per-evaluation code at its occupancy, with no memory traffic or
+        barriers: the ceiling of its arithmetic."""
+        r = C.c_double()
+        L.check(L.lib().nts_probe_blend_mix(self.h, int(tile_size), int(bwd_batch), C.byref(r)))
+        return r.value
+
     def prof_report(self) -> dict:
         buf = C.create_string_buffer(1 << 16)
         L.check(L.lib().nts_prof_report(self.h, buf, len(buf)))
@@ -561,6 +569,19 @@ class ViewBatch:
         tile) or SCHED_PERSISTENT (resident workgroups take tiles from a
         device counter)."""
         L.check(L.lib().nts_batch_set_scheduler(self.ctx.h, self.h, int(scheduler)))
+
+    def set_eval_counting(self, on: bool = True) -> None:
+        """Run the blend kernels in their counting form (a measurement aid:
+        slower, same results; grid scheduler only)."""
+        L.check(L.lib().nts_batch_set_eval_counting(self.ctx.h, self.h, int(bool(on))))
+
+    def eval_counts(self) -> dict:
+        """The last step's counts while counting is on: (splat, wave)
+        evaluations and blended / contributing (pixel, splat) pairs of the
+        forward and the backward."""
+        c = (C.c_uint64 * 4)()
+        L.check(L.lib().nts_batch_eval_counts(self.ctx.h, self.h, c))
+        return dict(fwd_wave_evals=c[0], fwd_blended_pairs=c[1], bwd_wave_evals=c[2], bwd_contributing_pairs=c[3])
 
     def set_ssim_lambda(self, lam: float = 0.2) -> None:
         """Blend factor of LOSS_SSIM steps: (1 - lam) MSE + lam (1 - SSIM)

@@ -72,6 +72,8 @@ struct nts_batch {
   uint32_t fwd_batch = NTS_DEFAULT_FWD_BATCH, bwd_batch = NTS_DEFAULT_BWD_BATCH;
   // tile scheduler of the blend kernels (NTS_SCHED_*)
   int scheduler = NTS_SCHED_GRID;
+  // blend kernels in their counting form (evaluation counters in flags[8..15])
+  bool count_evals = false;
   // combined MSE + SSIM loss of NTS_LOSS_SSIM steps (planes allocated by the setter)
   float ssim_lambda = 0.2f;
   void* ssim_base = nullptr;
@@ -521,6 +523,8 @@ static int plan_batch(nts_batch* b, const nts_scene* scene, const nts_view_desc*
   // the persistent scheduler's tile counters: flags[4] (forward) and
   // flags[5] (backward), zeroed with the flags at the start of every launch
   pl.vt.tickets = b->scheduler == NTS_SCHED_PERSISTENT ? b->flags + 4 : nullptr;
+  // the evaluation counters: flags[8..15] as four 64-bit words (grid form only)
+  pl.vt.counts = b->count_evals && !pl.vt.tickets ? reinterpret_cast<unsigned long long*>(b->flags + 8) : nullptr;
   pl.pixels = pix;
   // culled pairs carry the all-ones key: tile ids below 2^tb - 1
   pl.tile_bits = nts_tile_bits_strict(max_tiles_view);
@@ -714,8 +718,8 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
   put(views, sizeof(nts_view_desc) * n_views);
   int lam_bits;
   memcpy(&lam_bits, &b->ssim_lambda, 4);
-  const int meta[8] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch,
-                       loss_kind == NTS_LOSS_SSIM ? lam_bits : 0, b->scheduler};
+  const int meta[9] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch,
+                       loss_kind == NTS_LOSS_SSIM ? lam_bits : 0, b->scheduler, (int)b->count_evals};
   put(meta, sizeof meta);
   put(cfg_pos, sizeof *cfg_pos);
   put(cfg_attr, sizeof *cfg_attr);
@@ -781,6 +785,21 @@ extern "C" int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* b, uint32_t fwd_
 extern "C" int nts_batch_set_scheduler(nts_ctx* ctx, nts_batch* b, int scheduler) {
   NTS_CHECK_ARG(ctx && b && (scheduler == NTS_SCHED_GRID || scheduler == NTS_SCHED_PERSISTENT));
   b->scheduler = scheduler;
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_set_eval_counting(nts_ctx* ctx, nts_batch* b, int on) {
+  NTS_CHECK_ARG(ctx && b);
+  b->count_evals = on != 0;
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_eval_counts(nts_ctx* ctx, const nts_batch* b, uint64_t* counts) {
+  NTS_CHECK_ARG(ctx && b && counts);
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  NTS_HIP_TRY(hipMemcpyAsync(ctx->host_word, b->flags + 8, 32, hipMemcpyDeviceToHost, ctx->stream));
+  NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  memcpy(counts, ctx->host_word, 32);
   return NTS_OK;
 }
 

@@ -30,6 +30,9 @@ struct Blend {
 };
 #define NTS_FWD_THREADS 1024
 #define NTS_BWD_THREADS 256
+#ifndef NTS_BWD_WAVES
+#define NTS_BWD_WAVES 8  // occupancy floor of the backward per SIMD (bounds its VGPRs)
+#endif
 
 // Tile-local pixel j of thread t.
 template <int TS, int MAXT>
@@ -211,10 +214,23 @@ __device__ __forceinline__ float4 pixel_loss_grad(float4 p, float4 t, float inv_
                      d2 > 0.f ? inv_n : (d2 < 0.f ? -inv_n : 0.f), 0.0f);
 }
 
+// Counting form of the blend kernels: adds a wave's evaluation count (the
+// same in every lane) and the sum of its lanes' pixel counts into
+// counts[0] and counts[1].
+__device__ __forceinline__ void flush_counts(unsigned long long* counts, uint32_t wave_count, uint32_t lane_count) {
+  uint32_t sum = lane_count;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if ((threadIdx.x & 63u) == 0) {
+    atomicAdd(counts, (unsigned long long)wave_count);
+    atomicAdd(counts + 1, (unsigned long long)sum);
+  }
+}
+
 // A 1024-thread workgroup is 4 waves per SIMD: two of them share a CU only
 // within 64 VGPRs (8 waves per SIMD), which the attribute holds the compiler
 // to; one per CU leaves half the SIMD slots idle at every barrier.
-template <int TS, int BATCH, bool LOSS, bool PERSIST>
+template <int TS, int BATCH, bool LOSS, int MODE>
 __global__ void __launch_bounds__((Blend<TS, NTS_FWD_THREADS>::NT))
 __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 1))) k_blend_fwd(const Projected* __restrict__ proj,
                                                              const uint32_t* __restrict__ payloads,
@@ -223,6 +239,8 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
                                                              uint32_t* __restrict__ last_idx, LossArgs la) {
   using BL = Blend<TS, NTS_FWD_THREADS>;
   constexpr int NT = BL::NT, PPT = BL::PPT, NM = BATCH / 64;
+  constexpr bool PERSIST = MODE == NTS_BLEND_PERSIST, COUNT = MODE == NTS_BLEND_COUNT;
+  uint32_t n_hits = 0, n_blend = 0;  // counting form: wave evaluations, lane blends
   __shared__ float4 s_a[BATCH];  // mean.x, mean.y, conic.a, conic.b
   __shared__ float4 s_b[BATCH];  // conic.c, opacity, r, g
   __shared__ float4 s_c[BATCH];  // b, power threshold, cull half-extents x / y
@@ -293,6 +311,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
         if (!__ballot(lv)) break;  // every pixel of the wave is finished
         const uint32_t k = (uint32_t)i * 64u + (uint32_t)__builtin_ctzll(m);
         m &= m - 1;
+        if (COUNT) ++n_hits;
         const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
         // All of the record is read here, from one address register: left to
         // itself the compiler sinks each read into the branch that uses it
@@ -308,6 +327,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
           const float ar = B.y * g;
           const float alpha = fminf(ar, NTS_ALPHA_MAX);  // min(alpha_raw, 0.99): a NaN gives 0.99
           if (alpha < NTS_ALPHA_MIN) continue;
+          if (COUNT) ++n_blend;
           const float ta = T[j] * alpha;
           C0[j] = C0[j] + ta * B.z;
           C1[j] = C1[j] + ta * B.w;
@@ -344,6 +364,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
   }
   if (!PERSIST) break;
   }
+  if (COUNT) flush_counts(vt.counts, n_hits, n_blend);
   if (la.clear) {
     const uint64_t per = (la.clear_n4 + gridDim.x - 1) / gridDim.x;
     const uint64_t c0 = (uint64_t)blockIdx.x * per;
@@ -441,19 +462,160 @@ __device__ __forceinline__ uint32_t grad_term_of_quad(uint32_t q) {
   return vi < 9u ? vi : 9u;
 }
 
-// A 256-thread workgroup is one wave per SIMD; 80 VGPRs (6 waves per SIMD)
-// hold without spills, and the LDS of a 128- or 256-splat batch allows that
-// many workgroups per CU.  The persistent form keeps some per-thread setup
-// live across its tile loop and would spill at 80: it is held to 5 waves per
-// SIMD (96 VGPRs).
-template <int TS, int BATCH, bool RECOMP, bool PERSIST>
+// The backward of one staged splat over one lane's PPT pixels: updates the
+// pixels' T and U, leaves the splat's nine per-lane gradient moments in acc
+// and returns whether any of the lane's pixels took part.  relk is the
+// splat's slice-relative index.  With COUNT, n_ok counts the lane's pixels
+// that took part.  (Shared by the backward kernel and its instruction-mix
+// probe, so that both run the same code.)
+template <int PPT, bool RECOMP, bool COUNT>
+__device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const float4 Cc, uint32_t relk,
+                                        const float (&pxf)[PPT], const float (&pyf)[PPT], float (&T)[PPT],
+                                        float (&U)[PPT], const float (&G0)[PPT], const float (&G1)[PPT],
+                                        const float (&G2)[PPT], uint32_t (&lastj)[PPT], float (&acc)[9],
+                                        uint32_t& n_ok) {
+  // Per-pixel moments of the splat's dL/dpower, mapped through the
+  // conic once per splat at the commit:
+  //   0, 1   sum gpow dx, sum gpow dy        -> dL/dmean = Q (X, Y)
+  //   2      sum dL/dalpha G                  -> dL/dopacity
+  //   3-5    sum gpow dx^2, dx dy, dy^2       -> dL/dconic (-1/2, -1, -1/2)
+  //   6-8    sum dL/dC_i T alpha              -> dL/dcolour
+  // (recompute mode: acc is set by the quad's first pixel)
+  if (!RECOMP) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
+  }
+  bool contrib = false;
+  const float sigma = B.y, c0 = B.z, c1 = B.w, c2 = Cc.x;
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
+    // the forward's power and thresholds, bit for bit
+    const float power = A.z * dx * dx + A.w * dx * dy + B.x * dy * dy;  // the staged -q/2
+    // alpha for every lane, then one branch on all the forward's tests
+    // recompute mode: hardware 2^x (error well under 1e-6 relative),
+    // and where alpha lies that close to 1/255 or 0.99 the forward's
+    // exact exponential, so every alpha test decides as in the
+    // forward.  Reference-parity mode keeps the exact exponential: its
+    // 1.0001 bail on the divided-back T is sensitive to the last bit.
+    // alpha = min(alpha_raw, 0.99), a NaN giving 0.99, taken next to each
+    // product (a product is canonical: the min needs no quieting first)
+    // The alpha >= 1/255 decision of the hardware value is taken as
+    // final only outside the band [1 - 4e-6, 1 + 4e-6] 1/255; inside
+    // it, and from 0.99 - 4e-6 up (the clamp), the exact exponential
+    // decides.  Three compares: alpha above the band, alpha in or above
+    // it, ar at or above the clamp's band.  A NaN ar gives alpha 0.99:
+    // kept, no exact pass, as in the forward.
+    float g, ar, alpha;
+    bool aok;
+    if (RECOMP) {
+      g = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
+      ar = sigma * g;
+      alpha = fminf(ar, NTS_ALPHA_MAX);
+      aok = alpha > NTS_ALPHA_MIN * (1.0f + 4e-6f);
+      if ((!aok && alpha >= NTS_ALPHA_MIN * (1.0f - 4e-6f)) || ar >= NTS_ALPHA_MAX - 4e-6f) {
+        g = nts_exp_blend(power);
+        ar = sigma * g;
+        alpha = fminf(ar, NTS_ALPHA_MAX);
+        aok = !(alpha < NTS_ALPHA_MIN);
+      }
+    } else {
+      g = nts_exp_blend(power);
+      ar = sigma * g;
+      alpha = fminf(ar, NTS_ALPHA_MAX);
+      aok = !(alpha < NTS_ALPHA_MIN);
+    }
+    // the exact complement of the forward's skip tests (a NaN threshold
+    // or alpha must be kept here as it is there)
+    const bool ok = relk < lastj[j] && power <= 0.0f && !(power < Cc.y) && aok;
+    if (RECOMP && j == 0) {
+      // The first pixel of the quad without a branch: a lane that fails
+      // the tests sets exact zeros and keeps T (acc needs no zeroing
+      // on a path around it, which costs nine moves per join, and
+      // starts from this pixel's terms rather than from zero).
+      const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
+      const float Tn = T[j] * r, Un = U[j] * r;
+      const float gc = __builtin_fmaf(G2[j], c2, __builtin_fmaf(G1[j], c1, G0[j] * c0));
+      const float gal = __builtin_fmaf(Tn, gc, -Un);
+      const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;
+      const float gpow = ok ? gar * ar : 0.0f;
+      const float og = ok ? gar * g : 0.0f;
+      const float ta = ok ? Tn * alpha : 0.0f;
+      const float gx = gpow * dx, gy = gpow * dy;
+      acc[0] = gx;
+      acc[1] = gy;
+      acc[2] = og;
+      acc[3] = gx * dx;
+      acc[4] = gx * dy;
+      acc[5] = gy * dy;
+      acc[6] = G0[j] * ta;
+      acc[7] = G1[j] * ta;
+      acc[8] = G2[j] * ta;
+      U[j] = __builtin_fmaf(ta, gc, U[j]);
+      T[j] = ok ? Tn : T[j];
+      contrib = contrib || ok;
+      if (COUNT) n_ok += ok ? 1u : 0u;
+      continue;
+    }
+    if (!ok) continue;
+    float Tn, Un;
+    if (RECOMP) {
+      const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
+      Tn = T[j] * r;
+      Un = U[j] * r;
+    } else {
+      const float oma = nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
+      Tn = T[j] / oma;
+      T[j] = Tn;
+      if (Tn > NTS_T_BAIL) {
+        lastj[j] = 0u;
+        continue;
+      }
+      Un = U[j] / oma;
+    }
+    T[j] = Tn;
+    // dL/dalpha = T (dL/dC . c) - dL/dC . S_behind (fused multiply-adds;
+    // no bit parity is needed in the gradient)
+    const float gc = __builtin_fmaf(G2[j], c2, __builtin_fmaf(G1[j], c1, G0[j] * c0));
+    const float gal = __builtin_fmaf(Tn, gc, -Un);
+    const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;  // alpha clamp gate
+    const float gpow = gar * ar;                           // dL/dpower = dL/dalpha sigma G
+    const float ta = Tn * alpha;
+    const float gx = gpow * dx, gy = gpow * dy;
+    acc[0] += gx;
+    acc[1] += gy;
+    acc[2] = __builtin_fmaf(gar, g, acc[2]);
+    acc[3] = __builtin_fmaf(gx, dx, acc[3]);
+    acc[4] = __builtin_fmaf(gx, dy, acc[4]);
+    acc[5] = __builtin_fmaf(gy, dy, acc[5]);
+    acc[6] = __builtin_fmaf(G0[j], ta, acc[6]);
+    acc[7] = __builtin_fmaf(G1[j], ta, acc[7]);
+    acc[8] = __builtin_fmaf(G2[j], ta, acc[8]);
+    U[j] = __builtin_fmaf(ta, gc, U[j]);  // S += T alpha c (S itself is not rescaled)
+    contrib = true;
+    if (COUNT) ++n_ok;
+  }
+  return contrib;
+}
+
+// A 256-thread workgroup is one wave per SIMD.  The kernel issues its own
+// arithmetic about as fast as a replay of it with no memory traffic or
+// barriers does (k_probe_bwd_mix), so more waves to cover that arithmetic's
+// latencies is what shortens it: it is held to 8 waves per SIMD (64 VGPRs,
+// a few spilled; 4.74 ms against 5.02 ms at 6 waves and 80 VGPRs on 1M
+// splats x 16 views), which the LDS of a 128- or 256-splat batch allows.
+// The persistent form keeps some per-thread setup live across its tile loop:
+// it is held to 5 waves per SIMD (96 VGPRs).
+template <int TS, int BATCH, bool RECOMP, int MODE>
 __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT))
-__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (PERSIST ? 5 : 6) : 1))) k_blend_bwd(
+__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE == NTS_BLEND_PERSIST ? 5 : NTS_BWD_WAVES) : 1))) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
     ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
     const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
   using BL = Blend<TS, NTS_BWD_THREADS>;
   constexpr int NT = BL::NT, PPT = BL::PPT, NM = BATCH / 64;
+  constexpr bool PERSIST = MODE == NTS_BLEND_PERSIST, COUNT = MODE == NTS_BLEND_COUNT;
+  uint32_t n_hits = 0, n_ok = 0;  // counting form: wave evaluations, contributing lane pixels
   __shared__ float4 s_a[BATCH];
   __shared__ float4 s_b[BATCH];
   __shared__ float4 s_c[BATCH];
@@ -553,126 +715,11 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (PERS
         const int bt = 63 - __builtin_clzll(m);
         m &= ~(1ull << bt);
         const uint32_t k = (uint32_t)i * 64u + (uint32_t)bt;
+        if (COUNT) ++n_hits;
         const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
-        // Per-pixel moments of the splat's dL/dpower, mapped through the
-        // conic once per splat at the commit:
-        //   0, 1   sum gpow dx, sum gpow dy        -> dL/dmean = Q (X, Y)
-        //   2      sum dL/dalpha G                  -> dL/dopacity
-        //   3-5    sum gpow dx^2, dx dy, dy^2       -> dL/dconic (-1/2, -1, -1/2)
-        //   6-8    sum dL/dC_i T alpha              -> dL/dcolour
-        float acc[9];  // recompute mode: set by the quad's first pixel
-        if (!RECOMP) {
-#pragma unroll
-          for (int q = 0; q < 9; ++q) acc[q] = 0.0f;
-        }
-        bool contrib = false;
-        const float sigma = B.y, c0 = B.z, c1 = B.w, c2 = Cc.x;
-#pragma unroll
-        for (int j = 0; j < PPT; ++j) {
-          const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
-          // the forward's power and thresholds, bit for bit
-          const float power = A.z * dx * dx + A.w * dx * dy + B.x * dy * dy;  // the staged -q/2
-          // alpha for every lane, then one branch on all the forward's tests
-          // recompute mode: hardware 2^x (error well under 1e-6 relative),
-          // and where alpha lies that close to 1/255 or 0.99 the forward's
-          // exact exponential, so every alpha test decides as in the
-          // forward.  Reference-parity mode keeps the exact exponential: its
-          // 1.0001 bail on the divided-back T is sensitive to the last bit.
-          // alpha = min(alpha_raw, 0.99), a NaN giving 0.99, taken next to each
-          // product (a product is canonical: the min needs no quieting first)
-          // The alpha >= 1/255 decision of the hardware value is taken as
-          // final only outside the band [1 - 4e-6, 1 + 4e-6] 1/255; inside
-          // it, and from 0.99 - 4e-6 up (the clamp), the exact exponential
-          // decides.  Three compares: alpha above the band, alpha in or above
-          // it, ar at or above the clamp's band.  A NaN ar gives alpha 0.99:
-          // kept, no exact pass, as in the forward.
-          float g, ar, alpha;
-          bool aok;
-          if (RECOMP) {
-            g = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
-            ar = sigma * g;
-            alpha = fminf(ar, NTS_ALPHA_MAX);
-            aok = alpha > NTS_ALPHA_MIN * (1.0f + 4e-6f);
-            if ((!aok && alpha >= NTS_ALPHA_MIN * (1.0f - 4e-6f)) || ar >= NTS_ALPHA_MAX - 4e-6f) {
-              g = nts_exp_blend(power);
-              ar = sigma * g;
-              alpha = fminf(ar, NTS_ALPHA_MAX);
-              aok = !(alpha < NTS_ALPHA_MIN);
-            }
-          } else {
-            g = nts_exp_blend(power);
-            ar = sigma * g;
-            alpha = fminf(ar, NTS_ALPHA_MAX);
-            aok = !(alpha < NTS_ALPHA_MIN);
-          }
-          // the exact complement of the forward's skip tests (a NaN threshold
-          // or alpha must be kept here as it is there)
-          const bool ok = rel0 + k < lastj[j] && power <= 0.0f && !(power < Cc.y) && aok;
-          if (RECOMP && j == 0) {
-            // The first pixel of the quad without a branch: a lane that fails
-            // the tests sets exact zeros and keeps T (acc needs no zeroing
-            // on a path around it, which costs nine moves per join, and
-            // starts from this pixel's terms rather than from zero).
-            const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
-            const float Tn = T[j] * r, Un = U[j] * r;
-            const float gc = __builtin_fmaf(G2[j], c2, __builtin_fmaf(G1[j], c1, G0[j] * c0));
-            const float gal = __builtin_fmaf(Tn, gc, -Un);
-            const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;
-            const float gpow = ok ? gar * ar : 0.0f;
-            const float og = ok ? gar * g : 0.0f;
-            const float ta = ok ? Tn * alpha : 0.0f;
-            const float gx = gpow * dx, gy = gpow * dy;
-            acc[0] = gx;
-            acc[1] = gy;
-            acc[2] = og;
-            acc[3] = gx * dx;
-            acc[4] = gx * dy;
-            acc[5] = gy * dy;
-            acc[6] = G0[j] * ta;
-            acc[7] = G1[j] * ta;
-            acc[8] = G2[j] * ta;
-            U[j] = __builtin_fmaf(ta, gc, U[j]);
-            T[j] = ok ? Tn : T[j];
-            contrib = contrib || ok;
-            continue;
-          }
-          if (!ok) continue;
-          float Tn, Un;
-          if (RECOMP) {
-            const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
-            Tn = T[j] * r;
-            Un = U[j] * r;
-          } else {
-            const float oma = nts_maxf(1.0f - alpha, NTS_ONE_MINUS_ALPHA_EPS);
-            Tn = T[j] / oma;
-            T[j] = Tn;
-            if (Tn > NTS_T_BAIL) {
-              lastj[j] = 0u;
-              continue;
-            }
-            Un = U[j] / oma;
-          }
-          T[j] = Tn;
-          // dL/dalpha = T (dL/dC . c) - dL/dC . S_behind (fused multiply-adds;
-          // no bit parity is needed in the gradient)
-          const float gc = __builtin_fmaf(G2[j], c2, __builtin_fmaf(G1[j], c1, G0[j] * c0));
-          const float gal = __builtin_fmaf(Tn, gc, -Un);
-          const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;  // alpha clamp gate
-          const float gpow = gar * ar;                           // dL/dpower = dL/dalpha sigma G
-          const float ta = Tn * alpha;
-          const float gx = gpow * dx, gy = gpow * dy;
-          acc[0] += gx;
-          acc[1] += gy;
-          acc[2] = __builtin_fmaf(gar, g, acc[2]);
-          acc[3] = __builtin_fmaf(gx, dx, acc[3]);
-          acc[4] = __builtin_fmaf(gx, dy, acc[4]);
-          acc[5] = __builtin_fmaf(gy, dy, acc[5]);
-          acc[6] = __builtin_fmaf(G0[j], ta, acc[6]);
-          acc[7] = __builtin_fmaf(G1[j], ta, acc[7]);
-          acc[8] = __builtin_fmaf(G2[j], ta, acc[8]);
-          U[j] = __builtin_fmaf(ta, gc, U[j]);  // S += T alpha c (S itself is not rescaled)
-          contrib = true;
-        }
+        float acc[9];
+        const bool contrib =
+            bwd_hit<PPT, RECOMP, COUNT>(A, B, Cc, rel0 + k, pxf, pyf, T, U, G0, G1, G2, lastj, acc, n_ok);
         if (__ballot(contrib)) {
           const float r = wave_reduce9(acc);
           if (term_lane) atomicAdd(&s_acc[k][my_term], r);
@@ -706,6 +753,84 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (PERS
   }
   if (!PERSIST) break;
   }
+  if (COUNT) flush_counts(vt.counts + 2, n_hits, n_ok);
+}
+
+// Instruction-mix probe of the recompute backward: every wave runs `hits`
+// evaluations of staged splats through bwd_hit, wave_reduce9 and the slot
+// table's LDS float atomic, as the kernel does for each (splat, wave) it
+// evaluates, with every pixel of the wave taking part, and with no global
+// memory access and no barrier in the loop.  Workgroup size, LDS and
+// occupancy bound are those of k_blend_bwd<TS, BATCH, true, grid>, so the
+// launch measures how fast the kernel's own instruction mix issues (the
+// host pads the LDS so that no more workgroups share a CU than the kernel's
+// VGPRs allow).  The staged splats are wide (every pixel of the tile well
+// inside their alpha band, alpha near 0.25); the pixel state comes from the
+// runtime value s1 (1.0) so that the compiler cannot fold it; T and U
+// restart every 64 evaluations so that they stay finite.
+template <int TS, int BATCH>
+__global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT))
+__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? NTS_BWD_WAVES : 1))) k_probe_bwd_mix(
+    uint32_t hits, float s1, float* __restrict__ sink) {
+  using BL = Blend<TS, NTS_BWD_THREADS>;
+  constexpr int NT = BL::NT, PPT = BL::PPT;
+  __shared__ float4 s_a[BATCH];
+  __shared__ float4 s_b[BATCH];
+  __shared__ float4 s_c[BATCH];
+  __shared__ float4 s_d[BATCH];
+  __shared__ float s_acc[BATCH][9];
+  for (uint32_t k = threadIdx.x; k < (uint32_t)BATCH; k += NT) {
+    Projected p;
+    const float off = 0.01f * (float)k;
+    p.a = make_float4(0.5f * TS + off, 0.5f * TS - off, 3.0f, 2.0f * TS);
+    p.b = make_float4(0.4f / (TS * TS), 0.0f, 0.4f / (TS * TS), 1.0f);
+    p.c = make_float4(0.3f, 0.5f, 0.7f, 0.3f);
+    stage_splat(p, s_a[k], s_b[k], s_c[k], s_d[k]);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) s_acc[k][q] = 0.0f;
+  }
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t my_term = (lane & 3u) == 0u ? grad_term_of_quad(lane >> 2) : 9u;
+  const bool term_lane = my_term < 9u;
+  float pxf[PPT], pyf[PPT], T[PPT], U[PPT], G0[PPT], G1[PPT], G2[PPT];
+  uint32_t lastj[PPT];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) {
+    uint32_t lx, ly;
+    tile_pixel<TS, NTS_BWD_THREADS>(threadIdx.x, j, lx, ly);
+    pxf[j] = (float)lx + 0.5f;
+    pyf[j] = (float)ly + 0.5f;
+    T[j] = 0.5f * s1;
+    U[j] = 0.0f * s1;
+    G0[j] = 0.01f * s1;
+    G1[j] = 0.02f * (float)(j + 1) * s1;
+    G2[j] = -0.01f * s1;
+    lastj[j] = 0xFFFFFF00u | (threadIdx.x & 0xFFu);
+  }
+  uint32_t n_ok = 0;
+  for (uint32_t it = 0; it < hits; ++it) {
+    const uint32_t k = it & (uint32_t)(BATCH - 1);
+    const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+    float acc[9];
+    const bool contrib = bwd_hit<PPT, true, false>(A, B, Cc, k, pxf, pyf, T, U, G0, G1, G2, lastj, acc, n_ok);
+    if (__ballot(contrib)) {
+      const float r = wave_reduce9(acc);
+      if (term_lane) atomicAdd(&s_acc[k][my_term], r);
+    }
+    if ((it & 63u) == 63u) {
+#pragma unroll
+      for (int j = 0; j < PPT; ++j) {
+        T[j] = 0.5f * s1;
+        U[j] = 0.0f * s1;
+      }
+    }
+  }
+  __syncthreads();
+  float keep = s_acc[threadIdx.x % BATCH][threadIdx.x % 9];
+#pragma unroll
+  for (int j = 0; j < PPT; ++j) keep += T[j] + U[j];
+  if (keep == 1234.5f) sink[blockIdx.x] = keep;  // a result no run produces: keeps the loop live
 }
 
 // ---------------------------------------------------------------------------
@@ -768,13 +893,16 @@ static int launch_fwd_kernel(nts_ctx* ctx, const char* name, const Projected* pr
                              uint32_t* last_idx, const LossArgs& la) {
   constexpr int NT = Blend<TS, NTS_FWD_THREADS>::NT;
   if (vt.tickets) {
-    const uint32_t r = resident_workgroups(k_blend_fwd<TS, B, LOSS, true>, NT);
+    const uint32_t r = resident_workgroups(k_blend_fwd<TS, B, LOSS, NTS_BLEND_PERSIST>, NT);
     const dim3 grid(r < vt.total_tiles ? r : vt.total_tiles);
-    NTS_LAUNCH(ctx, name, (k_blend_fwd<TS, B, LOSS, true>), grid, dim3(NT), 0, proj, payloads, ranges, vt, out,
-               final_t, last_idx, la);
+    NTS_LAUNCH(ctx, name, (k_blend_fwd<TS, B, LOSS, NTS_BLEND_PERSIST>), grid, dim3(NT), 0, proj, payloads, ranges,
+               vt, out, final_t, last_idx, la);
+  } else if (vt.counts) {
+    NTS_LAUNCH(ctx, name, (k_blend_fwd<TS, B, LOSS, NTS_BLEND_COUNT>), dim3(vt.total_tiles), dim3(NT), 0, proj,
+               payloads, ranges, vt, out, final_t, last_idx, la);
   } else {
-    NTS_LAUNCH(ctx, name, (k_blend_fwd<TS, B, LOSS, false>), dim3(vt.total_tiles), dim3(NT), 0, proj, payloads,
-               ranges, vt, out, final_t, last_idx, la);
+    NTS_LAUNCH(ctx, name, (k_blend_fwd<TS, B, LOSS, NTS_BLEND_GRID>), dim3(vt.total_tiles), dim3(NT), 0, proj,
+               payloads, ranges, vt, out, final_t, last_idx, la);
   }
   return NTS_OK;
 }
@@ -823,13 +951,16 @@ static int launch_bwd_kernel(nts_ctx* ctx, const char* name, const Projected* pr
                              const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
   constexpr int NT = Blend<TS, NTS_BWD_THREADS>::NT;
   if (vt.tickets) {
-    const uint32_t r = resident_workgroups(k_blend_bwd<TS, B, RECOMP, true>, NT);
+    const uint32_t r = resident_workgroups(k_blend_bwd<TS, B, RECOMP, NTS_BLEND_PERSIST>, NT);
     const dim3 grid(r < vt.total_tiles ? r : vt.total_tiles);
-    NTS_LAUNCH(ctx, name, (k_blend_bwd<TS, B, RECOMP, true>), grid, dim3(NT), 0, proj, payloads, ranges, vt,
-               fwd_out, final_t, last_idx, dl_dc, pgrad);
-  } else {
-    NTS_LAUNCH(ctx, name, (k_blend_bwd<TS, B, RECOMP, false>), dim3(vt.total_tiles), dim3(NT), 0, proj, payloads,
+    NTS_LAUNCH(ctx, name, (k_blend_bwd<TS, B, RECOMP, NTS_BLEND_PERSIST>), grid, dim3(NT), 0, proj, payloads,
                ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+  } else if (vt.counts) {
+    NTS_LAUNCH(ctx, name, (k_blend_bwd<TS, B, RECOMP, NTS_BLEND_COUNT>), dim3(vt.total_tiles), dim3(NT), 0, proj,
+               payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
+  } else {
+    NTS_LAUNCH(ctx, name, (k_blend_bwd<TS, B, RECOMP, NTS_BLEND_GRID>), dim3(vt.total_tiles), dim3(NT), 0, proj,
+               payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
   }
   return NTS_OK;
 }
@@ -868,6 +999,66 @@ int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, int
     case 16: return bwd_batch_switch<16>(ctx, lds_batch, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
     case 32: return bwd_batch_switch<32>(ctx, lds_batch, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
     default: return nts_fail(NTS_E_INVALID, "tile size %u not in {8, 16, 32}", tile_size);
+  }
+}
+
+template <int TS, int B>
+static int probe_bwd_mix(nts_ctx* ctx, double* hits_per_s) {
+  constexpr int NT = Blend<TS, NTS_BWD_THREADS>::NT;
+  // as many workgroups per CU as the backward kernel holds: LDS padding
+  // caps the probe (whose VGPRs would allow more) at the kernel's count
+  int dev = 0, cus = 0, per_cu = 0, lds_cu = 0;
+  NTS_HIP_TRY(hipGetDevice(&dev));
+  NTS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  NTS_HIP_TRY(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+  NTS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_blend_bwd<TS, B, true, NTS_BLEND_GRID>, NT, 0));
+  if (per_cu < 1 || cus < 1 || lds_cu < 1) return nts_fail(NTS_E_HIP, "occupancy query");
+  hipFuncAttributes fa;
+  NTS_HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_probe_bwd_mix<TS, B>)));
+  const size_t per_wg = (size_t)lds_cu / (size_t)per_cu;
+  const size_t pad = per_wg > fa.sharedSizeBytes + 256 ? per_wg - fa.sharedSizeBytes - 256 : 0;
+  const uint32_t grid = (uint32_t)(per_cu * cus);
+  const uint32_t hits = 4096;
+  void* sink = nullptr;
+  int rc = nts_dev_alloc(ctx, &sink, (uint64_t)grid * 4 + 256);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 1e30;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = nts_fail(NTS_E_HIP, "events");
+  for (int r = 0; r < 4 && rc == NTS_OK; ++r) {
+    (void)hipEventRecord(e0, ctx->stream);
+    hipLaunchKernelGGL((k_probe_bwd_mix<TS, B>), dim3(grid), dim3(NT), pad, ctx->stream, hits, 1.0f,
+                       (float*)sink);
+    (void)hipEventRecord(e1, ctx->stream);
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = nts_fail(NTS_E_HIP, "blend mix probe failed");
+      break;
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (r > 0 && ms < best) best = ms;  // the first launch warms up
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  nts_dev_free(ctx, sink, (uint64_t)grid * 4 + 256);
+  if (rc) return rc;
+  *hits_per_s = (double)grid * (NT / 64) * hits / (best * 1e-3);
+  return NTS_OK;
+}
+
+extern "C" int nts_probe_blend_mix(nts_ctx* ctx, uint32_t tile_size, uint32_t bwd_batch, double* bwd_hits_per_s) {
+  NTS_CHECK_ARG(ctx && bwd_hits_per_s);
+  switch (tile_size * 1000 + bwd_batch) {
+    case 8064: return probe_bwd_mix<8, 64>(ctx, bwd_hits_per_s);
+    case 8128: return probe_bwd_mix<8, 128>(ctx, bwd_hits_per_s);
+    case 8256: return probe_bwd_mix<8, 256>(ctx, bwd_hits_per_s);
+    case 16064: return probe_bwd_mix<16, 64>(ctx, bwd_hits_per_s);
+    case 16128: return probe_bwd_mix<16, 128>(ctx, bwd_hits_per_s);
+    case 16256: return probe_bwd_mix<16, 256>(ctx, bwd_hits_per_s);
+    case 32064: return probe_bwd_mix<32, 64>(ctx, bwd_hits_per_s);
+    case 32128: return probe_bwd_mix<32, 128>(ctx, bwd_hits_per_s);
+    case 32256: return probe_bwd_mix<32, 256>(ctx, bwd_hits_per_s);
+    default: return nts_fail(NTS_E_INVALID, "tile %u / backward batch %u", tile_size, bwd_batch);
   }
 }
 

@@ -40,8 +40,19 @@ struct ViewTable {
   // resident workgroups takes tiles, in `order`, from these two counters
   // (forward, backward), which must be zero when the launch starts.
   uint32_t* tickets;
+  // Evaluation counters (nullptr: none), added into by the counting form of
+  // the blend kernels: [0] (splat, wave) evaluations of the forward, [1] its
+  // blended (pixel, splat) pairs, [2] (splat, wave) evaluations of the
+  // backward, [3] its contributing (pixel, splat) pairs.
+  unsigned long long* counts;
   ViewDesc v[NTS_MAX_VIEWS];
 };
+
+// Forms of the blend kernels: one workgroup per tile, the persistent tile
+// scheduler, or one workgroup per tile with the evaluation counters.
+#define NTS_BLEND_GRID 0
+#define NTS_BLEND_PERSIST 1
+#define NTS_BLEND_COUNT 2
 
 // View of global tile gt (tile_base ascending).
 __device__ __forceinline__ uint32_t view_of_tile(const ViewTable& vt, uint32_t gt) {

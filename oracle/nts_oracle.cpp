@@ -537,6 +537,11 @@ void orc_composite(const float* projected, const uint32_t* payloads, const uint3
   composite_t<float>(projected, payloads, ranges, t, out, final_t, last_idx);
 }
 
+void orc_composite_used(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
+                        const orc_tiling* t, float* out, uint32_t* n_used) {
+  composite_t<float>(projected, payloads, ranges, t, out, nullptr, nullptr, n_used);
+}
+
 void orc_composite_f64(const double* projected, const uint32_t* payloads, const uint32_t* ranges,
                        const orc_tiling* t, double* out, uint32_t* last_idx, uint32_t* n_used) {
   composite_t<double>(projected, payloads, ranges, t, out, nullptr, last_idx, n_used);

@@ -96,6 +96,9 @@ uint64_t orc_bin_ex(const float* projected, uint32_t n, const orc_tiling* t, int
 // contributed to the pixel.
 void orc_composite(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
                    const orc_tiling* t, float* out, float* final_t, uint32_t* last_idx);
+// orc_composite's image and, per pixel, the number of splats blended into it.
+void orc_composite_used(const float* projected, const uint32_t* payloads, const uint32_t* ranges,
+                        const orc_tiling* t, float* out, uint32_t* n_used);
 // float64 blend; last_idx / n_used (contributing splats per pixel) may be NULL.
 void orc_composite_f64(const double* projected, const uint32_t* payloads, const uint32_t* ranges,
                        const orc_tiling* t, double* out, uint32_t* last_idx, uint32_t* n_used);

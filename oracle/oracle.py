@@ -215,6 +215,19 @@ def composite(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t
     return (img, ft, li) if with_state else img
 
 
+def composite_used(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t: Tiling) -> np.ndarray:
+    """Per pixel, the number of splats the forward blend blends into it."""
+    projected = np.ascontiguousarray(projected, np.float32)
+    payloads = np.ascontiguousarray(payloads, np.uint32)
+    if payloads.size == 0:
+        payloads = np.zeros(1, np.uint32)
+    ranges = np.ascontiguousarray(ranges, np.uint32)
+    img = np.zeros((t.height, t.width, 4), np.float32)
+    used = np.zeros((t.height, t.width), np.uint32)
+    lib().orc_composite_used(_p(projected), _p(payloads), _p(ranges), C.byref(t), _p(img), _p(used))
+    return used
+
+
 def composite_f64(projected: np.ndarray, payloads: np.ndarray, ranges: np.ndarray, t: Tiling,
                   with_state: bool = False):
     """float64 blend; with_state also returns (last_idx, n_used) per pixel."""
