@@ -487,6 +487,12 @@ __device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const fl
   }
   bool contrib = false;
   const float sigma = B.y, c0 = B.z, c1 = B.w, c2 = Cc.x;
+  // A lane's 2x2 quad (PPT 4): pixel j lies at x + (j & 1), y + (j >> 1).
+  // Its five positional moments (terms 0, 1, 3-5) follow from the four
+  // pixels' dL/dpower at the end, Sum g (dx0 + ox)^a (dy0 + oy)^b with
+  // ox, oy in {0, 1}, instead of two products and five sums per pixel.
+  constexpr bool QUAD = PPT == 4;
+  float gp[PPT];
 #pragma unroll
   for (int j = 0; j < PPT; ++j) {
     const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
@@ -541,13 +547,17 @@ __device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const fl
       const float gpow = ok ? gar * ar : 0.0f;
       const float og = ok ? gar * g : 0.0f;
       const float ta = ok ? Tn * alpha : 0.0f;
-      const float gx = gpow * dx, gy = gpow * dy;
-      acc[0] = gx;
-      acc[1] = gy;
+      if (QUAD) {
+        gp[0] = gpow;
+      } else {
+        const float gx = gpow * dx, gy = gpow * dy;
+        acc[0] = gx;
+        acc[1] = gy;
+        acc[3] = gx * dx;
+        acc[4] = gx * dy;
+        acc[5] = gy * dy;
+      }
       acc[2] = og;
-      acc[3] = gx * dx;
-      acc[4] = gx * dy;
-      acc[5] = gy * dy;
       acc[6] = G0[j] * ta;
       acc[7] = G1[j] * ta;
       acc[8] = G2[j] * ta;
@@ -557,6 +567,7 @@ __device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const fl
       if (COUNT) n_ok += ok ? 1u : 0u;
       continue;
     }
+    if (QUAD) gp[j] = 0.0f;
     if (!ok) continue;
     float Tn, Un;
     if (RECOMP) {
@@ -581,19 +592,33 @@ __device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const fl
     const float gar = (ar < NTS_ALPHA_MAX) ? gal : 0.0f;  // alpha clamp gate
     const float gpow = gar * ar;                           // dL/dpower = dL/dalpha sigma G
     const float ta = Tn * alpha;
-    const float gx = gpow * dx, gy = gpow * dy;
-    acc[0] += gx;
-    acc[1] += gy;
+    if (QUAD) {
+      gp[j] = gpow;
+    } else {
+      const float gx = gpow * dx, gy = gpow * dy;
+      acc[0] += gx;
+      acc[1] += gy;
+      acc[3] = __builtin_fmaf(gx, dx, acc[3]);
+      acc[4] = __builtin_fmaf(gx, dy, acc[4]);
+      acc[5] = __builtin_fmaf(gy, dy, acc[5]);
+    }
     acc[2] = __builtin_fmaf(gar, g, acc[2]);
-    acc[3] = __builtin_fmaf(gx, dx, acc[3]);
-    acc[4] = __builtin_fmaf(gx, dy, acc[4]);
-    acc[5] = __builtin_fmaf(gy, dy, acc[5]);
     acc[6] = __builtin_fmaf(G0[j], ta, acc[6]);
     acc[7] = __builtin_fmaf(G1[j], ta, acc[7]);
     acc[8] = __builtin_fmaf(G2[j], ta, acc[8]);
     U[j] = __builtin_fmaf(ta, gc, U[j]);  // S += T alpha c (S itself is not rescaled)
     contrib = true;
     if (COUNT) ++n_ok;
+  }
+  if (QUAD) {
+    const float dx0 = pxf[0] - A.x, dy0 = pyf[0] - A.y;
+    const float S = (gp[0] + gp[1]) + (gp[2] + gp[3]);
+    const float Sx = gp[1] + gp[3], Sy = gp[2] + gp[3];
+    acc[0] = __builtin_fmaf(S, dx0, Sx);
+    acc[1] = __builtin_fmaf(S, dy0, Sy);
+    acc[3] = __builtin_fmaf(dx0, __builtin_fmaf(S, dx0, Sx + Sx), Sx);
+    acc[4] = __builtin_fmaf(dx0, __builtin_fmaf(S, dy0, Sy), __builtin_fmaf(dy0, Sx, gp[3]));
+    acc[5] = __builtin_fmaf(dy0, __builtin_fmaf(S, dy0, Sy + Sy), Sy);
   }
   return contrib;
 }
@@ -809,8 +834,12 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? NTS_B
     lastj[j] = 0xFFFFFF00u | (threadIdx.x & 0xFFu);
   }
   uint32_t n_ok = 0;
+  // the waves of a workgroup walk the slots from different places, as their
+  // own hit masks make them do in the kernel (in step, their LDS atomics
+  // would all meet on one slot)
+  const uint32_t k0 = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) * 37u);
   for (uint32_t it = 0; it < hits; ++it) {
-    const uint32_t k = it & (uint32_t)(BATCH - 1);
+    const uint32_t k = (it + k0) & (uint32_t)(BATCH - 1);
     const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
     float acc[9];
     const bool contrib = bwd_hit<PPT, true, false>(A, B, Cc, k, pxf, pyf, T, U, G0, G1, G2, lastj, acc, n_ok);

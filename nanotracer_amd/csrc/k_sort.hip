@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
             atomicOr(err, 1u);
             break;
           }
-          __builtin_amdgcn_s_sleep(1);
+          if (RS_LB_SLEEP) __builtin_amdgcn_s_sleep(RS_LB_SLEEP);
         }
       }
       __hip_atomic_store(&st[(size_t)p * RS_RADIX + d], FLAG_INC | ((prefix + tot) & VALUE_MASK), __ATOMIC_RELAXED,

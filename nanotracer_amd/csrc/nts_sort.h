@@ -17,6 +17,9 @@
 #ifndef RS_WAVES_PER_EU
 #define RS_WAVES_PER_EU 3  // onesweep occupancy floor per SIMD (bounds its VGPRs)
 #endif
+#ifndef RS_LB_SLEEP
+#define RS_LB_SLEEP 1  // s_sleep between look-back polls (0: none)
+#endif
 #ifndef RS_LB_WIN
 #define RS_LB_WIN 4  // predecessors whose look-back words are read per round trip
 #endif
