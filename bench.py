@@ -102,18 +102,18 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 
 
 # HBM bytes per launch measured with rocprofv3 (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors
-# read on a 1 GiB copy kernel) at the (kernel, tile size, that kernel's LDS batch) it was profiled at,
-# with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (2.8567e+09, 45_319_956),
-                    ("blend_fwd_loss", 32, 512): (4.6451e+09, 45_319_956)}
+# read on the 4 GiB copy kernel of the peak probe) at the (kernel, tile size, that kernel's LDS batch)
+# it was profiled at, with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
+PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (4.0085e+09, 45_319_956),
+                    ("blend_fwd_loss", 32, 512): (4.6423e+09, 45_319_956)}
 # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
-PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.7587e9, 45_319_956),
-                 ("blend_fwd_loss", 32, 512): (2.1923e9, 45_319_956)}
+PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.6775e9, 45_319_956),
+                 ("blend_fwd_loss", 32, 512): (2.2026e9, 45_319_956)}
 # LDS bytes per launch moved by the lanes (rocprofv3 SQ_INSTS_LDS_LOAD_BANDWIDTH + _STORE_BANDWIDTH,
 # 64-byte units) and the share of the kernel's cycles the LDS spends on indexed operations
 # (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((792269317.5 + 15983137.5) * 64, 45_319_956, 0.234),
-                ("blend_fwd_loss", 32, 512): ((1957194823.6 + 13700604.4) * 64, 45_319_956, 0.389)}
+PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((792269345.4 + 15983137.4) * 64, 45_319_956, 0.250),
+                ("blend_fwd_loss", 32, 512): ((1957194914.8 + 13700604.4) * 64, 45_319_956, 0.385)}
 N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 # Independent FP32 FMA chains at full occupancy issue 0.244 wave instructions per SIMD per cycle
 # (a wave64 FMA occupies a SIMD for 4 cycles). The blend kernels' mixed code issues above that

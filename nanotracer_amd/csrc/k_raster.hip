@@ -30,6 +30,9 @@ struct Blend {
 };
 #define NTS_FWD_THREADS 1024
 #define NTS_BWD_THREADS 256
+#ifndef NTS_FWD_LIVE_CHECK
+#define NTS_FWD_LIVE_CHECK 1  // the forward leaves a batch once every pixel of the wave is finished
+#endif
 #ifndef NTS_BWD_WAVES
 #define NTS_BWD_WAVES 8  // occupancy floor of the backward per SIMD (bounds its VGPRs)
 #endif
@@ -305,10 +308,12 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
     for (int i = 0; i < NM; ++i) {
       uint64_t m = mk[i];
       while (m) {
-        int lv = 0;
+        if (NTS_FWD_LIVE_CHECK) {
+          int lv = 0;
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) lv |= pxf[j] == pxf[j];
-        if (!__ballot(lv)) break;  // every pixel of the wave is finished
+          for (int j = 0; j < PPT; ++j) lv |= pxf[j] == pxf[j];
+          if (!__ballot(lv)) break;  // every pixel of the wave is finished
+        }
         const uint32_t k = (uint32_t)i * 64u + (uint32_t)__builtin_ctzll(m);
         m &= m - 1;
         if (COUNT) ++n_hits;
