@@ -477,6 +477,8 @@ def run_batched(args, ctx):
         b.set_scheduler(L.SCHED_PERSISTENT)
     ssim = ssim_step(ctx, b, g, specs, tbuf, t, tile, mode, min(args.steps, 10), s.n, visible, pairs, pixels,
                      n_tiles, args.ssim_lambda) if args.ssim_lambda > 0 else None
+    # (the SSIM leg advanced the Adam step by fewer than 100)
+    refbin = reference_binning_step(ctx, b, g, specs, tbuf, t + 100, tile, mode, min(args.steps, 10))
     b.free()
     out = dict(
         metric="train iters/s + render Mpx/s at 1M splats 1080p; p50/p99 render latency",
@@ -512,6 +514,7 @@ def run_batched(args, ctx):
     out["blend_evals"] = blend_evals_of(ctx, evals, kernels, (tile, lds[0], lds[1]), pairs)
     if ssim:
         out["ssim_step"] = ssim
+    out["reference_binning_step"] = refbin
     if tune:
         out["autotune"] = dict(predicted_step_ms=round(tune.step_ms, 3), blend_ms_by_scheduler=tune.blend_ms,
                                table=tune.table)
@@ -533,6 +536,34 @@ def run_batched(args, ctx):
             out["vs_cpu"]["plain_over_cpu_1080p"] = round(out["vs_cpu"]["plain_1080p_views_per_s"]
                                                           / out["vs_cpu"]["cpu_1080p_views_per_s"], 1)
     return out
+
+
+def reference_binning_step(ctx, b, g, specs, tbuf, t, tile, mode, steps):
+    """The same batched iteration with the reference's binning (BIN_REFERENCE:
+    TileBinner::bin's pairs, so every view's keys are the reference's), graph
+    replays timed as in the headline run; the batch goes back to the default
+    culled rule afterwards."""
+    from nanotracer_amd import _lib as L
+
+    b.set_binning(L.BIN_REFERENCE)
+    try:
+        for _ in range(2):
+            b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, mode, graph=True)
+            t += 1
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.train_step(g, specs, tbuf, t, tile, L.LOSS_L1, mode, graph=True)
+            t += 1
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        pairs, _, overflow = b.stats()
+        if overflow:
+            raise RuntimeError("pair capacity overflowed in the reference-binning steps")
+    finally:
+        b.set_binning(L.BIN_CULLED)
+    return dict(binning="reference (TileBinner::bin pairs; keys bit-exact)", steps=steps,
+                ms_per_step=round(dt * 1e3 / steps, 3), iters_s=round(steps / dt, 3), pairs=pairs)
 
 
 def ssim_step(ctx, b, g, specs, tbuf, t, tile, mode, steps, N, Nv, K, P, T, lam=0.2):

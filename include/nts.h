@@ -351,6 +351,17 @@ int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* batch, uint32_t fwd_batch, 
 #define NTS_SCHED_GRID 0
 #define NTS_SCHED_PERSISTENT 1
 int nts_batch_set_scheduler(nts_ctx* ctx, nts_batch* batch, int scheduler);
+/* Binning rule of the batch's steps and renders.  NTS_BIN_CULLED (the
+ * default): a splat gets pairs only for the tiles of its alpha >= 1/255
+ * box, and a pair of such a tile that the alpha >= 1/255 ellipse cannot
+ * reach is keyed all ones (sorted past every tile, in no range); no pair
+ * that can change a pixel is dropped.  NTS_BIN_REFERENCE: the pairs of
+ * TileBinner::bin (tile_binner.rs:181-382, tile_count.wgsl / tile_emit.wgsl),
+ * so each view's keys, payloads and ranges (nts_batch_download_bins) are
+ * the reference's.  Images are identical under both rules. */
+#define NTS_BIN_CULLED 0
+#define NTS_BIN_REFERENCE 1
+int nts_batch_set_binning(nts_ctx* ctx, nts_batch* batch, int binning);
 /* Evaluation counting (no reference counterpart; a measurement aid, off by
  * default).  While on, the batch's steps run the grid-scheduler blend
  * kernels in a form that also counts their work, and nts_batch_eval_counts

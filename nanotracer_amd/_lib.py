@@ -19,6 +19,7 @@ ERRORS = {-1: "NTS_E_INVALID", -2: "NTS_E_HIP", -3: "NTS_E_NOMEM", -4: "NTS_E_CA
 LOSS_L1, LOSS_MSE, LOSS_SSIM = 0, 1, 2
 SCENE_TRAINABLE = 1
 SCHED_GRID, SCHED_PERSISTENT = 0, 1
+BIN_CULLED, BIN_REFERENCE = 0, 1
 SORT_NOALLOC_MAX_PAIRS = 1 << 21
 BWD_REF_PARITY, BWD_RECOMPUTE = 0, 1
 
@@ -210,6 +211,7 @@ _SIGS = {
     "nts_batch_set_ssim_lambda": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
     "nts_batch_set_scheduler": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "nts_batch_set_eval_counting": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "nts_batch_set_binning": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "nts_batch_eval_counts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nts_batch_losses": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nts_batch_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),

@@ -570,6 +570,14 @@ class ViewBatch:
         device counter)."""
         L.check(L.lib().nts_batch_set_scheduler(self.ctx.h, self.h, int(scheduler)))
 
+    def set_binning(self, binning: int = L.BIN_CULLED) -> None:
+        """Binning rule of the batch's steps: BIN_CULLED (pairs only for the
+        tiles a splat's alpha >= 1/255 region reaches; the default) or
+        BIN_REFERENCE (TileBinner::bin's tile rectangle, tile_binner.rs:181-382:
+        every view's keys, payloads and ranges are the reference's).  Images
+        are the same under both."""
+        L.check(L.lib().nts_batch_set_binning(self.ctx.h, self.h, int(binning)))
+
     def set_eval_counting(self, on: bool = True) -> None:
         """Run the blend kernels in their counting form (a measurement aid:
         slower, same results; grid scheduler only)."""

@@ -74,6 +74,8 @@ struct nts_batch {
   int scheduler = NTS_SCHED_GRID;
   // blend kernels in their counting form (evaluation counters in flags[8..15])
   bool count_evals = false;
+  // binning rule (NTS_BIN_*): the alpha >= 1/255 region, or the reference's box
+  int binning = NTS_BIN_CULLED;
   // combined MSE + SSIM loss of NTS_LOSS_SSIM steps (planes allocated by the setter)
   float ssim_lambda = 0.2f;
   void* ssim_base = nullptr;
@@ -109,6 +111,7 @@ __global__ void k_latch_errors(const uint32_t* __restrict__ flags, const uint32_
 // ---------------------------------------------------------------------------
 // Held to 4 waves per SIMD (125 VGPRs, no spills; the compiler's own choice
 // is 139 VGPRs and 3 waves): 6% faster over 1M splats x 16 views.
+template <bool CULL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_project_views(const float* __restrict__ P, uint32_t cap, uint32_t n,
                                                        const DevCamera* __restrict__ cams, TileTable tt,
                                                        Projected* __restrict__ proj, uint32_t* __restrict__ counts,
@@ -125,9 +128,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       uint32_t c = 0;
       int32_t x0, y0, x1, y1;
       const ViewTiles vt = tt.t[v];
-      // only the tiles the splat's alpha >= 1/255 ellipse can reach (the
-      // emit uses the same rule)
-      if (record_tiles<true>(p, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1, y1)) {
+      // CULL: only the tiles the splat's alpha >= 1/255 ellipse can reach;
+      // otherwise the reference's tile rectangle (the emit uses the same rule)
+      if (record_tiles<CULL>(p, (float)vt.tile_size, (int32_t)vt.tiles_x, (int32_t)vt.tiles_y, x0, y0, x1, y1)) {
         c = (uint32_t)((x1 - x0 + 1) * (y1 - y0 + 1));
         ++vis;
       }
@@ -540,9 +543,15 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
   const uint64_t vn = (uint64_t)V * n;
   int rc = nts_fill_u32(ctx, b->flags, 0, 16);
   if (rc) return rc;
-  if (n > 0)
-    NTS_LAUNCH(ctx, "project_views", k_project_views, dim3(nts_div_up(n, 256)), dim3(256), 0, scene->params,
-               scene->cap, n, (const DevCamera*)b->d_cams, pl.tt, b->proj, b->counts, b->flags + 1);
+  {
+    const bool cull = b->binning == NTS_BIN_CULLED;
+    if (n > 0 && cull)
+      NTS_LAUNCH(ctx, "project_views", k_project_views<true>, dim3(nts_div_up(n, 256)), dim3(256), 0, scene->params,
+                 scene->cap, n, (const DevCamera*)b->d_cams, pl.tt, b->proj, b->counts, b->flags + 1);
+    else if (n > 0)
+      NTS_LAUNCH(ctx, "project_views", k_project_views<false>, dim3(nts_div_up(n, 256)), dim3(256), 0, scene->params,
+                 scene->cap, n, (const DevCamera*)b->d_cams, pl.tt, b->proj, b->counts, b->flags + 1);
+  }
   rc = nts_scan_u32_ws(ctx, b->counts, b->counts, vn, b->d_total, b->scan_ws);
   if (rc) return rc;
   RecTable rt;
@@ -554,12 +563,21 @@ static int launch_forward(nts_ctx* ctx, const nts_scene* scene, nts_batch* b, co
   bool alt = false;
   if (n > 0) {
     const uint32_t nrec = (uint32_t)vn;
-    if (pl.key_bytes == 4)
+    const bool cull = b->binning == NTS_BIN_CULLED;
+    if (pl.key_bytes == 4 && cull)
       NTS_LAUNCH(ctx, "emit_views", (k_emit_balanced<uint32_t, true>), dim3(nts_div_up(nrec, 256)), dim3(256), 0,
                  (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
                  b->pair_cap, (uint32_t*)b->keys_a, b->vals_a);
-    else
+    else if (pl.key_bytes == 4)
+      NTS_LAUNCH(ctx, "emit_views", (k_emit_balanced<uint32_t, false>), dim3(nts_div_up(nrec, 256)), dim3(256), 0,
+                 (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
+                 b->pair_cap, (uint32_t*)b->keys_a, b->vals_a);
+    else if (cull)
       NTS_LAUNCH(ctx, "emit_views", (k_emit_balanced<uint64_t, true>), dim3(nts_div_up(nrec, 256)), dim3(256), 0,
+                 (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
+                 b->pair_cap, (uint64_t*)b->keys_a, b->vals_a);
+    else
+      NTS_LAUNCH(ctx, "emit_views", (k_emit_balanced<uint64_t, false>), dim3(nts_div_up(nrec, 256)), dim3(256), 0,
                  (const Projected*)b->proj, rt, pl.tt, (const uint32_t*)b->counts, (const uint64_t*)b->d_total,
                  b->pair_cap, (uint64_t*)b->keys_a, b->vals_a);
   }
@@ -718,8 +736,8 @@ extern "C" int nts_train_step_views(nts_ctx* ctx, nts_scene* scene, nts_batch* b
   put(views, sizeof(nts_view_desc) * n_views);
   int lam_bits;
   memcpy(&lam_bits, &b->ssim_lambda, 4);
-  const int meta[9] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch,
-                       loss_kind == NTS_LOSS_SSIM ? lam_bits : 0, b->scheduler, (int)b->count_evals};
+  const int meta[10] = {(int)tile_size, loss_kind, bwd_mode, write_grads, (int)b->fwd_batch, (int)b->bwd_batch,
+                        loss_kind == NTS_LOSS_SSIM ? lam_bits : 0, b->scheduler, (int)b->count_evals, b->binning};
   put(meta, sizeof meta);
   put(cfg_pos, sizeof *cfg_pos);
   put(cfg_attr, sizeof *cfg_attr);
@@ -785,6 +803,12 @@ extern "C" int nts_batch_set_lds_batch(nts_ctx* ctx, nts_batch* b, uint32_t fwd_
 extern "C" int nts_batch_set_scheduler(nts_ctx* ctx, nts_batch* b, int scheduler) {
   NTS_CHECK_ARG(ctx && b && (scheduler == NTS_SCHED_GRID || scheduler == NTS_SCHED_PERSISTENT));
   b->scheduler = scheduler;
+  return NTS_OK;
+}
+
+extern "C" int nts_batch_set_binning(nts_ctx* ctx, nts_batch* b, int binning) {
+  NTS_CHECK_ARG(ctx && b && (binning == NTS_BIN_CULLED || binning == NTS_BIN_REFERENCE));
+  b->binning = binning;
   return NTS_OK;
 }
 

@@ -1,8 +1,9 @@
 """The batched multi-view step (one launch per stage across all views)
 against the oracle run view by view: keys, ranges and images bit-exact
-(the batched path bins only the tiles a splat's alpha >= 1/255 ellipse can
-reach: its keys are the oracle's with that cull, its images those of the
-reference binning),
+(by default the batched path bins only the tiles a splat's alpha >= 1/255
+ellipse can reach: its keys are the oracle's with that cull, its images
+those of the reference binning; with BIN_REFERENCE its keys are the
+reference's),
 per-view losses, summed gradients within 1e-4 norm-wise, parameters after
 Adam; and hipGraph replays against eager steps."""
 import math
@@ -76,6 +77,78 @@ def test_render_views_bit_exact_per_view(ctx, tile):
         assert np.array_equal(gimg.view(np.uint32), img.view(np.uint32))
         gft, gli = b.state(k)
         assert np.array_equal(gft.view(np.uint32), ft.view(np.uint32)) and np.array_equal(gli, li)
+    b.free()
+
+
+@pytest.mark.parametrize("tile", [16, 8, 32])
+def test_reference_binning_keys_are_the_references(ctx, tile):
+    """With BIN_REFERENCE the batched path emits TileBinner::bin's pairs: every
+    view's keys, payloads and ranges bit-exact against the oracle's reference
+    binning (tile_binner.rs:181-382), the images those of the culled rule."""
+    s = synth.make_scene(20_000, 0x5EED0002, 3.0)
+    views = _views(s.n)
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 4_000_000)
+    b.render(g, [v[0] for v in views], tile)
+    culled = [b.image(k) for k in range(len(views))]
+    culled_pairs = b.stats()[0]
+    b.set_binning(L.BIN_REFERENCE)
+    b.render(g, [v[0] for v in views], tile)
+    tp, vis, ov = b.stats()
+    assert ov == 0 and tp > culled_pairs
+    total = 0
+    for k, (spec, ocam, _) in enumerate(views):
+        t = O.tiling(spec.width, spec.height, tile, spec.depth_max)
+        op = O.project(s, ocam)
+        rk, rp, rr = O.bin(op, t)
+        keys, pay, rng = b.bins(k, tile)
+        assert np.array_equal(keys, rk) and np.array_equal(pay, rp) and np.array_equal(rng, rr), (tile, k)
+        total += len(rk)
+        img = O.composite(op, rp, rr, t)
+        assert np.array_equal(b.image(k).view(np.uint32), img.view(np.uint32)), (tile, k)
+        assert np.array_equal(culled[k].view(np.uint32), img.view(np.uint32)), (tile, k)
+    assert total == tp
+    # back to the default rule
+    b.set_binning(L.BIN_CULLED)
+    b.render(g, [v[0] for v in views], tile)
+    assert b.stats()[0] == culled_pairs
+    b.free()
+
+
+def test_reference_binning_step_matches_oracle(ctx):
+    """A training step with the reference's binning: the oracle step view by
+    view (train.rs:155-250), as for the culled rule, eager and graph-replayed."""
+    s = synth.make_scene(20_000, 0x5EED0002, 3.0)
+    views = _views(s.n)
+    tile = 16
+    targets = _targets(synth.perturb_means(s, 77), views, tile)
+    tbuf = ctx.storage_buffer(np.concatenate([x.reshape(-1) for x in targets]))
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 4_000_000)
+    b.set_binning(L.BIN_REFERENCE)
+    cfg = AdamConfig()
+    for step in (1, 2, 3):
+        before = g.readback()
+        mg, vg = g.moments()
+        m = [a.reshape(-1).copy() for a in mg.arrays()]
+        v = [a.reshape(-1).copy() for a in vg.arrays()]
+        acc = g.grad_acc()
+        b.train_step(g, [x[0] for x in views], tbuf, step, tile, L.LOSS_L1, L.BWD_RECOMPUTE, graph=step > 1,
+                     write_grads=True)
+        ref_s, ref_g, ref_losses = _oracle_batch_step(before, views, targets, tile, L.BWD_RECOMPUTE, m, v, acc, step,
+                                                      cfg)
+        assert np.allclose(b.losses(), ref_losses, rtol=1e-9, atol=0)
+        for name, x, y in zip(("pos", "rot", "scale", "logit", "dc", "rest"), g.grads().arrays(), ref_g.arrays()):
+            assert rel_norm_err(x, y) < 1e-4, (step, name, rel_norm_err(x, y))
+        for x, y in zip(g.readback().arrays(), ref_s.arrays()):
+            d = np.abs(x.astype(np.float64) - y)
+            assert d.max() <= 2.0 * cfg.lr + 1e-6
+            assert (d > 1e-6).mean() < 2e-3
+        spec, ocam, _ = views[0]
+        tt = O.tiling(spec.width, spec.height, tile, spec.depth_max)
+        rk, rp, rr = O.bin(O.project(before, ocam), tt)
+        keys, pay, rng = b.bins(0, tile)
+        assert np.array_equal(keys, rk) and np.array_equal(pay, rp) and np.array_equal(rng, rr), step
     b.free()
 
 

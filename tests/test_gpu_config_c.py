@@ -9,7 +9,8 @@ then at tile 16.  Against the oracle run view by view
 (train.rs:155-250 once per view, gradients summed over views):
 
 - keys, payloads and ranges of three views, one of them 2560x1440, bit-exact
-  against the oracle's culled binning;
+  against the oracle's culled binning, and with the reference's binning
+  (BIN_REFERENCE) against TileBinner::bin's;
 - those views' image, final T and last contributor bit-exact;
 - all 16 per-view losses to 1e-9 relative;
 - the summed parameter gradient within 1e-4 norm-wise per attribute;
@@ -112,6 +113,25 @@ def test_config_c_full_size_graph_replay(ctx):
     assert ov == 0
     print(f"tile 32: {pairs} pairs, {visible} visible records", flush=True)
     _check_step(b, g, views, targets, 32, before, 2, picked)
+    # the reference's binning at tile 32: the picked views' keys, payloads and
+    # ranges bit-exact against TileBinner::bin, images unchanged
+    s_now = g.readback()
+    b.render(g, specs, 32)
+    culled = {k: b.image(k) for k in picked}
+    b.set_binning(L.BIN_REFERENCE)
+    b.render(g, specs, 32)
+    rpairs, _, ov = b.stats()
+    assert ov == 0 and rpairs > pairs
+    for k in picked:
+        spec, ocam, _ = views[k]
+        tt = O.tiling(spec.width, spec.height, 32, spec.depth_max)
+        op = O.project(s_now, ocam)
+        rk, rp, rr = O.bin(op, tt)
+        keys, pay, rng = b.bins(k, 32)
+        assert np.array_equal(keys, rk) and np.array_equal(pay, rp) and np.array_equal(rng, rr), k
+        assert np.array_equal(b.image(k).view(np.uint32), culled[k].view(np.uint32)), k
+        print(f"reference binning: view {k} ({spec.width}x{spec.height}, {len(rk)} pairs) keys bit-exact", flush=True)
+    b.set_binning(L.BIN_CULLED)
     # tile 16: a new capture, then a replay
     b.set_lds_batch(256, 256)
     b.train_step(g, specs, tbuf, 3, 16, L.LOSS_L1, L.BWD_RECOMPUTE, graph=True, write_grads=True)
