@@ -14,9 +14,9 @@
 
 // Thread layout of a (view, tile) workgroup with at most MAXT threads.
 // Every wave covers a square sub-tile: 8x8 pixels one per lane, or 16x16
-// as a 2x2 quad per lane when the tile has more pixels than threads; the
-// waves tile the tile row-major.  Square wave rectangles are what the
-// per-wave culling rejects best.  The forward runs one pixel per thread
+// as a 2x2 quad per lane when the tile has more pixels than threads (each
+// row of 16 lanes then an 8x8 square of it); the waves tile the tile
+// row-major.  Square rectangles are what the per-wave culling rejects best.  The forward runs one pixel per thread
 // (up to 1024 threads); the backward caps the workgroup at 256 threads,
 // which keeps its per-batch barriers and the LDS partials of its waves
 // small.
@@ -33,9 +33,21 @@ struct Blend {
 #ifndef NTS_FWD_LIVE_CHECK
 #define NTS_FWD_LIVE_CHECK 1  // the forward leaves a batch once every pixel of the wave is finished
 #endif
-#ifndef NTS_BWD_WAVES
-#define NTS_BWD_WAVES 8  // occupancy floor of the backward per SIMD (bounds its VGPRs)
+#ifndef NTS_BWD_QUARTERS
+// The quad backward's rows of 16 lanes walk their own splat streams: each
+// row takes the splats that reach its 8x8-pixel square, independently of the
+// other rows of the wave.
+#define NTS_BWD_QUARTERS 1
 #endif
+#define NTS_NO_SLOT 0xFFFFFFFFu
+#ifndef NTS_BWD_WAVES
+#define NTS_BWD_WAVES 8  // occupancy floor of the one-pixel backward per SIMD (bounds its VGPRs)
+#endif
+#ifndef NTS_BWD_QUAD_WAVES
+#define NTS_BWD_QUAD_WAVES 7  // the same for the quad backward (72 VGPRs: at 64 it spills in its loop)
+#endif
+// Occupancy floor of a 256-thread backward workgroup with PPT pixels per lane.
+#define NTS_BWD_OCC(PPT) ((PPT) == 4 && NTS_BWD_QUARTERS ? NTS_BWD_QUAD_WAVES : NTS_BWD_WAVES)
 
 // Tile-local pixel j of thread t.
 template <int TS, int MAXT>
@@ -47,6 +59,13 @@ __device__ __forceinline__ void tile_pixel(uint32_t t, int j, uint32_t& lx, uint
   if (B::PPT == 1) {
     lx = bx + (l & 7u);
     ly = by + (l >> 3);
+  } else if (NTS_BWD_QUARTERS) {
+    // each row of 16 lanes covers an 8x8-pixel square of the wave (4x4
+    // quads): row r = l >> 4 at (8 (r & 1), 8 (r >> 1)), its lane q = l & 15
+    // at quad (q & 3, q >> 2)
+    const uint32_t r = l >> 4, q = l & 15u;
+    lx = bx + 8u * (r & 1u) + 2u * (q & 3u) + ((uint32_t)j & 1u);
+    ly = by + 8u * (r >> 1) + 2u * (q >> 2) + ((uint32_t)j >> 1);
   } else {
     lx = bx + 2u * (l & 7u) + ((uint32_t)j & 1u);
     ly = by + 2u * (l >> 3) + ((uint32_t)j >> 1);
@@ -189,6 +208,42 @@ __device__ __forceinline__ void wave_masks(const WaveRect& wr, const float4* s_a
     bool h = false;
     if (k < klim) h = wave_hits(wr, s_a[k], s_c[k], s_d[k]);
     mk[i] = __ballot(h);
+  }
+}
+
+// The 8x8-pixel square of row q (16 lanes) inside the wave's 16x16
+// rectangle (the quad layout with NTS_BWD_QUARTERS).  Pixel centres are
+// integers + 0.5, so the offsets are exact.
+__device__ __forceinline__ WaveRect quarter_rect(const WaveRect& w, int q) {
+  const float x0 = w.x0 + ((q & 1) ? 8.0f : 0.0f), y0 = w.y0 + ((q & 2) ? 8.0f : 0.0f);
+  const float x1 = x0 + 7.0f, y1 = y0 + 7.0f;
+  return WaveRect{x0, x1, y0, y1, 0.5f * (x0 + x1), 0.5f * (y0 + y1), 3.5f, 3.5f};
+}
+
+// Hit masks of the staged batch per row of 16 lanes: bit b of mq[r][i] is
+// set when slot 64 i + 63 - b (< klim) can reach a pixel of row r's square
+// (reversed, so that the lowest bit is the last slot).  Each lane tests its
+// slots against the four squares directly: a test of the wave's rectangle
+// first measured slower (nearly every batch has a slot that passes it).
+template <int NM>
+__device__ __forceinline__ void quarter_masks(const WaveRect& wr, const float4* s_a, const float4* s_c,
+                                              const float4* s_d, uint32_t klim, uint64_t (&mq)[4][NM]) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    const uint32_t k = (uint32_t)i * 64u + 63u - lane;
+    bool h0 = false, h1 = false, h2 = false, h3 = false;
+    if (k < klim) {
+      const float4 A = s_a[k], Cc = s_c[k], D = s_d[k];
+      h0 = wave_hits(quarter_rect(wr, 0), A, Cc, D);
+      h1 = wave_hits(quarter_rect(wr, 1), A, Cc, D);
+      h2 = wave_hits(quarter_rect(wr, 2), A, Cc, D);
+      h3 = wave_hits(quarter_rect(wr, 3), A, Cc, D);
+    }
+    mq[0][i] = __ballot(h0);
+    mq[1][i] = __ballot(h1);
+    mq[2][i] = __ballot(h2);
+    mq[3][i] = __ballot(h3);
   }
 }
 
@@ -455,6 +510,20 @@ __device__ __forceinline__ float wave_reduce9(const float (&v)[9]) {
   return r;
 }
 
+// Reduce-scatter of the 9 per-lane terms within each row of 16 lanes (DPP
+// halvings across bits 3, 2, 1 and 0: 9 -> 5 -> 3 -> 2 -> 1).  Afterwards
+// lane l holds its row's sum of term grad_term_of_quad(l & 15) (none when
+// that is >= 9).
+__device__ __forceinline__ float row_reduce9(const float (&v)[9]) {
+  const uint32_t lane = threadIdx.x & 63u;
+  float w[5], x[3], y[2], z[1];
+  rs_stage<9, 5, NTS_DPP_ROW_MIRROR>(v, w, lane & 8u);
+  rs_stage<5, 3, NTS_DPP_ROW_HALF_MIRROR>(w, x, lane & 4u);
+  rs_stage<3, 2, NTS_DPP_QUAD_XOR2>(x, y, lane & 2u);
+  rs_stage<2, 1, NTS_DPP_QUAD_XOR1>(y, z, lane & 1u);
+  return z[0];
+}
+
 // Term index held by quad q after wave_reduce9 (9 = none): bit 5 of the
 // lane picks terms 5-8, bit 4 the upper of 5, bit 3 the upper of 3, bit 2
 // the upper of 2.
@@ -631,14 +700,15 @@ __device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const fl
 // A 256-thread workgroup is one wave per SIMD.  The kernel issues its own
 // arithmetic about as fast as a replay of it with no memory traffic or
 // barriers does (k_probe_bwd_mix), so more waves to cover that arithmetic's
-// latencies is what shortens it: it is held to 8 waves per SIMD (64 VGPRs,
-// a few spilled; 4.74 ms against 5.02 ms at 6 waves and 80 VGPRs on 1M
-// splats x 16 views), which the LDS of a 128- or 256-splat batch allows.
-// The persistent form keeps some per-thread setup live across its tile loop:
-// it is held to 5 waves per SIMD (96 VGPRs).
+// latencies is what shortens it, up to the point where the loop spills: the
+// one-pixel form is held to 8 waves per SIMD (64 VGPRs), the quad form, whose
+// rows walk their own splat streams, to 7 (72 VGPRs; 4.31 ms on 1M splats x
+// 16 views against 5.0 ms at 8 waves, where it reloads spilled registers in
+// its loop, and 4.45 ms at 6).  The persistent form keeps some per-thread
+// setup live across its tile loop: it is held to 5 waves per SIMD (96 VGPRs).
 template <int TS, int BATCH, bool RECOMP, int MODE>
 __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT))
-__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE == NTS_BLEND_PERSIST ? 5 : NTS_BWD_WAVES) : 1))) k_blend_bwd(
+__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE == NTS_BLEND_PERSIST ? 5 : NTS_BWD_OCC((Blend<TS, NTS_BWD_THREADS>::PPT))) : 1))) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
     ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
     const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
@@ -674,8 +744,11 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE
   const uint2 rg = ranges[gt];
   const uint32_t begin = rg.x, end = rg.y;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t my_term = (lane & 3u) == 0u ? grad_term_of_quad(lane >> 2) : 9u;
+  constexpr bool QUART = PPT == 4 && NTS_BWD_QUARTERS;
+  const uint32_t my_term = QUART ? grad_term_of_quad(lane & 15u)
+                                 : ((lane & 3u) == 0u ? grad_term_of_quad(lane >> 2) : 9u);
   const bool term_lane = my_term < 9u;
+  const uint32_t row = lane >> 4;
   const WaveRect wr = wave_rect<TS, NTS_BWD_THREADS>(tx, ty);
 
   // Per pixel: T walked back, dL/dC, U = dL/dC . S with S the sum of
@@ -736,23 +809,52 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE
     __syncthreads();
     uint32_t klim = 0;
     if (__ballot(live) && wave_last > rel0) klim = wave_last - rel0 < take ? wave_last - rel0 : take;
-    uint64_t mk[NM];
-    wave_masks<NM>(wr, s_a, s_c, s_d, klim, mk);
+    if constexpr (QUART) {
+      // Every row walks the slots that reach its own square, last first; a
+      // row whose slots are used up evaluates slot 0 with an index past
+      // every pixel's last contributor, so none of its pixels takes part.
+      uint64_t mq[4][NM];
+      quarter_masks<NM>(wr, s_a, s_c, s_d, klim, mq);
 #pragma unroll
-    for (int i = NM - 1; i >= 0; --i) {
-      uint64_t m = mk[i];
-      while (m) {
-        const int bt = 63 - __builtin_clzll(m);
-        m &= ~(1ull << bt);
-        const uint32_t k = (uint32_t)i * 64u + (uint32_t)bt;
-        if (COUNT) ++n_hits;
-        const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
-        float acc[9];
-        const bool contrib =
-            bwd_hit<PPT, RECOMP, COUNT>(A, B, Cc, rel0 + k, pxf, pyf, T, U, G0, G1, G2, lastj, acc, n_ok);
-        if (__ballot(contrib)) {
-          const float r = wave_reduce9(acc);
-          if (term_lane) atomicAdd(&s_acc[k][my_term], r);
+      for (int i = NM - 1; i >= 0; --i) {
+        // this lane's row's mask of word i, walked lowest bit (last slot)
+        // first in vector registers: no scalar work per evaluation
+        uint64_t m = (row & 2u) ? ((row & 1u) ? mq[3][i] : mq[2][i]) : ((row & 1u) ? mq[1][i] : mq[0][i]);
+        while (__ballot(m != 0ull)) {
+          if (COUNT) ++n_hits;
+          const bool none = m == 0ull;
+          const uint32_t k = none ? 0u : (uint32_t)i * 64u + 63u - (uint32_t)__builtin_ctzll(m | (1ull << 63));
+          m &= m - 1ull;
+          const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+          float acc[9];
+          const bool contrib = bwd_hit<PPT, RECOMP, COUNT>(A, B, Cc, none ? 0xFFFFFFFFu : rel0 + k, pxf, pyf, T,
+                                                           U, G0, G1, G2, lastj, acc, n_ok);
+          const uint64_t cm = __ballot(contrib);
+          if (cm) {
+            const float r = row_reduce9(acc);
+            if (term_lane && ((cm >> (lane & 48u)) & 0xFFFFull)) atomicAdd(&s_acc[k][my_term], r);
+          }
+        }
+      }
+    } else {
+      uint64_t mk[NM];
+      wave_masks<NM>(wr, s_a, s_c, s_d, klim, mk);
+#pragma unroll
+      for (int i = NM - 1; i >= 0; --i) {
+        uint64_t m = mk[i];
+        while (m) {
+          const int bt = 63 - __builtin_clzll(m);
+          m &= ~(1ull << bt);
+          const uint32_t k = (uint32_t)i * 64u + (uint32_t)bt;
+          if (COUNT) ++n_hits;
+          const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
+          float acc[9];
+          const bool contrib =
+              bwd_hit<PPT, RECOMP, COUNT>(A, B, Cc, rel0 + k, pxf, pyf, T, U, G0, G1, G2, lastj, acc, n_ok);
+          if (__ballot(contrib)) {
+            const float r = wave_reduce9(acc);
+            if (term_lane) atomicAdd(&s_acc[k][my_term], r);
+          }
         }
       }
     }
@@ -800,7 +902,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE
 // restart every 64 evaluations so that they stay finite.
 template <int TS, int BATCH>
 __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT))
-__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? NTS_BWD_WAVES : 1))) k_probe_bwd_mix(
+__attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? NTS_BWD_OCC((Blend<TS, NTS_BWD_THREADS>::PPT)) : 1))) k_probe_bwd_mix(
     uint32_t hits, float s1, float* __restrict__ sink) {
   using BL = Blend<TS, NTS_BWD_THREADS>;
   constexpr int NT = BL::NT, PPT = BL::PPT;
@@ -821,8 +923,11 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? NTS_B
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t my_term = (lane & 3u) == 0u ? grad_term_of_quad(lane >> 2) : 9u;
+  constexpr bool QUART = PPT == 4 && NTS_BWD_QUARTERS;
+  const uint32_t my_term = QUART ? grad_term_of_quad(lane & 15u)
+                                 : ((lane & 3u) == 0u ? grad_term_of_quad(lane >> 2) : 9u);
   const bool term_lane = my_term < 9u;
+  const uint32_t row = lane >> 4;
   float pxf[PPT], pyf[PPT], T[PPT], U[PPT], G0[PPT], G1[PPT], G2[PPT];
   uint32_t lastj[PPT];
 #pragma unroll
@@ -844,13 +949,26 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? NTS_B
   // would all meet on one slot)
   const uint32_t k0 = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) * 37u);
   for (uint32_t it = 0; it < hits; ++it) {
-    const uint32_t k = (it + k0) & (uint32_t)(BATCH - 1);
+    uint32_t k = (it + k0) & (uint32_t)(BATCH - 1);
+    if (QUART) {
+      // the rows take different slots, as their own masks make them do
+      uint32_t kq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) kq[q] = __builtin_amdgcn_readfirstlane((k + 5u * (uint32_t)q) & (uint32_t)(BATCH - 1));
+      k = (row & 2u) ? ((row & 1u) ? kq[3] : kq[2]) : ((row & 1u) ? kq[1] : kq[0]);
+    }
     const float4 A = s_a[k], B = s_b[k], Cc = s_c[k];
     float acc[9];
     const bool contrib = bwd_hit<PPT, true, false>(A, B, Cc, k, pxf, pyf, T, U, G0, G1, G2, lastj, acc, n_ok);
-    if (__ballot(contrib)) {
-      const float r = wave_reduce9(acc);
-      if (term_lane) atomicAdd(&s_acc[k][my_term], r);
+    const uint64_t cm = __ballot(contrib);
+    if (cm) {
+      if (QUART) {
+        const float r = row_reduce9(acc);
+        if (term_lane && ((cm >> (lane & 48u)) & 0xFFFFull)) atomicAdd(&s_acc[k][my_term], r);
+      } else {
+        const float r = wave_reduce9(acc);
+        if (term_lane) atomicAdd(&s_acc[k][my_term], r);
+      }
     }
     if ((it & 63u) == 63u) {
 #pragma unroll
