@@ -221,24 +221,25 @@ __device__ __forceinline__ WaveRect quarter_rect(const WaveRect& w, int q) {
 }
 
 // Hit masks of the staged batch per row of 16 lanes: bit b of mq[r][i] is
-// set when slot 64 i + 63 - b (< klim) can reach a pixel of row r's square
+// set when slot 64 i + 63 - b (< klq[r]) can reach a pixel of row r's square
 // (reversed, so that the lowest bit is the last slot).  Each lane tests its
 // slots against the four squares directly: a test of the wave's rectangle
 // first measured slower (nearly every batch has a slot that passes it).
 template <int NM>
 __device__ __forceinline__ void quarter_masks(const WaveRect& wr, const float4* s_a, const float4* s_c,
-                                              const float4* s_d, uint32_t klim, uint64_t (&mq)[4][NM]) {
+                                              const float4* s_d, const uint32_t (&klq)[4], uint64_t (&mq)[4][NM]) {
   const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t klim = max(max(klq[0], klq[1]), max(klq[2], klq[3]));
 #pragma unroll
   for (int i = 0; i < NM; ++i) {
     const uint32_t k = (uint32_t)i * 64u + 63u - lane;
     bool h0 = false, h1 = false, h2 = false, h3 = false;
     if (k < klim) {
       const float4 A = s_a[k], Cc = s_c[k], D = s_d[k];
-      h0 = wave_hits(quarter_rect(wr, 0), A, Cc, D);
-      h1 = wave_hits(quarter_rect(wr, 1), A, Cc, D);
-      h2 = wave_hits(quarter_rect(wr, 2), A, Cc, D);
-      h3 = wave_hits(quarter_rect(wr, 3), A, Cc, D);
+      h0 = k < klq[0] && wave_hits(quarter_rect(wr, 0), A, Cc, D);
+      h1 = k < klq[1] && wave_hits(quarter_rect(wr, 1), A, Cc, D);
+      h2 = k < klq[2] && wave_hits(quarter_rect(wr, 2), A, Cc, D);
+      h3 = k < klq[3] && wave_hits(quarter_rect(wr, 3), A, Cc, D);
     }
     mq[0][i] = __ballot(h0);
     mq[1][i] = __ballot(h1);
@@ -789,6 +790,18 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE
     my_max = lastj[j] > my_max ? lastj[j] : my_max;
   }
   const uint32_t wave_last = wave_max_u32(my_max);  // slice entries this wave can need
+  // and each row of 16 lanes (the quad form's splat streams)
+  uint32_t row_last[4] = {0u, 0u, 0u, 0u};
+  if constexpr (QUART) {
+    uint32_t rl = my_max;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)rl, off, 64);
+      rl = o > rl ? o : rl;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) row_last[q] = (uint32_t)__builtin_amdgcn_readlane((int)rl, 16 * q);
+  }
   __syncthreads();
   if (my_max) atomicMax(&s_max, my_max);
   __syncthreads();
@@ -813,8 +826,12 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE
       // Every row walks the slots that reach its own square, last first; a
       // row whose slots are used up evaluates slot 0 with an index past
       // every pixel's last contributor, so none of its pixels takes part.
+      uint32_t klq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        klq[q] = (klim && row_last[q] > rel0) ? (row_last[q] - rel0 < take ? row_last[q] - rel0 : take) : 0u;
       uint64_t mq[4][NM];
-      quarter_masks<NM>(wr, s_a, s_c, s_d, klim, mq);
+      quarter_masks<NM>(wr, s_a, s_c, s_d, klq, mq);
 #pragma unroll
       for (int i = NM - 1; i >= 0; --i) {
         // this lane's row's mask of word i, walked lowest bit (last slot)
@@ -832,7 +849,9 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE
           const uint64_t cm = __ballot(contrib);
           if (cm) {
             const float r = row_reduce9(acc);
-            if (term_lane && ((cm >> (lane & 48u)) & 0xFFFFull)) atomicAdd(&s_acc[k][my_term], r);
+            // (a row none of whose lanes took part adds exact zeros: cheaper
+            // than testing for it)
+            if (term_lane) atomicAdd(&s_acc[k][my_term], r);
           }
         }
       }
@@ -964,7 +983,7 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? NTS_B
     if (cm) {
       if (QUART) {
         const float r = row_reduce9(acc);
-        if (term_lane && ((cm >> (lane & 48u)) & 0xFFFFull)) atomicAdd(&s_acc[k][my_term], r);
+        if (term_lane) atomicAdd(&s_acc[k][my_term], r);
       } else {
         const float r = wave_reduce9(acc);
         if (term_lane) atomicAdd(&s_acc[k][my_term], r);
