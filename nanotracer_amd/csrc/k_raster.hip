@@ -390,9 +390,11 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
           if (alpha < NTS_ALPHA_MIN) continue;
           if (COUNT) ++n_blend;
           const float ta = T[j] * alpha;
-          C0[j] = C0[j] + ta * B.z;
-          C1[j] = C1[j] + ta * B.w;
-          C2[j] = C2[j] + ta * Cc.x;
+          // color + (T alpha) c, contracted (one rounding), as the host
+          // restatement does
+          C0[j] = __builtin_fmaf(ta, B.z, C0[j]);
+          C1[j] = __builtin_fmaf(ta, B.w, C1[j]);
+          C2[j] = __builtin_fmaf(ta, Cc.x, C2[j]);
           T[j] = T[j] * (1.0f - alpha);
           last[j] = base + k - begin + 1;
           pxf[j] = T[j] < NTS_T_MIN ? __builtin_nanf("") : pxf[j];

@@ -29,12 +29,14 @@ template <> struct M<float> {
   static float sqrt(float x) { return __builtin_sqrtf(x); }
   static float ceil(float x) { return __builtin_ceilf(x); }
   static float floor(float x) { return __builtin_floorf(x); }
+  static float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };
 template <> struct M<double> {
   static double exp(double x) { return ::exp(x); }
   static double sqrt(double x) { return ::sqrt(x); }
   static double ceil(double x) { return ::ceil(x); }
   static double floor(double x) { return ::floor(x); }
+  static double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 };
 
 template <typename T> struct V3 { T x, y, z; };
@@ -278,10 +280,13 @@ void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* r
           const T ar = p[11] * g;
           const T alpha = ar < T(NTS_ALPHA_MAX) ? ar : T(NTS_ALPHA_MAX);
           if (alpha < T(NTS_ALPHA_MIN)) continue;
+          // color = color + T alpha c (rasterize.wgsl:103), with the
+          // product-add contracted into one fused multiply-add (a form the
+          // WGSL's compilation may take; one rounding)
           const T ta = Tr * alpha;
-          C[0] = C[0] + ta * p[8];
-          C[1] = C[1] + ta * p[9];
-          C[2] = C[2] + ta * p[10];
+          C[0] = M<T>::fma(ta, p[8], C[0]);
+          C[1] = M<T>::fma(ta, p[9], C[1]);
+          C[2] = M<T>::fma(ta, p[10], C[2]);
           Tr = Tr * (T(1) - alpha);
           last = i - begin + 1;
           ++used;

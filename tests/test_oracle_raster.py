@@ -104,6 +104,46 @@ def test_near_and_frustum_cull_write_zero_records():
     assert not p[0].any() and not p[1].any() and p[2, 7] == 1.0
 
 
+def _fma32(a, b, c):
+    """float32 fused multiply-add, exactly: a b + c in rationals, rounded
+    once to the nearest float32 (ties to even)."""
+    from fractions import Fraction
+
+    x = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    if x == 0:
+        return np.float32(0.0) if (np.float32(a) * np.float32(b) + np.float32(c)) >= 0 else np.float32(-0.0)
+    sgn, x = (1, x) if x > 0 else (-1, -x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    if Fraction(2) ** e > x:
+        e -= 1
+    q = max(e, -126) - 23  # the float32 ulp at x
+    m = x / Fraction(2) ** q
+    r = m.numerator // m.denominator
+    rem = m - r
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and r % 2 == 1):
+        r += 1
+    return np.float32(sgn * float(r * Fraction(2) ** q))
+
+
+def test_fma32_rounds_once():
+    from fractions import Fraction
+
+    f = np.float32
+    # a product-then-add loses 2^-24 of (1 + 2^-12)^2 - 1; the fused form keeps it
+    a = f(1 + 2.0 ** -12)
+    assert _fma32(a, a, f(-1)) == f(2.0 ** -11 + 2.0 ** -24)
+    assert f(a * a) - f(1) == f(2.0 ** -11)
+    # against the exact value: no float32 neighbour is closer (ties to even)
+    rng = np.random.default_rng(11)
+    for a, b, c in rng.uniform(-2, 2, (300, 3)).astype(np.float32):
+        x = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+        r = _fma32(a, b, c)
+        d = abs(Fraction(float(r)) - x)
+        for nb in (np.nextafter(r, f(np.inf)), np.nextafter(r, f(-np.inf))):
+            dn = abs(Fraction(float(nb)) - x)
+            assert d < dn or (d == dn and (r.view(np.uint32) & 1) == 0)
+
+
 def _brute_force_pixel(proj, px, py, t):
     """Front-to-back over every visible splat whose tile box covers the
     pixel's tile, ordered by (depth16, index) - no ranges, no sort."""
@@ -136,7 +176,7 @@ def _brute_force_pixel(proj, px, py, t):
         if alpha < f(1.0 / 255.0):
             continue
         for c in range(3):
-            C[c] = f(C[c] + f(T * alpha) * p[8 + c])
+            C[c] = _fma32(f(T * alpha), p[8 + c], C[c])
         T = f(T * f(f(1) - alpha))
         if T < f(1e-4):
             break
