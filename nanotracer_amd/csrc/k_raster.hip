@@ -128,10 +128,11 @@ __device__ __forceinline__ WaveRect wave_rect(uint32_t tx, uint32_t ty) {
 __device__ __forceinline__ void stage_splat(float4 pa, float4 pb, float4 pc, float4& a, float4& b, float4& c,
                                             float4& d) {
   // The conic is staged as the power's own coefficients: power =
-  // -a/2 dx^2 - b dx dy - c/2 dy^2 rounds at every step exactly as
-  // -0.5 (a dx^2 + 2b dx dy + c dy^2) does (scaling by a power of two
-  // commutes with rounding; where an intermediate is subnormal the power is
-  // within 2^-126 of zero and exp gives 1 either way), one multiply fewer.
+  // fma(-c/2 dy, dy, fma(-a/2 dx, dx, -b dx dy)) rounds at every step
+  // exactly as nts_blend_power's -0.5 fma(c dy, dy, fma(a dx, dx, 2b dx dy))
+  // does (scaling by a power of two commutes with rounding; where an
+  // intermediate is subnormal the power is within 2^-126 of zero and exp
+  // gives 1 either way), one multiply fewer.
   a = make_float4(pa.x, pa.y, -0.5f * pb.x, -pb.y);
   b = make_float4(-0.5f * pb.z, pc.w, pc.x, pc.y);
   const float o = pc.w, qa = pb.x, qb = pb.y, qc = pb.z;
@@ -381,7 +382,8 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_FWD_THREADS>::NT == 1024 ? 8 : 
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
           const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
-          const float power = A.z * dx * dx + A.w * dx * dy + B.x * dy * dy;  // the staged -q/2
+          // nts_blend_power with the staged coefficients (-a/2, -b, -c/2)
+          const float power = __builtin_fmaf(B.x * dy, dy, __builtin_fmaf(A.z * dx, dx, A.w * dx * dy));
           // power > 0, a finished pixel (NaN), or below the alpha >= 1/255 bound
           if (!(power <= 0.0f) || power < Cc.y) continue;
           const float g = nts_exp_blend(power);
@@ -574,7 +576,7 @@ __device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const fl
   for (int j = 0; j < PPT; ++j) {
     const float dx = pxf[j] - A.x, dy = pyf[j] - A.y;
     // the forward's power and thresholds, bit for bit
-    const float power = A.z * dx * dx + A.w * dx * dy + B.x * dy * dy;  // the staged -q/2
+    const float power = __builtin_fmaf(B.x * dy, dy, __builtin_fmaf(A.z * dx, dx, A.w * dx * dy));
     // alpha for every lane, then one branch on all the forward's tests
     // recompute mode: hardware 2^x (error well under 1e-6 relative),
     // and where alpha lies that close to 1/255 or 0.99 the forward's
@@ -1360,7 +1362,7 @@ __global__ void __launch_bounds__(256) k_tonemap(const float4* __restrict__ img,
   for (int k = 0; k < 3; ++k) {
     float tm = c[k] / (1.0f + nts_maxf(c[k], 0.0f));
     tm = nts_minf(nts_maxf(tm, 0.0f), 1.0f);
-    const float s = tm > 0.0031308f ? 1.055f * powf(tm, 1.0f / 2.4f) - 0.055f : tm * 12.92f;
+    const float s = tm > 0.0031308f ? 1.055f * nts_srgb_pow(tm) - 0.055f : tm * 12.92f;
     o |= to_u8(s) << (8 * k);
   }
   o |= to_u8(p.w) << 24;

@@ -32,6 +32,16 @@ NTS_HD uint32_t nts_f32_to_bits(float f) { return __builtin_bit_cast(uint32_t, f
 
 NTS_HD float nts_fmaf(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// The blend's power -1/2 (a dx^2 + 2 b dx dy + c dy^2) of conic (a, b, c)
+// at offset (dx, dy) (rasterize.wgsl:91-95), in its contracted form: the
+// sum of the three products taken as two fused multiply-adds,
+// -1/2 fma(c dy, dy, fma(a dx, dx, (2 b dx) dy)).  The device evaluates the
+// same with the factor -1/2 folded into the coefficients (exact: a power of
+// two).
+NTS_HD float nts_blend_power(float a, float b, float c, float dx, float dy) {
+  return -0.5f * nts_fmaf(c * dy, dy, nts_fmaf(a * dx, dx, 2.0f * b * dx * dy));
+}
+
 // 2^k for k in [-126, 127] as an exact float.
 NTS_HD float nts_pow2i(int k) { return nts_bits_to_f32((uint32_t)(k + 127) << 23); }
 
@@ -139,6 +149,12 @@ NTS_HD float nts_logf(float x) {
   const float fe = (float)e;
   return nts_fmaf(fe, 0.693359375f, nts_fmaf(fe, -2.12194440e-4f, lm));
 }
+
+// t^(1/2.4) of the sRGB transfer (tonemap.wgsl:27-41) for the t above its
+// cutoff (0.0031308 < t <= 1: positive normal): exp(ln(t) / 2.4) with
+// nts_logf and nts_expf, so that the device and the host restatement give
+// the same bits (a libm powf and the device's do not always).
+NTS_HD float nts_srgb_pow(float t) { return nts_expf(nts_logf(t) * (1.0f / 2.4f)); }
 
 // cos(2 pi u): v = u - rint(u) in [-1/2, 1/2] (exact), a = |v|, then the
 // octant reduction a -> 1/4 - a or 1/2 - a (exact for the octants that use

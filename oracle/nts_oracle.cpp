@@ -274,7 +274,7 @@ void composite_t(const T* projected, const uint32_t* payloads, const uint32_t* r
         for (uint32_t i = begin; i < end; ++i) {
           const T* p = projected + 12 * (size_t)payloads[i];
           const T dx = pxf - p[0], dy = pyf - p[1];
-          const T power = T(-0.5) * (p[4] * dx * dx + T(2) * p[5] * dx * dy + p[6] * dy * dy);
+          const T power = T(-0.5) * M<T>::fma(p[6] * dy, dy, M<T>::fma(p[4] * dx, dx, T(2) * p[5] * dx * dy));
           if (!(power <= T(0))) continue;  // also a NaN power
           const T g = M<T>::exp(power);
           const T ar = p[11] * g;
@@ -592,7 +592,7 @@ void orc_composite_backward_ref(const float* projected, const uint32_t* payloads
             const float* p = projected + 12 * (size_t)sidx;
             const float dx = pxf - p[0], dy = pyf - p[1];
             const float ca = p[4], cb = p[5], cc = p[6];
-            const float power = -0.5f * (ca * dx * dx + 2.0f * cb * dx * dy + cc * dy * dy);
+            const float power = nts_blend_power(ca, cb, cc, dx, dy);
             if (!(power <= 0.0f)) continue;  // also a NaN power
             const float g = nts_expf(power);
             const float sigma = p[11];
@@ -683,7 +683,7 @@ void orc_composite_backward_recompute(const float* projected, const uint32_t* pa
             const float* p = projected + 12 * (size_t)sidx;
             const float dx = pxf - p[0], dy = pyf - p[1];
             const float ca = p[4], cb = p[5], cc = p[6];
-            const float power = -0.5f * (ca * dx * dx + 2.0f * cb * dx * dy + cc * dy * dy);
+            const float power = nts_blend_power(ca, cb, cc, dx, dy);
             if (!(power <= 0.0f)) continue;  // also a NaN power
             const float g = nts_expf(power);
             const float sigma = p[11];
@@ -1203,7 +1203,7 @@ uint64_t pixel_sig_f32(const float* proj32, const uint32_t* payloads, uint32_t b
   for (uint32_t i = begin; i < end; ++i) {
     const float* p = proj32 + 12 * (size_t)payloads[i];
     const float dx = pxf - p[0], dy = pyf - p[1];
-    const float power = -0.5f * (p[4] * dx * dx + 2.0f * p[5] * dx * dy + p[6] * dy * dy);
+    const float power = nts_blend_power(p[4], p[5], p[6], dx, dy);
     if (!(power <= 0.0f)) continue;  // also a NaN power
     const float ar = p[11] * nts_expf(power);
     const bool clamped = !(ar < NTS_ALPHA_MAX);
@@ -1448,7 +1448,7 @@ void orc_tonemap(const float* image, uint32_t w, uint32_t h, uint8_t* out) {
       const float x = image[4 * i + c];
       float tm = x / (1.0f + nts_maxf(x, 0.0f));
       tm = nts_minf(nts_maxf(tm, 0.0f), 1.0f);
-      const float s = tm > 0.0031308f ? 1.055f * powf(tm, 1.0f / 2.4f) - 0.055f : tm * 12.92f;
+      const float s = tm > 0.0031308f ? 1.055f * nts_srgb_pow(tm) - 0.055f : tm * 12.92f;
       out[4 * i + c] = (uint8_t)lrintf(nts_minf(nts_maxf(s, 0.0f), 1.0f) * 255.0f);
     }
     const float a = nts_minf(nts_maxf(image[4 * i + 3], 0.0f), 1.0f);

@@ -168,7 +168,7 @@ def _brute_force_pixel(proj, px, py, t):
     for _, i in cand:
         p = proj[i]
         dx, dy = f(f(px) + f(0.5)) - p[0], f(f(py) + f(0.5)) - p[1]
-        power = f(-0.5) * (p[4] * dx * dx + f(2) * p[5] * dx * dy + p[6] * dy * dy)
+        power = f(-0.5) * _fma32(f(p[6] * dy), dy, _fma32(f(p[4] * dx), dx, f(f(f(2) * p[5]) * dx) * dy))
         if power > 0:
             continue
         g = O.expf(np.array([power], np.float32))[0]
