@@ -103,23 +103,19 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 
 # HBM bytes per launch measured with rocprofv3 (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, the factors
 # read on the 4 GiB copy kernel of the peak probe) at the (kernel, tile size, that kernel's LDS batch)
-# it was profiled at, with the pair count of that run; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (4.0085e+09, 45_319_956),
-                    ("blend_fwd_loss", 32, 512): (4.6423e+09, 45_319_956)}
+# it was profiled at, with the pair count of that run (the steps with the default culled binning);
+# profiles/hbm_counters_train_step_1M_16views.md.
+PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (4.8927e+09, 45_319_957),
+                    ("blend_fwd_loss", 32, 512): (4.6413e+09, 45_319_957)}
 # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
-PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.6775e9, 45_319_956),
-                 ("blend_fwd_loss", 32, 512): (2.2026e9, 45_319_956)}
-# LDS bytes per launch moved by the lanes (rocprofv3 SQ_INSTS_LDS_LOAD_BANDWIDTH + _STORE_BANDWIDTH,
-# 64-byte units) and the share of the kernel's cycles the LDS spends on indexed operations
-# (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((792269345.4 + 15983137.4) * 64, 45_319_956, 0.250),
-                ("blend_fwd_loss", 32, 512): ((1957194914.8 + 13700604.4) * 64, 45_319_956, 0.385)}
+PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.3063e9, 45_319_957),
+                 ("blend_fwd_loss", 32, 512): (1.9748e9, 45_319_957)}
+# LDS bytes per launch moved by the lanes (rocprofv3 SQ_INSTS_LDS_LOAD_BANDWIDTH + _STORE_BANDWIDTH +
+# _ATOMIC_BANDWIDTH, 64-byte units) and the share of the kernel's cycles the LDS spends on indexed
+# operations (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
+PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((600067089.1 + 15984651.8 + 27816890.2) * 64, 45_319_957, 0.660),
+                ("blend_fwd_loss", 32, 512): ((1957269199.0 + 13701201.5) * 64, 45_319_957, 0.403)}
 N_SIMDS = 1024  # 256 CUs x 4 SIMDs
-# Independent FP32 FMA chains at full occupancy issue 0.244 wave instructions per SIMD per cycle
-# (a wave64 FMA occupies a SIMD for 4 cycles). The blend kernels' mixed code issues above that
-# rate, and fewer VALU instructions alone did not shorten them: they also wait on LDS, barriers
-# and memory (the stall counters in the same profile).
-FMA_PROBE_RATE = 0.244
 
 
 def blend_evals_of(ctx, evals, kernels, cfg, pairs):
@@ -174,8 +170,9 @@ def roofline_of(kernels, cfg=None, pairs=None):
 def valu_of(kernels, cfg, pairs, peaks):
     """VALU issue rate and LDS bandwidth of the blend kernels: the profiled instruction count and LDS
     bytes, scaled by this run's pairs, over this run's HIP-event kernel time; the issue rate (at the
-    shader clock measured on this device) beside the FMA-chain probe's, the LDS rate beside the LDS
-    read bandwidth measured on this device."""
+    shader clock measured on this device) beside the rate of independent FMA chains measured on this
+    device (the VALU's issue ceiling for FP32 work), the LDS rate beside the LDS read bandwidth
+    measured on this device."""
     CLOCK_HZ = peaks["clock_ghz"] * 1e9
     LDS_PEAK_TBS = peaks["lds_read_tbs"]
     out = {}
@@ -187,8 +184,10 @@ def valu_of(kernels, cfg, pairs, peaks):
         sec = kernels[name]["avg_ms"] * 1e-3
         instr = PROFILED_VALU[key][0] * pairs / PROFILED_VALU[key][1]
         rate = instr / N_SIMDS / (sec * CLOCK_HZ)
+        fma = peaks.get("fma_per_simd_cycle") or None
         out[name] = dict(valu_instructions=round(instr), valu_issue_per_simd_cycle=round(rate, 4),
-                         fma_probe_rate=FMA_PROBE_RATE, vs_fma_probe=round(rate / FMA_PROBE_RATE, 3))
+                         fma_probe_rate=round(fma, 4) if fma else None,
+                         vs_fma_probe=round(rate / fma, 3) if fma else None)
         if key in PROFILED_LDS:
             lb, lp, busy = PROFILED_LDS[key]
             tbs = lb * pairs / lp / sec / 1e12
@@ -496,12 +495,16 @@ def run_batched(args, ctx):
     )
     out.update(render_latency(ctx, g, s.n, tile, lds))
     peaks = ctx.probe_peaks(4 << 30)
+    peaks.update(ctx.probe_valu())
     out["roofline"] = roofline_of(kernels, (tile, lds[0], lds[1]), pairs)
     out["roofline"]["frac_of_measured_copy"] = round(out["roofline"]["achieved"] / peaks["hbm_copy_gbs"], 4)
     out["peaks"] = dict(hbm_copy_gbs=round(peaks["hbm_copy_gbs"], 1), lds_read_tbs=round(peaks["lds_read_tbs"], 2),
                         clock_ghz=round(peaks["clock_ghz"], 3), hbm_vendor_gbs=HBM_PEAK_GBS,
+                        valu_fp32_fma_tflops=round(peaks["fma_tflops"], 1),
+                        valu_fma_per_simd_cycle=round(peaks["fma_per_simd_cycle"], 4),
                         how="copy of 4 GiB (read + write bytes); conflict-free ds_read_b128 over all CUs; "
-                            "shader cycle counter over the wall clock during the LDS run")
+                            "shader cycle counter over the wall clock during the LDS run; independent "
+                            "v_fma_f32 chains at 8 waves per SIMD")
     # the whole step against HBM: compulsory bytes of the step (params, Adam, records, pairs, pixels, tiles)
     b_step = 2368.0 * s.n + 132.0 * visible + 116.0 * pairs + 76.0 * pixels + 8.0 * n_tiles
     ach = b_step / (out["ms_per_step"] * 1e-3) / 1e9

@@ -107,6 +107,11 @@ int nts_prof_report(nts_ctx* ctx, char* buf, size_t cap);
  * the cycle counter over the wall clock).  Allocates 2 x copy_bytes. */
 int nts_probe_peaks(nts_ctx* ctx, uint64_t copy_bytes, double* hbm_copy_gbs, double* lds_read_tbs,
                     double* clock_ghz);
+/* The FP32 VALU ceiling measured on the device (no reference counterpart):
+ * independent v_fma_f32 chains at eight waves per SIMD, as TFLOP/s (2 flops
+ * per lane FMA) and as wave instructions issued per SIMD per shader cycle
+ * (the clock of the same run). */
+int nts_probe_valu(nts_ctx* ctx, double* fma_tflops, double* fma_per_simd_cycle);
 /* Instruction-mix ceiling of the recompute blend backward (no reference
  * counterpart): the (splat, wave) evaluations per second a launch of the
  * backward's own per-evaluation code reaches at the kernel's workgroup size,

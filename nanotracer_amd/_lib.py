@@ -125,6 +125,7 @@ _SIGS = {
     "nts_probe_peaks": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
     "nts_probe_blend_mix": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
+    "nts_probe_valu": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nts_buffer_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "nts_buffer_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nts_buffer_size": (C.c_uint64, [C.c_void_p]),

@@ -113,6 +113,13 @@ class GpuCtx:
         L.check(L.lib().nts_probe_peaks(self.h, int(copy_bytes), C.byref(h), C.byref(l), C.byref(c)))
         return dict(hbm_copy_gbs=h.value, lds_read_tbs=l.value, clock_ghz=c.value)
 
+    def probe_valu(self) -> dict:
+        """Measured FP32 FMA ceiling of this device: TFLOP/s and wave
+        instructions per SIMD per shader cycle."""
+        t, r = C.c_double(), C.c_double()
+        L.check(L.lib().nts_probe_valu(self.h, C.byref(t), C.byref(r)))
+        return dict(fma_tflops=t.value, fma_per_simd_cycle=r.value)
+
     def probe_blend_mix(self, tile_size: int = 32, bwd_batch: int = 128) -> float:
         """(splat, wave) evaluations per second of the recompute backward's
         own per-evaluation code at its occupancy, with no memory traffic or

@@ -1,7 +1,7 @@
 // Measurement probes of the device's own ceilings, for the roofline figures:
 // the HBM bandwidth a plain copy reaches, the LDS bandwidth of 128-bit reads
-// (ds_read_b128), and the shader clock during that LDS probe (cycle counter
-// over the kernel's own time).  No reference counterpart: the reference has
+// (ds_read_b128), the shader clock during that LDS probe (cycle counter
+// over the kernel's own time), and the FP32 FMA issue rate of the VALU.  No reference counterpart: the reference has
 // no timing at all.
 #include <functional>
 
@@ -63,6 +63,40 @@ __global__ void __launch_bounds__(256) k_probe_lds(uint32_t iters, uint32_t* __r
   const uint64_t w1 = wall_clock64();
   // the words are below 2^14, so is their XOR: this store never happens
   if (acc == 0xdeadbeefu) sink[threadIdx.x & 255u] = acc;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    cycles[0] = c1 - c0;
+    cycles[1] = w1 - w0;
+  }
+}
+
+// Eight independent FP32 FMA chains per lane (v_fma_f32 on wave64), the
+// loop counter and branch scalar: the VALU issue ceiling for FP32 work.  The
+// operands come from the runtime value s (1.0) so nothing folds; a result no
+// launch produces would be stored.  Block 0 times its loop with the cycle
+// counter and the wall clock (for the clock of this run).
+#define PROBE_FMA_CHAINS 8
+__global__ void __launch_bounds__(256) k_probe_fma(uint32_t iters, float s, float* __restrict__ sink,
+                                                   uint64_t* __restrict__ cycles) {
+  const float lane = (float)(threadIdx.x & 63u);
+  float a0 = lane * 1e-3f * s, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f;
+  float a4 = a0 + 4.0f, a5 = a0 + 5.0f, a6 = a0 + 6.0f, a7 = a0 + 7.0f;
+  const float b = 0.999f * s, c = 1e-3f * s;
+  const uint64_t c0 = __builtin_readcyclecounter();
+  const uint64_t w0 = wall_clock64();
+  for (uint32_t it = 0; it < iters; ++it) {
+    a0 = __builtin_fmaf(a0, b, c);
+    a1 = __builtin_fmaf(a1, b, c);
+    a2 = __builtin_fmaf(a2, b, c);
+    a3 = __builtin_fmaf(a3, b, c);
+    a4 = __builtin_fmaf(a4, b, c);
+    a5 = __builtin_fmaf(a5, b, c);
+    a6 = __builtin_fmaf(a6, b, c);
+    a7 = __builtin_fmaf(a7, b, c);
+  }
+  const uint64_t c1 = __builtin_readcyclecounter();
+  const uint64_t w1 = wall_clock64();
+  const float r = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+  if (r == -1.0f) sink[threadIdx.x] = r;  // the chains stay positive
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     cycles[0] = c1 - c0;
     cycles[1] = w1 - w0;
@@ -132,5 +166,39 @@ extern "C" int nts_probe_peaks(nts_ctx* ctx, uint64_t copy_bytes, double* hbm_co
       cyc[1] > 0)
     *clock_ghz = (double)cyc[0] / ((double)cyc[1] / ((double)wall_khz * 1e3)) / 1e9;
   (void)hipGetLastError();
+  return NTS_OK;
+}
+
+extern "C" int nts_probe_valu(nts_ctx* ctx, double* fma_tflops, double* fma_per_simd_cycle) {
+  NTS_CHECK_ARG(ctx && fma_tflops && fma_per_simd_cycle);
+  int dev = 0, cus = 0, wall_khz = 0;
+  NTS_HIP_TRY(hipGetDevice(&dev));
+  NTS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  NTS_HIP_TRY(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev));
+  if (cus < 1) return nts_fail(NTS_E_HIP, "no compute units");
+  void* buf = nullptr;
+  int rc = nts_dev_alloc(ctx, &buf, 4096);
+  if (rc) return rc;
+  uint64_t* cycles = (uint64_t*)buf;
+  float* sink = (float*)((char*)buf + 256);
+  // eight 256-lane workgroups per CU: eight waves per SIMD
+  const uint32_t grid = (uint32_t)cus * 8u, iters = 16384;
+  double ms = 0.0;
+  rc = time_launches(ctx, 5, &ms, [&]() {
+    NTS_LAUNCH(ctx, "probe_fma", k_probe_fma, dim3(grid), dim3(256), 0, iters, 1.0f, sink, cycles);
+    return NTS_OK;
+  });
+  uint64_t cyc[2] = {0, 0};
+  if (rc == NTS_OK && hipMemcpy(cyc, cycles, 16, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = nts_fail(NTS_E_HIP, "probe readback");
+  (void)hipStreamSynchronize(ctx->stream);
+  nts_dev_free(ctx, buf, 4096);
+  if (rc) return rc;
+  const double lane_fmas = (double)grid * 256.0 * (double)iters * PROBE_FMA_CHAINS;
+  *fma_tflops = 2.0 * lane_fmas / (ms * 1e-3) / 1e12;
+  // wave instructions per SIMD per shader cycle, at the clock of this run
+  double clk = 0.0;
+  if (wall_khz > 0 && cyc[1] > 0) clk = (double)cyc[0] / ((double)cyc[1] / ((double)wall_khz * 1e3));
+  *fma_per_simd_cycle = clk > 0.0 ? (lane_fmas / 64.0) / (4.0 * cus) / (ms * 1e-3 * clk) : 0.0;
   return NTS_OK;
 }
