@@ -188,10 +188,11 @@ __device__ __forceinline__ bool wave_hits(const WaveRect& r, float4 A, float4 Cc
   const bool out_x = u0 > 0.0f || u1 < 0.0f, out_y = v0 > 0.0f || v1 < 0.0f;
   const float U = u0 > 0.0f ? u0 : u1;
   const float dy = __builtin_amdgcn_fmed3f(U * D.y, v0, v1);
-  const float qx = A.z * U * U + A.w * U * dy + D.w * dy * dy;
+  // (fused multiply-adds: fewer roundings than the bound above allows for)
+  const float qx = __builtin_fmaf(A.z * U, U, __builtin_fmaf(A.w * U, dy, D.w * dy * dy));
   const float V = v0 > 0.0f ? v0 : v1;
   const float dx = __builtin_amdgcn_fmed3f(V * D.z, u0, u1);
-  const float qy = A.z * dx * dx + A.w * dx * V + D.w * V * V;
+  const float qy = __builtin_fmaf(A.z * dx, dx, __builtin_fmaf(A.w * dx, V, D.w * V * V));
   const float q = fmaxf(out_x ? qx : -3.4e38f, out_y ? qy : -3.4e38f);
   return (!out_x && !out_y) || q >= D.x;
 }
@@ -610,9 +611,12 @@ __device__ __forceinline__ bool bwd_hit(const float4 A, const float4 B, const fl
       alpha = fminf(ar, NTS_ALPHA_MAX);
       aok = !(alpha < NTS_ALPHA_MIN);
     }
-    // the exact complement of the forward's skip tests (a NaN threshold
-    // or alpha must be kept here as it is there)
-    const bool ok = relk < lastj[j] && power <= 0.0f && !(power < Cc.y) && aok;
+    // the exact complement of the forward's skip tests (a NaN alpha must
+    // be kept here as it is there).  The forward's power threshold needs no
+    // test of its own: it lies 1e-3 below -ln(255 opacity), so a power under
+    // it gives alpha below 0.999 / 255, which aok rejects (outside the band
+    // of the exact exponential); a NaN threshold skips nothing on either side.
+    const bool ok = relk < lastj[j] && power <= 0.0f && aok;
     if (RECOMP && j == 0) {
       // The first pixel of the quad without a branch: a lane that fails
       // the tests sets exact zeros and keeps T (acc needs no zeroing
