@@ -10,7 +10,8 @@ backward's instruction-mix probe.
   forward blended (it repeats the forward's tests from the last contributor
   back), so its contributing count equals the forward's blended count.
 - The probe reports a positive, finite rate for every tile size and LDS
-  batch of the backward."""
+  batch of the backward.
+- The FP32 FMA ceiling probe reports a consistent rate."""
 import math
 
 import numpy as np
@@ -84,3 +85,18 @@ def test_counted_step_equals_uncounted_and_the_oracle(ctx, tile):
 def test_backward_mix_probe(ctx, tile, batch):
     r = ctx.probe_blend_mix(tile, batch)
     assert math.isfinite(r) and r > 1e6
+
+
+def test_valu_probe_reports_the_fma_ceiling(ctx):
+    """nts_probe_valu: independent FP32 FMA chains at eight waves per SIMD.
+    A wave64 FMA cannot issue faster than one per SIMD-cycle, and the two
+    figures the probe returns agree with each other through the device's
+    compute units (TFLOP/s = SIMD-cycle rate x 4 SIMDs x CUs x 64 lanes x
+    2 flops x clock): an MI355X has 256 CUs."""
+    v = ctx.probe_valu()
+    r, t = v["fma_per_simd_cycle"], v["fma_tflops"]
+    assert 0.1 < r < 1.0, v
+    assert 10.0 < t < 400.0, v
+    pk = ctx.probe_peaks(64 << 20)
+    cus = t * 1e12 / (r * 4 * 64 * 2 * pk["clock_ghz"] * 1e9)
+    assert 200 < cus < 320, (v, pk, cus)
