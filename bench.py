@@ -75,6 +75,7 @@ def alg_bytes(name: str, N: int, Nv: int, K: int, P: int, T: int, V: int, kb: in
         "tile_emit": 24 * N + (kb + 4) * K,
         "emit_views": 24 * N * V + (kb + 4) * K,
         "sort_hist": kb * K,
+        "sort_part_count": kb * K,
         "sort_onesweep": 2 * (kb + 4) * K,
         "tile_ranges": kb * K + 8 * T,
         "blend_fwd": 8 * T + 40 * K + 16 * P,

@@ -216,7 +216,7 @@ int nts_sort_tile_pairs(nts_ctx* ctx, nts_buffer* keys, nts_buffer* vals, uint64
  * hipMalloc).  nts_sort_pairs_u32 with the same result and errors, but every
  * buffer beyond keys and vals (the ping-pong pairs, digit histograms and
  * offsets, look-back status words, counters) lives in one static device
- * array of 18.9 MB per GPU, held until the process ends: nothing is
+ * array of 19.0 MB per GPU, held until the process ends: nothing is
  * allocated.  At most NTS_SORT_NOALLOC_MAX_PAIRS pairs (NTS_E_CAPACITY
  * beyond).  All calls on one GPU share that array, so calls that overlap on
  * one GPU (several contexts or threads) race; sequential calls are safe.

@@ -148,46 +148,76 @@ extern "C" int nts_scan_u32(nts_ctx* ctx, nts_buffer* data, uint64_t n) {
 #define FLAG_MASK (3u << 30)
 #define VALUE_MASK ((1u << 30) - 1u)
 
+// part_begin[s]: the first partition of segment s; chunk_begin[s]: its first
+// chunk (RS_CHUNK consecutive partitions of one segment).
 __global__ void k_sort_parts(const uint64_t* __restrict__ seg_begin, uint32_t num_segs,
-                             uint32_t* __restrict__ part_begin) {
+                             uint32_t* __restrict__ part_begin, uint32_t* __restrict__ chunk_begin) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t p = 0;
+  uint32_t p = 0, c = 0;
   for (uint32_t s = 0; s < num_segs; ++s) {
     part_begin[s] = p;
+    chunk_begin[s] = c;
     const uint64_t len = seg_begin[s + 1] - seg_begin[s];
-    p += (uint32_t)((len + RS_TILE - 1) / RS_TILE);
+    const uint32_t np = (uint32_t)((len + RS_TILE - 1) / RS_TILE);
+    p += np;
+    c += (np + RS_CHUNK - 1) / RS_CHUNK;
   }
   part_begin[num_segs] = p;
+  chunk_begin[num_segs] = c;
+}
+
+// Chunks the count tables hold for max_pairs pairs in num_segs segments (the
+// stride of the chunk-base table, which is digit-major).
+static inline uint32_t rs_max_chunks(uint32_t num_segs, uint64_t max_pairs) {
+  return (uint32_t)((max_pairs + RS_TILE - 1) / RS_TILE / RS_CHUNK + 2ull * num_segs + 1);
 }
 
 // Digit histograms of every pass for every segment (pass ps sorts bits
-// [lo_bit + 8 ps, lo_bit + 8 ps + 8)).  A fixed grid takes
-// contiguous runs of partitions (the live count is only known on the device,
-// so a launch sized for the capacity costs nothing for empty partitions) and
-// keeps its counts in LDS until the segment changes: the global atomics are
-// one set per (block, segment) rather than one per partition.
+// [lo_bit + 8 ps, lo_bit + 8 ps + 8)), and, where cnt is given, pass 0's
+// count table: cnt[p][d] = the keys of digit d in partition p.  A fixed grid
+// takes contiguous runs of partitions (the live count is only known on the
+// device, so a launch sized for the capacity costs nothing for empty
+// partitions) and keeps the histograms in LDS until the segment changes: the
+// global atomics are one set per (block, segment) rather than one per
+// partition.  The partition counts go through two LDS rows used in turn, so
+// one barrier per partition orders both the row's readers and its reuse.
+template <typename K>
+__device__ __forceinline__ void hist_load(const K* __restrict__ keys, uint64_t base, uint64_t end, uint32_t tid,
+                                          K (&key)[RS_ITEMS]) {
+#pragma unroll
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    const uint64_t i = base + (uint64_t)k * RS_BLOCK + tid;
+    key[k] = i < end ? keys[i] : (K)0;
+  }
+}
+
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                         const uint32_t* __restrict__ part_begin, uint32_t num_segs,
-                                                        uint32_t lo_bit, uint32_t passes, uint32_t* __restrict__ hist) {
+                                                        uint32_t lo_bit, uint32_t passes, uint32_t* __restrict__ hist,
+                                                        uint32_t* __restrict__ cnt) {
   __shared__ uint32_t s_hist[8 * RS_RADIX];
+  __shared__ uint32_t s_part[2][RS_RADIX];
   const uint32_t total_parts = part_begin[num_segs];
   const uint32_t per = (total_parts + gridDim.x - 1) / gridDim.x;
   const uint32_t p0 = blockIdx.x * per;
   const uint32_t p1 = p0 + per < total_parts ? p0 + per : total_parts;
   if (p0 >= p1) return;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t nbins = passes * RS_RADIX;
   const auto flush = [&](uint32_t seg) {
     __syncthreads();
     uint32_t* h = hist + (size_t)seg * nbins;
-    for (uint32_t i = threadIdx.x; i < nbins; i += RS_BLOCK) {
+    for (uint32_t i = tid; i < nbins; i += RS_BLOCK) {
       const uint32_t c = s_hist[i];
       if (c) atomicAdd(&h[i], c);
       s_hist[i] = 0;
     }
     __syncthreads();
   };
-  for (uint32_t i = threadIdx.x; i < nbins; i += RS_BLOCK) s_hist[i] = 0;
+  for (uint32_t i = tid; i < nbins; i += RS_BLOCK) s_hist[i] = 0;
+  s_part[0][tid] = 0;
+  s_part[1][tid] = 0;
   __syncthreads();
   uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p0);
   for (uint32_t p = p0; p < p1; ++p) {
@@ -197,21 +227,22 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
     }
     const uint64_t base = seg_begin[s] + (uint64_t)(p - part_begin[s]) * RS_TILE;
     const uint64_t end = seg_begin[s + 1];
+    uint32_t* row = s_part[p & 1u];
+    K key[RS_ITEMS];
+    hist_load<K>(keys, base, end, tid, key);
     // Neighbouring pairs mostly come from one record (same depth, neighbouring
     // tiles), so a wave's digits come in runs: each run's first lane adds the
     // run's length, one LDS atomic per run instead of one per key.  The valid
     // lanes of a wave are a prefix (consecutive indices).
-    const uint32_t lane = threadIdx.x & 63u;
-#pragma unroll 4
+#pragma unroll
     for (int k = 0; k < RS_ITEMS; ++k) {
-      const uint64_t i = base + (uint64_t)k * RS_BLOCK + threadIdx.x;
+      const uint64_t i = base + (uint64_t)k * RS_BLOCK + tid;
       const bool valid = i < end;
-      const K key = valid ? keys[i] : (K)0;
       const uint64_t vmask = __ballot(valid);
       if (vmask == 0) continue;
       const uint32_t n_valid = (uint32_t)__popcll(vmask);
       for (uint32_t ps = 0; ps < passes; ++ps) {
-        const uint32_t d = (uint32_t)((key >> (lo_bit + 8 * ps)) & 255u);
+        const uint32_t d = (uint32_t)((key[k] >> (lo_bit + 8 * ps)) & 255u);
         const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, 64);
         const bool head = valid && (lane == 0 || prev != d);
         const uint64_t heads = __ballot(head);
@@ -219,11 +250,112 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
           const uint64_t above = lane < 63u ? heads & (~0ull << (lane + 1u)) : 0ull;
           const uint32_t next = above ? (uint32_t)__builtin_ctzll(above) : n_valid;
           atomicAdd(&s_hist[ps * RS_RADIX + d], next - lane);
+          if (ps == 0 && cnt) atomicAdd(&row[d], next - lane);
         }
       }
     }
+    if (cnt) {
+      __syncthreads();
+      cnt[(size_t)p * RS_RADIX + tid] = row[tid];
+      row[tid] = 0;
+    }
   }
   flush(s);
+}
+
+// The count table of the digit at bits [shift, shift + 8) over keys in
+// partition order: cnt[p][d] = the keys of digit d in partition p.  Lanes of
+// a wave with the same digit are found with ballots, as the onesweep ranks
+// them, and their leader adds their number (the digits of a sorted pass's
+// output repeat without forming runs).  Two LDS rows in turn, as in
+// k_sort_hist.
+template <typename K>
+__global__ void __launch_bounds__(RS_BLOCK) k_part_count(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
+                                                         const uint32_t* __restrict__ part_begin, uint32_t num_segs,
+                                                         uint32_t shift, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t s_part[2][RS_RADIX];
+  const uint32_t total_parts = part_begin[num_segs];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  s_part[0][tid] = 0;
+  s_part[1][tid] = 0;
+  __syncthreads();
+  uint32_t it = 0;
+  for (uint32_t p = blockIdx.x; p < total_parts; p += gridDim.x, ++it) {
+    const uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p);
+    const uint64_t base = seg_begin[s] + (uint64_t)(p - part_begin[s]) * RS_TILE;
+    const uint64_t end = seg_begin[s + 1];
+    uint32_t* row = s_part[it & 1u];
+    K key[RS_ITEMS];
+    hist_load<K>(keys, base, end, tid, key);
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+      const uint64_t i = base + (uint64_t)k * RS_BLOCK + tid;
+      const uint64_t vb = __ballot(i < end);
+      if (vb == 0) continue;
+      const uint32_t d = (uint32_t)((key[k] >> shift) & 255u);
+      uint32_t mlo = (uint32_t)vb, mhi = (uint32_t)(vb >> 32);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint32_t neg = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t bal = __ballot(neg != 0u);
+        mlo &= ~((uint32_t)bal ^ neg);
+        mhi &= ~((uint32_t)(bal >> 32) ^ neg);
+      }
+      const uint64_t match = i < end ? (((uint64_t)mhi << 32) | mlo) : 0ull;
+      if (match && lane == (uint32_t)__builtin_ctzll(match)) atomicAdd(&row[d], (uint32_t)__popcll(match));
+    }
+    __syncthreads();
+    cnt[(size_t)p * RS_RADIX + tid] = row[tid];
+    row[tid] = 0;
+  }
+}
+
+// Chunk prefixes of a count table: within each chunk (RS_CHUNK consecutive
+// partitions of one segment) cnt[p][d] becomes the keys of digit d in the
+// chunk's partitions before p, and ctot[d][c] receives the chunk's total
+// (digit-major, stride max_chunks).  One block per chunk, one thread per digit.
+__global__ void __launch_bounds__(RS_RADIX) k_chunk_prefix(uint32_t* __restrict__ cnt,
+                                                           const uint32_t* __restrict__ part_begin,
+                                                           const uint32_t* __restrict__ chunk_begin, uint32_t num_segs,
+                                                           uint32_t* __restrict__ ctot, uint32_t max_chunks) {
+  const uint32_t total_chunks = chunk_begin[num_segs];
+  const uint32_t d = threadIdx.x;
+  for (uint32_t c = blockIdx.x; c < total_chunks; c += gridDim.x) {
+    const uint32_t s = seg_of<uint32_t>(chunk_begin, num_segs, c);
+    const uint32_t pa = part_begin[s] + (c - chunk_begin[s]) * RS_CHUNK;
+    const uint32_t pb = pa + RS_CHUNK < part_begin[s + 1] ? pa + RS_CHUNK : part_begin[s + 1];
+    uint32_t v[RS_CHUNK];
+#pragma unroll
+    for (int k = 0; k < RS_CHUNK; ++k) v[k] = pa + k < pb ? cnt[(size_t)(pa + k) * RS_RADIX + d] : 0u;
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < RS_CHUNK; ++k) {
+      if (pa + k < pb) cnt[(size_t)(pa + k) * RS_RADIX + d] = run;
+      run += v[k];
+    }
+    ctot[(size_t)d * max_chunks + c] = run;
+  }
+}
+
+// Chunk bases of one pass: ctot[d][c] (the keys of digit d in chunk c) becomes
+// offsets[s][pass][d] plus the keys of digit d in the chunks of segment s
+// before c.  One wave per (segment, digit), 64 chunks per step.
+__global__ void __launch_bounds__(256) k_chunk_scan(uint32_t* __restrict__ ctot, const uint32_t* __restrict__ chunk_begin,
+                                                    uint32_t num_segs, const uint32_t* __restrict__ offsets,
+                                                    uint32_t passes, uint32_t pass, uint32_t max_chunks) {
+  const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  const uint32_t s = wave >> 8, d = wave & 255u;
+  if (s >= num_segs) return;
+  const uint32_t cb = chunk_begin[s], ce = chunk_begin[s + 1];
+  uint32_t carry = offsets[((size_t)s * passes + pass) * RS_RADIX + d];
+  uint32_t* col = ctot + (size_t)d * max_chunks;
+  for (uint32_t c = cb; c < ce; c += 64u) {
+    const uint32_t i = c + lane;
+    const uint32_t v = i < ce ? col[i] : 0u;
+    const uint32_t incl = wave_incl_scan(v);
+    if (i < ce) col[i] = carry + incl - v;
+    carry += __shfl(incl, 63, 64);
+  }
 }
 
 // Clears the look-back status words of the live partitions of every pass
@@ -231,10 +363,11 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
 // on the device, so a launch sized for the capacity clears only what is used.
 __global__ void __launch_bounds__(256) k_clear_status(uint32_t* __restrict__ status,
                                                       const uint32_t* __restrict__ part_begin, uint32_t num_segs,
-                                                      uint32_t passes, uint32_t max_parts) {
+                                                      uint32_t passes, uint32_t max_parts, uint32_t skip_mask) {
   const uint64_t words = (uint64_t)part_begin[num_segs] * RS_RADIX;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint32_t ps = 0; ps < passes; ++ps) {
+    if ((skip_mask >> ps) & 1u) continue;  // a pass that takes its offsets from a count table
     uint4* st = reinterpret_cast<uint4*>(status + (uint64_t)ps * max_parts * RS_RADIX);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words / 4; i += stride)
       st[i] = make_uint4(0u, 0u, 0u, 0u);
@@ -253,14 +386,21 @@ __global__ void __launch_bounds__(RS_RADIX) k_sort_offsets(const uint32_t* __res
   offsets[idx] = (uint32_t)seg_begin[s] + ex;
 }
 
-template <typename K>
+//
+// TABLE: the block's digit bases come from count tables written before the
+// pass (tcnt[p][d]: digit d's keys in the partitions of p's chunk before p;
+// tctot[d][c]: where chunk c's keys of digit d begin) instead of the
+// look-back, so no block waits on another.
+template <typename K, bool TABLE>
 __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(RS_WAVES_PER_EU))) k_onesweep(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
                                                        K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                        const uint64_t* __restrict__ seg_begin,
                                                        const uint32_t* __restrict__ part_begin, uint32_t num_segs,
                                                        uint32_t shift, uint32_t passes, uint32_t pass,
                                                        const uint32_t* __restrict__ offsets, uint32_t* status,
-                                                       uint32_t* part_counter, uint32_t* err) {
+                                                       uint32_t* part_counter, uint32_t* err,
+                                                       const uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ tctot,
+                                                       const uint32_t* __restrict__ chunk_begin, uint32_t max_chunks) {
   __shared__ K s_keys[RS_TILE];
   __shared__ uint32_t s_vals[RS_TILE];
   __shared__ uint32_t s_wave_hist[4][RS_RADIX];
@@ -285,6 +425,10 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
     const uint32_t first_part = part_begin[s];
     const uint64_t base = seg_begin[s] + (uint64_t)(p - first_part) * RS_TILE;
     const uint64_t end = seg_begin[s + 1];
+    uint32_t tbase = 0;  // TABLE: digit tid's first output position for this block
+    if (TABLE)
+      tbase = tctot[(size_t)tid * max_chunks + chunk_begin[s] + (p - first_part) / RS_CHUNK] +
+              tcnt[(size_t)p * RS_RADIX + tid];
 
     // ---- load and rank (order within the block = (wave, round, lane)) ----
     K key[RS_ITEMS];
@@ -336,6 +480,9 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
     const uint32_t ex = block_excl_scan<RS_BLOCK>(tot, s_scan, block_total);
     s_excl[d] = ex;
 
+    if (TABLE) {
+      s_gbase[d] = tbase - ex;
+    } else {
     // ---- decoupled look-back over the preceding partitions of this segment ----
     uint32_t* st = status;  // this pass's status words (offset by the host)
     uint32_t prefix = 0;
@@ -387,6 +534,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
                          __HIP_MEMORY_SCOPE_AGENT);
     }
     s_gbase[d] = offsets[((size_t)s * passes + pass) * RS_RADIX + d] + prefix - ex;
+    }
     __syncthreads();
 
     // ---- reorder in LDS by digit, then write digit runs contiguously ----
@@ -419,9 +567,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
 // Workspace of the segmented sort, sized for max_pairs pairs in num_segs
 // segments and keys of key_bytes bytes (so for up to key_bytes passes).
 uint64_t nts_sort_ws_bytes(uint32_t num_segs, uint64_t max_pairs, uint32_t key_bytes) {
-  const uint64_t max_parts = (max_pairs + RS_TILE - 1) / RS_TILE + num_segs;
-  const uint64_t n_hist = (uint64_t)num_segs * key_bytes * RS_RADIX;
-  return 4 * (256 + 2 * n_hist + 64 + (uint64_t)key_bytes * max_parts * RS_RADIX);
+  return rs_ws_bytes(num_segs, max_pairs, key_bytes);
 }
 
 // Launch the onesweep passes over bits [lo_bit, lo_bit + 8 passes) of pairs
@@ -442,13 +588,17 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
                                                  (unsigned long long)max_pairs);
   if (lo_bit + 8 * passes > 8 * sizeof(K)) return nts_fail(NTS_E_INVALID, "sort bits beyond the key");
   const uint32_t max_parts = nts_div_up(max_pairs, RS_TILE) + num_segs;
+  const uint32_t max_chunks = rs_max_chunks(num_segs, max_pairs);
   const uint64_t n_hist = (uint64_t)num_segs * sizeof(K) * RS_RADIX;
   uint32_t* part_begin = (uint32_t*)ws;
+  uint32_t* chunk_begin = part_begin + 128;
   uint32_t* hist = part_begin + 256;
   uint32_t* offsets = hist + n_hist;
   uint32_t* counters = offsets + n_hist;
   uint32_t* err = counters + 32;
-  uint32_t* status = counters + 64;
+  uint32_t* status = counters + 64;  // per pass: look-back words, or the count table
+  uint32_t* ctot = status + (size_t)sizeof(K) * max_parts * RS_RADIX;
+  const uint32_t table = RS_TABLE_MASK & ((1u << passes) - 1u);
   if (err_out) *err_out = err;
   if (offsets_out) *offsets_out = offsets;
   if (counters_out) *counters_out = counters;  // words 40-63 are free for the caller
@@ -457,12 +607,16 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   rc = nts_fill_u32(ctx, counters, 0, 64);
   if (rc) return rc;
   if (passes == 0 || max_pairs == 0) return NTS_OK;
-  NTS_LAUNCH(ctx, "sort_parts", k_sort_parts, dim3(1), dim3(64), 0, d_seg_begin, num_segs, part_begin);
-  NTS_LAUNCH(ctx, "sort_clear", k_clear_status, dim3(1024), dim3(256), 0, status, (const uint32_t*)part_begin,
-             num_segs, passes, max_parts);
+  NTS_LAUNCH(ctx, "sort_parts", k_sort_parts, dim3(1), dim3(64), 0, d_seg_begin, num_segs, part_begin, chunk_begin);
+  if (table != (1u << passes) - 1u)
+    NTS_LAUNCH(ctx, "sort_clear", k_clear_status, dim3(1024), dim3(256), 0, status, (const uint32_t*)part_begin,
+               num_segs, passes, max_parts, table);
   const uint32_t grid = max_parts < RS_GRID ? max_parts : RS_GRID;
+  const uint32_t cgrid = max_chunks < RS_GRID ? max_chunks : RS_GRID;
+  const uint32_t sgrid = nts_div_up(num_segs * RS_RADIX, 4);  // one wave per (segment, digit)
+  // the segment histograms of every pass, and pass 0's count table
   NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
-             (const uint32_t*)part_begin, num_segs, lo_bit, passes, hist);
+             (const uint32_t*)part_begin, num_segs, lo_bit, passes, hist, (table & 1u) ? status : (uint32_t*)nullptr);
   NTS_LAUNCH(ctx, "sort_offsets", k_sort_offsets, dim3(num_segs * passes), dim3(RS_RADIX), 0, (const uint32_t*)hist,
              d_seg_begin, passes, offsets);
   K* src_k = keys;
@@ -470,10 +624,25 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   K* dst_k = keys_alt;
   uint32_t* dst_v = vals_alt;
   for (uint32_t ps = 0; ps < passes; ++ps) {
-    NTS_LAUNCH(ctx, "sort_onesweep", k_onesweep<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
-               (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs,
-               lo_bit + 8 * ps, passes, ps, (const uint32_t*)offsets, status + (size_t)ps * max_parts * RS_RADIX,
-               counters + ps, err);
+    uint32_t* st = status + (size_t)ps * max_parts * RS_RADIX;
+    if ((table >> ps) & 1u) {
+      if (ps > 0)  // the count table of this pass's digit over the previous pass's output
+        NTS_LAUNCH(ctx, "sort_part_count", k_part_count<K>, dim3(max_parts), dim3(RS_BLOCK), 0, (const K*)src_k,
+                   d_seg_begin, (const uint32_t*)part_begin, num_segs, lo_bit + 8 * ps, st);
+      NTS_LAUNCH(ctx, "sort_chunk_prefix", k_chunk_prefix, dim3(cgrid), dim3(RS_RADIX), 0, st,
+                 (const uint32_t*)part_begin, (const uint32_t*)chunk_begin, num_segs, ctot, max_chunks);
+      NTS_LAUNCH(ctx, "sort_chunk_scan", k_chunk_scan, dim3(sgrid), dim3(256), 0, ctot,
+                 (const uint32_t*)chunk_begin, num_segs, (const uint32_t*)offsets, passes, ps, max_chunks);
+      NTS_LAUNCH(ctx, "sort_onesweep", (k_onesweep<K, true>), dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
+                 (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs,
+                 lo_bit + 8 * ps, passes, ps, (const uint32_t*)offsets, st, counters + ps, err, (const uint32_t*)st,
+                 (const uint32_t*)ctot, (const uint32_t*)chunk_begin, max_chunks);
+    } else {
+      NTS_LAUNCH(ctx, "sort_onesweep", (k_onesweep<K, false>), dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
+                 (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs,
+                 lo_bit + 8 * ps, passes, ps, (const uint32_t*)offsets, st, counters + ps, err,
+                 (const uint32_t*)nullptr, (const uint32_t*)nullptr, (const uint32_t*)chunk_begin, max_chunks);
+    }
     K* tk = src_k; src_k = dst_k; dst_k = tk;
     uint32_t* tv = src_v; src_v = dst_v; dst_v = tv;
   }
@@ -1207,7 +1376,7 @@ extern "C" int nts_sort_tile_pairs(nts_ctx* ctx, nts_buffer* keys, nts_buffer* v
 // that cannot allocate device memory.  The array holds the segment bounds,
 // the ping-pong keys and values and the sort workspace (histograms, digit
 // offsets, look-back status words, partition counters, the error word) for
-// up to NTS_SORT_NOALLOC_MAX_PAIRS pairs: 18.9 MB, held on each GPU the
+// up to NTS_SORT_NOALLOC_MAX_PAIRS pairs: 19.0 MB, held on each GPU the
 // library runs on until the process ends.  Every call on a GPU uses that
 // GPU's one copy, so calls that overlap on one GPU race; whatever must start
 // at zero is cleared by each call (fill kernels), nothing is allocated.
@@ -1215,8 +1384,7 @@ extern "C" int nts_sort_tile_pairs(nts_ctx* ctx, nts_buffer* keys, nts_buffer* v
 namespace {
 constexpr uint64_t na_al(uint64_t x) { return (x + 255) & ~255ull; }
 constexpr uint64_t kNaMax = NTS_SORT_NOALLOC_MAX_PAIRS;
-constexpr uint64_t kNaParts = (kNaMax + RS_TILE - 1) / RS_TILE + 1;
-constexpr uint64_t kNaWs = 4 * (256 + 2 * (4 * RS_RADIX) + 64 + 4 * kNaParts * RS_RADIX);  // nts_sort_ws_bytes(1, max, 4)
+constexpr uint64_t kNaWs = rs_ws_bytes(1, kNaMax, 4);  // nts_sort_ws_bytes(1, max, 4)
 constexpr uint64_t kNaOffKeys = 256;
 constexpr uint64_t kNaOffVals = kNaOffKeys + na_al(4 * kNaMax);
 constexpr uint64_t kNaOffWs = kNaOffVals + na_al(4 * kNaMax);

@@ -5,7 +5,12 @@
 // single kernel in which a block ranks its 4096 keys per digit with wave64
 // ballots, publishes its per-digit counts and looks back over the preceding
 // blocks of its segment (decoupled look-back) for its global digit offsets,
-// reorders its keys in LDS and writes them out digit-contiguously.
+// reorders its keys in LDS and writes them out digit-contiguously.  The
+// passes in RS_TABLE_MASK take their block offsets from per-partition digit
+// counts instead (pass 0's from the histogram pass, which reads the keys
+// anyway; later passes' from a counting pass over the previous pass's
+// output), scanned over 16-partition chunks and then over each segment's
+// chunks: no block waits on another.
 #pragma once
 #include "nts_internal.h"
 
@@ -26,6 +31,23 @@
 #define RS_RADIX 256
 #define RS_MAX_SEGS 64
 #define RS_GRID 2048  // blocks of the partition-walking sort kernels
+#ifndef RS_CHUNK
+#define RS_CHUNK 16  // partitions per chunk of the digit-count tables
+#endif
+#ifndef RS_TABLE_MASK
+#define RS_TABLE_MASK 3u  // passes (bit ps) whose block offsets come from count tables, not the look-back
+#endif
+
+// Workspace bytes of the segmented sort for max_pairs pairs in num_segs
+// segments and keys of key_bytes bytes (so up to key_bytes passes): segment
+// and chunk starts, digit histograms and offsets, counters, per pass one word
+// per (partition, digit) (look-back status or partition counts), and one
+// word per (chunk, digit) of chunk bases.
+constexpr uint64_t rs_ws_bytes(uint64_t num_segs, uint64_t max_pairs, uint64_t key_bytes) {
+  return 4 * (256 + 2 * num_segs * key_bytes * RS_RADIX + 64 +
+              key_bytes * ((max_pairs + RS_TILE - 1) / RS_TILE + num_segs) * RS_RADIX +
+              ((max_pairs + RS_TILE - 1) / RS_TILE / RS_CHUNK + 2 * num_segs + 1) * RS_RADIX);
+}
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
