@@ -79,15 +79,15 @@ def alg_bytes(name: str, N: int, Nv: int, K: int, P: int, T: int, V: int, kb: in
         "sort_onesweep": 2 * (kb + 4) * K,
         "tile_ranges": kb * K + 8 * T,
         "blend_fwd": 8 * T + 40 * K + 16 * P,
-        # the batched step's forward also clears the V x N projected-gradient records (48 B)
-        "blend_fwd_loss": 8 * T + 40 * K + 16 * P + 16 * P + 16 * P + 8 * P + 48 * N * V,
+        # the batched step's forward also clears the V x N projected-gradient records (36 B)
+        "blend_fwd_loss": 8 * T + 40 * K + 16 * P + 16 * P + 16 * P + 8 * P + 36 * N * V,
         "loss": 48 * P,
         "blend_bwd": 8 * T + 40 * K + 32 * P + 36 * Nv,
         "blend_bwd_recompute": 8 * T + 40 * K + 24 * P + 36 * Nv,
         "project_backward": (236 + 48 + 236) * N,
         "project_backward_full": (236 + 48 + 236) * N,
         "adam_step": 7 * 236 * N + 8 * N,
-        "backward_adam_views": 236 * N + 48 * N * V + 6 * 236 * N + 8 * N,
+        "backward_adam_views": 236 * N + 36 * N * V + 6 * 236 * N + 8 * N,
         "ssim_moments": 32 * P + 60 * P,
         "ssim_grad": 60 * P + 32 * P + 16 * P,
     }

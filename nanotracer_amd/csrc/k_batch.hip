@@ -18,6 +18,9 @@
 //                          into Adam + constraints + grad_acc.
 // All buffers live in a preallocated workspace and nothing synchronises with
 // the host, so the whole step can be captured once in a hipGraph and replayed.
+#include <cstring>
+#include <vector>
+
 #include "nts_adam.h"
 #include "nts_bin.h"
 #include "nts_project.h"
@@ -30,7 +33,7 @@ struct nts_batch {
   uint64_t bytes = 0;
   DevCamera* d_cams = nullptr;
   Projected* proj = nullptr;
-  ProjectedGrad* pgrad = nullptr;
+  float* pgrad = nullptr;  // compact per-(view, splat) gradient records (NTS_GRAD9_WORDS words)
   uint32_t* counts = nullptr;  // exclusive offsets after the scan
   uint64_t* d_total = nullptr;
   uint64_t* seg_begin = nullptr;
@@ -241,12 +244,21 @@ struct AdamCfgs {
 // rotation, scale and opacity planes: no more than 23 parameters and sums
 // are live across a view loop, so several waves share each SIMD.
 #define ADAM_LDS_VIEWS 16
+// A compact gradient record (NTS_GRAD9_WORDS words) as the three float4s of
+// the ProjectedGrad layout.
+__device__ __forceinline__ ProjectedGrad load_grad9(const float* __restrict__ g) {
+  ProjectedGrad r;
+  r.a = make_float4(g[0], g[1], g[2], 0.0f);
+  r.b = make_float4(g[3], g[4], g[5], 0.0f);
+  r.c = make_float4(g[6], g[7], g[8], 0.0f);
+  return r;
+}
 template <bool FULL>
 __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__ P, float* __restrict__ G,
                                                              float* __restrict__ M, float* __restrict__ V,
                                                              float* __restrict__ grad_acc, uint32_t cap, uint32_t n,
                                                              uint32_t n_views, const DevCamera* __restrict__ cams,
-                                                             const ProjectedGrad* __restrict__ pgrad,
+                                                             const float* __restrict__ pgrad,
                                                              const uint64_t* __restrict__ d_step, AdamCfgs cfg,
                                                              int write_grads, const uint32_t* __restrict__ flags,
                                                              const uint32_t* __restrict__ sort_err,
@@ -297,10 +309,10 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
   __shared__ float s_col[3][ADAM_LDS_VIEWS][256];
   const bool stash = n_views <= ADAM_LDS_VIEWS;
   {
-    ProjectedGrad next = pgrad[i];  // the next view's record loads while this one is chained back
+    ProjectedGrad next = load_grad9(pgrad + (size_t)i * NTS_GRAD9_WORDS);  // the next view's record loads while this one is chained back
     for (uint32_t v = 0; v < n_views; ++v) {
       const ProjectedGrad pg = next;
-      if (v + 1 < n_views) next = pgrad[(size_t)(v + 1) * n + i];
+      if (v + 1 < n_views) next = load_grad9(pgrad + ((size_t)(v + 1) * n + i) * NTS_GRAD9_WORDS);
       if (pg.a.x == 0.0f && pg.a.y == 0.0f && pg.a.z == 0.0f && pg.b.x == 0.0f && pg.b.y == 0.0f &&
           pg.b.z == 0.0f && pg.c.x == 0.0f && pg.c.y == 0.0f && pg.c.z == 0.0f)
         continue;  // the view gave this splat no gradient: its terms are all zero
@@ -333,8 +345,7 @@ __global__ void __launch_bounds__(256) k_backward_adam_views(float* __restrict__
       if (stash) {
         dcol = s_col[ch][v][threadIdx.x];
       } else {
-        const float4 pc = pgrad[(size_t)v * n + i].c;
-        dcol = ch == 0 ? pc.x : (ch == 1 ? pc.y : pc.z);
+        dcol = pgrad[((size_t)v * n + i) * NTS_GRAD9_WORDS + 6 + ch];
       }
       project_backward_colour<FULL>(q[0], q[1], q[2], dc, rest, cams[v], dcol, gdc, grest, psh);
     }
@@ -390,7 +401,7 @@ extern "C" int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_s
   auto take = [&](uint64_t bytes) { const uint64_t o = off; off += al256(bytes); return o; };
   const uint64_t o_cams = take(sizeof(DevCamera) * max_views);
   const uint64_t o_proj = take(vn * sizeof(Projected));
-  const uint64_t o_pgrad = take(vn * sizeof(ProjectedGrad));
+  const uint64_t o_pgrad = take((vn * NTS_GRAD9_WORDS * 4 + 15) & ~15ull);  // whole float4s (the forward clears it)
   const uint64_t o_counts = take(vn * 4);
   const uint64_t o_total = take(8);
   const uint64_t o_seg = take(8 * (max_views + 1));
@@ -420,7 +431,7 @@ extern "C" int nts_batch_create(nts_ctx* ctx, uint32_t max_views, uint32_t max_s
   char* p = (char*)b->base;
   b->d_cams = (DevCamera*)(p + o_cams);
   b->proj = (Projected*)(p + o_proj);
-  b->pgrad = (ProjectedGrad*)(p + o_pgrad);
+  b->pgrad = (float*)(p + o_pgrad);
   b->counts = (uint32_t*)(p + o_counts);
   b->d_total = (uint64_t*)(p + o_total);
   b->seg_begin = (uint64_t*)(p + o_seg);
@@ -492,6 +503,7 @@ static int plan_batch(nts_batch* b, const nts_scene* scene, const nts_view_desc*
   pl.tile_size = tile_size;
   pl.n = scene->n;
   pl.vt.n = n_views;
+  pl.vt.grad_words = NTS_GRAD9_WORDS;
   pl.tt.n = n_views;
   uint64_t pix = 0, tiles = 0;
   uint32_t max_tiles_view = 1;
@@ -642,8 +654,7 @@ static int record_step(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const Batch
   // The projected gradients start at zero: cleared by the forward blend behind
   // its tiles (its HBM is mostly idle), or by a fill where that launch has no
   // loss epilogue or no tiles.
-  const uint64_t pgrad_n4 = (uint64_t)V * n * (sizeof(ProjectedGrad) / 16);
-  static_assert(sizeof(ProjectedGrad) % 16 == 0, "projected gradient records are whole float4s");
+  const uint64_t pgrad_n4 = ((uint64_t)V * n * NTS_GRAD9_WORDS + 3) / 4;  // within the allocation's whole float4s
   if (!ssim && pl.vt.total_tiles > 0) {
     la.clear = (float4*)b->pgrad;
     la.clear_n4 = pgrad_n4;
@@ -684,12 +695,12 @@ static int record_step(nts_ctx* ctx, nts_scene* scene, nts_batch* b, const Batch
     if (mode == NTS_BWD_RECOMPUTE)
       NTS_LAUNCH(ctx, "backward_adam_views", k_backward_adam_views<true>, dim3(nts_div_up(n, 256)), dim3(256), 0,
                  scene->params, scene->grads, scene->m, scene->v, scene->grad_acc, scene->cap, n, V,
-                 (const DevCamera*)b->d_cams, (const ProjectedGrad*)b->pgrad, (const uint64_t*)b->d_step, cfg,
+                 (const DevCamera*)b->d_cams, (const float*)b->pgrad, (const uint64_t*)b->d_step, cfg,
                  write_grads, (const uint32_t*)b->flags, (const uint32_t*)b->sort_err, b->sticky);
     else
       NTS_LAUNCH(ctx, "backward_adam_views", k_backward_adam_views<false>, dim3(nts_div_up(n, 256)), dim3(256), 0,
                  scene->params, scene->grads, scene->m, scene->v, scene->grad_acc, scene->cap, n, V,
-                 (const DevCamera*)b->d_cams, (const ProjectedGrad*)b->pgrad, (const uint64_t*)b->d_step, cfg,
+                 (const DevCamera*)b->d_cams, (const float*)b->pgrad, (const uint64_t*)b->d_step, cfg,
                  write_grads, (const uint32_t*)b->flags, (const uint32_t*)b->sort_err, b->sticky);
   }
   return NTS_OK;
@@ -903,10 +914,18 @@ extern "C" int nts_batch_download_state(nts_ctx* ctx, const nts_batch* b, uint32
 
 extern "C" int nts_batch_download_projected_grad(nts_ctx* ctx, const nts_batch* b, uint32_t view, uint32_t n,
                                                  float* host) {
-  NTS_CHECK_ARG(ctx && b && host && view < b->n_views);
-  NTS_HIP_TRY(hipMemcpyAsync(host, b->pgrad + (size_t)view * n, (size_t)n * sizeof(ProjectedGrad),
+  NTS_CHECK_ARG(ctx && b && host && view < b->n_views && n <= b->max_splats);
+  // the compact records, spread into the ProjectedGrad layout (zero padding words)
+  std::vector<float> g9((size_t)n * NTS_GRAD9_WORDS);
+  NTS_HIP_TRY(hipMemcpyAsync(g9.data(), b->pgrad + (size_t)view * n * NTS_GRAD9_WORDS, g9.size() * 4,
                              hipMemcpyDeviceToHost, ctx->stream));
   NTS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i < n; ++i) {
+    const float* g = &g9[i * NTS_GRAD9_WORDS];
+    float* o = host + i * NTS_GRAD_WORDS;
+    const float rec[12] = {g[0], g[1], g[2], 0.0f, g[3], g[4], g[5], 0.0f, g[6], g[7], g[8], 0.0f};
+    memcpy(o, rec, sizeof rec);
+  }
   return NTS_OK;
 }
 

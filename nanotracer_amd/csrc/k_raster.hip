@@ -720,7 +720,7 @@ __global__ void __launch_bounds__((Blend<TS, NTS_BWD_THREADS>::NT))
 __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE == NTS_BLEND_PERSIST ? 5 : NTS_BWD_OCC((Blend<TS, NTS_BWD_THREADS>::PPT))) : 1))) k_blend_bwd(
     const Projected* __restrict__ proj, const uint32_t* __restrict__ payloads, const uint2* __restrict__ ranges,
     ViewTable vt, const float4* __restrict__ fwd_out, const float* __restrict__ final_t,
-    const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, ProjectedGrad* __restrict__ pgrad) {
+    const uint32_t* __restrict__ last_idx, const float4* __restrict__ dl_dc, float* __restrict__ pgrad) {
   using BL = Blend<TS, NTS_BWD_THREADS>;
   constexpr int NT = BL::NT, PPT = BL::PPT, NM = BATCH / 64;
   constexpr bool PERSIST = MODE == NTS_BLEND_PERSIST, COUNT = MODE == NTS_BLEND_COUNT;
@@ -904,8 +904,8 @@ __attribute__((amdgpu_waves_per_eu(Blend<TS, NTS_BWD_THREADS>::NT == 256 ? (MODE
         sum = -sum;
       }
       if (sum != 0.0f) {
-        float* o = reinterpret_cast<float*>(pgrad + vd.proj_base + s_idx[k]);
-        unsafeAtomicAdd(o + (q < 3u ? q : (q < 6u ? q + 1u : q + 2u)), sum);
+        float* o = pgrad + (size_t)(vd.proj_base + s_idx[k]) * vt.grad_words;
+        unsafeAtomicAdd(o + (vt.grad_words == NTS_GRAD9_WORDS ? q : (q < 3u ? q : (q < 6u ? q + 1u : q + 2u))), sum);
       }
     }
     rem -= take;
@@ -1027,6 +1027,7 @@ static int fill_single_view(const nts_bins* bins, const nts_tiling* t, ViewTable
   vt.v[0].proj_base = 0;
   vt.v[0].pix_base = 0;
   vt.order = bins->order;
+  vt.grad_words = NTS_GRAD_WORDS;
   if (bins->tile_size != t->tile_size || bins->tiles_x != (t->width + t->tile_size - 1) / t->tile_size ||
       bins->tiles_y != (t->height + t->tile_size - 1) / t->tile_size)
     return nts_fail(NTS_E_INVALID, "tiling does not match the bins");
@@ -1127,7 +1128,7 @@ int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, cons
 template <int TS, int B, bool RECOMP>
 static int launch_bwd_kernel(nts_ctx* ctx, const char* name, const Projected* proj, const uint32_t* payloads,
                              const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
-                             const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
+                             const uint32_t* last_idx, const float4* dl_dc, float* pgrad) {
   constexpr int NT = Blend<TS, NTS_BWD_THREADS>::NT;
   if (vt.tickets) {
     const uint32_t r = resident_workgroups(k_blend_bwd<TS, B, RECOMP, NTS_BLEND_PERSIST>, NT);
@@ -1147,7 +1148,7 @@ static int launch_bwd_kernel(nts_ctx* ctx, const char* name, const Projected* pr
 template <int TS, int B>
 static int launch_bwd(nts_ctx* ctx, dim3 grid, int mode, const Projected* proj, const uint32_t* payloads,
                        const uint2* ranges, const ViewTable& vt, const float4* fwd_out, const float* final_t,
-                       const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
+                       const uint32_t* last_idx, const float4* dl_dc, float* pgrad) {
   if (mode == NTS_BWD_RECOMPUTE)
     return launch_bwd_kernel<TS, B, true>(ctx, "blend_bwd_recompute", proj, payloads, ranges, vt, fwd_out, final_t,
                                           last_idx, dl_dc, pgrad);
@@ -1159,7 +1160,7 @@ template <int TS>
 static int bwd_batch_switch(nts_ctx* ctx, uint32_t lds_batch, dim3 grid, int mode, const Projected* proj,
                             const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, const float4* fwd_out,
                             const float* final_t, const uint32_t* last_idx, const float4* dl_dc,
-                            ProjectedGrad* pgrad) {
+                            float* pgrad) {
   switch (lds_batch) {
     case 64: return launch_bwd<TS, 64>(ctx, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
     case 128: return launch_bwd<TS, 128>(ctx, grid, mode, proj, payloads, ranges, vt, fwd_out, final_t, last_idx, dl_dc, pgrad);
@@ -1170,7 +1171,7 @@ static int bwd_batch_switch(nts_ctx* ctx, uint32_t lds_batch, dim3 grid, int mod
 
 int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, int mode, const Projected* proj,
                        const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, const float4* fwd_out,
-                       const float* final_t, const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad) {
+                       const float* final_t, const uint32_t* last_idx, const float4* dl_dc, float* pgrad) {
   if (vt.total_tiles == 0) return NTS_OK;
   const dim3 grid(vt.total_tiles);
   switch (tile_size) {
@@ -1282,7 +1283,7 @@ extern "C" int nts_composite_backward_ex(nts_ctx* ctx, const nts_buffer* project
                             bins->ranges, vt, forward_out ? (const float4*)forward_out->ptr : nullptr,
                             final_t ? (const float*)final_t->ptr : nullptr,
                             last_idx ? (const uint32_t*)last_idx->ptr : nullptr, (const float4*)dl_dc->ptr,
-                            (ProjectedGrad*)projected_grad->ptr);
+                            (float*)projected_grad->ptr);
 }
 
 extern "C" int nts_composite_backward(nts_ctx* ctx, const nts_buffer* projected, const nts_bins* bins,

@@ -36,6 +36,11 @@ struct __align__(16) Projected {
 struct __align__(16) ProjectedGrad {
   float4 a, b, c;
 };
+// The batched step keeps its per-(view, splat) gradients in compact records
+// of 9 words, the ProjectedGrad terms without the padding words:
+// {dmean.x, dmean.y, dopacity, dconic.a, .b, .c, dr, dg, db}.
+#define NTS_GRAD_WORDS 12u
+#define NTS_GRAD9_WORDS 9u
 
 struct DevCamera {  // the 96-byte camera, passed by value to kernels
   float view[16];

@@ -45,6 +45,9 @@ struct ViewTable {
   // blended (pixel, splat) pairs, [2] (splat, wave) evaluations of the
   // backward, [3] its contributing (pixel, splat) pairs.
   unsigned long long* counts;
+  // Words per projected-gradient record the backward adds into:
+  // NTS_GRAD_WORDS (the ProjectedGrad layout) or NTS_GRAD9_WORDS (compact).
+  uint32_t grad_words;
   ViewDesc v[NTS_MAX_VIEWS];
 };
 
@@ -120,9 +123,10 @@ int nts_tile_order(nts_ctx* ctx, const uint2* ranges, uint32_t n_tiles, uint32_t
 int nts_blend_forward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, const Projected* proj,
                       const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, float4* out, float* final_t,
                       uint32_t* last_idx, const LossArgs* loss);
+// pgrad: records of vt.grad_words words per (view, splat), added into.
 int nts_blend_backward(nts_ctx* ctx, uint32_t tile_size, uint32_t lds_batch, int mode, const Projected* proj,
                        const uint32_t* payloads, const uint2* ranges, const ViewTable& vt, const float4* fwd_out,
-                       const float* final_t, const uint32_t* last_idx, const float4* dl_dc, ProjectedGrad* pgrad);
+                       const float* final_t, const uint32_t* last_idx, const float4* dl_dc, float* pgrad);
 
 // ---- combined MSE + SSIM loss (k_ssim.hip) ---------------------------------
 // Normalised 11-tap Gaussian window (sigma 1.5), taps computed on the host.

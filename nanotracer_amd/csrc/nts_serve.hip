@@ -284,6 +284,7 @@ int launch_batch(nts_server* sv, Slot& sl) {
   memset(&rt, 0, sizeof rt);
   memset(&tt, 0, sizeof tt);
   memset(&vt, 0, sizeof vt);
+  vt.grad_words = NTS_GRAD_WORDS;
   memset(&tb, 0, sizeof tb);
   rt.n = tt.n = vt.n = V;
   tb.n = V + 1;
