@@ -172,8 +172,9 @@ static inline uint32_t rs_max_chunks(uint32_t num_segs, uint64_t max_pairs) {
   return (uint32_t)((max_pairs + RS_TILE - 1) / RS_TILE / RS_CHUNK + 2ull * num_segs + 1);
 }
 
-// Digit histograms of every pass for every segment (pass ps sorts bits
-// [lo_bit + 8 ps, lo_bit + 8 ps + 8)), and, where cnt is given, pass 0's
+// Digit histograms of every pass for every segment (pass 0 sorts bits
+// [lo_bit, lo_bit + first_bits), pass ps > 0 the 8 bits from
+// lo_bit + first_bits + 8 (ps - 1)), and, where cnt is given, pass 0's
 // count table: cnt[p][d] = the keys of digit d in partition p.  A fixed grid
 // takes contiguous runs of partitions (the live count is only known on the
 // device, so a launch sized for the capacity costs nothing for empty
@@ -194,8 +195,8 @@ __device__ __forceinline__ void hist_load(const K* __restrict__ keys, uint64_t b
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                         const uint32_t* __restrict__ part_begin, uint32_t num_segs,
-                                                        uint32_t lo_bit, uint32_t passes, uint32_t* __restrict__ hist,
-                                                        uint32_t* __restrict__ cnt) {
+                                                        uint32_t lo_bit, uint32_t first_bits, uint32_t passes,
+                                                        uint32_t* __restrict__ hist, uint32_t* __restrict__ cnt) {
   __shared__ uint32_t s_hist[8 * RS_RADIX];
   __shared__ uint32_t s_part[2][RS_RADIX];
   const uint32_t total_parts = part_begin[num_segs];
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
   if (p0 >= p1) return;
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t nbins = passes * RS_RADIX;
+  const uint32_t mask0 = (1u << first_bits) - 1u;
   const auto flush = [&](uint32_t seg) {
     __syncthreads();
     uint32_t* h = hist + (size_t)seg * nbins;
@@ -241,18 +243,21 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
       const uint64_t vmask = __ballot(valid);
       if (vmask == 0) continue;
       const uint32_t n_valid = (uint32_t)__popcll(vmask);
-      for (uint32_t ps = 0; ps < passes; ++ps) {
-        const uint32_t d = (uint32_t)((key[k] >> (lo_bit + 8 * ps)) & 255u);
+      // runs of equal digits: the run's first lane adds its length
+      const auto add_runs = [&](uint32_t d, uint32_t* h, uint32_t* h2) {
         const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, 64);
         const bool head = valid && (lane == 0 || prev != d);
         const uint64_t heads = __ballot(head);
         if (head) {
           const uint64_t above = lane < 63u ? heads & (~0ull << (lane + 1u)) : 0ull;
           const uint32_t next = above ? (uint32_t)__builtin_ctzll(above) : n_valid;
-          atomicAdd(&s_hist[ps * RS_RADIX + d], next - lane);
-          if (ps == 0 && cnt) atomicAdd(&row[d], next - lane);
+          atomicAdd(&h[d], next - lane);
+          if (h2) atomicAdd(&h2[d], next - lane);
         }
-      }
+      };
+      add_runs((uint32_t)(key[k] >> lo_bit) & mask0, s_hist, cnt ? row : nullptr);
+      for (uint32_t ps = 1; ps < passes; ++ps)
+        add_runs((uint32_t)(key[k] >> (lo_bit + first_bits + 8 * (ps - 1))) & 255u, s_hist + ps * RS_RADIX, nullptr);
     }
     if (cnt) {
       __syncthreads();
@@ -396,7 +401,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
                                                        K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                        const uint64_t* __restrict__ seg_begin,
                                                        const uint32_t* __restrict__ part_begin, uint32_t num_segs,
-                                                       uint32_t shift, uint32_t passes, uint32_t pass,
+                                                       uint32_t shift, uint32_t dmask, uint32_t passes, uint32_t pass,
                                                        const uint32_t* __restrict__ offsets, uint32_t* status,
                                                        uint32_t* part_counter, uint32_t* err,
                                                        const uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ tctot,
@@ -444,7 +449,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
     }
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
-      const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+      const uint32_t d = (uint32_t)(key[r] >> shift) & dmask;
       // lanes with this lane's digit: the AND over the digit's bits of the
       // bit's ballot, complemented where this lane's bit is clear, kept as
       // two 32-bit halves (neg is the bit sign-extended: 0 or all ones)
@@ -540,7 +545,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
     // ---- reorder in LDS by digit, then write digit runs contiguously ----
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
-      const uint32_t dd = (uint32_t)(key[r] >> shift) & 255u;
+      const uint32_t dd = (uint32_t)(key[r] >> shift) & dmask;
       const uint32_t lp = s_excl[dd] + s_wave_hist[wave][dd] + rank[r];
       s_keys[lp] = key[r];
       s_vals[lp] = val[r];
@@ -552,7 +557,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
       const uint32_t j = (uint32_t)r * RS_BLOCK + tid;
       if (j < valid_n) {
         const K k = s_keys[j];
-        const uint32_t dd = (uint32_t)(k >> shift) & 255u;
+        const uint32_t dd = (uint32_t)(k >> shift) & dmask;
         const uint32_t pos = s_gbase[dd] + j;
         if ((uint64_t)pos < end) {  // always true unless a look-back timed out
           keys_out[pos] = k;
@@ -570,8 +575,9 @@ uint64_t nts_sort_ws_bytes(uint32_t num_segs, uint64_t max_pairs, uint32_t key_b
   return rs_ws_bytes(num_segs, max_pairs, key_bytes);
 }
 
-// Launch the onesweep passes over bits [lo_bit, lo_bit + 8 passes) of pairs
-// of segments (result in keys/vals, or in the alt buffers when
+// Launch the onesweep passes over bits [lo_bit, lo_bit + first_bits +
+// 8 (passes - 1)) of pairs of segments (pass 0 sorts the first first_bits
+// bits, each later pass 8; result in keys/vals, or in the alt buffers when
 // *result_in_alt is set) with no host synchronisation.  seg_begin (device,
 // num_segs + 1 entries) bounds the segments; max_pairs bounds
 // seg_begin[num_segs].  A look-back that never completes sets *err (checked
@@ -581,12 +587,15 @@ template <typename K>
 static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt,
                        const uint64_t* d_seg_begin, uint32_t num_segs, uint64_t max_pairs, uint32_t lo_bit,
                        uint32_t passes, void* ws, bool* result_in_alt, uint32_t** err_out,
-                       const uint32_t** offsets_out = nullptr, uint32_t** counters_out = nullptr) {
+                       const uint32_t** offsets_out = nullptr, uint32_t** counters_out = nullptr,
+                       uint32_t first_bits = 8) {
   *result_in_alt = false;
   if (num_segs == 0 || num_segs > RS_MAX_SEGS) return nts_fail(NTS_E_INVALID, "segments %u", num_segs);
   if (max_pairs >= (1ull << 30)) return nts_fail(NTS_E_CAPACITY, "sort supports < 2^30 pairs (%llu)",
                                                  (unsigned long long)max_pairs);
-  if (lo_bit + 8 * passes > 8 * sizeof(K)) return nts_fail(NTS_E_INVALID, "sort bits beyond the key");
+  if (first_bits == 0 || first_bits > 8) return nts_fail(NTS_E_INVALID, "first digit of %u bits", first_bits);
+  if (passes && lo_bit + first_bits + 8 * (passes - 1) > 8 * sizeof(K))
+    return nts_fail(NTS_E_INVALID, "sort bits beyond the key");
   const uint32_t max_parts = nts_div_up(max_pairs, RS_TILE) + num_segs;
   const uint32_t max_chunks = rs_max_chunks(num_segs, max_pairs);
   const uint64_t n_hist = (uint64_t)num_segs * sizeof(K) * RS_RADIX;
@@ -616,7 +625,8 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   const uint32_t sgrid = nts_div_up(num_segs * RS_RADIX, 4);  // one wave per (segment, digit)
   // the segment histograms of every pass, and pass 0's count table
   NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
-             (const uint32_t*)part_begin, num_segs, lo_bit, passes, hist, (table & 1u) ? status : (uint32_t*)nullptr);
+             (const uint32_t*)part_begin, num_segs, lo_bit, first_bits, passes, hist,
+             (table & 1u) ? status : (uint32_t*)nullptr);
   NTS_LAUNCH(ctx, "sort_offsets", k_sort_offsets, dim3(num_segs * passes), dim3(RS_RADIX), 0, (const uint32_t*)hist,
              d_seg_begin, passes, offsets);
   K* src_k = keys;
@@ -625,22 +635,24 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   uint32_t* dst_v = vals_alt;
   for (uint32_t ps = 0; ps < passes; ++ps) {
     uint32_t* st = status + (size_t)ps * max_parts * RS_RADIX;
+    const uint32_t shift = ps ? lo_bit + first_bits + 8 * (ps - 1) : lo_bit;
+    const uint32_t dmask = ps ? 255u : (1u << first_bits) - 1u;
     if ((table >> ps) & 1u) {
       if (ps > 0)  // the count table of this pass's digit over the previous pass's output
         NTS_LAUNCH(ctx, "sort_part_count", k_part_count<K>, dim3(max_parts), dim3(RS_BLOCK), 0, (const K*)src_k,
-                   d_seg_begin, (const uint32_t*)part_begin, num_segs, lo_bit + 8 * ps, st);
+                   d_seg_begin, (const uint32_t*)part_begin, num_segs, shift, st);
       NTS_LAUNCH(ctx, "sort_chunk_prefix", k_chunk_prefix, dim3(cgrid), dim3(RS_RADIX), 0, st,
                  (const uint32_t*)part_begin, (const uint32_t*)chunk_begin, num_segs, ctot, max_chunks);
       NTS_LAUNCH(ctx, "sort_chunk_scan", k_chunk_scan, dim3(sgrid), dim3(256), 0, ctot,
                  (const uint32_t*)chunk_begin, num_segs, (const uint32_t*)offsets, passes, ps, max_chunks);
       NTS_LAUNCH(ctx, "sort_onesweep", (k_onesweep<K, true>), dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
                  (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs,
-                 lo_bit + 8 * ps, passes, ps, (const uint32_t*)offsets, st, counters + ps, err, (const uint32_t*)st,
+                 shift, dmask, passes, ps, (const uint32_t*)offsets, st, counters + ps, err, (const uint32_t*)st,
                  (const uint32_t*)ctot, (const uint32_t*)chunk_begin, max_chunks);
     } else {
       NTS_LAUNCH(ctx, "sort_onesweep", (k_onesweep<K, false>), dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
                  (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs,
-                 lo_bit + 8 * ps, passes, ps, (const uint32_t*)offsets, st, counters + ps, err,
+                 shift, dmask, passes, ps, (const uint32_t*)offsets, st, counters + ps, err,
                  (const uint32_t*)nullptr, (const uint32_t*)nullptr, (const uint32_t*)chunk_begin, max_chunks);
     }
     K* tk = src_k; src_k = dst_k; dst_k = tk;
@@ -1194,13 +1206,16 @@ static int bin_sort_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
     return nts_fail(NTS_E_INVALID, "%u tile bits do not fit %u-byte keys", tile_bits, (unsigned)sizeof(K));
   const bool by_digit = tile_bits <= 8 + BS_GMAX;
   const uint32_t g = by_digit && tile_bits > 8 ? tile_bits - 8 : 0;
-  uint32_t passes = by_digit ? 1 : (tile_bits + 7) / 8;
-  if (16 + 8 * passes > 8 * sizeof(K)) passes = (8 * (uint32_t)sizeof(K) - 16) / 8;
+  // Exactly the tile bits: the first pass takes the remainder of a whole
+  // number of bytes (fewer digits, so longer runs of each digit per block
+  // and fuller cache lines in its scatter), the later passes 8 bits each.
+  const uint32_t passes = by_digit ? 1 : (tile_bits + 7) / 8;
+  const uint32_t first_bits = by_digit ? 8 : tile_bits - 8 * (passes - 1);
   const uint32_t* offsets = nullptr;
   bool alt = false;
   uint32_t* counters = nullptr;
   int rc = onesweep_ws<K>(ctx, keys, vals, keys_alt, vals_alt, d_seg_begin, num_segs, max_pairs, 16 + g, passes, ws,
-                          &alt, err_out, &offsets, &counters);
+                          &alt, err_out, &offsets, &counters, first_bits);
   if (rc) return rc;
   *result_in_alt = alt;
   if (max_pairs == 0) return nts_tile_order(ctx, ranges, total_tiles, order);
