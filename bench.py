@@ -107,16 +107,17 @@ def kernel_table(rep, N, Nv, K, P, T, V):
 # read on the 4 GiB copy kernel of the peak probe) at the (kernel, tile size, that kernel's LDS batch)
 # it was profiled at, with the pair count of that run (the steps with the default culled binning);
 # profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (4.8927e+09, 45_319_957),
-                    ("blend_fwd_loss", 32, 512): (4.6413e+09, 45_319_957)}
+PROFILED_TRAFFIC = {("blend_bwd_recompute", 32, 128): (4.8937e+09, 45_319_957),
+                    ("blend_fwd_loss", 32, 512): (4.4507e+09, 45_319_957)}
 # Wave-level VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same profile and pair count).
-PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.3063e9, 45_319_957),
-                 ("blend_fwd_loss", 32, 512): (1.9748e9, 45_319_957)}
+PROFILED_VALU = {("blend_bwd_recompute", 32, 128): (3.2419e9, 45_319_957),
+                 ("blend_fwd_loss", 32, 512): (1.9617e9, 45_319_957)}
 # LDS bytes per launch moved by the lanes (rocprofv3 SQ_INSTS_LDS_LOAD_BANDWIDTH + _STORE_BANDWIDTH +
 # _ATOMIC_BANDWIDTH, 64-byte units) and the share of the kernel's cycles the LDS spends on indexed
-# operations (SQ_LDS_IDX_ACTIVE over 256 CUs), same profile; profiles/hbm_counters_train_step_1M_16views.md.
-PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((600067089.1 + 15984651.8 + 27816890.2) * 64, 45_319_957, 0.660),
-                ("blend_fwd_loss", 32, 512): ((1957269199.0 + 13701201.5) * 64, 45_319_957, 0.403)}
+# operations (SQ_LDS_IDX_ACTIVE over 256 CUs against GRBM_GUI_ACTIVE over 8 XCDs), same profile;
+# profiles/hbm_counters_train_step_1M_16views.md.
+PROFILED_LDS = {("blend_bwd_recompute", 32, 128): ((543362000.0 + 15983100.0 + 27815900.0) * 64, 45_319_957, 0.670),
+                ("blend_fwd_loss", 32, 512): ((1957190000.0 + 13700600.0) * 64, 45_319_957, 0.405)}
 N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 
 
