@@ -607,7 +607,9 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   uint32_t* err = counters + 32;
   uint32_t* status = counters + 64;  // per pass: look-back words, or the count table
   uint32_t* ctot = status + (size_t)sizeof(K) * max_parts * RS_RADIX;
-  const uint32_t table = RS_TABLE_MASK & ((1u << passes) - 1u);
+  // Count tables only for large sorts: below RS_TABLE_MIN_PAIRS their extra
+  // launches cost more than the look-back they replace.
+  const uint32_t table = max_pairs >= RS_TABLE_MIN_PAIRS ? RS_TABLE_MASK & ((1u << passes) - 1u) : 0u;
   if (err_out) *err_out = err;
   if (offsets_out) *offsets_out = offsets;
   if (counters_out) *counters_out = counters;  // words 40-63 are free for the caller

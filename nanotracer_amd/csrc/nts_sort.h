@@ -37,6 +37,9 @@
 #ifndef RS_TABLE_MASK
 #define RS_TABLE_MASK 3u  // passes (bit ps) whose block offsets come from count tables, not the look-back
 #endif
+#ifndef RS_TABLE_MIN_PAIRS
+#define RS_TABLE_MIN_PAIRS (16ull << 20)  // sorts sized for fewer pairs keep the look-back in every pass
+#endif
 
 // Workspace bytes of the segmented sort for max_pairs pairs in num_segs
 // segments and keys of key_bytes bytes (so up to key_bytes passes): segment

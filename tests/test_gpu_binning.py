@@ -178,6 +178,49 @@ def test_tile_sort_u64_keys(ctx):
     assert np.array_equal(sr, _ranges_u64(live, num_tiles))
 
 
+# Sorts of 16M pairs or more take their onesweep offsets from per-partition
+# digit-count tables (passes 0 and 1) instead of the look-back; the smaller
+# cases above keep the look-back in every pass.
+TABLE_MIN_PAIRS = 16 << 20
+
+
+def test_tile_sort_count_tables_u32(ctx):
+    # 31,100 tiles (15 tile bits: a 7-bit first pass, an 8-bit second)
+    rng = np.random.default_rng(31100)
+    num_tiles, n = 31_100, TABLE_MIN_PAIRS + 700_001
+    keys = _tile_keys(rng, n, num_tiles, 0.2, 0.05, np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv, sr = _tile_sort(ctx, keys, vals, 15, num_tiles)
+    ok, ov = O.sort_pairs(keys, vals)
+    assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
+    assert np.array_equal(sr, O.tile_ranges(ok, num_tiles))
+
+
+def test_tile_sort_count_tables_u64(ctx):
+    # 17 tile bits in 64-bit keys: a 1-bit first pass and two 8-bit passes,
+    # the last with the look-back
+    rng = np.random.default_rng(129_600)
+    num_tiles, n = 129_600, TABLE_MIN_PAIRS + 3
+    keys = _tile_keys(rng, n, num_tiles, 0.1, 0.05, np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv, sr = _tile_sort(ctx, keys, vals, 17, num_tiles)
+    ok, ov = O.sort_pairs(keys, vals)
+    assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
+    live = ok[ok != np.uint64(2**64 - 1)]
+    assert np.array_equal(sr, _ranges_u64(live, num_tiles))
+
+
+def test_sort_u32_count_tables_vs_oracle(ctx):
+    n = TABLE_MIN_PAIRS + 4097
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    keys[: n // 4] = keys[n // 4: 2 * (n // 4)]  # ties
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv = _sort(ctx, keys, vals, 32)
+    ok, ov = O.sort_pairs(keys, vals)
+    assert np.array_equal(sk, ok) and np.array_equal(sv, ov)
+
+
 @pytest.mark.parametrize("n", [0, 1, 2, 4097])
 def test_tile_sort_small_and_edge_sizes(ctx, n):
     rng = np.random.default_rng(n)
