@@ -255,7 +255,15 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
           if (h2) atomicAdd(&h2[d], next - lane);
         }
       };
-      add_runs((uint32_t)(key[k] >> lo_bit) & mask0, s_hist, cnt ? row : nullptr);
+      if (RS_HIST_DIRECT0) {  // the first digit's runs are short (neighbouring tiles): one add per key
+        if (valid) {
+          const uint32_t d0 = (uint32_t)(key[k] >> lo_bit) & mask0;
+          atomicAdd(&s_hist[d0], 1u);
+          if (cnt) atomicAdd(&row[d0], 1u);
+        }
+      } else {
+        add_runs((uint32_t)(key[k] >> lo_bit) & mask0, s_hist, cnt ? row : nullptr);
+      }
       for (uint32_t ps = 1; ps < passes; ++ps)
         add_runs((uint32_t)(key[k] >> (lo_bit + first_bits + 8 * (ps - 1))) & 255u, s_hist + ps * RS_RADIX, nullptr);
     }
@@ -269,49 +277,40 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
 }
 
 // The count table of the digit at bits [shift, shift + 8) over keys in
-// partition order: cnt[p][d] = the keys of digit d in partition p.  Lanes of
-// a wave with the same digit are found with ballots, as the onesweep ranks
-// them, and their leader adds their number (the digits of a sorted pass's
-// output repeat without forming runs).  Two LDS rows in turn, as in
-// k_sort_hist.
+// partition order: cnt[p][d] = the keys of digit d in partition p (the
+// digits of a sorted pass's output repeat without forming runs, so each key
+// adds one into a private copy of the histogram).
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_part_count(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                          const uint32_t* __restrict__ part_begin, uint32_t num_segs,
                                                          uint32_t shift, uint32_t* __restrict__ cnt) {
-  __shared__ uint32_t s_part[2][RS_RADIX];
+  // RS_PC_COPIES private copies of the histogram, lane l adding into copy
+  // l % RS_PC_COPIES; rows padded to 257 words so that the copies of one
+  // digit sit in different banks
+  constexpr uint32_t C = RS_PC_COPIES, ROW = RS_RADIX + 1;
+  __shared__ uint32_t s_h[C * ROW];
   const uint32_t total_parts = part_begin[num_segs];
-  const uint32_t tid = threadIdx.x, lane = tid & 63u;
-  s_part[0][tid] = 0;
-  s_part[1][tid] = 0;
-  __syncthreads();
-  uint32_t it = 0;
-  for (uint32_t p = blockIdx.x; p < total_parts; p += gridDim.x, ++it) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t* mine = s_h + (tid & (C - 1u)) * ROW;
+  for (uint32_t p = blockIdx.x; p < total_parts; p += gridDim.x) {
+    for (uint32_t w = tid; w < C * ROW; w += RS_BLOCK) s_h[w] = 0;
     const uint32_t s = seg_of<uint32_t>(part_begin, num_segs, p);
     const uint64_t base = seg_begin[s] + (uint64_t)(p - part_begin[s]) * RS_TILE;
     const uint64_t end = seg_begin[s + 1];
-    uint32_t* row = s_part[it & 1u];
     K key[RS_ITEMS];
     hist_load<K>(keys, base, end, tid, key);
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; ++k) {
       const uint64_t i = base + (uint64_t)k * RS_BLOCK + tid;
-      const uint64_t vb = __ballot(i < end);
-      if (vb == 0) continue;
-      const uint32_t d = (uint32_t)((key[k] >> shift) & 255u);
-      uint32_t mlo = (uint32_t)vb, mhi = (uint32_t)(vb >> 32);
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const uint32_t neg = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
-        const uint64_t bal = __ballot(neg != 0u);
-        mlo &= ~((uint32_t)bal ^ neg);
-        mhi &= ~((uint32_t)(bal >> 32) ^ neg);
-      }
-      const uint64_t match = i < end ? (((uint64_t)mhi << 32) | mlo) : 0ull;
-      if (match && lane == (uint32_t)__builtin_ctzll(match)) atomicAdd(&row[d], (uint32_t)__popcll(match));
+      if (i < end) atomicAdd(&mine[(uint32_t)((key[k] >> shift) & 255u)], 1u);
     }
     __syncthreads();
-    cnt[(size_t)p * RS_RADIX + tid] = row[tid];
-    row[tid] = 0;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < C; ++c) sum += s_h[c * ROW + tid];
+    cnt[(size_t)p * RS_RADIX + tid] = sum;
+    __syncthreads();  // the copies are cleared for the next partition
   }
 }
 

@@ -37,6 +37,12 @@
 #ifndef RS_TABLE_MASK
 #define RS_TABLE_MASK 3u  // passes (bit ps) whose block offsets come from count tables, not the look-back
 #endif
+#ifndef RS_PC_COPIES
+#define RS_PC_COPIES 4  // private histogram copies of the partition count (a power of two)
+#endif
+#ifndef RS_HIST_DIRECT0
+#define RS_HIST_DIRECT0 1  // the histogram pass adds the first digit key by key, not by runs
+#endif
 #ifndef RS_TABLE_MIN_PAIRS
 #define RS_TABLE_MIN_PAIRS (16ull << 20)  // sorts sized for fewer pairs keep the look-back in every pass
 #endif
