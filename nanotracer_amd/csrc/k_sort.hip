@@ -199,6 +199,10 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
                                                         uint32_t* __restrict__ hist, uint32_t* __restrict__ cnt) {
   __shared__ uint32_t s_hist[8 * RS_RADIX];
   __shared__ uint32_t s_part[2][RS_RADIX];
+  // the second digit, key by key into private copies (lane l into copy
+  // l % 4, rows of 257 words: one digit's copies in different banks)
+  constexpr uint32_t C1 = RS_HIST_DIRECT1 ? 4 : 1, ROW1 = RS_RADIX + 1;
+  __shared__ uint32_t s_h1[C1 * ROW1];
   const uint32_t total_parts = part_begin[num_segs];
   const uint32_t per = (total_parts + gridDim.x - 1) / gridDim.x;
   const uint32_t p0 = blockIdx.x * per;
@@ -207,17 +211,27 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t nbins = passes * RS_RADIX;
   const uint32_t mask0 = (1u << first_bits) - 1u;
+  const bool direct1 = RS_HIST_DIRECT1 && passes > 1;
+  uint32_t* mine1 = s_h1 + (tid & (C1 - 1u)) * ROW1;
   const auto flush = [&](uint32_t seg) {
     __syncthreads();
     uint32_t* h = hist + (size_t)seg * nbins;
     for (uint32_t i = tid; i < nbins; i += RS_BLOCK) {
-      const uint32_t c = s_hist[i];
+      uint32_t c = s_hist[i];
+      if (direct1 && i >= RS_RADIX && i < 2 * RS_RADIX) {
+#pragma unroll
+        for (uint32_t q = 0; q < C1; ++q) {
+          c += s_h1[q * ROW1 + i - RS_RADIX];
+          s_h1[q * ROW1 + i - RS_RADIX] = 0;
+        }
+      }
       if (c) atomicAdd(&h[i], c);
       s_hist[i] = 0;
     }
     __syncthreads();
   };
   for (uint32_t i = tid; i < nbins; i += RS_BLOCK) s_hist[i] = 0;
+  for (uint32_t i = tid; i < C1 * ROW1; i += RS_BLOCK) s_h1[i] = 0;
   s_part[0][tid] = 0;
   s_part[1][tid] = 0;
   __syncthreads();
@@ -232,10 +246,11 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
     uint32_t* row = s_part[p & 1u];
     K key[RS_ITEMS];
     hist_load<K>(keys, base, end, tid, key);
-    // Neighbouring pairs mostly come from one record (same depth, neighbouring
-    // tiles), so a wave's digits come in runs: each run's first lane adds the
-    // run's length, one LDS atomic per run instead of one per key.  The valid
-    // lanes of a wave are a prefix (consecutive indices).
+    // The first digit (neighbouring tiles differ in it) and the second add
+    // key by key, the second into private copies; the digits of later
+    // passes come in runs in emit order (a record's pairs share them), so
+    // each run's first lane adds the run's length, one LDS atomic per run.
+    // The valid lanes of a wave are a prefix (consecutive indices).
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; ++k) {
       const uint64_t i = base + (uint64_t)k * RS_BLOCK + tid;
@@ -264,8 +279,14 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
       } else {
         add_runs((uint32_t)(key[k] >> lo_bit) & mask0, s_hist, cnt ? row : nullptr);
       }
-      for (uint32_t ps = 1; ps < passes; ++ps)
-        add_runs((uint32_t)(key[k] >> (lo_bit + first_bits + 8 * (ps - 1))) & 255u, s_hist + ps * RS_RADIX, nullptr);
+      for (uint32_t ps = 1; ps < passes; ++ps) {
+        const uint32_t d = (uint32_t)(key[k] >> (lo_bit + first_bits + 8 * (ps - 1))) & 255u;
+        if (ps == 1 && direct1) {
+          if (valid) atomicAdd(&mine1[d], 1u);
+        } else {
+          add_runs(d, s_hist + ps * RS_RADIX, nullptr);
+        }
+      }
     }
     if (cnt) {
       __syncthreads();

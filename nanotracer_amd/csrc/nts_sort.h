@@ -43,6 +43,9 @@
 #ifndef RS_HIST_DIRECT0
 #define RS_HIST_DIRECT0 1  // the histogram pass adds the first digit key by key, not by runs
 #endif
+#ifndef RS_HIST_DIRECT1
+#define RS_HIST_DIRECT1 1  // ... and the second key by key into private copies
+#endif
 #ifndef RS_TABLE_MIN_PAIRS
 #define RS_TABLE_MIN_PAIRS (16ull << 20)  // sorts sized for fewer pairs keep the look-back in every pass
 #endif
