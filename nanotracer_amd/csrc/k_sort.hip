@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
 template <typename K>
 __global__ void __launch_bounds__(RS_BLOCK) k_part_count(const K* __restrict__ keys, const uint64_t* __restrict__ seg_begin,
                                                          const uint32_t* __restrict__ part_begin, uint32_t num_segs,
-                                                         uint32_t shift, uint32_t* __restrict__ cnt) {
+                                                         uint32_t shift, uint32_t dmask, uint32_t* __restrict__ cnt) {
   // RS_PC_COPIES private copies of the histogram, lane l adding into copy
   // l % RS_PC_COPIES; rows padded to 257 words so that the copies of one
   // digit sit in different banks
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_part_count(const K* __restrict__ k
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; ++k) {
       const uint64_t i = base + (uint64_t)k * RS_BLOCK + tid;
-      if (i < end) atomicAdd(&mine[(uint32_t)((key[k] >> shift) & 255u)], 1u);
+      if (i < end) atomicAdd(&mine[(uint32_t)(key[k] >> shift) & dmask], 1u);
     }
     __syncthreads();
     uint32_t sum = 0;
@@ -363,16 +363,18 @@ __global__ void __launch_bounds__(RS_RADIX) k_chunk_prefix(uint32_t* __restrict_
 }
 
 // Chunk bases of one pass: ctot[d][c] (the keys of digit d in chunk c) becomes
-// offsets[s][pass][d] plus the keys of digit d in the chunks of segment s
-// before c.  One wave per (segment, digit), 64 chunks per step.
+// the keys of digit d in the chunks of segment s before c, and, where hist is
+// given, hist[s][pass][d] receives the segment's keys of digit d (the digit
+// histogram the offsets come from).  One wave per (segment, digit), 64
+// chunks per step.
 __global__ void __launch_bounds__(256) k_chunk_scan(uint32_t* __restrict__ ctot, const uint32_t* __restrict__ chunk_begin,
-                                                    uint32_t num_segs, const uint32_t* __restrict__ offsets,
+                                                    uint32_t num_segs, uint32_t* __restrict__ hist,
                                                     uint32_t passes, uint32_t pass, uint32_t max_chunks) {
   const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
   const uint32_t s = wave >> 8, d = wave & 255u;
   if (s >= num_segs) return;
   const uint32_t cb = chunk_begin[s], ce = chunk_begin[s + 1];
-  uint32_t carry = offsets[((size_t)s * passes + pass) * RS_RADIX + d];
+  uint32_t carry = 0;
   uint32_t* col = ctot + (size_t)d * max_chunks;
   for (uint32_t c = cb; c < ce; c += 64u) {
     const uint32_t i = c + lane;
@@ -381,6 +383,7 @@ __global__ void __launch_bounds__(256) k_chunk_scan(uint32_t* __restrict__ ctot,
     if (i < ce) col[i] = carry + incl - v;
     carry += __shfl(incl, 63, 64);
   }
+  if (hist && lane == 0) hist[((size_t)s * passes + pass) * RS_RADIX + d] = carry;
 }
 
 // Clears the look-back status words of the live partitions of every pass
@@ -400,12 +403,14 @@ __global__ void __launch_bounds__(256) k_clear_status(uint32_t* __restrict__ sta
 }
 
 // offsets[s][pass][d] = seg_begin[s] + sum of hist[s][pass][d' < d].
+// One block per (segment, pass) of passes [first_pass, first_pass + npass).
 __global__ void __launch_bounds__(RS_RADIX) k_sort_offsets(const uint32_t* __restrict__ hist,
                                                            const uint64_t* __restrict__ seg_begin, uint32_t passes,
+                                                           uint32_t first_pass, uint32_t npass,
                                                            uint32_t* __restrict__ offsets) {
   __shared__ uint32_t s_wave[4];
-  const uint32_t s = blockIdx.x / passes;
-  const uint32_t idx = blockIdx.x * RS_RADIX + threadIdx.x;
+  const uint32_t s = blockIdx.x / npass, ps = first_pass + blockIdx.x % npass;
+  const uint32_t idx = ((s * passes) + ps) * RS_RADIX + threadIdx.x;
   uint32_t total;
   const uint32_t ex = block_excl_scan<RS_RADIX>(hist[idx], s_wave, total);
   offsets[idx] = (uint32_t)seg_begin[s] + ex;
@@ -506,7 +511,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
     s_excl[d] = ex;
 
     if (TABLE) {
-      s_gbase[d] = tbase - ex;
+      s_gbase[d] = offsets[((size_t)s * passes + pass) * RS_RADIX + d] + tbase - ex;
     } else {
     // ---- decoupled look-back over the preceding partitions of this segment ----
     uint32_t* st = status;  // this pass's status words (offset by the host)
@@ -645,12 +650,18 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
   const uint32_t grid = max_parts < RS_GRID ? max_parts : RS_GRID;
   const uint32_t cgrid = max_chunks < RS_GRID ? max_chunks : RS_GRID;
   const uint32_t sgrid = nts_div_up(num_segs * RS_RADIX, 4);  // one wave per (segment, digit)
-  // the segment histograms of every pass, and pass 0's count table
-  NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
-             (const uint32_t*)part_begin, num_segs, lo_bit, first_bits, passes, hist,
-             (table & 1u) ? status : (uint32_t*)nullptr);
-  NTS_LAUNCH(ctx, "sort_offsets", k_sort_offsets, dim3(num_segs * passes), dim3(RS_RADIX), 0, (const uint32_t*)hist,
-             d_seg_begin, passes, offsets);
+  // With every pass on count tables, each pass's digit histogram is its
+  // table's total (k_chunk_scan writes it) and no histogram pass runs;
+  // otherwise the histogram pass counts every pass's digits, and pass 0's
+  // count table if it has one.
+  const bool all_tables = table == (1u << passes) - 1u;
+  if (!all_tables) {
+    NTS_LAUNCH(ctx, "sort_hist", k_sort_hist<K>, dim3(grid), dim3(RS_BLOCK), 0, (const K*)keys, d_seg_begin,
+               (const uint32_t*)part_begin, num_segs, lo_bit, first_bits, passes, hist,
+               (table & 1u) ? status : (uint32_t*)nullptr);
+    NTS_LAUNCH(ctx, "sort_offsets", k_sort_offsets, dim3(num_segs * passes), dim3(RS_RADIX), 0,
+               (const uint32_t*)hist, d_seg_begin, passes, 0u, passes, offsets);
+  }
   K* src_k = keys;
   uint32_t* src_v = vals;
   K* dst_k = keys_alt;
@@ -660,13 +671,17 @@ static int onesweep_ws(nts_ctx* ctx, K* keys, uint32_t* vals, K* keys_alt, uint3
     const uint32_t shift = ps ? lo_bit + first_bits + 8 * (ps - 1) : lo_bit;
     const uint32_t dmask = ps ? 255u : (1u << first_bits) - 1u;
     if ((table >> ps) & 1u) {
-      if (ps > 0)  // the count table of this pass's digit over the previous pass's output
+      if (ps > 0 || all_tables)  // the count table of this pass's digit over this pass's input
         NTS_LAUNCH(ctx, "sort_part_count", k_part_count<K>, dim3(max_parts), dim3(RS_BLOCK), 0, (const K*)src_k,
-                   d_seg_begin, (const uint32_t*)part_begin, num_segs, shift, st);
+                   d_seg_begin, (const uint32_t*)part_begin, num_segs, shift, dmask, st);
       NTS_LAUNCH(ctx, "sort_chunk_prefix", k_chunk_prefix, dim3(cgrid), dim3(RS_RADIX), 0, st,
                  (const uint32_t*)part_begin, (const uint32_t*)chunk_begin, num_segs, ctot, max_chunks);
       NTS_LAUNCH(ctx, "sort_chunk_scan", k_chunk_scan, dim3(sgrid), dim3(256), 0, ctot,
-                 (const uint32_t*)chunk_begin, num_segs, (const uint32_t*)offsets, passes, ps, max_chunks);
+                 (const uint32_t*)chunk_begin, num_segs, all_tables ? hist : (uint32_t*)nullptr, passes, ps,
+                 max_chunks);
+      if (all_tables)
+        NTS_LAUNCH(ctx, "sort_offsets", k_sort_offsets, dim3(num_segs), dim3(RS_RADIX), 0, (const uint32_t*)hist,
+                   d_seg_begin, passes, ps, 1u, offsets);
       NTS_LAUNCH(ctx, "sort_onesweep", (k_onesweep<K, true>), dim3(grid), dim3(RS_BLOCK), 0, (const K*)src_k,
                  (const uint32_t*)src_v, dst_k, dst_v, d_seg_begin, (const uint32_t*)part_begin, num_segs,
                  shift, dmask, passes, ps, (const uint32_t*)offsets, st, counters + ps, err, (const uint32_t*)st,
