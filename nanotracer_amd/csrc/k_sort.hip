@@ -457,7 +457,8 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
     const uint64_t end = seg_begin[s + 1];
     uint32_t tbase = 0;  // TABLE: digit tid's first output position for this block
     if (TABLE)
-      tbase = tctot[(size_t)tid * max_chunks + chunk_begin[s] + (p - first_part) / RS_CHUNK] +
+      tbase = offsets[((size_t)s * passes + pass) * RS_RADIX + tid] +
+              tctot[(size_t)tid * max_chunks + chunk_begin[s] + (p - first_part) / RS_CHUNK] +
               tcnt[(size_t)p * RS_RADIX + tid];
 
     // ---- load and rank (order within the block = (wave, round, lane)) ----
@@ -511,7 +512,7 @@ __global__ void __launch_bounds__(RS_BLOCK) __attribute__((amdgpu_waves_per_eu(R
     s_excl[d] = ex;
 
     if (TABLE) {
-      s_gbase[d] = offsets[((size_t)s * passes + pass) * RS_RADIX + d] + tbase - ex;
+      s_gbase[d] = tbase - ex;
     } else {
     // ---- decoupled look-back over the preceding partitions of this segment ----
     uint32_t* st = status;  // this pass's status words (offset by the host)
