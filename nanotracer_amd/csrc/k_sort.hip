@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(RS_BLOCK) k_sort_hist(const K* __restrict__ ke
   flush(s);
 }
 
-// The count table of the digit at bits [shift, shift + 8) over keys in
+// The count table of the digit (key >> shift) & dmask over keys in
 // partition order: cnt[p][d] = the keys of digit d in partition p (the
 // digits of a sorted pass's output repeat without forming runs, so each key
 // adds one into a private copy of the histogram).

@@ -5,12 +5,13 @@
 // single kernel in which a block ranks its 4096 keys per digit with wave64
 // ballots, publishes its per-digit counts and looks back over the preceding
 // blocks of its segment (decoupled look-back) for its global digit offsets,
-// reorders its keys in LDS and writes them out digit-contiguously.  The
-// passes in RS_TABLE_MASK take their block offsets from per-partition digit
-// counts instead (pass 0's from the histogram pass, which reads the keys
-// anyway; later passes' from a counting pass over the previous pass's
-// output), scanned over 16-partition chunks and then over each segment's
-// chunks: no block waits on another.
+// reorders its keys in LDS and writes them out digit-contiguously.  In sorts
+// of RS_TABLE_MIN_PAIRS pairs or more, the passes in RS_TABLE_MASK take their
+// block offsets from per-partition digit counts instead (a counting pass over
+// the pass's input; pass 0's written by the histogram pass when one runs),
+// scanned over 16-partition chunks and then over each segment's chunks: no
+// block waits on another.  When every pass takes counts, the chunk scans also
+// give the digit histograms and no histogram pass runs.
 #pragma once
 #include "nts_internal.h"
 
