@@ -80,6 +80,36 @@ def test_render_views_bit_exact_per_view(ctx, tile):
     b.free()
 
 
+@pytest.mark.parametrize("tile", [32, 16, 8])
+def test_render_views_count_table_sort_with_an_empty_view(ctx, tile):
+    # a pair capacity of 16M or more: the binning sort takes its offsets from
+    # count tables (tile bits 8, 9 and 11 at tiles 32, 16 and 8: one pass by
+    # digit, then 1 + 8 and 3 + 8 bits); one view looks away and has no pair
+    s = synth.make_scene(20_000, 0x5EED0002, 3.0)
+    views = _views(s.n)
+    w, h = 400, 300
+    cam = CameraUniform.from_pose((0, 0, 8), (0, 0, 16), (0, 1, 0), math.pi / 3, w, h, s.n)
+    views.insert(2, (ViewSpec(cam, w, h, 12.0), O.camera_like(cam), (0, 0, 8)))
+    g = GpuSplatBuffer.upload(ctx, s)
+    b = ViewBatch(ctx, 8, s.n, sum(v[0].width * v[0].height for v in views), 20_000_000)
+    b.render(g, [v[0] for v in views], tile)
+    tp, vis, ov = b.stats()
+    assert ov == 0 and tp > 0
+    for k, (spec, ocam, _) in enumerate(views):
+        t = O.tiling(spec.width, spec.height, tile, spec.depth_max)
+        op = O.project(s, ocam)
+        ok, opay, orr = O.bin(op, t, cull=True)
+        keys, pay, rng = b.bins(k, tile)
+        assert np.array_equal(keys, ok) and np.array_equal(pay, opay) and np.array_equal(rng, orr)
+        if k == 2:
+            assert len(ok) == 0 and not rng.any()
+        img, ft, li = O.composite(op, opay, orr, t, with_state=True)
+        assert np.array_equal(b.image(k).view(np.uint32), img.view(np.uint32))
+        gft, gli = b.state(k)
+        assert np.array_equal(gft.view(np.uint32), ft.view(np.uint32)) and np.array_equal(gli, li)
+    b.free()
+
+
 @pytest.mark.parametrize("tile", [16, 8, 32])
 def test_reference_binning_keys_are_the_references(ctx, tile):
     """With BIN_REFERENCE the batched path emits TileBinner::bin's pairs: every
