@@ -209,15 +209,17 @@ def _oracle_batch_step(s, views, targets, tile, mode, m, v, acc, t, cfg, ssim_la
     return _oracle_fused_step(s, total, m, v, acc, t, cfg, cfg), total, losses
 
 
-@pytest.mark.parametrize("mode", [L.BWD_RECOMPUTE, L.BWD_REF_PARITY])
-def test_train_step_views_matches_oracle(ctx, mode):
+# a pair capacity of 16M or more takes the binning sort through its count tables
+@pytest.mark.parametrize("mode,cap", [(L.BWD_RECOMPUTE, 2_000_000), (L.BWD_REF_PARITY, 2_000_000),
+                                      (L.BWD_RECOMPUTE, 20_000_000)])
+def test_train_step_views_matches_oracle(ctx, mode, cap):
     s = synth.make_scene(20_000, 0x5EED0002, 3.0)
     views = _views(s.n)
     tile = 16
     targets = _targets(synth.perturb_means(s, 77), views, tile)
     tbuf = ctx.storage_buffer(np.concatenate([x.reshape(-1) for x in targets]))
     g = GpuSplatBuffer.upload(ctx, s)
-    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), 2_000_000)
+    b = ViewBatch(ctx, 8, s.n, sum(w * h for w, h in RES), cap)
     cfg = AdamConfig()
     for step in (1, 2, 3):
         before = g.readback()
